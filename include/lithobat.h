@@ -1,0 +1,155 @@
+/*
+ * lithobat.h -- C ABI of the MI355X (gfx950) partially coherent imaging
+ * library: the Abbe source-point sum, its sum-of-coherent-systems (SOCS /
+ * Hopkins) form, and the adjoint of both for inverse lithography.
+ *
+ * Plain C types, device pointers and sizes only.  All device buffers are
+ * owned by the caller except plans, which the library allocates and frees.
+ * Every call returns an lb_status; on failure lb_last_error() (thread-local)
+ * describes it.  Calls that take a hipStream_t (passed as void*) are
+ * asynchronous on that stream.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference checkout, src/litho_sim/):
+ *   lb_abbe_plan_create   expose/aerial_image.py:275-304 (source_points) and
+ *                         :392-419 (per-call geometry set-up of _abbe_sum)
+ *   lb_forward            expose/aerial_image.py:351-494 (_abbe_sum) behind
+ *                         compute_aerial_image :502-548 and
+ *                         compute_aerial_planes :551-587; for SOCS plans
+ *                         expose/hopkins.py:205-218 (SOCSKernels.raw) and
+ *                         opc/ilt.py:208-232 (socs_fields, intensity part)
+ *   lb_backward           opc/ilt.py:235-266 (socs_gradient) -- the adjoint
+ *                         of lb_forward, recomputing the fields
+ *   lb_socs_plan_create   expose/hopkins.py:103-198 (SOCSKernels.build)
+ *   lb_socs_kernels       the SOCSKernels.kernels attribute (hopkins.py:87)
+ *   lb_fields             opc/ilt.py:208-232 (socs_fields, the fields part)
+ *   lb_plan_clear         the `clear` scalar of _abbe_sum / SOCSKernels
+ */
+#ifndef LITHOBAT_H_
+#define LITHOBAT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LB_OK = 0,
+  LB_ERR_ARG = 1,          /* bad argument, size or shape */
+  LB_ERR_UNSUPPORTED = 2,  /* valid request this build does not implement */
+  LB_ERR_HIP = 3,          /* HIP runtime error */
+  LB_ERR_OOM = 4,          /* device allocation failed */
+  LB_ERR_WORKSPACE = 5     /* workspace too small */
+} lb_status;
+
+typedef enum {
+  LB_POL_UNPOLARISED = 0,
+  LB_POL_X = 1,
+  LB_POL_Y = 2,
+  LB_POL_TE = 3,
+  LB_POL_TM = 4
+} lb_polarisation;
+
+typedef struct lb_plan_s* lb_plan;
+
+/* Optics, grid and source of one imaging set-up (OpticsConfig + GridConfig +
+ * SourcePoints of the reference, core/config.py:383-406, 741-744;
+ * expose/aerial_image.py:249-272). */
+typedef struct {
+  int32_t n;               /* grid side N (pixels) */
+  double pixel_size;       /* [m] */
+  double wavelength;       /* [m] */
+  double na;
+  int32_t vector;          /* 0 scalar, 1 vector (three field components) */
+  int32_t polarisation;    /* lb_polarisation, vector model only */
+  int32_t obliquity;       /* sqrt(cos theta) factor, vector model only */
+  int32_t exact_defocus;   /* non-paraxial defocus OPD (forced on for vector) */
+  int32_t n_planes;        /* image planes imaged together */
+  const double* defocus;   /* [n_planes] defocus [m] */
+  const double* n_image;   /* [n_planes] image-medium index */
+  int32_t n_zernike;       /* Noll terms, j in 1..21 (others contribute 0) */
+  const int32_t* zernike_j;
+  const double* zernike_c; /* [n_zernike] coefficients in waves */
+  int32_t n_source;        /* source points */
+  const double* src_weight;
+  const double* src_fx;    /* [1/m] */
+  const double* src_fy;    /* [1/m] */
+  const double* src_phi;   /* azimuth [rad] */
+  int32_t precision;       /* 32: complex64 / FP32 accumulate; 64: complex128 */
+  int32_t coarse_m;        /* 0 = automatic coarse-grid side, else forced */
+  int32_t chunk;           /* 0 = automatic systems per work item */
+} lb_imaging_desc;
+
+typedef struct {
+  int32_t n;               /* N */
+  int32_t m;               /* coarse grid side (== n when imaged directly) */
+  int32_t box;             /* W: pupil box side in frequency bins */
+  int32_t band;            /* side of the mask-spectrum band */
+  int32_t n_planes;
+  int32_t n_systems;       /* coherent systems per plane */
+  int32_t precision;
+  int32_t n_kernels;       /* SOCS plans: kernels kept (else 0) */
+  int32_t block_threads;   /* chosen coherent-system kernel shape */
+  int32_t column_splits;
+  int32_t chunk;           /* systems per work item */
+  int32_t grid_blocks;     /* persistent blocks of the coherent-system kernel */
+} lb_plan_info;
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int32_t lb_version(void);
+/* Last error message of this thread ("" when none). */
+const char* lb_last_error(void);
+
+/* Abbe plan: builds the source table, pupil boxes and the per-system transfer
+ * tables (FP64 on the device) for every plane, on the current device. */
+lb_status lb_abbe_plan_create(const lb_imaging_desc* desc, lb_plan* out);
+
+/* SOCS plan (scalar model, one plane): TCC eigen-decomposition in FP64 on the
+ * device, kernels kept where lambda > tol * lambda_max and lambda > 0, capped at
+ * max_kernels (< 0: no cap).  The kernels are used in desc->precision. */
+lb_status lb_socs_plan_create(const lb_imaging_desc* desc, double tol, int32_t max_kernels,
+                              lb_plan* out);
+
+lb_status lb_plan_destroy(lb_plan plan);
+lb_status lb_plan_info_get(lb_plan plan, lb_plan_info* info);
+
+/* Open-frame throughput per plane (host array of n_planes). */
+lb_status lb_plan_clear(lb_plan plan, double* clear_per_plane);
+
+/* SOCS plans: eigenvalues of the kept kernels (host, n_kernels), the
+ * trace-normalised energy kept, and the band size n_band. */
+lb_status lb_socs_weights(lb_plan plan, double* weights, double* energy_kept,
+                          int32_t* n_band, int32_t* n_source);
+/* SOCS plans: kernels on the full grid, FFT order, complex128, [K][N][N]
+ * (device, caller allocated). */
+lb_status lb_socs_kernels(lb_plan plan, void* kernels_c128, void* stream);
+
+/* Workspace bytes lb_forward / lb_backward need for n_masks masks. */
+size_t lb_workspace_bytes(lb_plan plan, int32_t n_masks, int32_t backward);
+
+/* Raw (un-normalised, un-dosed) images of n_masks masks on every plane:
+ * images[mask][plane][N][N] (real, precision of the plan) and the per-image
+ * peak (may be NULL).  masks: [n_masks][N][N] real, or interleaved complex
+ * when mask_complex != 0, in the plan's precision. */
+lb_status lb_forward(lb_plan plan, const void* masks, int32_t mask_complex, int32_t n_masks,
+                     void* images, void* peak, void* workspace, size_t workspace_bytes,
+                     void* stream);
+
+/* d_masks[mask][N][N] = d/d mask of sum_{plane,x} d_images[mask][plane][x] * I(x)
+ * for real masks (fields recomputed, not stored). */
+lb_status lb_backward(lb_plan plan, const void* masks, int32_t n_masks, const void* d_images,
+                      void* d_masks, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Coherent fields of each system on the full grid, complex, [n_masks][n_sys][N][N]
+ * for plane 0 (SOCS: one per kernel, the `fields` of socs_fields). */
+lb_status lb_fields(lb_plan plan, const void* masks, int32_t mask_complex, int32_t n_masks,
+                    void* fields, void* workspace, size_t workspace_bytes, void* stream);
+size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LITHOBAT_H_ */

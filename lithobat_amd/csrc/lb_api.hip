@@ -1,0 +1,843 @@
+// C ABI of the imaging library (declared in include/lithobat.h).
+//
+// A plan is one optical set-up: its coherent systems (Abbe: source point x
+// plane x field component; SOCS: eigen-kernel), their transfer tables on the
+// device, the box / band geometry and the coherent-system kernel shape.
+// lb_forward / lb_backward are pure launch sequences on the caller's stream
+// over a caller-provided workspace: no allocation, no host synchronisation,
+// so they can be captured into a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/lithobat.h"
+#include "lb_kernels.h"
+
+using namespace lb;
+
+namespace {
+
+thread_local std::string g_err;
+
+lb_status fail(lb_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define LB_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(e_ == hipErrorOutOfMemory ? LB_ERR_OOM : LB_ERR_HIP,                  \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+constexpr size_t kLdsLimit = 160 * 1024;
+const int kFieldSizes[] = {16, 32, 64, 128, 256};
+
+// Noll j -> (n, m), expose/pupil.py:56-78.
+const int kNoll[22][2] = {{0, 0},  {0, 0},  {1, 1},  {1, -1}, {2, 0},  {2, -2}, {2, 2},  {3, -1},
+                          {3, 1},  {3, -3}, {3, 3},  {4, 0},  {4, 2},  {4, -2}, {4, 4},  {4, -4},
+                          {5, 1},  {5, -1}, {5, 3},  {5, -3}, {5, 5},  {5, -5}};
+
+double factorial(int n) {
+  double f = 1;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return f;
+}
+
+struct Plan {
+  int precision = 32;
+  int N = 0;
+  double px = 0, lam = 0, na = 0, k_max = 0, val = 0;
+  int vector = 0, pol = 0, obliq = 1, exact = 0;
+  std::vector<double> defocus, n_image;
+  PupilParams pp{};
+  int S = 0;
+  // geometry
+  int W = 0, M = 0, direct = 0;
+  int QI = 0, WbI = 0;
+  int nu0m = 0, Wbm = 0, wrapm = 0;
+  int n_planes = 1, spp = 0;  // systems per plane
+  int chunk_override = 0;
+  FieldConfig fwd{}, adj{};
+  int cus = 256;
+  // device tables
+  void* H = nullptr;
+  int2* box = nullptr;
+  void* wsys = nullptr;
+  std::vector<double> clear;
+  // SOCS
+  int socs = 0;
+  int K = 0;
+  std::vector<double> weights;
+  double energy = 1.0;
+  int n_band = 0;
+  void* phi128 = nullptr;  // [K][W][W] complex128
+
+  ~Plan() {
+    if (H) (void)hipFree(H);
+    if (box) (void)hipFree(box);
+    if (wsys) (void)hipFree(wsys);
+    if (phi128) (void)hipFree(phi128);
+  }
+  size_t csz() const { return precision == 64 ? 16 : 8; }
+  size_t rsz() const { return precision == 64 ? 8 : 4; }
+};
+
+bool field_size(int m) {
+  for (int s : kFieldSizes)
+    if (s == m) return true;
+  return false;
+}
+
+// Coherent-system kernel shape for (M, W): prefer no column split, then the
+// smallest split whose LDS fits (the adjoint keeps two W x M/CS tables).
+bool pick_field_configs(Plan& p) {
+  struct Cand {
+    int m, nt, cs;
+  };
+  const Cand cands[] = {{16, 64, 1},   {32, 128, 1},  {64, 256, 1},  {64, 128, 1},
+                        {64, 256, 2},  {128, 256, 1}, {128, 512, 1}, {128, 256, 2},
+                        {128, 256, 4}, {256, 256, 2}, {256, 256, 4}, {256, 256, 8}};
+  bool got_f = false, got_a = false;
+  for (const Cand& c : cands) {
+    if (c.m != p.M) continue;
+    FieldConfig fc{c.m, c.nt, c.cs, 0};
+    if (!got_f && field_smem_bytes(p.precision, fc, p.W, false) <= kLdsLimit) {
+      p.fwd = fc;
+      got_f = true;
+    }
+    if (!got_a && field_smem_bytes(p.precision, fc, p.W, true) <= kLdsLimit) {
+      p.adj = fc;
+      got_a = true;
+    }
+  }
+  return got_f && got_a;
+}
+
+lb_status set_geometry_common(Plan& p, const lb_imaging_desc* d) {
+  p.precision = d->precision == 64 ? 64 : 32;
+  p.N = d->n;
+  p.px = d->pixel_size;
+  p.lam = d->wavelength;
+  p.na = d->na;
+  p.k_max = d->na / d->wavelength;  // aerial_image.py:131
+  p.val = 1.0 / ((double)d->n * d->pixel_size);  // numpy fftfreq step
+  p.vector = d->vector ? 1 : 0;
+  p.pol = d->polarisation;
+  p.obliq = d->obliquity ? 1 : 0;
+  p.exact = (d->exact_defocus || d->vector) ? 1 : 0;  // aerial_image.py:392
+  p.chunk_override = d->chunk > 0 ? d->chunk : 0;
+  PupilParams& pp = p.pp;
+  std::memset(&pp, 0, sizeof(pp));
+  pp.N = p.N;
+  pp.val = p.val;
+  pp.k_max = p.k_max;
+  pp.wavelength = p.lam;
+  pp.NA = p.na;
+  pp.exact = p.exact;
+  pp.obliquity = p.obliq;
+  int nz = 0;
+  for (int t = 0; t < d->n_zernike; ++t) {
+    const int j = d->zernike_j[t];
+    if (j < 1 || j > 21) continue;  // pupil.py:128-130: unknown Noll index contributes 0
+    if (nz >= kMaxZernike) return fail(LB_ERR_ARG, "too many Zernike terms");
+    const int n = kNoll[j][0], m = kNoll[j][1], ma = std::abs(m);
+    pp.zn[nz] = n;
+    pp.zm[nz] = m;
+    pp.znorm[nz] = m == 0 ? std::sqrt((double)(n + 1)) : std::sqrt(2.0 * (n + 1));
+    pp.zcoef[nz] = d->zernike_c[t];
+    for (int s = 0; s < 3; ++s) pp.zrad[nz][s] = 0.0;
+    for (int s = 0; s <= (n - ma) / 2; ++s) {  // pupil.py:89-99
+      pp.zrad[nz][s] = ((s % 2) ? -1.0 : 1.0) * factorial(n - s) /
+                       (factorial(s) * factorial((n + ma) / 2 - s) * factorial((n - ma) / 2 - s));
+    }
+    ++nz;
+  }
+  pp.n_zern = nz;
+  int dev = 0;
+  LB_HIP(hipGetDevice(&dev));
+  LB_HIP(hipDeviceGetAttribute(&p.cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return LB_OK;
+}
+
+lb_status check_desc(const lb_imaging_desc* d) {
+  if (!d) return fail(LB_ERR_ARG, "null descriptor");
+  if (d->n < 2) return fail(LB_ERR_ARG, "grid side must be >= 2");
+  if (!band_size_supported(d->n))
+    return fail(LB_ERR_UNSUPPORTED,
+                "grid side " + std::to_string(d->n) +
+                    " is not a supported FFT size (2^k for 8..2048, or 3*2^k for 48..1536)");
+  if (!(d->pixel_size > 0) || !(d->wavelength > 0) || !(d->na > 0))
+    return fail(LB_ERR_ARG, "pixel_size, wavelength and NA must be > 0");
+  if (d->n_planes < 1 || !d->defocus || !d->n_image) return fail(LB_ERR_ARG, "need >= 1 plane");
+  if (d->n_source < 0) return fail(LB_ERR_ARG, "negative source count");
+  if (d->n_source > 0 && (!d->src_weight || !d->src_fx || !d->src_fy || !d->src_phi))
+    return fail(LB_ERR_ARG, "null source arrays");
+  if (d->polarisation < 0 || d->polarisation > 4) return fail(LB_ERR_ARG, "bad polarisation");
+  if (d->n_zernike > 0 && (!d->zernike_j || !d->zernike_c))
+    return fail(LB_ERR_ARG, "null Zernike arrays");
+  return LB_OK;
+}
+
+// Box side, coarse grid side, image band.
+lb_status choose_grid(Plan& p, int Wbox, int forced_m) {
+  p.W = std::min(Wbox, p.N);
+  int M = 0;
+  if (forced_m > 0) {
+    if (forced_m != p.N && forced_m < 2 * p.W - 1)
+      return fail(LB_ERR_ARG, "coarse grid side must be >= 2W-1 or equal N");
+    M = forced_m;
+  } else {
+    for (int s : kFieldSizes)
+      if (s >= 2 * p.W - 1) {
+        M = s;
+        break;
+      }
+    if (M == 0 || M >= p.N) M = p.N;
+  }
+  if (!field_size(M))
+    return fail(LB_ERR_UNSUPPORTED, "coarse grid side " + std::to_string(M) +
+                                        " unsupported (pupil box " + std::to_string(p.W) +
+                                        " bins on a " + std::to_string(p.N) + " grid)");
+  p.M = M;
+  p.direct = (M == p.N) ? 1 : 0;
+  p.QI = p.W - 1;
+  p.WbI = 2 * p.W - 1;
+  if (!p.direct && !band_size_supported(M))
+    return fail(LB_ERR_UNSUPPORTED, "coarse grid side not a band size");
+  if (!pick_field_configs(p))
+    return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel shape fits LDS for M=" +
+                                        std::to_string(M) + " W=" + std::to_string(p.W));
+  return LB_OK;
+}
+
+template <typename T>
+lb_status upload_tables(Plan& p, const std::vector<PupilSys>& sys, const std::vector<int2>& boxes,
+                        const std::vector<double>& w, std::vector<double>* dc_host) {
+  const int n = (int)sys.size();
+  PupilSys* sys_dev = nullptr;
+  double* dc = nullptr;
+  LB_HIP(hipMalloc(&p.H, sizeof(cx<T>) * (size_t)std::max(n, 1) * p.W * p.W));
+  LB_HIP(hipMalloc(&p.box, sizeof(int2) * std::max(n, 1)));
+  LB_HIP(hipMalloc(&p.wsys, sizeof(T) * std::max(n, 1)));
+  LB_HIP(hipMalloc(&sys_dev, sizeof(PupilSys) * std::max(n, 1)));
+  LB_HIP(hipMalloc(&dc, sizeof(double) * std::max(n, 1)));
+  std::vector<T> wt(w.begin(), w.end());
+  lb_status rc = LB_OK;
+  if (n > 0) {
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMemcpy(p.box, boxes.data(), sizeof(int2) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p.wsys, wt.data(), sizeof(T) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(sys_dev, sys.data(), sizeof(PupilSys) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dc, 0, sizeof(double) * n);
+    p.pp.W = p.W;
+    if (e == hipSuccess) e = pupil_tables<T>(sys_dev, n, p.pp, (cx<T>*)p.H, dc, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && dc_host) {
+      dc_host->resize(n);
+      e = hipMemcpy(dc_host->data(), dc, sizeof(double) * n, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) rc = fail(LB_ERR_HIP, std::string("pupil tables: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(sys_dev);
+  (void)hipFree(dc);
+  return rc;
+}
+
+// ---------------------------------------------------------------- workspace
+struct Layout {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  }
+};
+
+struct Work {
+  int n_masks, n_img, n_chunks, chunk, n_items;
+  size_t Xm, RTm, partial, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, peak;
+  size_t total;
+};
+
+Work layout(const Plan& p, int n_masks, bool backward) {
+  Work w{};
+  w.n_masks = n_masks;
+  w.n_img = n_masks * p.n_planes;
+  int chunk = p.chunk_override;
+  if (chunk <= 0) {
+    const long long total = (long long)w.n_img * p.spp;
+    const long long target = 4LL * p.cus;
+    int n_chunks = (int)std::max<long long>(1, std::min<long long>(p.spp, (target + w.n_img - 1) / std::max(w.n_img, 1)));
+    (void)total;
+    chunk = (p.spp + n_chunks - 1) / n_chunks;
+  }
+  chunk = std::max(1, std::min(chunk, std::max(p.spp, 1)));
+  w.chunk = chunk;
+  w.n_chunks = std::max(1, (p.spp + chunk - 1) / chunk);
+  w.n_items = w.n_img * w.n_chunks;
+  const size_t C = p.csz(), R = p.rsz();
+  Layout L;
+  w.Xm = L.take(C * n_masks * p.Wbm * p.Wbm);
+  w.RTm = L.take(C * (size_t)n_masks * p.Wbm * p.N);
+  w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
+  w.peak = L.take(R * (size_t)w.n_img);
+  if (!p.direct) {
+    w.RTc = L.take(C * (size_t)w.n_img * p.WbI * p.M);
+    w.Y = L.take(C * (size_t)w.n_img * p.WbI * p.WbI);
+    w.TT = L.take(C * (size_t)w.n_img * p.WbI * p.N);
+  }
+  if (backward) {
+    if (!p.direct) {
+      w.gM = L.take(R * (size_t)w.n_img * p.M * p.M);
+      w.RTg = L.take(C * (size_t)w.n_img * p.WbI * p.N);
+      w.GY = L.take(C * (size_t)w.n_img * p.WbI * p.WbI);
+      w.TTc = L.take(C * (size_t)w.n_img * p.WbI * p.M);
+    }
+    w.gX = L.take(C * (size_t)w.n_items * p.Wbm * p.Wbm);
+    w.GX = L.take(C * (size_t)n_masks * p.Wbm * p.Wbm);
+    w.TTm = L.take(C * (size_t)n_masks * p.Wbm * p.N);
+  }
+  w.total = L.off;
+  return w;
+}
+
+FieldArgs field_args(const Plan& p, const void* Xm, const Work& w) {
+  FieldArgs a{};
+  a.X = Xm;
+  a.Wb = p.Wbm;
+  a.wrapN = p.wrapm;
+  a.H = p.H;
+  a.W = p.W;
+  a.box = p.box;
+  a.wsys = p.wsys;
+  a.map = ItemMap{w.n_chunks, p.n_planes, p.spp, w.chunk, w.n_items};
+  return a;
+}
+
+int grid_for(const Plan& p, const FieldConfig& fc, int n_items, bool adjoint) {
+  const size_t sm = field_smem_bytes(p.precision, fc, p.W, adjoint);
+  int occ = (int)std::max<size_t>(1, kLdsLimit / std::max<size_t>(sm, 1));
+  occ = std::min(occ, std::max(1, 2048 / fc.nt));
+  return std::max(1, std::min(n_items, p.cus * occ));
+}
+
+template <typename T>
+lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* images, void* peak,
+                    char* ws, const Work& w, hipStream_t st) {
+  const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
+  cx<T>* Xm = (cx<T>*)(ws + w.Xm);
+  LB_HIP(band_analysis<T>(p.N, masks, cplx, (long long)p.N * p.N * (cplx ? 2 : 1), 0, 1, n_masks,
+                          p.nu0m, p.Wbm, invN2, (cx<T>*)(ws + w.RTm), Xm, st));
+  FieldConfig fc = p.fwd;
+  fc.grid = grid_for(p, fc, w.n_items, false);
+  FieldArgs a = field_args(p, Xm, w);
+  T* partial = (T*)(ws + w.partial);
+  LB_HIP(field_forward(p.precision, fc, a, partial, st));
+  T* pk = peak ? (T*)peak : (T*)(ws + w.peak);
+  LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
+  if (p.direct) {
+    LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.N * p.N, (T*)images, pk, st));
+  } else {
+    const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
+    cx<T>* Y = (cx<T>*)(ws + w.Y);
+    LB_HIP(band_analysis<T>(p.M, partial, 0, (long long)w.n_chunks * p.M * p.M,
+                            (long long)p.M * p.M, w.n_chunks, w.n_img, -p.QI, p.WbI, invM2,
+                            (cx<T>*)(ws + w.RTc), Y, st));
+    LB_HIP(band_synthesis<T>(p.N, Y, w.n_img, -p.QI, p.WbI, 1, (T)1, 0, (cx<T>*)(ws + w.TT),
+                             (T*)images, pk, st));
+  }
+  return LB_OK;
+}
+
+template <typename T>
+lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, void* dmask,
+                     char* ws, const Work& w, hipStream_t st) {
+  const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
+  cx<T>* Xm = (cx<T>*)(ws + w.Xm);
+  LB_HIP(band_analysis<T>(p.N, masks, 0, (long long)p.N * p.N, 0, 1, n_masks, p.nu0m, p.Wbm,
+                          invN2, (cx<T>*)(ws + w.RTm), Xm, st));
+  const T* gM = (const T*)dimg;
+  if (!p.direct) {
+    const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
+    cx<T>* GY = (cx<T>*)(ws + w.GY);
+    LB_HIP(band_analysis<T>(p.N, dimg, 0, (long long)p.N * p.N, 0, 1, w.n_img, -p.QI, p.WbI,
+                            (T)1, (cx<T>*)(ws + w.RTg), GY, st));
+    LB_HIP(band_synthesis<T>(p.M, GY, w.n_img, -p.QI, p.WbI, 1, invM2, 0, (cx<T>*)(ws + w.TTc),
+                             (T*)(ws + w.gM), (T*)nullptr, st));
+    gM = (const T*)(ws + w.gM);
+  }
+  FieldConfig fc = p.adj;
+  fc.grid = grid_for(p, fc, w.n_items, true);
+  FieldArgs a = field_args(p, Xm, w);
+  cx<T>* gX = (cx<T>*)(ws + w.gX);
+  LB_HIP(field_adjoint(p.precision, fc, a, gM, gX, st));
+  cx<T>* GX = (cx<T>*)(ws + w.GX);
+  LB_HIP(segment_sum<T>((const T*)gX, n_masks, p.n_planes * w.n_chunks,
+                        2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, st));
+  LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, 1, invN2, 0, (cx<T>*)(ws + w.TTm),
+                           (T*)dmask, (T*)nullptr, st));
+  return LB_OK;
+}
+
+// mask band covering every box: symmetric [-Q, Q], or the whole grid
+void set_mask_band(Plan& p, int Q) {
+  if (2 * Q + 1 > p.N) {
+    p.nu0m = -(p.N / 2);
+    p.Wbm = p.N;
+    p.wrapm = p.N;
+  } else {
+    p.nu0m = -Q;
+    p.Wbm = 2 * Q + 1;
+    p.wrapm = 0;
+  }
+}
+
+}  // namespace
+
+struct lb_plan_s {
+  Plan p;
+};
+
+extern "C" {
+
+int32_t lb_version(void) { return 100; }
+
+const char* lb_last_error(void) { return g_err.c_str(); }
+
+lb_status lb_abbe_plan_create(const lb_imaging_desc* d, lb_plan* out) {
+  g_err.clear();
+  if (!out) return fail(LB_ERR_ARG, "null output plan");
+  *out = nullptr;
+  lb_status rc = check_desc(d);
+  if (rc != LB_OK) return rc;
+  lb_plan h = new lb_plan_s();
+  Plan& p = h->p;
+  rc = set_geometry_common(p, d);
+  if (rc != LB_OK) {
+    delete h;
+    return rc;
+  }
+  p.S = d->n_source;
+  p.n_planes = d->n_planes;
+  p.defocus.assign(d->defocus, d->defocus + d->n_planes);
+  p.n_image.assign(d->n_image, d->n_image + d->n_planes);
+  const double r = p.k_max / p.val;  // pupil radius in bins
+  const int Wbox = (int)std::floor(2.0 * r) + 3;
+  rc = choose_grid(p, Wbox, d->coarse_m);
+  if (rc != LB_OK) {
+    delete h;
+    return rc;
+  }
+  // boxes per source point (box coordinates: origin frequency index)
+  std::vector<int> oy(p.S), ox(p.S);
+  int Q = 0;
+  for (int s = 0; s < p.S; ++s) {
+    oy[s] = (int)std::floor(d->src_fy[s] / p.val - r);
+    ox[s] = (int)std::floor(d->src_fx[s] / p.val - r);
+    Q = std::max({Q, std::abs(oy[s]), std::abs(oy[s] + p.W - 1), std::abs(ox[s]),
+                  std::abs(ox[s] + p.W - 1)});
+  }
+  set_mask_band(p, Q);
+  // Jones states, pupil.py:272-285
+  struct St {
+    double jx, jy, w;
+  };
+  auto states = [&](double phi_s) {
+    std::vector<St> v;
+    switch (p.pol) {
+      case LB_POL_X: v.push_back({1.0, 0.0, 1.0}); break;
+      case LB_POL_Y: v.push_back({0.0, 1.0, 1.0}); break;
+      case LB_POL_TE: v.push_back({-std::sin(phi_s), std::cos(phi_s), 1.0}); break;
+      case LB_POL_TM: v.push_back({std::cos(phi_s), std::sin(phi_s), 1.0}); break;
+      default:
+        v.push_back({1.0, 0.0, 0.5});
+        v.push_back({0.0, 1.0, 0.5});
+    }
+    return v;
+  };
+  const int per_source = p.vector ? (p.pol == LB_POL_UNPOLARISED ? 6 : 3) : 1;
+  p.spp = p.S * per_source;
+  std::vector<PupilSys> sys;
+  std::vector<int2> boxes;
+  std::vector<double> wts;
+  std::vector<int> plane_of;
+  sys.reserve((size_t)p.spp * p.n_planes);
+  for (int pl = 0; pl < p.n_planes; ++pl) {
+    for (int s = 0; s < p.S; ++s) {
+      PupilSys base{};
+      base.fsx = d->src_fx[s];
+      base.fsy = d->src_fy[s];
+      base.weight = 1.0;
+      base.defocus = p.defocus[pl];
+      base.n_image = p.n_image[pl];
+      base.box_y = oy[s];
+      base.box_x = ox[s];
+      const int2 bi = make_int2(oy[s] - p.nu0m, ox[s] - p.nu0m);
+      if (!p.vector) {
+        base.comp = -1;
+        sys.push_back(base);
+        boxes.push_back(bi);
+        wts.push_back(d->src_weight[s]);
+        plane_of.push_back(pl);
+        continue;
+      }
+      for (const St& stt : states(d->src_phi[s])) {
+        for (int c = 0; c < 3; ++c) {
+          PupilSys v = base;
+          v.jx = stt.jx;
+          v.jy = stt.jy;
+          v.comp = c;
+          sys.push_back(v);
+          boxes.push_back(bi);
+          wts.push_back(d->src_weight[s] * stt.w);
+          plane_of.push_back(pl);
+        }
+      }
+    }
+  }
+  std::vector<double> dc;
+  rc = p.precision == 64 ? upload_tables<double>(p, sys, boxes, wts, &dc)
+                         : upload_tables<float>(p, sys, boxes, wts, &dc);
+  if (rc != LB_OK) {
+    delete h;
+    return rc;
+  }
+  p.clear.assign(p.n_planes, 0.0);
+  for (size_t j = 0; j < sys.size(); ++j) p.clear[plane_of[j]] += wts[j] * dc[j];
+  *out = h;
+  return LB_OK;
+}
+
+lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_kernels,
+                              lb_plan* out) {
+  g_err.clear();
+  if (!out) return fail(LB_ERR_ARG, "null output plan");
+  *out = nullptr;
+  lb_status rc = check_desc(d);
+  if (rc != LB_OK) return rc;
+  if (d->vector) return fail(LB_ERR_ARG, "SOCS kernels apply to the scalar thin-mask process only");
+  if (d->n_planes != 1) return fail(LB_ERR_ARG, "SOCS plans image one plane");
+  lb_plan h = new lb_plan_s();
+  Plan& p = h->p;
+  rc = set_geometry_common(p, d);
+  if (rc != LB_OK) {
+    delete h;
+    return rc;
+  }
+  p.socs = 1;
+  p.S = d->n_source;
+  p.n_planes = 1;
+  p.defocus.assign(1, d->defocus[0]);
+  p.n_image.assign(1, d->n_image[0]);
+  const double r = p.k_max / p.val;
+  // the band |f| <= 2 NA / lambda (hopkins.py:144-147), as a square box
+  int Qb = (int)std::floor(2.0 * r * (1.0 + 1e-9)) + 1;
+  set_mask_band(p, Qb);
+  const int Wt = p.Wbm;
+  rc = choose_grid(p, Wt, d->coarse_m);
+  if (rc != LB_OK) {
+    delete h;
+    return rc;
+  }
+  if (p.W != Wt) {
+    delete h;
+    return fail(LB_ERR_UNSUPPORTED, "SOCS band wider than the grid");
+  }
+  // n_band as the reference counts it
+  {
+    int cnt = 0;
+    const double lim = 2.0 * p.k_max + 1e-9 * p.k_max;
+    for (int i = 0; i < p.N; ++i)
+      for (int j = 0; j < p.N; ++j) {
+        const int nuy = i <= (p.N - 1) / 2 ? i : i - p.N;
+        const int nux = j <= (p.N - 1) / 2 ? j : j - p.N;
+        if (std::hypot(nux * p.val, nuy * p.val) <= lim) ++cnt;
+      }
+    p.n_band = cnt;
+  }
+  const int S = p.S;
+  const long long F = (long long)Wt * Wt;
+  // A[s][f] = sqrt(w_s) P_s(f) on the band box, FP64 (hopkins.py:155-167)
+  std::vector<PupilSys> sys(S);
+  for (int s = 0; s < S; ++s) {
+    PupilSys v{};
+    v.fsx = d->src_fx[s];
+    v.fsy = d->src_fy[s];
+    v.weight = std::sqrt(d->src_weight[s]);
+    v.defocus = p.defocus[0];
+    v.n_image = p.n_image[0];
+    v.comp = -1;
+    v.box_y = p.nu0m;
+    v.box_x = p.nu0m;
+    sys[s] = v;
+  }
+  cx<double>* A = nullptr;
+  cx<double>* G = nullptr;
+  cx<double>* V = nullptr;
+  double* ev = nullptr;
+  int* sweeps = nullptr;
+  PupilSys* sys_dev = nullptr;
+  double* dc = nullptr;
+  int* idx_dev = nullptr;
+  double* isq_dev = nullptr;
+  auto cleanup = [&]() {
+    (void)hipFree(A);
+    (void)hipFree(G);
+    (void)hipFree(V);
+    (void)hipFree(ev);
+    (void)hipFree(sweeps);
+    (void)hipFree(sys_dev);
+    (void)hipFree(dc);
+    (void)hipFree(idx_dev);
+    (void)hipFree(isq_dev);
+  };
+#define LB_HIP_C(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      cleanup();                                                                        \
+      delete h;                                                                         \
+      return fail(e_ == hipErrorOutOfMemory ? LB_ERR_OOM : LB_ERR_HIP,                  \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    }                                                                                   \
+  } while (0)
+  const int Sa = std::max(S, 1);
+  LB_HIP_C(hipMalloc(&A, sizeof(cx<double>) * Sa * F));
+  LB_HIP_C(hipMalloc(&G, sizeof(cx<double>) * Sa * Sa));
+  LB_HIP_C(hipMalloc(&V, sizeof(cx<double>) * Sa * Sa));
+  LB_HIP_C(hipMalloc(&ev, sizeof(double) * Sa));
+  LB_HIP_C(hipMalloc(&sweeps, sizeof(int)));
+  LB_HIP_C(hipMalloc(&sys_dev, sizeof(PupilSys) * Sa));
+  LB_HIP_C(hipMalloc(&dc, sizeof(double) * Sa));
+  std::vector<double> evals(S);
+  if (S > 0) {
+    LB_HIP_C(hipMemcpy(sys_dev, sys.data(), sizeof(PupilSys) * S, hipMemcpyHostToDevice));
+    p.pp.W = Wt;
+    LB_HIP_C(pupil_tables<double>(sys_dev, S, p.pp, A, dc, 0));
+    LB_HIP_C(gram_f64(A, S, F, G, 0));
+    LB_HIP_C(jacobi_eigh_f64(G, V, S, 40, ev, sweeps, 0));
+    LB_HIP_C(hipDeviceSynchronize());
+    LB_HIP_C(hipMemcpy(evals.data(), ev, sizeof(double) * S, hipMemcpyDeviceToHost));
+  }
+  // order, truncation: hopkins.py:172-182
+  std::vector<int> order(S);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return evals[a] > evals[b]; });
+  double trace = 0;
+  for (double e : evals) trace += std::max(e, 0.0);
+  const double lmax = S ? evals[order[0]] : 0.0;
+  std::vector<int> keep;
+  std::vector<double> isq;
+  for (int i = 0; i < S; ++i) {
+    const double e = evals[order[i]];
+    if (!(e > tol * lmax) || !(e > 0.0)) continue;
+    if (max_kernels >= 0 && (int)keep.size() >= max_kernels) break;
+    keep.push_back(order[i]);
+    p.weights.push_back(e);
+    isq.push_back(1.0 / std::sqrt(e));
+  }
+  // (the reference applies the cap by position after the tolerance; kernels
+  // failing the tolerance are all at the tail of the descending order)
+  p.K = (int)keep.size();
+  double kept = 0;
+  for (double e : p.weights) kept += e;
+  p.energy = trace > 0 ? kept / trace : 1.0;
+  const int Ka = std::max(p.K, 1);
+  LB_HIP_C(hipMalloc(&p.phi128, sizeof(cx<double>) * Ka * F));
+  LB_HIP_C(hipMalloc(&idx_dev, sizeof(int) * Ka));
+  LB_HIP_C(hipMalloc(&isq_dev, sizeof(double) * Ka));
+  if (p.K > 0) {
+    LB_HIP_C(hipMemcpy(idx_dev, keep.data(), sizeof(int) * p.K, hipMemcpyHostToDevice));
+    LB_HIP_C(hipMemcpy(isq_dev, isq.data(), sizeof(double) * p.K, hipMemcpyHostToDevice));
+    LB_HIP_C(socs_phi_f64(A, V, S, F, idx_dev, isq_dev, p.K, (cx<double>*)p.phi128, 0));
+  }
+  // systems: one per kernel, weight lambda_k, box = the whole band
+  p.spp = p.K;
+  const size_t csz = p.csz();
+  LB_HIP_C(hipMalloc(&p.H, csz * Ka * F));
+  LB_HIP_C(hipMalloc(&p.box, sizeof(int2) * Ka));
+  LB_HIP_C(hipMalloc(&p.wsys, p.rsz() * Ka));
+  if (p.K > 0) {
+    if (p.precision == 64) {
+      LB_HIP_C(hipMemcpy(p.H, p.phi128, csz * p.K * F, hipMemcpyDeviceToDevice));
+      LB_HIP_C(hipMemcpy(p.wsys, p.weights.data(), sizeof(double) * p.K, hipMemcpyHostToDevice));
+    } else {
+      LB_HIP_C(cast_c128_c64((const cx<double>*)p.phi128, (cx<float>*)p.H, (long long)p.K * F, 0));
+      std::vector<float> wf(p.weights.begin(), p.weights.end());
+      LB_HIP_C(hipMemcpy(p.wsys, wf.data(), sizeof(float) * p.K, hipMemcpyHostToDevice));
+    }
+    std::vector<int2> boxes(p.K, make_int2(0, 0));
+    LB_HIP_C(hipMemcpy(p.box, boxes.data(), sizeof(int2) * p.K, hipMemcpyHostToDevice));
+    LB_HIP_C(hipDeviceSynchronize());
+  }
+  // clear = sum_k lambda_k |phi_k(0)|^2 (hopkins.py:190-191)
+  double clear = 0;
+  if (p.K > 0) {
+    const long long dc_idx = (long long)(-p.nu0m) * Wt + (-p.nu0m);
+    std::vector<cx<double>> at(p.K);
+    for (int k = 0; k < p.K; ++k)
+      LB_HIP_C(hipMemcpy(&at[k], (cx<double>*)p.phi128 + (long long)k * F + dc_idx,
+                         sizeof(cx<double>), hipMemcpyDeviceToHost));
+    for (int k = 0; k < p.K; ++k)
+      clear += p.weights[k] * (at[k].re * at[k].re + at[k].im * at[k].im);
+  }
+  p.clear.assign(1, clear);
+#undef LB_HIP_C
+  cleanup();
+  *out = h;
+  return LB_OK;
+}
+
+lb_status lb_plan_destroy(lb_plan plan) {
+  delete plan;
+  return LB_OK;
+}
+
+lb_status lb_plan_info_get(lb_plan plan, lb_plan_info* info) {
+  if (!plan || !info) return fail(LB_ERR_ARG, "null plan or info");
+  const Plan& p = plan->p;
+  Work w = layout(p, 1, false);
+  info->n = p.N;
+  info->m = p.M;
+  info->box = p.W;
+  info->band = p.Wbm;
+  info->n_planes = p.n_planes;
+  info->n_systems = p.spp;
+  info->precision = p.precision;
+  info->n_kernels = p.socs ? p.K : 0;
+  info->block_threads = p.fwd.nt;
+  info->column_splits = p.fwd.csplit;
+  info->chunk = w.chunk;
+  info->grid_blocks = grid_for(p, p.fwd, w.n_items, false);
+  return LB_OK;
+}
+
+lb_status lb_plan_clear(lb_plan plan, double* clear) {
+  if (!plan || !clear) return fail(LB_ERR_ARG, "null plan or output");
+  for (size_t i = 0; i < plan->p.clear.size(); ++i) clear[i] = plan->p.clear[i];
+  return LB_OK;
+}
+
+lb_status lb_socs_weights(lb_plan plan, double* weights, double* energy, int32_t* n_band,
+                          int32_t* n_source) {
+  if (!plan) return fail(LB_ERR_ARG, "null plan");
+  const Plan& p = plan->p;
+  if (!p.socs) return fail(LB_ERR_ARG, "not a SOCS plan");
+  if (weights)
+    for (int k = 0; k < p.K; ++k) weights[k] = p.weights[k];
+  if (energy) *energy = p.energy;
+  if (n_band) *n_band = p.n_band;
+  if (n_source) *n_source = p.S;
+  return LB_OK;
+}
+
+size_t lb_workspace_bytes(lb_plan plan, int32_t n_masks, int32_t backward) {
+  if (!plan || n_masks < 1) return 0;
+  return layout(plan->p, n_masks, backward != 0).total;
+}
+
+lb_status lb_forward(lb_plan plan, const void* masks, int32_t cplx, int32_t n_masks, void* images,
+                     void* peak, void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !masks || !images) return fail(LB_ERR_ARG, "null plan, masks or images");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  Work w = layout(p, n_masks, false);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? forward_t<double>(p, masks, cplx, n_masks, images, peak, (char*)workspace, w, st)
+             : forward_t<float>(p, masks, cplx, n_masks, images, peak, (char*)workspace, w, st);
+}
+
+lb_status lb_backward(lb_plan plan, const void* masks, int32_t n_masks, const void* dimg,
+                      void* dmask, void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !masks || !dimg || !dmask) return fail(LB_ERR_ARG, "null argument");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  Work w = layout(p, n_masks, true);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? backward_t<double>(p, masks, n_masks, dimg, dmask, (char*)workspace, w, st)
+             : backward_t<float>(p, masks, n_masks, dimg, dmask, (char*)workspace, w, st);
+}
+
+lb_status lb_socs_kernels(lb_plan plan, void* kernels, void* stream) {
+  g_err.clear();
+  if (!plan || !kernels) return fail(LB_ERR_ARG, "null argument");
+  Plan& p = plan->p;
+  if (!p.socs) return fail(LB_ERR_ARG, "not a SOCS plan");
+  if (p.K == 0) return LB_OK;
+  hipStream_t st = (hipStream_t)stream;
+  LB_HIP(fill<double>((double*)kernels, 0.0, 2LL * p.K * p.N * p.N, st));
+  LB_HIP(scatter_band<double>((const cx<double>*)p.phi128, p.K, p.Wbm, p.nu0m, p.N,
+                              (cx<double>*)kernels, st));
+  return LB_OK;
+}
+
+size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks) {
+  if (!plan || n_masks < 1) return 0;
+  const Plan& p = plan->p;
+  Layout L;
+  L.take(p.csz() * (size_t)n_masks * p.Wbm * p.Wbm);
+  L.take(p.csz() * (size_t)n_masks * p.Wbm * p.N);
+  L.take(p.csz() * (size_t)n_masks * p.spp * p.Wbm * p.Wbm);
+  L.take(p.csz() * (size_t)n_masks * p.spp * p.Wbm * p.N);
+  return L.off;
+}
+
+}  // extern "C"
+
+template <typename T>
+static lb_status fields_t(Plan& p, const void* masks, int cplx, int n_masks, void* fields,
+                          char* ws, hipStream_t st) {
+  Layout L;
+  const size_t C = p.csz();
+  cx<T>* Xm = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.Wbm * p.Wbm));
+  cx<T>* RT = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.Wbm * p.N));
+  cx<T>* Y = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.spp * p.Wbm * p.Wbm));
+  cx<T>* TT = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.spp * p.Wbm * p.N));
+  const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
+  LB_HIP(band_analysis<T>(p.N, masks, cplx, (long long)p.N * p.N * (cplx ? 2 : 1), 0, 1, n_masks,
+                          p.nu0m, p.Wbm, invN2, RT, Xm, st));
+  LB_HIP(band_mul<T>(Xm, (const cx<T>*)p.H, n_masks, p.spp, (long long)p.Wbm * p.Wbm, Y, st));
+  LB_HIP(band_synthesis<T>(p.N, Y, n_masks * p.spp, p.nu0m, p.Wbm, 0, (T)1, 1, TT, (T*)fields,
+                           (T*)nullptr, st));
+  return LB_OK;
+}
+
+extern "C" {
+
+lb_status lb_fields(lb_plan plan, const void* masks, int32_t cplx, int32_t n_masks, void* fields,
+                    void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !masks || !fields) return fail(LB_ERR_ARG, "null argument");
+  Plan& p = plan->p;
+  if (!p.socs) return fail(LB_ERR_UNSUPPORTED, "fields are exported for SOCS plans only");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  if (p.spp < 1) return LB_OK;
+  const size_t need = lb_fields_workspace_bytes(plan, n_masks);
+  if (!workspace || ws_bytes < need)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64 ? fields_t<double>(p, masks, cplx, n_masks, fields, (char*)workspace, st)
+                           : fields_t<float>(p, masks, cplx, n_masks, fields, (char*)workspace, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+// Device building blocks shared by every kernel of the imaging path:
+// a complex type, in-register radix-2/3/4/8/16 DFTs, and one Stockham
+// autosort stage over LDS (Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors", SC 2008).
+//
+// Sign convention: DIR = +1 is the inverse transform (e^{+2 pi i n k / L},
+// unnormalised), DIR = -1 the forward one (e^{-2 pi i n k / L}).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+namespace lb {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T r, T i) { return cx<T>{r, i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> scale(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cx<T> mulc(cx<T> a, cx<T> b) {
+  return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+template <typename T>
+__device__ __forceinline__ T norm2(cx<T> a) { return a.re * a.re + a.im * a.im; }
+// multiply by (DIR * i)
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> mul_i(cx<T> a) {
+  return DIR > 0 ? cx<T>{-a.im, a.re} : cx<T>{a.im, -a.re};
+}
+
+// cos / sin (2 pi k / 16), k = 0..4; the rest follow by symmetry.
+#define LB_C16_1 0.92387953251128674
+#define LB_S16_1 0.38268343236508978
+#define LB_R2_2 0.70710678118654752
+
+// e^{DIR * 2 pi i k / R} for compile-time R | 16 and k < R.
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cx<T> root(int k) {
+  const int q = (k * (16 / R)) & 15;  // in units of 2 pi / 16
+  T c, s;
+  switch (q) {
+    case 0: c = 1; s = 0; break;
+    case 1: c = (T)LB_C16_1; s = (T)LB_S16_1; break;
+    case 2: c = (T)LB_R2_2; s = (T)LB_R2_2; break;
+    case 3: c = (T)LB_S16_1; s = (T)LB_C16_1; break;
+    case 4: c = 0; s = 1; break;
+    case 5: c = -(T)LB_S16_1; s = (T)LB_C16_1; break;
+    case 6: c = -(T)LB_R2_2; s = (T)LB_R2_2; break;
+    case 7: c = -(T)LB_C16_1; s = (T)LB_S16_1; break;
+    case 8: c = -1; s = 0; break;
+    case 9: c = -(T)LB_C16_1; s = -(T)LB_S16_1; break;
+    case 10: c = -(T)LB_R2_2; s = -(T)LB_R2_2; break;
+    case 11: c = -(T)LB_S16_1; s = -(T)LB_C16_1; break;
+    case 12: c = 0; s = -1; break;
+    case 13: c = (T)LB_S16_1; s = -(T)LB_C16_1; break;
+    case 14: c = (T)LB_R2_2; s = -(T)LB_R2_2; break;
+    default: c = (T)LB_C16_1; s = -(T)LB_S16_1; break;
+  }
+  return cx<T>{c, DIR > 0 ? s : -s};
+}
+
+// Multiply by a compile-time root, skipping the trivial cases.
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cx<T> mul_root(cx<T> a, int k) {
+  const int q = (k * (16 / R)) & 15;
+  if (q == 0) return a;
+  if (q == 4) return mul_i<DIR>(a);
+  if (q == 8) return cx<T>{-a.re, -a.im};
+  if (q == 12) return mul_i<-DIR>(a);
+  return a * root<R, DIR, T>(k);
+}
+
+// In-register DFT of size R (power of two <= 16), natural order in and out.
+// Radix-2 decimation in time, fully unrolled at compile time.
+template <int R, int DIR, typename T>
+struct DFT2k {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    DFT2k<R / 2, DIR, T>::run(e);
+    DFT2k<R / 2, DIR, T>::run(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      cx<T> t = mul_root<R, DIR, T>(o[k], k);
+      v[k] = e[k] + t;
+      v[k + R / 2] = e[k] - t;
+    }
+  }
+};
+template <int DIR, typename T>
+struct DFT2k<1, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>*) {}
+};
+template <int DIR, typename T>
+struct DFT2k<2, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  }
+};
+template <int DIR, typename T>
+struct DFT2k<4, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> t0 = v[0] + v[2], t1 = v[0] - v[2];
+    cx<T> t2 = v[1] + v[3], t3 = mul_i<DIR>(v[1] - v[3]);
+    v[0] = t0 + t2;
+    v[2] = t0 - t2;
+    v[1] = t1 + t3;
+    v[3] = t1 - t3;
+  }
+};
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft(cx<T>* v) {
+  if constexpr (R == 3) {
+    const T h = (T)0.86602540378443865;  // sqrt(3)/2
+    cx<T> s = v[1] + v[2];
+    cx<T> d = mul_i<DIR>(v[1] - v[2]);
+    cx<T> m = v[0] - scale(s, (T)0.5);
+    v[0] = v[0] + s;
+    v[1] = m + scale(d, h);
+    v[2] = m - scale(d, h);
+  } else {
+    DFT2k<R, DIR, T>::run(v);
+  }
+}
+
+// e^{DIR 2 pi i t / L} from a table of the L-th roots of unity (tw[t] holds
+// the e^{+2 pi i t / L} values).
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> tw_get(const cx<T>* tw, int t) {
+  cx<T> w = tw[t];
+  return DIR > 0 ? w : conj(w);
+}
+
+// Fill tw[0..L) with e^{+2 pi i t / L}, computed in double.
+template <typename T>
+__device__ __forceinline__ void fill_roots(cx<T>* tw, int L, int tid, int nt) {
+  for (int t = tid; t < L; t += nt) {
+    double s, c;
+    sincospi(2.0 * (double)t / (double)L, &s, &c);
+    tw[t] = cx<T>{(T)c, (T)s};
+  }
+}
+
+// One Stockham stage of radix R with span Ns on a length-L transform held
+// contiguously at in / out (out of place).  Task j in [0, L/R).
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void stockham_stage(const cx<T>* in, cx<T>* out, int L, int Ns, int j,
+                                               const cx<T>* tw) {
+  cx<T> v[R];
+  const int stride = L / R;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = in[j + r * stride];
+  const int k = j % Ns;
+  if (Ns > 1) {
+    const int step = L / (Ns * R);
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = v[r] * tw_get<DIR>(tw, r * k * step);
+  }
+  dft<R, DIR>(v);
+  const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[base + r * Ns] = v[r];
+}
+
+// A full length-L transform of `ntr` independent rows laid out at
+// buf + row * ld (row < ntr), by all threads of the block, radices R0..R3
+// (1 = unused, product must equal L).  Ping-pongs between a and b; returns
+// the buffer that holds the result.  Synchronises the block between stages
+// and at the end.
+template <typename T, int DIR, int L, int R0, int R1, int R2, int R3>
+__device__ cx<T>* fft_rows_lds(cx<T>* a, cx<T>* b, int ntr, int ld, const cx<T>* tw, int tid,
+                               int nt) {
+  static_assert(R0 * R1 * R2 * R3 == L, "radix plan must multiply to L");
+  cx<T>* src = a;
+  cx<T>* dst = b;
+  int Ns = 1;
+  auto stage = [&](auto rc) {
+    constexpr int R = decltype(rc)::value;
+    if constexpr (R > 1) {
+      const int tasks = L / R;
+      for (int t = tid; t < ntr * tasks; t += nt) {
+        const int row = t / tasks, j = t - row * tasks;
+        stockham_stage<R, DIR, T>(src + row * ld, dst + row * ld, L, Ns, j, tw);
+      }
+      __syncthreads();
+      cx<T>* x = src;
+      src = dst;
+      dst = x;
+      Ns *= R;
+    }
+  };
+  stage(std::integral_constant<int, R0>{});
+  stage(std::integral_constant<int, R1>{});
+  stage(std::integral_constant<int, R2>{});
+  stage(std::integral_constant<int, R3>{});
+  return src;
+}
+
+// Frequency index nu (any integer) -> FFT bin in [0, n).
+__device__ __forceinline__ int bin_of(int nu, int n) {
+  int b = nu % n;
+  return b < 0 ? b + n : b;
+}
+
+}  // namespace lb

@@ -1,0 +1,444 @@
+// Coherent-system kernels: the Abbe source-point sum (one system per source
+// point x plane x field component) and the SOCS sum (one system per kernel).
+//
+// A system j images the band spectrum X of a mask through its transfer table
+// H_j, held on a W x W box of frequencies starting at box[j]:
+//
+//     C[a][b]   = X[box_y + a][box_x + b] * H_j[a][b]
+//     E[y'][x'] = sum_{a,b < W} C[a][b] e^{+2 pi i (a y' + b x') / M}
+//     I[y'][x'] += w_j |E[y'][x']|^2
+//
+// on an M x M grid with M >= 2W - 1.  |E|^2 only holds frequency differences
+// inside the box, so M >= 2W - 1 samples it without aliasing and the fine
+// N x N image is its exact band-limited interpolation (done by the band
+// synthesis afterwards); the box-origin phase ramp is a unit-modulus factor
+// and cancels in |E|^2.  When M == N the coarse grid is the image grid.
+//
+// One block owns one work item (a chunk of systems of one image) and loops
+// over its systems; the intensity accumulates in registers, so each item
+// writes its M x M partial once.  Per system: the row transforms (W rows,
+// input pruned to W points) run in blocks of CPR rows through LDS into the
+// W x (M/CSPLIT) table Tb; the column transforms (input pruned to W rows) run
+// in rounds of CPR columns; the last radix stage's outputs are squared and
+// accumulated in place.  Both 1-D transforms are two Stockham stages,
+// M = R1 * R2, with the R1 stage first.
+//
+// The adjoint kernel is the exact transpose of that linear chain: it
+// recomputes E, multiplies by the coarse sensitivity gM, and runs the
+// transposed stages (forward DFTs, conjugate twiddles, outputs pruned to the
+// box) to get G_C = 2 w FFT_M(gM E) on the box, then accumulates
+// G_C * conj(H) into the item's band-gradient partial.
+#include "lb_kernels.h"
+
+namespace lb {
+
+template <int M>
+struct FieldRadix;
+template <>
+struct FieldRadix<16> {
+  static constexpr int R1 = 4, R2 = 4;
+};
+template <>
+struct FieldRadix<32> {
+  static constexpr int R1 = 8, R2 = 4;
+};
+template <>
+struct FieldRadix<64> {
+  static constexpr int R1 = 8, R2 = 8;
+};
+template <>
+struct FieldRadix<128> {
+  static constexpr int R1 = 16, R2 = 8;
+};
+template <>
+struct FieldRadix<256> {
+  static constexpr int R1 = 16, R2 = 16;
+};
+
+template <typename T, int M, int NT, int CSPLIT>
+struct FieldShape {
+  static constexpr int R1 = FieldRadix<M>::R1, R2 = FieldRadix<M>::R2;
+  static constexpr int T1 = M / R1;  // stage-1 tasks per line
+  static constexpr int T2 = M / R2;  // stage-2 tasks per line (== R1)
+  static constexpr int CPR = NT / T1;
+  static constexpr int MC = M / CSPLIT;
+  static constexpr int ROUNDS = MC / CPR;
+  static constexpr int Q2 = R1 / R2;
+  static constexpr int NACC = ROUNDS * Q2 * R2;
+  static constexpr int LDC = M + 1;  // padded line stride of the round buffer
+  static_assert(T2 == R1, "two-stage plan");
+  static_assert(CPR * T1 == NT, "stage 1 maps one task per thread");
+  static_assert(CPR <= MC && MC % CPR == 0, "rounds tile the split");
+  static_assert(R1 % R2 == 0, "stage-2 tasks per thread");
+};
+
+__device__ __forceinline__ int wrap_idx(int i, int n) {
+  if (n <= 0) return i;
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// Row pass of one system for one split: W rows of C, inverse length-M
+// transforms, columns of the split stored into Tb (W x MC).
+template <typename T, int M, int NT, int CSPLIT>
+__device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
+                                         const cx<T>* __restrict__ Hj, int W, int2 bx,
+                                         const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs, int tid) {
+  using F = FieldShape<T, M, NT, CSPLIT>;
+  const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
+  for (int a0 = 0; a0 < W; a0 += F::CPR) {
+    {
+      const int a = a0 + rl1;
+      if (a < W) {
+        cx<T> v[F::R1];
+        const int xr = wrap_idx(bx.x + a, wrapN);
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) {
+          const int b = j1 + r * F::T1;
+          if (b < W) {
+            const cx<T> x = Xm[(long long)xr * Wb + wrap_idx(bx.y + b, wrapN)];
+            v[r] = x * Hj[a * W + b];
+          } else {
+            v[r] = cx<T>{0, 0};
+          }
+        }
+        dft<F::R1, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) S[rl1 * F::LDC + j1 * F::R1 + r] = v[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < F::Q2; ++q) {
+      const int t = tid + q * NT;
+      const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+      const int a = a0 + rl;
+      if (a < W) {
+        cx<T> v[F::R2];
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          v[r] = S[rl * F::LDC + j2 + r * F::R1];
+          if (r > 0) v[r] = v[r] * tw[r * j2];
+        }
+        dft<F::R2, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          const int x = j2 + r * F::R1 - cs * F::MC;
+          if (CSPLIT == 1 || (x >= 0 && x < F::MC)) Tb[a * F::MC + x] = v[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int M, int NT, int CSPLIT>
+__global__ __launch_bounds__(NT) void k_field_fwd(const cx<T>* __restrict__ X, int Wb, int wrapN,
+                                                  const cx<T>* __restrict__ H, int W,
+                                                  const int2* __restrict__ box,
+                                                  const T* __restrict__ wsys, ItemMap map,
+                                                  T* __restrict__ partial) {
+  using F = FieldShape<T, M, NT, CSPLIT>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
+  cx<T>* Cb = tw + M;                              // CPR x LDC (row scratch too)
+  cx<T>* Tb = Cb + F::CPR * F::LDC;                // W x MC
+  const int tid = threadIdx.x;
+  fill_roots(tw, M, tid, NT);
+  __syncthreads();
+  const int c1 = tid % F::CPR, jc1 = tid / F::CPR;  // column stage-1 mapping
+
+  for (int item = blockIdx.x; item < map.n_items; item += gridDim.x) {
+    const int img = item / map.n_chunks, chunk = item - img * map.n_chunks;
+    const int plane = img % map.n_planes, mask = img / map.n_planes;
+    const int s0 = plane * map.sys_per_plane + chunk * map.chunk_sz;
+    const int s1 = min(s0 + map.chunk_sz, (plane + 1) * map.sys_per_plane);
+    const cx<T>* Xm = X + (long long)mask * Wb * Wb;
+    for (int cs = 0; cs < CSPLIT; ++cs) {
+      T acc[F::NACC];
+#pragma unroll
+      for (int i = 0; i < F::NACC; ++i) acc[i] = 0;
+      for (int j = s0; j < s1; ++j) {
+        const int2 bx = box[j];
+        const T w = wsys[j];
+        row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, H + (long long)j * W * W, W, bx, tw, Cb, Tb, cs,
+                                   tid);
+#pragma unroll
+        for (int rd = 0; rd < F::ROUNDS; ++rd) {
+          {
+            cx<T> v[F::R1];
+            const int xl = rd * F::CPR + c1;
+#pragma unroll
+            for (int r = 0; r < F::R1; ++r) {
+              const int a = jc1 + r * F::T1;
+              v[r] = (a < W) ? Tb[a * F::MC + xl] : cx<T>{0, 0};
+            }
+            dft<F::R1, +1>(v);
+#pragma unroll
+            for (int r = 0; r < F::R1; ++r) Cb[c1 * F::LDC + jc1 * F::R1 + r] = v[r];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < F::Q2; ++q) {
+            const int t = tid + q * NT;
+            const int c2 = t % F::CPR, j2 = t / F::CPR;
+            cx<T> v[F::R2];
+#pragma unroll
+            for (int r = 0; r < F::R2; ++r) {
+              v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
+              if (r > 0) v[r] = v[r] * tw[r * j2];
+            }
+            dft<F::R2, +1>(v);
+#pragma unroll
+            for (int r = 0; r < F::R2; ++r) acc[(rd * F::Q2 + q) * F::R2 + r] += w * norm2(v[r]);
+          }
+          __syncthreads();
+        }
+      }
+      T* out = partial + (long long)item * M * M;
+#pragma unroll
+      for (int rd = 0; rd < F::ROUNDS; ++rd)
+#pragma unroll
+        for (int q = 0; q < F::Q2; ++q) {
+          const int t = tid + q * NT;
+          const int c2 = t % F::CPR, j2 = t / F::CPR;
+          const int x = cs * F::MC + rd * F::CPR + c2;
+#pragma unroll
+          for (int r = 0; r < F::R2; ++r)
+            out[(j2 + r * F::R1) * M + x] = acc[(rd * F::Q2 + q) * F::R2 + r];
+        }
+    }
+  }
+}
+
+template <typename T, int M, int NT, int CSPLIT>
+__global__ __launch_bounds__(NT) void k_field_adj(const cx<T>* __restrict__ X, int Wb, int wrapN,
+                                                  const cx<T>* __restrict__ H, int W,
+                                                  const int2* __restrict__ box,
+                                                  const T* __restrict__ wsys, ItemMap map,
+                                                  const T* __restrict__ gM,
+                                                  cx<T>* __restrict__ gX) {
+  using F = FieldShape<T, M, NT, CSPLIT>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
+  cx<T>* Cb = tw + M;                              // CPR x LDC
+  cx<T>* Tb = Cb + F::CPR * F::LDC;                // W x MC
+  cx<T>* Tp = Tb + W * F::MC;                      // W x MC
+  const int tid = threadIdx.x;
+  fill_roots(tw, M, tid, NT);
+  __syncthreads();
+  const int c1 = tid % F::CPR, jc1 = tid / F::CPR;
+  const int rl1 = tid / F::T1, j1r = tid - (tid / F::T1) * F::T1;
+
+  for (int item = blockIdx.x; item < map.n_items; item += gridDim.x) {
+    const int img = item / map.n_chunks, chunk = item - img * map.n_chunks;
+    const int plane = img % map.n_planes, mask = img / map.n_planes;
+    const int s0 = plane * map.sys_per_plane + chunk * map.chunk_sz;
+    const int s1 = min(s0 + map.chunk_sz, (plane + 1) * map.sys_per_plane);
+    const cx<T>* Xm = X + (long long)mask * Wb * Wb;
+    const T* g = gM + (long long)img * M * M;
+    cx<T>* gout = gX + (long long)item * Wb * Wb;
+    for (int e = tid; e < Wb * Wb; e += NT) gout[e] = cx<T>{0, 0};
+    __syncthreads();
+    for (int cs = 0; cs < CSPLIT; ++cs) {
+      T gm[F::NACC];
+#pragma unroll
+      for (int rd = 0; rd < F::ROUNDS; ++rd)
+#pragma unroll
+        for (int q = 0; q < F::Q2; ++q) {
+          const int t = tid + q * NT;
+          const int c2 = t % F::CPR, j2 = t / F::CPR;
+          const int x = cs * F::MC + rd * F::CPR + c2;
+#pragma unroll
+          for (int r = 0; r < F::R2; ++r)
+            gm[(rd * F::Q2 + q) * F::R2 + r] = g[(j2 + r * F::R1) * M + x];
+        }
+      for (int j = s0; j < s1; ++j) {
+        const int2 bx = box[j];
+        const T w2 = (T)2 * wsys[j];
+        const cx<T>* Hj = H + (long long)j * W * W;
+        row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, Hj, W, bx, tw, Cb, Tb, cs, tid);
+#pragma unroll
+        for (int rd = 0; rd < F::ROUNDS; ++rd) {
+          const int xl = rd * F::CPR + c1;
+          {
+            cx<T> v[F::R1];
+#pragma unroll
+            for (int r = 0; r < F::R1; ++r) {
+              const int a = jc1 + r * F::T1;
+              v[r] = (a < W) ? Tb[a * F::MC + xl] : cx<T>{0, 0};
+            }
+            dft<F::R1, +1>(v);
+#pragma unroll
+            for (int r = 0; r < F::R1; ++r) Cb[c1 * F::LDC + jc1 * F::R1 + r] = v[r];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < F::Q2; ++q) {
+            const int t = tid + q * NT;
+            const int c2 = t % F::CPR, j2 = t / F::CPR;
+            cx<T> v[F::R2];
+#pragma unroll
+            for (int r = 0; r < F::R2; ++r) {
+              v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
+              if (r > 0) v[r] = v[r] * tw[r * j2];
+            }
+            dft<F::R2, +1>(v);
+            // h = gM * E, then the transposed stage 2: forward DFT, conj twiddle
+#pragma unroll
+            for (int r = 0; r < F::R2; ++r) v[r] = scale(v[r], gm[(rd * F::Q2 + q) * F::R2 + r]);
+            dft<F::R2, -1>(v);
+#pragma unroll
+            for (int r = 0; r < F::R2; ++r) {
+              if (r > 0) v[r] = mulc(v[r], tw[r * j2]);
+              Cb[c2 * F::LDC + j2 + r * F::R1] = v[r];
+            }
+          }
+          __syncthreads();
+          {
+            // transposed stage 1: outputs pruned to the W box rows
+            cx<T> v[F::R1];
+#pragma unroll
+            for (int r = 0; r < F::R1; ++r) v[r] = Cb[c1 * F::LDC + jc1 * F::R1 + r];
+            dft<F::R1, -1>(v);
+#pragma unroll
+            for (int r = 0; r < F::R1; ++r) {
+              const int a = jc1 + r * F::T1;
+              if (a < W) Tp[a * F::MC + xl] = v[r];
+            }
+          }
+          __syncthreads();
+        }
+        // transposed row pass over Tp, in blocks of CPR rows through Cb
+        for (int a0 = 0; a0 < W; a0 += F::CPR) {
+          for (int e = tid; e < F::CPR * M; e += NT) {
+            const int rl = e / M, x = e - rl * M;
+            const int a = a0 + rl;
+            const int xl = x - cs * F::MC;
+            cx<T> val = cx<T>{0, 0};
+            if (a < W && (CSPLIT == 1 || (xl >= 0 && xl < F::MC))) val = Tp[a * F::MC + xl];
+            Cb[rl * F::LDC + x] = val;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < F::Q2; ++q) {
+            const int t = tid + q * NT;
+            const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+            cx<T> v[F::R2];
+#pragma unroll
+            for (int r = 0; r < F::R2; ++r) v[r] = Cb[rl * F::LDC + j2 + r * F::R1];
+            dft<F::R2, -1>(v);
+#pragma unroll
+            for (int r = 0; r < F::R2; ++r) {
+              if (r > 0) v[r] = mulc(v[r], tw[r * j2]);
+              Cb[rl * F::LDC + j2 + r * F::R1] = v[r];
+            }
+          }
+          __syncthreads();
+          {
+            const int a = a0 + rl1;
+            cx<T> v[F::R1];
+#pragma unroll
+            for (int r = 0; r < F::R1; ++r) v[r] = Cb[rl1 * F::LDC + j1r * F::R1 + r];
+            dft<F::R1, -1>(v);
+            if (a < W) {
+              const int xr = wrap_idx(bx.x + a, wrapN);
+#pragma unroll
+              for (int r = 0; r < F::R1; ++r) {
+                const int b = j1r + r * F::T1;
+                if (b < W) {
+                  const cx<T> gc = scale(mulc(v[r], Hj[a * W + b]), w2);
+                  cx<T>* dst = gout + (long long)xr * Wb + wrap_idx(bx.y + b, wrapN);
+                  *dst = *dst + gc;
+                }
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- dispatch
+#define LB_FIELD_CONFIGS(X)                                                                  \
+  X(16, 64, 1) X(32, 128, 1) X(64, 256, 1) X(64, 128, 1) X(64, 256, 2) X(128, 256, 1)        \
+      X(128, 512, 1) X(128, 256, 2) X(128, 256, 4) X(256, 256, 2) X(256, 256, 4) X(256, 256, 8)
+
+bool field_config_supported(int precision, const FieldConfig& c) {
+  (void)precision;
+#define LB_CASE(M_, NT_, CS_) \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == CS_) return true;
+  LB_FIELD_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return false;
+}
+
+template <typename T, int M, int NT, int CS>
+static size_t smem_for(int W, bool adjoint) {
+  using F = FieldShape<T, M, NT, CS>;
+  size_t n = M + F::CPR * F::LDC + (size_t)W * F::MC * (adjoint ? 2 : 1);
+  return n * sizeof(cx<T>);
+}
+
+size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint) {
+#define LB_CASE(M_, NT_, CS_)                                                   \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == CS_)                              \
+    return precision == 64 ? smem_for<double, M_, NT_, CS_>(W, adjoint)         \
+                           : smem_for<float, M_, NT_, CS_>(W, adjoint);
+  LB_FIELD_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return 0;
+}
+
+template <typename T, int M, int NT, int CS>
+static hipError_t launch_fwd(const FieldConfig& c, const FieldArgs& a, void* partial,
+                             hipStream_t st) {
+  auto k = k_field_fwd<T, M, NT, CS>;
+  const size_t sm = smem_for<T, M, NT, CS>(a.W, false);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st, (const cx<T>*)a.X, a.Wb, a.wrapN,
+                     (const cx<T>*)a.H, a.W, a.box, (const T*)a.wsys, a.map, (T*)partial);
+  return hipGetLastError();
+}
+
+template <typename T, int M, int NT, int CS>
+static hipError_t launch_adj(const FieldConfig& c, const FieldArgs& a, const void* gM, void* gX,
+                             hipStream_t st) {
+  auto k = k_field_adj<T, M, NT, CS>;
+  const size_t sm = smem_for<T, M, NT, CS>(a.W, true);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st, (const cx<T>*)a.X, a.Wb, a.wrapN,
+                     (const cx<T>*)a.H, a.W, a.box, (const T*)a.wsys, a.map, (const T*)gM,
+                     (cx<T>*)gX);
+  return hipGetLastError();
+}
+
+hipError_t field_forward(int precision, const FieldConfig& c, const FieldArgs& a, void* partial,
+                         hipStream_t st) {
+#define LB_CASE(M_, NT_, CS_)                                                           \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == CS_)                                      \
+    return precision == 64 ? launch_fwd<double, M_, NT_, CS_>(c, a, partial, st)        \
+                           : launch_fwd<float, M_, NT_, CS_>(c, a, partial, st);
+  LB_FIELD_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t field_adjoint(int precision, const FieldConfig& c, const FieldArgs& a,
+                         const void* gM, void* gX, hipStream_t st) {
+#define LB_CASE(M_, NT_, CS_)                                                           \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == CS_)                                      \
+    return precision == 64 ? launch_adj<double, M_, NT_, CS_>(c, a, gM, gX, st)         \
+                           : launch_adj<float, M_, NT_, CS_>(c, a, gM, gX, st);
+  LB_FIELD_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace lb

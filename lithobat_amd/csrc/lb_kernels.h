@@ -1,0 +1,124 @@
+// Host-side launchers of the device kernels (internal to the library).
+#pragma once
+#include "lb_common.h"
+
+namespace lb {
+
+// ---- band transforms (lb_band.hip) ---------------------------------------
+bool band_size_supported(int n);
+
+// X[img][u][v] = s * fft2(f_img)[nu0+u][nu0+v]; f_img = sum over nsum slices
+// (f real, or complex interleaved when cplx).  RT: scratch img x Wb x N.
+template <typename T>
+hipError_t band_analysis(int N, const void* in, int cplx, long long img_stride,
+                         long long slice_stride, int nsum, int nimg, int nu0, int Wb, T s,
+                         cx<T>* RT, cx<T>* X, hipStream_t st);
+
+// out[img] = s * Re sum X e^{+...} (cout = 0; X Hermitianised when herm) or the
+// complex sum (cout = 1, out holds complex).  TT: scratch img x Wb x N.
+// peak (nullable, real output only): running max per image, must start at 0.
+template <typename T>
+hipError_t band_synthesis(int N, const cx<T>* Y, int nimg, int nu0, int Wb, int herm, T s,
+                          int cout, cx<T>* TT, T* out, T* peak, hipStream_t st);
+
+// ---- coherent-system kernels (lb_field.hip) -------------------------------
+// A work item is a range of coherent systems that all image one mask into one
+// image.  Item i covers image i / n_chunks, systems
+// [plane * sys_per_plane + chunk * chunk_sz, ...) of plane (image % n_planes),
+// mask image / n_planes.
+struct ItemMap {
+  int n_chunks;
+  int n_planes;
+  int sys_per_plane;
+  int chunk_sz;
+  int n_items;
+};
+
+struct FieldArgs {
+  const void* X;      // band spectra [n_masks][Wb][Wb] complex
+  int Wb;
+  int wrapN;          // > 0: band spans the whole grid, index mod wrapN
+  const void* H;      // transfer tables [n_sys][W][W] complex
+  int W;
+  const int2* box;    // [n_sys] band index (row, col) of each box origin
+  const void* wsys;   // [n_sys] real weights
+  ItemMap map;
+};
+
+struct FieldConfig {
+  int M;       // coarse grid side
+  int nt;      // threads per block
+  int csplit;  // column splits
+  int grid;    // blocks (persistent)
+};
+
+bool field_config_supported(int precision, const FieldConfig& c);
+size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint);
+
+// partial[item][M][M] = sum over the item's systems of w |E|^2 on the coarse grid
+hipError_t field_forward(int precision, const FieldConfig& c, const FieldArgs& a, void* partial,
+                         hipStream_t st);
+// gX[item][Wb][Wb] = sum over the item's systems of 2 w conj(H) FFT(gM E) on the box
+hipError_t field_adjoint(int precision, const FieldConfig& c, const FieldArgs& a,
+                         const void* gM, void* gX, hipStream_t st);
+
+// ---- small helpers (lb_util.hip) ------------------------------------------
+template <typename T>
+hipError_t fill(T* p, T v, long long n, hipStream_t st);
+// out[g][e] = sum_{k < cnt} in[(g*cnt + k)][e], e < len (deterministic order)
+template <typename T>
+hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, T* peak,
+                       hipStream_t st);
+
+template <typename T>
+hipError_t scatter_band(const cx<T>* in, int K, int Wb, int nu0, int N, cx<T>* out,
+                        hipStream_t st);
+template <typename T>
+hipError_t band_mul(const cx<T>* X, const cx<T>* Hk, int n_masks, int K, long long len, cx<T>* Y,
+                    hipStream_t st);
+
+// ---- pupil tables (lb_pupil.hip) ------------------------------------------
+struct PupilSys {
+  double fsx, fsy;       // source-point frequency offset [1/m]
+  double weight;         // multiplies H (1 for Abbe systems, sqrt(w_s) for the TCC rows)
+  double jx, jy;         // input Jones vector (vector model)
+  double defocus;        // [m]
+  double n_image;
+  int comp;              // -1 scalar, 0..2 = x, y, z component
+  int box_y, box_x;      // frequency index (box coordinates) of the box origin
+  int pad;
+};
+
+constexpr int kMaxZernike = 21;
+struct PupilParams {
+  int N;
+  int W;
+  double val;            // 1 / (N * pixel_size), the fftfreq step
+  double k_max;          // NA / wavelength
+  double wavelength;
+  double NA;
+  int exact;             // exact (non-paraxial) defocus
+  int obliquity;
+  int n_zern;
+  int zn[kMaxZernike], zm[kMaxZernike];
+  double znorm[kMaxZernike], zcoef[kMaxZernike], zrad[kMaxZernike][3];
+};
+
+// H[j][a][b] (complex T) and dc[j] = weight^2-free DC power of system j.
+template <typename T>
+hipError_t pupil_tables(const PupilSys* sys_dev, int n_sys, const PupilParams& pp, cx<T>* H,
+                        double* dc, hipStream_t st);
+
+// ---- SOCS decomposition (lb_socs.hip), FP64 ------------------------------
+// G[s][t] = sum_f conj(A[s][f]) A[t][f]
+hipError_t gram_f64(const cx<double>* A, int S, long long F, cx<double>* G, hipStream_t st);
+// Hermitian Jacobi: G overwritten (diagonal = eigenvalues), V = eigenvectors (columns)
+hipError_t jacobi_eigh_f64(cx<double>* G, cx<double>* V, int S, int max_sweeps, double* evals,
+                           int* sweeps_out, hipStream_t st);
+// phi[k][f] = sum_s A[s][f] V[s][idx[k]] * inv_sqrt[k]
+hipError_t socs_phi_f64(const cx<double>* A, const cx<double>* V, int S, long long F,
+                        const int* idx, const double* inv_sqrt, int K, cx<double>* phi,
+                        hipStream_t st);
+hipError_t cast_c128_c64(const cx<double>* in, cx<float>* out, long long n, hipStream_t st);
+
+}  // namespace lb

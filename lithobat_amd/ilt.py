@@ -1,0 +1,108 @@
+"""Inverse-lithography gradient (opc/ilt.py:208-389 of the reference).
+
+``AbbeIntensity`` is a ``torch.autograd.Function`` mask -> raw intensity whose
+backward is the device adjoint (``lb_backward``): it recomputes every
+per-system coherent field from the mask instead of storing them, and returns
+d/dmask of <dI, I> for real masks.  It works for Abbe plans (scalar or
+vector, several planes) and SOCS plans alike.
+
+``socs_fields`` / ``socs_gradient`` keep the reference's two-call contract
+(fields are returned, then handed back with the sensitivity); the fields
+array remembers its mask, and the gradient is the recomputing adjoint.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .expose import _to_device
+from .plan import ImagingPlan
+
+__all__ = ["AbbeIntensity", "abbe_intensity", "socs_fields", "socs_gradient",
+           "relaxed_resist_loss", "mask_of"]
+
+
+class AbbeIntensity(torch.autograd.Function):
+    """raw[B, P, N, N] = plan(mask[B, N, N]); backward by the recomputing adjoint."""
+
+    @staticmethod
+    def forward(ctx, mask: torch.Tensor, plan: ImagingPlan):
+        raw, _ = plan.forward(mask, with_peak=False)
+        ctx.plan = plan
+        ctx.save_for_backward(mask)
+        return raw
+
+    @staticmethod
+    def backward(ctx, grad_raw: torch.Tensor):
+        (mask,) = ctx.saved_tensors
+        squeeze = mask.dim() == 2
+        g = ctx.plan.backward(mask, grad_raw.contiguous())
+        g = g.to(mask.dtype)
+        return (g[0] if squeeze else g), None
+
+
+def abbe_intensity(mask: torch.Tensor, plan: ImagingPlan) -> torch.Tensor:
+    return AbbeIntensity.apply(mask, plan)
+
+
+def mask_of(theta: torch.Tensor, beta: float) -> torch.Tensor:
+    """ilt.py:368-378 with no curvature smoothing: sigmoid(clip(beta * theta, +-60))."""
+    return torch.sigmoid(torch.clamp(beta * theta, -60.0, 60.0))
+
+
+def relaxed_resist_loss(raw: torch.Tensor, target: torch.Tensor, scale: float, level: float,
+                        steepness: float, sign: float = 1.0) -> torch.Tensor:
+    """ilt.py:311-330 with sigma_px = 0 and no penalties: mean (sigmoid(a s) - t)^2.
+
+    a = steepness / |level| and s = sign * (raw * scale - level), the reference's
+    relaxed resist.  The mean runs over the last two (image) axes and is
+    summed over any leading axes.
+    """
+    signed = sign * (raw * scale - level)
+    a = steepness / max(abs(level), 1e-12)
+    z = torch.sigmoid(torch.clamp(a * signed, -60.0, 60.0))
+    resid = z - target
+    return (resid * resid).mean(dim=(-2, -1)).sum()
+
+
+class FieldStack(np.ndarray):
+    """The (K, n, n) fields of ``socs_fields``, remembering the mask they came from."""
+
+    def __new__(cls, arr, mask):
+        obj = np.asarray(arr).view(cls)
+        obj.mask = mask
+        return obj
+
+    def __array_finalize__(self, obj):
+        self.mask = getattr(obj, "mask", None)
+
+
+def socs_fields(kernels, mask):
+    """ilt.py:208-232: (fields (K, n, n) complex, raw intensity (n, n))."""
+    n = kernels.grid.n_pixels
+    if tuple(np.shape(mask)) != (n, n):
+        raise ValueError(f"mask must be {(n, n)} for these kernels, got {np.shape(mask)}")
+    m, was_np, _ = _to_device(mask, kernels.plan.precision)
+    fields = kernels.plan.fields(m)[0]
+    raw, _ = kernels.plan.forward(m, with_peak=False)
+    if was_np:
+        return FieldStack(fields.cpu().numpy(), np.array(mask, copy=True)), raw[0, 0].double().cpu().numpy()
+    return fields, raw[0, 0]
+
+
+def socs_gradient(kernels, fields, sensitivity, mask=None):
+    """ilt.py:235-266: d/dm of sum(sensitivity * I) for the raw SOCS intensity.
+
+    The fields are recomputed from their mask (carried by the array
+    ``socs_fields`` returned, or passed as ``mask``).
+    """
+    if mask is None:
+        mask = getattr(fields, "mask", None)
+    if mask is None:
+        raise ValueError("socs_gradient recomputes the fields: pass the fields returned by "
+                         "socs_fields, or the mask they came from")
+    m, was_np, _ = _to_device(mask, kernels.plan.precision)
+    g = torch.as_tensor(np.asarray(sensitivity) if was_np else sensitivity,
+                        device=m.device, dtype=kernels.plan.real_dtype)
+    out = kernels.plan.backward(m.reshape(1, *m.shape[-2:]), g.reshape(1, 1, *g.shape[-2:]))[0]
+    return out.double().cpu().numpy() if was_np else out

@@ -1,0 +1,268 @@
+"""Imaging plans: one optical set-up bound to the device library.
+
+An ``ImagingPlan`` owns a C-ABI plan (the coherent systems of one optics /
+grid / plane set, their transfer tables on the device, and the kernel shape)
+and a growable device workspace.  ``forward`` and ``backward`` take and return
+torch tensors on the current device and run on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import threading
+from collections import OrderedDict
+from dataclasses import asdict, is_dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .source import SourcePoints, source_points
+
+_DTYPES = {32: (torch.float32, torch.complex64), 64: (torch.float64, torch.complex128)}
+
+
+def _image_index(optics) -> float:
+    n_image = getattr(optics, "n_image", None)
+    return optics.n_immersion if n_image is None else n_image
+
+
+def _dptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class ImagingPlan:
+    """A device plan for the Abbe sum (``kind="abbe"``) or its SOCS form (``kind="socs"``)."""
+
+    def __init__(self, optics, grid, planes=None, precision: int = 32, kind: str = "abbe",
+                 tol: float = 1e-8, max_kernels: int | None = None, src: SourcePoints | None = None,
+                 coarse_m: int = 0, chunk: int = 0):
+        lib = _lib.load()
+        if precision not in (32, 64):
+            raise ValueError("precision must be 32 or 64")
+        if not torch.cuda.is_available():
+            raise RuntimeError("the imaging library needs a HIP device (torch.cuda is unavailable)")
+        model = optics.imaging_model.lower()
+        if model not in ("scalar", "vector"):
+            raise ValueError(f"imaging_model must be 'scalar' or 'vector', got "
+                             f"'{optics.imaging_model}'")
+        mask_model = getattr(optics, "mask_model", "thin")
+        if mask_model != "thin":
+            raise NotImplementedError(
+                f"mask_model={mask_model!r}: only the thin-mask spectrum is implemented")
+        if planes is None:
+            planes = [(optics.defocus, _image_index(optics))]
+        self.planes = [(float(d), float(n)) for d, n in planes]
+        self.precision = precision
+        self.kind = kind
+        self.n = int(grid.n_pixels)
+        self.src = src if src is not None else source_points(optics)
+        zern = {int(k): float(v) for k, v in (optics.zernike_coeffs or {}).items()}
+        pol = optics.polarisation.lower()
+        if pol not in _lib.POLARISATION_CODES:
+            raise ValueError(f"polarisation must be one of {list(_lib.POLARISATION_CODES)}, "
+                             f"got '{optics.polarisation}'")
+        self._keep = []  # arrays the descriptor points into
+
+        def arr(x, dt=np.float64):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dt))
+            self._keep.append(a)
+            return a
+
+        defocus = arr([p[0] for p in self.planes])
+        n_image = arr([p[1] for p in self.planes])
+        zj = arr(list(zern.keys()) or [0], np.int32)
+        zc = arr(list(zern.values()) or [0.0])
+        w, fx, fy, phi = (arr(self.src.weight), arr(self.src.fx), arr(self.src.fy),
+                          arr(self.src.phi))
+        d = _lib.ImagingDesc()
+        d.n = self.n
+        d.pixel_size = float(grid.pixel_size)
+        d.wavelength = float(optics.wavelength)
+        d.na = float(optics.NA)
+        d.vector = 1 if model == "vector" else 0
+        d.polarisation = _lib.POLARISATION_CODES[pol]
+        d.obliquity = 1 if optics.obliquity else 0
+        d.exact_defocus = 1 if optics.exact_defocus else 0
+        d.n_planes = len(self.planes)
+        d.defocus = _dptr(defocus, C.c_double)
+        d.n_image = _dptr(n_image, C.c_double)
+        d.n_zernike = len(zern)
+        d.zernike_j = _dptr(zj, C.c_int32)
+        d.zernike_c = _dptr(zc, C.c_double)
+        d.n_source = len(self.src)
+        d.src_weight = _dptr(w, C.c_double)
+        d.src_fx = _dptr(fx, C.c_double)
+        d.src_fy = _dptr(fy, C.c_double)
+        d.src_phi = _dptr(phi, C.c_double)
+        d.precision = precision
+        d.coarse_m = int(coarse_m)
+        d.chunk = int(chunk)
+        handle = C.c_void_p()
+        if kind == "abbe":
+            _lib.check(lib.lb_abbe_plan_create(C.byref(d), C.byref(handle)))
+        elif kind == "socs":
+            mk = -1 if max_kernels is None else int(max_kernels)
+            _lib.check(lib.lb_socs_plan_create(C.byref(d), float(tol), mk, C.byref(handle)))
+        else:
+            raise ValueError(f"unknown plan kind {kind!r}")
+        self._h = handle
+        self._lib = lib
+        self._ws = None
+        self._lock = threading.Lock()
+        info = _lib.PlanInfo()
+        _lib.check(lib.lb_plan_info_get(self._h, C.byref(info)))
+        self.info = {k: getattr(info, k) for k, _ in _lib.PlanInfo._fields_}
+        clear = np.zeros(len(self.planes))
+        _lib.check(lib.lb_plan_clear(self._h, _dptr(clear, C.c_double)))
+        self.clear = clear
+        if kind == "socs":
+            K = self.info["n_kernels"]
+            wts = np.zeros(max(K, 1))
+            e = C.c_double()
+            nb = C.c_int32()
+            ns = C.c_int32()
+            _lib.check(lib.lb_socs_weights(self._h, _dptr(wts, C.c_double), C.byref(e),
+                                           C.byref(nb), C.byref(ns)))
+            self.weights = wts[:K].copy()
+            self.energy_kept = float(e.value)
+            self.n_band = int(nb.value)
+            self.n_source = int(ns.value)
+
+    # ------------------------------------------------------------------
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.lb_plan_destroy(h)
+            except Exception:  # noqa: BLE001 - interpreter shutdown
+                pass
+            self._h = None
+
+    @property
+    def real_dtype(self):
+        return _DTYPES[self.precision][0]
+
+    @property
+    def complex_dtype(self):
+        return _DTYPES[self.precision][1]
+
+    def _workspace(self, nbytes: int, device) -> torch.Tensor:
+        if self._ws is None or self._ws.numel() < nbytes or self._ws.device != device:
+            self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+        return self._ws
+
+    def workspace_bytes(self, n_masks: int, backward: bool = False) -> int:
+        return int(self._lib.lb_workspace_bytes(self._h, int(n_masks), 1 if backward else 0))
+
+    def _check_masks(self, masks: torch.Tensor) -> tuple[torch.Tensor, int]:
+        if not isinstance(masks, torch.Tensor) or masks.device.type != "cuda":
+            raise TypeError("masks must be a torch tensor on the HIP device")
+        if masks.dim() == 2:
+            masks = masks.unsqueeze(0)
+        if masks.dim() != 3 or masks.shape[-1] != self.n or masks.shape[-2] != self.n:
+            raise ValueError(f"mask must be {(self.n, self.n)} for this plan, got "
+                             f"{tuple(masks.shape[-2:])}")
+        if masks.is_complex():
+            masks = torch.view_as_real(masks.to(self.complex_dtype).contiguous())
+            return masks, 1
+        return masks.to(self.real_dtype).contiguous(), 0
+
+    def forward(self, masks: torch.Tensor, with_peak: bool = True):
+        """Raw images [B, P, N, N] (and peaks [B, P]) of masks [B, N, N] or [N, N]."""
+        m, cplx = self._check_masks(masks)
+        B = m.shape[0]
+        P = len(self.planes)
+        out = torch.empty((B, P, self.n, self.n), dtype=self.real_dtype, device=m.device)
+        peak = torch.empty((B, P), dtype=self.real_dtype, device=m.device) if with_peak else None
+        with self._lock:
+            nb = self.workspace_bytes(B, False)
+            ws = self._workspace(nb, m.device)
+            st = torch.cuda.current_stream(m.device).cuda_stream
+            _lib.check(self._lib.lb_forward(self._h, m.data_ptr(), cplx, B, out.data_ptr(),
+                                            peak.data_ptr() if peak is not None else None,
+                                            ws.data_ptr(), ws.numel(), st))
+        return out, peak
+
+    def backward(self, masks: torch.Tensor, d_images: torch.Tensor) -> torch.Tensor:
+        """d/dmask of sum(d_images * I) for real masks [B, N, N]; d_images [B, P, N, N]."""
+        m, cplx = self._check_masks(masks)
+        if cplx:
+            raise NotImplementedError("the adjoint is implemented for real masks")
+        B = m.shape[0]
+        P = len(self.planes)
+        g = d_images.to(self.real_dtype).reshape(B, P, self.n, self.n).contiguous()
+        out = torch.empty((B, self.n, self.n), dtype=self.real_dtype, device=m.device)
+        with self._lock:
+            nb = self.workspace_bytes(B, True)
+            ws = self._workspace(nb, m.device)
+            st = torch.cuda.current_stream(m.device).cuda_stream
+            _lib.check(self._lib.lb_backward(self._h, m.data_ptr(), B, g.data_ptr(),
+                                             out.data_ptr(), ws.data_ptr(), ws.numel(), st))
+        return out
+
+    def fields(self, masks: torch.Tensor) -> torch.Tensor:
+        """SOCS plans: per-kernel coherent fields [B, K, N, N] complex on the full grid."""
+        m, cplx = self._check_masks(masks)
+        B = m.shape[0]
+        K = self.info["n_systems"]
+        out = torch.empty((B, K, self.n, self.n), dtype=self.complex_dtype, device=m.device)
+        with self._lock:
+            nb = int(self._lib.lb_fields_workspace_bytes(self._h, B))
+            ws = self._workspace(nb, m.device)
+            st = torch.cuda.current_stream(m.device).cuda_stream
+            _lib.check(self._lib.lb_fields(self._h, m.data_ptr(), cplx, B, out.data_ptr(),
+                                           ws.data_ptr(), ws.numel(), st))
+        return out
+
+    def kernels(self, device=None) -> torch.Tensor:
+        """SOCS plans: the kernels on the full grid, FFT order, complex128 [K, N, N]."""
+        K = self.info["n_kernels"]
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        out = torch.empty((K, self.n, self.n), dtype=torch.complex128, device=dev)
+        if K:
+            st = torch.cuda.current_stream(dev).cuda_stream
+            _lib.check(self._lib.lb_socs_kernels(self._h, out.data_ptr(), st))
+        return out
+
+
+# ----------------------------------------------------------------------------
+# Plan cache keyed on the optical set-up (the role of PrintModel.kernels'
+# cache in the reference, opc/model.py:201-221).
+# ----------------------------------------------------------------------------
+
+_CACHE: "OrderedDict[str, ImagingPlan]" = OrderedDict()
+_CACHE_LOCK = threading.Lock()
+_CACHE_SIZE = 16
+
+
+def _key(optics, grid, planes, precision, kind, extra) -> str:
+    od = asdict(optics) if is_dataclass(optics) else dict(vars(optics))
+    gd = asdict(grid) if is_dataclass(grid) else dict(vars(grid))
+    od["zernike_coeffs"] = {str(k): v for k, v in (od.get("zernike_coeffs") or {}).items()}
+    dev = torch.cuda.current_device() if torch.cuda.is_available() else -1
+    return json.dumps([od, gd, planes, precision, kind, extra, dev], sort_keys=True, default=str)
+
+
+def get_plan(optics, grid, planes=None, precision: int = 32, kind: str = "abbe",
+             **kw) -> ImagingPlan:
+    if planes is None:
+        planes = [(optics.defocus, _image_index(optics))]
+    planes = [(float(d), float(n)) for d, n in planes]
+    key = _key(optics, grid, planes, precision, kind, kw)
+    with _CACHE_LOCK:
+        plan = _CACHE.get(key)
+        if plan is not None:
+            _CACHE.move_to_end(key)
+            return plan
+    plan = ImagingPlan(optics, grid, planes, precision=precision, kind=kind, **kw)
+    with _CACHE_LOCK:
+        _CACHE[key] = plan
+        while len(_CACHE) > _CACHE_SIZE:
+            _CACHE.popitem(last=False)
+    return plan
+
+
+def clear_plan_cache() -> None:
+    with _CACHE_LOCK:
+        _CACHE.clear()

@@ -1,0 +1,134 @@
+"""Device Abbe sum vs the float64 oracle and the reference's golden images.
+
+FP64 path: max |dI| <= 1e-12 * peak.  FP32 path: max |dI| / max I <= 1e-5.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import litho_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _cfg(d):
+    from lithobat_amd.config import OpticsConfig
+    d = dict(d)
+    d["zernike_coeffs"] = {int(k): v for k, v in (d.get("zernike_coeffs") or {}).items()}
+    return OpticsConfig(**d)
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+CASES = {
+    # name: (optics kwargs, n, pixel)  -- coarse-grid and direct paths both covered
+    "conv_64_4nm": (dict(NA=0.93, sigma_outer=0.8, source_grid=9), 64, 4e-9),
+    "annular_defocus_zern_64_8nm": (dict(NA=0.93, source_type="annular", sigma_outer=0.85,
+                                         sigma_inner=0.5, source_grid=11, defocus=120e-9,
+                                         zernike_coeffs={7: 0.03, 11: -0.02}), 64, 8e-9),
+    "direct_tilt_64_32nm": (dict(NA=0.93, sigma_outer=0.5, source_grid=5,
+                                 zernike_coeffs={4: 0.05}, exact_defocus=True), 64, 32e-9),
+    "dipole_exact_128_4nm": (dict(NA=1.35, n_immersion=1.44, n_image=1.7, source_type="dipole",
+                                  source_kwargs={"axis": "x"}, sigma_outer=0.9, sigma_inner=0.6,
+                                  source_grid=11, defocus=-60e-9, exact_defocus=True), 128, 4e-9),
+    "quad_256_5nm": (dict(NA=0.85, source_type="quadrupole", sigma_outer=0.85,
+                          sigma_inner=0.55, source_grid=15, defocus=40e-9), 256, 5e-9),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("precision", [64, 32])
+def test_scalar_raw_matches_oracle(device, name, precision):
+    from lithobat_amd import compute_aerial_image
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    kw, n, px = CASES[name]
+    optics = OpticsConfig(normalisation="none", **kw)
+    grid = GridConfig(n_pixels=n, pixel_size=px)
+    rng = np.random.default_rng(n)
+    mask = (rng.random((n, n)) > 0.5).astype(np.float64)
+    ref, rpk, rcl = O.compute_aerial_image(mask, optics, grid, return_raw=True)
+    raw, pk, cl = compute_aerial_image(mask, optics, grid, return_raw=True, precision=precision)
+    tol = 1e-12 if precision == 64 else 1e-5
+    assert _rel(raw, ref) <= tol, (name, _rel(raw, ref))
+    assert pk == pytest.approx(rpk, rel=tol * 10)
+    assert cl == pytest.approx(rcl, rel=1e-12)
+
+
+def test_golden_scalar_planes(device):
+    from lithobat_amd import compute_aerial_planes
+    from lithobat_amd.config import GridConfig
+    z = np.load(os.path.join(GOLD, "scalar128.npz"))
+    grid = GridConfig(n_pixels=128, pixel_size=4e-9)
+    planes = [tuple(p) for p in z["planes"]]
+    for c in z["cases"]:
+        optics = _cfg(json.loads(str(z[f"{c}__optics"])))
+        for prec, tol in ((64, 1e-12), (32, 1e-5)):
+            st = compute_aerial_planes(z["mask"], optics, grid, planes, precision=prec)
+            assert _rel(st, z[f"{c}__raw"]) <= tol, (c, prec, _rel(st, z[f"{c}__raw"]))
+
+
+def test_golden_phase_shift_mask(device):
+    from lithobat_amd import compute_aerial_image
+    from lithobat_amd.config import GridConfig
+    z = np.load(os.path.join(GOLD, "scalar128.npz"))
+    optics = _cfg(json.loads(str(z["psm__optics"])))
+    grid = GridConfig(n_pixels=128, pixel_size=4e-9)
+    raw, pk, cl = compute_aerial_image(z["psm_mask"], optics, grid, return_raw=True)
+    assert _rel(raw, z["psm__raw"]) <= 1e-12
+    assert cl == pytest.approx(float(z["psm__clear"]), rel=1e-12)
+
+
+def test_golden_vector_every_polarisation(device):
+    from lithobat_amd import compute_aerial_image
+    from lithobat_amd.config import GridConfig
+    z = np.load(os.path.join(GOLD, "vector64.npz"))
+    grid = GridConfig(n_pixels=64, pixel_size=4e-9)
+    for c in z["cases"]:
+        optics = _cfg(json.loads(str(z[f"{c}__optics"])))
+        for prec, tol in ((64, 1e-12), (32, 1e-5)):
+            raw, pk, cl = compute_aerial_image(z["mask"], optics, grid, return_raw=True,
+                                               precision=prec)
+            assert _rel(raw, z[f"{c}__raw"]) <= tol, (c, prec, _rel(raw, z[f"{c}__raw"]))
+            assert cl == pytest.approx(float(z[f"{c}__clear"]), rel=1e-12)
+
+
+def test_batched_masks_and_planes_equal_single_calls(device):
+    import torch
+    from lithobat_amd import compute_aerial_batch, compute_aerial_image
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.masks import manhattan
+    grid = GridConfig(n_pixels=128, pixel_size=4e-9)
+    optics = OpticsConfig(NA=0.93, source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55,
+                          source_grid=21, normalisation="none")
+    masks = np.stack([manhattan(128, 4e-9, seed=s, n_rect=(5, 15)) for s in range(3)])
+    planes = [(-50e-9, 1.0), (0.0, 1.0), (70e-9, 1.0)]
+    mt = torch.tensor(masks, device=device, dtype=torch.float32)
+    batch = compute_aerial_batch(mt, optics, grid, planes).cpu().numpy()
+    for b in range(3):
+        for p, (dz, ni) in enumerate(planes):
+            o = OpticsConfig(**{**optics.__dict__, "defocus": dz, "n_image": ni})
+            ref = O.compute_aerial_image(masks[b], o, grid, return_raw=True)[0]
+            assert _rel(batch[b, p], ref) <= 1e-5
+            one = compute_aerial_image(mt[b], o, grid, return_raw=True)[0].cpu().numpy()
+            assert _rel(batch[b, p], one) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_1024_quadrupole_defocus_matches_oracle(device, precision):
+    """The batched-benchmark geometry at one mask and one plane."""
+    from lithobat_amd import compute_aerial_image
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.masks import manhattan
+    grid = GridConfig(n_pixels=1024, pixel_size=4e-9)
+    optics = OpticsConfig(NA=0.93, source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55,
+                          source_grid=47, defocus=-100e-9, normalisation="none")
+    mask = manhattan(1024, 4e-9, seed=1000)
+    ref = O.compute_aerial_image(mask, optics, grid, return_raw=True)[0]
+    raw = compute_aerial_image(mask, optics, grid, return_raw=True, precision=precision)[0]
+    tol = 1e-12 if precision == 64 else 1e-5
+    assert _rel(raw, ref) <= tol, _rel(raw, ref)

@@ -1,0 +1,354 @@
+"""Pins the float64 oracle (oracle/litho_oracle.py) before anything is checked against it.
+
+Each oracle function is held to something independent of its own formula:
+the golden vectors written by the reference (tests/golden), closed forms,
+invariants, or a brute-force direct DFT.  A few tests plant a plausible
+mistake in the oracle and check that the pins catch it.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import litho_oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _optics(d):
+    d = dict(d)
+    d["zernike_coeffs"] = {int(k): v for k, v in (d.get("zernike_coeffs") or {}).items()}
+    return O.Optics(**d)
+
+
+# ------------------------------------------------------------------ grids
+
+def test_freq_grid_is_the_fftfreq_definition():
+    n, px = 12, 5e-9
+    FX, FY = O.make_freq_grid(n, px)
+    nu = np.array([k if k <= (n - 1) // 2 else k - n for k in range(n)], dtype=float)
+    assert np.array_equal(FX[0], nu * (1.0 / (n * px)))
+    assert np.array_equal(FY[:, 0], nu * (1.0 / (n * px)))
+    assert np.all(FX == FX[0][None, :]) and np.all(FY == FY[:, 0][:, None])
+
+
+def test_pupil_grid_rows_are_eta():
+    rho, phi, xi, eta = O.pupil_grid(5)
+    assert xi[0, 0] == -1 and xi[0, -1] == 1 and eta[0, 0] == -1 and eta[-1, 0] == 1
+    assert rho[2, 2] == 0 and phi[-1, 2] == pytest.approx(np.pi / 2)
+
+
+# ----------------------------------------------------------------- pupil
+
+def test_zernike_low_orders_closed_forms():
+    rng = np.random.default_rng(0)
+    rho, phi = rng.random(50), rng.uniform(-np.pi, np.pi, 50)
+    assert np.allclose(O.zernike_noll(1, rho, phi), 1.0, atol=1e-14)
+    assert np.allclose(O.zernike_noll(2, rho, phi), 2 * rho * np.cos(phi), atol=1e-13)
+    assert np.allclose(O.zernike_noll(3, rho, phi), 2 * rho * np.sin(phi), atol=1e-13)
+    assert np.allclose(O.zernike_noll(4, rho, phi), np.sqrt(3) * (2 * rho ** 2 - 1), atol=1e-13)
+    assert np.allclose(O.zernike_noll(11, rho, phi),
+                       np.sqrt(5) * (6 * rho ** 4 - 6 * rho ** 2 + 1), atol=1e-12)
+    assert np.allclose(O.zernike_noll(22, rho, phi), 0.0)
+
+
+def test_zernike_noll_set_is_orthonormal_on_the_disc():
+    # polar quadrature: Gauss-Legendre in rho (exact for these polynomials), uniform in phi
+    x, w = np.polynomial.legendre.leggauss(16)
+    r = 0.5 * (x + 1)
+    wr = 0.5 * w * r
+    phi = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    R, P = np.meshgrid(r, phi, indexing="ij")
+    Wt = wr[:, None] * (2 * np.pi / 64) / np.pi
+    Z = np.stack([O.zernike_noll(j, R, P) for j in range(1, 22)])
+    G = np.einsum("aij,bij,ij->ab", Z, Z, Wt)
+    assert np.allclose(G, np.eye(21), atol=1e-10)
+
+
+def test_defocus_paraxial_closed_form_and_zero():
+    rho = np.linspace(0, 1, 11)
+    lam, NA, n, dz = 193e-9, 0.93, 1.0, 150e-9
+    assert np.allclose(O.defocus_opd(rho, lam, NA, dz, n),
+                       np.pi * NA ** 2 * rho ** 2 * dz / (lam * n), rtol=1e-12)
+    for exact in (False, True):
+        assert np.array_equal(O.defocus_opd(rho, lam, 1.35, 0.0, 1.44, exact), np.zeros(11))
+
+
+def test_exact_defocus_understates_a_third_at_immersion():
+    par = O.defocus_opd(np.array([1.0]), 193e-9, 1.35, 100e-9, 1.44, exact=False)[0]
+    exa = O.defocus_opd(np.array([1.0]), 193e-9, 1.35, 100e-9, 1.44, exact=True)[0]
+    assert 30.0 < (exa - par) / exa * 100 < 35.0
+    # exact = (2 pi / lam) n dz (1 - cos theta), sin theta = NA/n
+    assert exa == pytest.approx(2 * np.pi / 193e-9 * 1.44 * 100e-9
+                                * (1 - np.sqrt(1 - (1.35 / 1.44) ** 2)), rel=1e-13)
+
+
+def test_vector_two_beam_laws():
+    """TE overlap 1, TM overlap cos 2 theta, |v_z| = sin theta (TM), 0 (TE)."""
+    stm = 1.35 / 1.44
+    for rho in (0.3, 0.7, 1.0):
+        th = np.arcsin(rho * stm)
+        # two beams at +-theta along x; TE polarisation is along y
+        a = [np.array(O.vector_coefficients(rho, p, 0.0, 1.0, stm, obliquity=False)).real
+             for p in (0.0, np.pi)]
+        assert np.dot(a[0], a[1]) == pytest.approx(1.0, abs=1e-12)
+        assert abs(a[0][2]) < 1e-15
+        b = [np.array(O.vector_coefficients(rho, p, 1.0, 0.0, stm, obliquity=False)).real
+             for p in (0.0, np.pi)]
+        assert np.dot(b[0], b[1]) == pytest.approx(np.cos(2 * th), abs=1e-12)
+        assert abs(b[0][2]) == pytest.approx(np.sin(th), abs=1e-12)
+        # obliquity scales power by cos theta
+        c = np.array(O.vector_coefficients(rho, 0.4, 0.6, 0.8, stm, obliquity=True)).real
+        assert np.sum(c ** 2) == pytest.approx(np.cos(th), abs=1e-12)
+
+
+# ---------------------------------------------------------------- sources
+
+def test_source_tables_match_the_reference():
+    z = np.load(os.path.join(GOLD, "sources.npz"))
+    names = sorted({k.split("__")[0] for k in z.files if "__" in k})
+    assert len(names) == 5
+    for name in names:
+        o = _optics(json.loads(str(z[f"{name}__optics"])))
+        s = O.source_points(o)
+        for f in ("weight", "fx", "fy", "phi"):
+            assert np.array_equal(getattr(s, f), z[f"{name}__{f}"]), (name, f)
+    counts = {n: z[f"{n}__weight"].size for n in names}
+    assert counts["annular_0.5_0.8_g41_na0.85"] == 488
+    assert counts["quad_0.85_0.55_g47_na0.93"] == 148
+    assert counts["annular_0.9_0.6_g21_na1.35"] == 142
+
+
+def test_sources_are_normalised_and_bounded():
+    o = O.Optics(source_type="annular", sigma_outer=0.8, sigma_inner=0.5, source_grid=15)
+    s = O.source_points(o)
+    k = o.NA / o.wavelength
+    assert s.weight.sum() == pytest.approx(1.0)
+    r = np.hypot(s.fx, s.fy)
+    assert np.all(r <= k * (1 + 1e-12)) and np.all(r >= 0.5 * k * (1 - 1e-12))
+
+
+# ------------------------------------------------------------- Abbe sum
+
+def test_abbe_sum_equals_the_direct_dft():
+    """Independent of any FFT: explicit exponentials, defocus and aberrations on."""
+    g = O.Grid(16, 25e-9)
+    rng = np.random.default_rng(3)
+    mask = (rng.random((16, 16)) > 0.6).astype(float)
+    o = O.Optics(NA=0.93, sigma_outer=0.7, source_grid=7, defocus=60e-9,
+                 zernike_coeffs={7: 0.03, 5: -0.02}, exact_defocus=True, normalisation="none")
+    raw, peak, _ = O.compute_aerial_image(mask, o, g, return_raw=True)
+    ref = O.abbe_direct_dft(mask, o, g)
+    assert np.abs(raw - ref).max() < 1e-13 * ref.max()
+
+
+def test_source_point_tilts_the_mask_the_other_way():
+    """aerial_image.py:60-73 / tests/test_aerial_image.py:136-202, restated."""
+    n, dx = 64, 32e-9
+    mask = np.random.default_rng(1).random((n, n))
+    sx, sy = 3, -2
+    lam, NA = 193e-9, 0.93
+    fsx, fsy = sx / (n * dx), sy / (n * dx)
+    FX, FY = O.make_freq_grid(n, dx)
+
+    def pupil(fx0, fy0):
+        rho, phi, inside = O.pupil_geometry(FX, FY, lam, NA, fx0, fy0, need_phi=True)
+        W = O.pupil_phase(rho, phi, lam, NA, 0.0, 1.0, {4: 0.05}, True)
+        return np.where(inside, np.exp(1j * W), 0)
+
+    eng = np.abs(np.fft.ifft2(np.fft.fft2(mask) * pupil(fsx, fsy))) ** 2
+    X, Y = np.meshgrid(np.arange(n), np.arange(n))
+
+    def tilted(sign):
+        t = np.exp(sign * 2j * np.pi * (sx * X + sy * Y) / n)
+        return np.abs(np.fft.ifft2(np.fft.fft2(mask * t) * pupil(0.0, 0.0))) ** 2
+
+    assert np.abs(eng - tilted(-1)).max() < 1e-12 * eng.max()
+    assert np.abs(eng - tilted(+1)).max() > 1e-3 * eng.max()
+
+
+@pytest.mark.parametrize("pol", ["te", "tm", "x", "y", "unpolarised"])
+def test_clear_field_prints_one_and_flat(pol):
+    g = O.Grid(32, 4e-9)
+    model = "scalar" if pol == "x" else "vector"
+    o = O.Optics(NA=1.35, n_immersion=1.44, imaging_model=model, polarisation=pol,
+                 source_type="annular", sigma_outer=0.9, sigma_inner=0.6, source_grid=9,
+                 normalisation="clear")
+    img = O.compute_aerial_image(np.ones((32, 32)), o, g)
+    assert img.mean() == pytest.approx(1.0, rel=1e-9)
+    assert np.ptp(img) < 1e-12
+
+
+def test_planes_equal_single_images():
+    g = O.Grid(32, 8e-9)
+    mask = O.lines_and_spaces(32, 8e-9, 128e-9, 64e-9)
+    o = O.Optics(NA=0.93, source_type="annular", sigma_outer=0.85, sigma_inner=0.5,
+                 source_grid=9, zernike_coeffs={7: 0.03})
+    planes = [(-100e-9, 1.0), (45e-9, 1.2)]
+    stack = O.compute_aerial_planes(mask, o, g, planes)
+    for p, (dz, ni) in enumerate(planes):
+        one = O.compute_aerial_image(mask, O.Optics(**{**o.__dict__, "defocus": dz,
+                                                       "n_image": ni}), g)
+        assert np.abs(stack[p] - one).max() < 1e-12 * one.max()
+
+
+def _gold_vector():
+    return np.load(os.path.join(GOLD, "vector64.npz"))
+
+
+def test_vector_images_match_the_reference():
+    z = _gold_vector()
+    g = O.Grid(64, 4e-9)
+    for c in z["cases"]:
+        o = _optics(json.loads(str(z[f"{c}__optics"])))
+        raw, pk, cl = O.compute_aerial_image(z["mask"], o, g, return_raw=True)
+        assert np.abs(raw - z[f"{c}__raw"]).max() <= 1e-12 * pk, c
+        assert cl == pytest.approx(float(z[f"{c}__clear"]), rel=1e-12)
+
+
+def test_scalar_planes_match_the_reference():
+    z = np.load(os.path.join(GOLD, "scalar128.npz"))
+    g = O.Grid(128, 4e-9)
+    planes = [tuple(p) for p in z["planes"]]
+    for c in z["cases"]:
+        o = _optics(json.loads(str(z[f"{c}__optics"])))
+        st = O.compute_aerial_planes(z["mask"], o, g, planes)
+        assert np.abs(st - z[f"{c}__raw"]).max() <= 1e-12 * st.max(), c
+    o = _optics(json.loads(str(z["psm__optics"])))
+    raw, pk, cl = O.compute_aerial_image(z["psm_mask"], o, g, return_raw=True)
+    assert np.abs(raw - z["psm__raw"]).max() <= 1e-12 * pk
+
+
+# ------------------------------------------------------------------- SOCS
+
+@pytest.fixture(scope="module")
+def socs_gold():
+    z = np.load(os.path.join(GOLD, "socs64.npz"))
+    o = _optics(json.loads(str(z["optics"])))
+    return z, o, O.Grid(64, 8e-9)
+
+
+def test_socs_eigenvalues_and_images_match_the_reference(socs_gold):
+    z, o, g = socs_gold
+    for name, kw in (("full", dict(tol=0.0)), ("k24", dict(tol=0.0, max_kernels=24)),
+                     ("default", {})):
+        k = O.socs_build(o, g, **kw)
+        ref_w = z[f"{name}__weights"]
+        top = min(40, ref_w.size)
+        assert np.allclose(k.weights[:top], ref_w[:top], rtol=1e-10, atol=1e-14 * ref_w[0])
+        raw, pk, cl = O.socs_raw(k, z["mask"])
+        assert np.abs(raw - z[f"{name}__raw"]).max() <= 1e-11 * pk, name
+        assert cl == pytest.approx(float(z[f"{name}__clear"]), rel=1e-10)
+        assert k.n_band == int(z[f"{name}__n_band"])
+
+
+def test_full_rank_socs_is_the_abbe_sum(socs_gold):
+    z, o, g = socs_gold
+    k = O.socs_build(o, g, tol=0.0)
+    raw_k, pk_k, cl_k = O.socs_raw(k, z["mask"])
+    raw_a, pk_a, cl_a = O.compute_aerial_image(z["mask"], o, g, return_raw=True)
+    assert np.abs(raw_k - raw_a).max() < 1e-12 * pk_a
+    assert cl_k == pytest.approx(cl_a, rel=1e-12)
+
+
+# ---------------------------------------------------------------- adjoints
+
+def _fd(f, x, idx, h=1e-6):
+    out = []
+    for i in idx:
+        xp, xm = x.copy(), x.copy()
+        xp.flat[i] += h
+        xm.flat[i] -= h
+        out.append((f(xp) - f(xm)) / (2 * h))
+    return np.array(out)
+
+
+def test_socs_gradient_matches_finite_differences():
+    g = O.Grid(32, 8e-9)
+    k = O.socs_build(O.Optics(), g, tol=1e-6)
+    rng = np.random.default_rng(5)
+    mask = rng.random((32, 32))
+    sens = rng.standard_normal((32, 32))
+    fields, _ = O.socs_fields(k, mask)
+    grad = O.socs_gradient(k, fields, sens)
+    idx = rng.choice(32 * 32, 8, replace=False)
+    fd = _fd(lambda m: float(np.sum(sens * O.socs_fields(k, m)[1])), mask, idx)
+    assert np.allclose(grad.flat[idx], fd, rtol=1e-5, atol=1e-9)
+
+
+@pytest.mark.parametrize("model,pol", [("scalar", "x"), ("vector", "te"), ("vector", "unpolarised")])
+def test_abbe_gradient_matches_finite_differences(model, pol):
+    g = O.Grid(32, 8e-9)
+    o = O.Optics(NA=1.35, n_immersion=1.44, imaging_model=model, polarisation=pol,
+                 source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55, source_grid=9,
+                 defocus=30e-9, zernike_coeffs={8: 0.02})
+    rng = np.random.default_rng(6)
+    mask = rng.random((32, 32))
+    sens = rng.standard_normal((32, 32))
+    grad = O.abbe_gradient(mask, o, g, sens)
+    idx = rng.choice(32 * 32, 6, replace=False)
+    fd = _fd(lambda m: float(np.sum(sens * O.compute_aerial_image(m, o, g, return_raw=True)[0])),
+             mask, idx)
+    assert np.allclose(grad.flat[idx], fd, rtol=1e-5, atol=1e-9)
+
+
+def test_abbe_gradient_equals_full_rank_socs_gradient():
+    g = O.Grid(32, 8e-9)
+    o = O.Optics(NA=0.93, source_type="annular", sigma_outer=0.8, sigma_inner=0.5, source_grid=9)
+    k = O.socs_build(o, g, tol=0.0)
+    rng = np.random.default_rng(8)
+    mask = rng.random((32, 32))
+    sens = rng.standard_normal((32, 32))
+    a = O.abbe_gradient(mask, o, g, sens)
+    s = O.socs_gradient(k, O.socs_fields(k, mask)[0], sens)
+    assert np.abs(a - s).max() < 1e-10 * np.abs(a).max()
+
+
+# ------------------------------------------------- planted mistakes are caught
+
+def test_a_wrong_tilt_sign_is_caught(monkeypatch):
+    """One off-axis source point (no centrosymmetric partner to cancel the sign)."""
+    n, dx = 32, 32e-9
+    g = O.Grid(n, dx)
+    mask = np.random.default_rng(4).random((n, n))
+    # an even aberration: with an odd one and a real mask the two signs image alike
+    o = O.Optics(NA=0.93, zernike_coeffs={4: 0.05}, normalisation="none")
+    one = O.SourcePts(weight=np.array([1.0]), fx=np.array([3 / (n * dx)]),
+                      fy=np.array([-2 / (n * dx)]), phi=np.array([0.0]), grid=3)
+    good = O.abbe_sum(mask, o, g, [(0.0, 1.0)], src=one)[0][0]
+    real = O.pupil_geometry
+    monkeypatch.setattr(O, "pupil_geometry",
+                        lambda FX, FY, lam, NA, fx=0.0, fy=0.0, need_phi=False:
+                        real(FX, FY, lam, NA, -fx, -fy, need_phi))
+    bad = O.abbe_sum(mask, o, g, [(0.0, 1.0)], src=one)[0][0]
+    assert np.abs(bad - good).max() > 1e-3 * good.max()
+
+
+def test_a_dropped_obliquity_fails_the_vector_golden(monkeypatch):
+    z = _gold_vector()
+    o = _optics(json.loads(str(z["te_obl1__optics"])))
+    real = O.vector_coefficients
+    monkeypatch.setattr(O, "vector_coefficients",
+                        lambda rho, phi, jx, jy, stm, obliquity=True:
+                        real(rho, phi, jx, jy, stm, obliquity=False))
+    raw = O.compute_aerial_image(z["mask"], o, O.Grid(64, 4e-9), return_raw=True)[0]
+    assert np.abs(raw - z["te_obl1__raw"]).max() > 1e-4 * raw.max()
+
+
+def test_transposed_source_rows_fail_the_golden_table():
+    z = np.load(os.path.join(GOLD, "sources.npz"))
+    o = _optics(json.loads(str(z["dipole_y_0.9_0.6_g11__optics"])))
+    s = O.source_points(o)
+    assert not np.array_equal(s.fy, z["dipole_y_0.9_0.6_g11__fx"])
+    assert np.array_equal(s.fy, z["dipole_y_0.9_0.6_g11__fy"])
+
+
+def test_lines_and_spaces_period_and_duty():
+    m = O.lines_and_spaces(200, 4e-9, 200e-9, 100e-9)
+    row = m[0]
+    # 200 nm pitch = 50 px (edge pixels may flip with float rounding of mod)
+    assert np.mean(row[:150] == row[50:200]) > 0.97
+    assert abs(row.mean() - 0.5) < 0.02                 # 100 nm CD
+    assert np.all(m == row[None, :])                    # vertical lines
