@@ -148,6 +148,13 @@ lb_status lb_fields(lb_plan plan, const void* masks, int32_t mask_complex, int32
                     void* fields, void* workspace, size_t workspace_bytes, void* stream);
 size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks);
 
+/* Timing aid (no counterpart in the reference): when enabled, lb_forward
+ * records a HIP event pair on its stream around each launch of the
+ * coherent-system kernel; lb_profile_read waits for them and returns the
+ * launch count and summed kernel time.  Enabling resets the record. */
+lb_status lb_profile_enable(lb_plan plan, int32_t on);
+lb_status lb_profile_read(lb_plan plan, int32_t* launches, double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif

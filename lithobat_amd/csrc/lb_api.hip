@@ -80,8 +80,13 @@ struct Plan {
   double energy = 1.0;
   int n_band = 0;
   void* phi128 = nullptr;  // [K][W][W] complex128
+  // optional event timing of the coherent-system kernel (lb_profile_*)
+  int profile = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_pairs;  // (start, stop) indices per launch
 
   ~Plan() {
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (H) (void)hipFree(H);
     if (box) (void)hipFree(box);
     if (wsys) (void)hipFree(wsys);
@@ -331,6 +336,19 @@ int grid_for(const Plan& p, const FieldConfig& fc, int n_items, bool adjoint) {
   return std::max(1, std::min(n_items, p.cus * occ));
 }
 
+hipError_t profile_events(Plan& p, int* e0, int* e1) {
+  for (int k = 0; k < 2; ++k) {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    p.ev_pool.push_back(e);
+  }
+  *e0 = (int)p.ev_pool.size() - 2;
+  *e1 = (int)p.ev_pool.size() - 1;
+  p.ev_pairs.emplace_back(*e0, *e1);
+  return hipSuccess;
+}
+
 template <typename T>
 lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* images, void* peak,
                     char* ws, const Work& w, hipStream_t st) {
@@ -342,7 +360,13 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
   fc.grid = grid_for(p, fc, w.n_items, false);
   FieldArgs a = field_args(p, Xm, w);
   T* partial = (T*)(ws + w.partial);
+  int e0 = -1, e1 = -1;
+  if (p.profile) {
+    LB_HIP(profile_events(p, &e0, &e1));
+    LB_HIP(hipEventRecord(p.ev_pool[e0], st));
+  }
   LB_HIP(field_forward(p.precision, fc, a, partial, st));
+  if (p.profile) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
   T* pk = peak ? (T*)peak : (T*)(ws + w.peak);
   LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
   if (p.direct) {
@@ -738,6 +762,31 @@ lb_status lb_socs_weights(lb_plan plan, double* weights, double* energy, int32_t
   if (energy) *energy = p.energy;
   if (n_band) *n_band = p.n_band;
   if (n_source) *n_source = p.S;
+  return LB_OK;
+}
+
+lb_status lb_profile_enable(lb_plan plan, int32_t on) {
+  if (!plan) return fail(LB_ERR_ARG, "null plan");
+  Plan& p = plan->p;
+  for (hipEvent_t e : p.ev_pool) (void)hipEventDestroy(e);
+  p.ev_pool.clear();
+  p.ev_pairs.clear();
+  p.profile = on ? 1 : 0;
+  return LB_OK;
+}
+
+lb_status lb_profile_read(lb_plan plan, int32_t* launches, double* total_ms) {
+  if (!plan || !launches || !total_ms) return fail(LB_ERR_ARG, "null argument");
+  Plan& p = plan->p;
+  double t = 0;
+  for (auto& pr : p.ev_pairs) {
+    LB_HIP(hipEventSynchronize(p.ev_pool[pr.second]));
+    float ms = 0;
+    LB_HIP(hipEventElapsedTime(&ms, p.ev_pool[pr.first], p.ev_pool[pr.second]));
+    t += ms;
+  }
+  *launches = (int32_t)p.ev_pairs.size();
+  *total_ms = t;
   return LB_OK;
 }
 

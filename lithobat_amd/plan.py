@@ -215,6 +215,17 @@ class ImagingPlan:
                                            ws.data_ptr(), ws.numel(), st))
         return out
 
+    def profile(self, on: bool) -> None:
+        """Record HIP events around every coherent-system kernel launch (resets)."""
+        _lib.check(self._lib.lb_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self) -> tuple[int, float]:
+        """(launches, summed kernel milliseconds) since ``profile(True)``."""
+        n = C.c_int32()
+        t = C.c_double()
+        _lib.check(self._lib.lb_profile_read(self._h, C.byref(n), C.byref(t)))
+        return int(n.value), float(t.value)
+
     def kernels(self, device=None) -> torch.Tensor:
         """SOCS plans: the kernels on the full grid, FFT order, complex128 [K, N, N]."""
         K = self.info["n_kernels"]
