@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--plain-images", type=int, default=20,
                     help="images timed for the plain single-image form")
     ap.add_argument("--ilt-steps", type=int, default=10)
+    ap.add_argument("--tune", metavar="TABLE", default=None,
+                    help="autotune the kernel shapes of this workload and write the table")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (--gpus {args.gpus}"
@@ -119,6 +121,19 @@ def main():
 
     # ---------------- batched: 64 masks x 5 planes per step --------------------
     plan = ImagingPlan(optics, grid, planes, precision=32)
+    ilt = ImagingPlan(optics_cfg4(), grid, precision=32)
+    if args.tune:
+        from lithobat_amd import autotune
+        res = autotune.tune(plan, masks, adjoint=False, save=False)
+        th = torch.randn((1, N, N), device=dev)
+        res.update(autotune.tune(ilt, torch.sigmoid(8 * th), adjoint=True, save=False))
+        one = ImagingPlan(optics, grid, planes[:1], precision=32)
+        res.update(autotune.tune(one, masks[:1], adjoint=False, save=False))
+        table = autotune.load_table()
+        table.update(res)
+        autotune.save_table(table, args.tune)
+        print(json.dumps(res), file=sys.stderr)
+    plan._use_tuning(N_MASKS)
     n_img = N_MASKS * len(planes)
     out = torch.empty((N_MASKS, len(planes), N, N), device=dev, dtype=torch.float32)
     peak = torch.empty((N_MASKS, len(planes)), device=dev, dtype=torch.float32)
@@ -189,7 +204,6 @@ def main():
     plain_images_per_s = 1.0 / plain_s
 
     # ---------------- ILT forward + backward at 1024^2 ---------------------------
-    ilt = ImagingPlan(optics_cfg4(), grid, precision=32)
     scale = 1.0 / float(ilt.clear[0])
     rng = np.random.default_rng(3000)
     theta = torch.tensor(rng.standard_normal((1, N, N)), device=dev, dtype=torch.float32,

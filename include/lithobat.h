@@ -148,6 +148,16 @@ lb_status lb_fields(lb_plan plan, const void* masks, int32_t mask_complex, int32
                     void* fields, void* workspace, size_t workspace_bytes, void* stream);
 size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks);
 
+/* Kernel-shape control for the autotuner (no counterpart in the reference):
+ * list the coherent-system kernel shapes (threads, column splits) that fit
+ * this plan, and select one for the forward (adjoint = 0) or adjoint
+ * (adjoint = 1) kernel; chunk >= 0 also sets the systems per work item
+ * (0 = automatic). */
+lb_status lb_plan_shapes(lb_plan plan, int32_t adjoint, int32_t* shapes_pairs,
+                         int32_t max_shapes, int32_t* n_shapes);
+lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads,
+                            int32_t column_splits, int32_t chunk);
+
 /* Timing aid (no counterpart in the reference): when enabled, lb_forward
  * records a HIP event pair on its stream around each launch of the
  * coherent-system kernel; lb_profile_read waits for them and returns the

@@ -43,7 +43,8 @@ EXPORTS = [
     "lb_version", "lb_last_error", "lb_abbe_plan_create", "lb_socs_plan_create",
     "lb_plan_destroy", "lb_plan_info_get", "lb_plan_clear", "lb_socs_weights",
     "lb_socs_kernels", "lb_workspace_bytes", "lb_forward", "lb_backward", "lb_fields",
-    "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read",
+    "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read", "lb_plan_shapes",
+    "lb_plan_set_shape",
 ]
 
 _lib = None
@@ -79,6 +80,8 @@ def load() -> C.CDLL:
         "lb_fields_workspace_bytes": ([vp, i32], sz),
         "lb_profile_enable": ([vp, i32], C.c_int),
         "lb_profile_read": ([vp, C.POINTER(i32), C.POINTER(C.c_double)], C.c_int),
+        "lb_plan_shapes": ([vp, i32, C.POINTER(i32), i32, C.POINTER(i32)], C.c_int),
+        "lb_plan_set_shape": ([vp, i32, i32, i32, i32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,133 @@
+"""Autotuner for the coherent-system kernel schedule.
+
+For a plan and a batch size it times every compiled kernel shape that fits
+(threads per block x column splits, i.e. the radix-stage tiling of the
+coarse-grid transforms) and a few work-item sizes (systems per block), with
+HIP events on the launch stream, and keeps the fastest.  Winners are stored in
+a JSON table keyed on (precision, coarse grid M, pupil box W, images per call
+and systems per image, both bucketed to powers of two), so one tuning run
+serves every later plan of the same shape.  The table lives at
+``$LITHOBAT_TUNING`` or ``lithobat_amd/tuning.json``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import threading
+
+import torch
+
+from . import _lib
+
+TABLE_PATH = os.environ.get(
+    "LITHOBAT_TUNING", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
+_lock = threading.Lock()
+_table: dict | None = None
+
+
+def _bucket(x: int) -> int:
+    b = 1
+    while b < x:
+        b *= 2
+    return b
+
+
+def key_of(plan, n_masks: int, adjoint: bool) -> str:
+    i = plan.info
+    n_img = n_masks * i["n_planes"]
+    return (f"p{plan.precision}-M{i['m']}-W{i['box']}-img{_bucket(n_img)}"
+            f"-sys{_bucket(i['n_systems'])}-{'adj' if adjoint else 'fwd'}")
+
+
+def load_table() -> dict:
+    global _table
+    with _lock:
+        if _table is None:
+            try:
+                with open(TABLE_PATH) as f:
+                    _table = json.load(f)
+            except (OSError, ValueError):
+                _table = {}
+        return _table
+
+
+def save_table(table: dict, path: str | None = None) -> None:
+    path = path or TABLE_PATH
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def shapes(plan, adjoint: bool) -> list[tuple[int, int]]:
+    lib = _lib.load()
+    buf = (C.c_int32 * 64)()
+    n = C.c_int32()
+    _lib.check(lib.lb_plan_shapes(plan._h, 1 if adjoint else 0, buf, 32, C.byref(n)))
+    return [(buf[2 * k], buf[2 * k + 1]) for k in range(min(n.value, 32))]
+
+
+def set_shape(plan, adjoint: bool, nt: int, cs: int, chunk: int) -> None:
+    _lib.check(_lib.load().lb_plan_set_shape(plan._h, 1 if adjoint else 0, nt, cs, chunk))
+
+
+def apply(plan, n_masks: int) -> bool:
+    """Select the tuned shapes for this batch size if the table has them."""
+    table = load_table()
+    hit = False
+    for adjoint in (False, True):
+        e = table.get(key_of(plan, n_masks, adjoint))
+        if e:
+            try:
+                set_shape(plan, adjoint, e["threads"], e["splits"], e["chunk"])
+                hit = True
+            except (ValueError, NotImplementedError):
+                pass
+    return hit
+
+
+def _time(fn, reps: int) -> float:
+    fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        fn()
+    stop.record()
+    stop.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
+         chunks=(0, 1, 2, 4, 8, 16, 32), save: bool = True) -> dict:
+    """Time every candidate for ``masks`` (the real batch) and record the winners."""
+    n_masks = masks.shape[0] if masks.dim() == 3 else 1
+    plan._tuned_for.add(n_masks)  # keep the table from overriding the candidates
+    P = len(plan.planes)
+    results = {}
+    spp = plan.info["n_systems"]
+    chunk_cands = sorted({c for c in chunks if c <= max(spp, 1)})
+    for adj in ([False, True] if adjoint else [False]):
+        best = None
+        d_img = torch.randn((n_masks, P, plan.n, plan.n), device=masks.device,
+                            dtype=plan.real_dtype) if adj else None
+        for nt, cs in shapes(plan, adj):
+            for ch in chunk_cands:
+                set_shape(plan, adj, nt, cs, ch)
+                if adj:
+                    ms = _time(lambda: plan.backward(masks, d_img), reps)
+                else:
+                    ms = _time(lambda: plan.forward(masks, with_peak=False), reps)
+                if best is None or ms < best[0]:
+                    best = (ms, nt, cs, ch)
+        ms, nt, cs, ch = best
+        set_shape(plan, adj, nt, cs, ch)
+        results[key_of(plan, n_masks, adj)] = {"threads": nt, "splits": cs, "chunk": ch,
+                                               "ms": round(ms, 4)}
+    if save:
+        table = load_table()
+        with _lock:
+            table.update(results)
+        save_table(table)
+    return results

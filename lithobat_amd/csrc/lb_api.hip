@@ -39,7 +39,7 @@ lb_status fail(lb_status s, const std::string& msg) {
   } while (0)
 
 constexpr size_t kLdsLimit = 160 * 1024;
-const int kFieldSizes[] = {16, 32, 64, 128, 256};
+const int kFieldSizes[] = {16, 32, 48, 64, 96, 128, 256};
 
 // Noll j -> (n, m), expose/pupil.py:56-78.
 const int kNoll[22][2] = {{0, 0},  {0, 0},  {1, 1},  {1, -1}, {2, 0},  {2, -2}, {2, 2},  {3, -1},
@@ -65,7 +65,7 @@ struct Plan {
   int QI = 0, WbI = 0;
   int nu0m = 0, Wbm = 0, wrapm = 0;
   int n_planes = 1, spp = 0;  // systems per plane
-  int chunk_override = 0;
+  int chunk_fwd = 0, chunk_adj = 0;  // systems per work item (0 = automatic)
   FieldConfig fwd{}, adj{};
   int cus = 256;
   // device tables
@@ -105,22 +105,15 @@ bool field_size(int m) {
 // Coherent-system kernel shape for (M, W): prefer no column split, then the
 // smallest split whose LDS fits (the adjoint keeps two W x M/CS tables).
 bool pick_field_configs(Plan& p) {
-  struct Cand {
-    int m, nt, cs;
-  };
-  const Cand cands[] = {{16, 64, 1},   {32, 128, 1},  {64, 256, 1},  {64, 128, 1},
-                        {64, 256, 2},  {128, 256, 1}, {128, 512, 1}, {128, 256, 2},
-                        {128, 256, 4}, {256, 256, 2}, {256, 256, 4}, {256, 256, 8}};
   bool got_f = false, got_a = false;
-  for (const Cand& c : cands) {
-    if (c.m != p.M) continue;
-    FieldConfig fc{c.m, c.nt, c.cs, 0};
-    if (!got_f && field_smem_bytes(p.precision, fc, p.W, false) <= kLdsLimit) {
-      p.fwd = fc;
+  for (const FieldConfig& c : field_configs()) {
+    if (c.M != p.M) continue;
+    if (!got_f && field_smem_bytes(p.precision, c, p.W, false) <= kLdsLimit) {
+      p.fwd = c;
       got_f = true;
     }
-    if (!got_a && field_smem_bytes(p.precision, fc, p.W, true) <= kLdsLimit) {
-      p.adj = fc;
+    if (!got_a && field_smem_bytes(p.precision, c, p.W, true) <= kLdsLimit) {
+      p.adj = c;
       got_a = true;
     }
   }
@@ -139,7 +132,7 @@ lb_status set_geometry_common(Plan& p, const lb_imaging_desc* d) {
   p.pol = d->polarisation;
   p.obliq = d->obliquity ? 1 : 0;
   p.exact = (d->exact_defocus || d->vector) ? 1 : 0;  // aerial_image.py:392
-  p.chunk_override = d->chunk > 0 ? d->chunk : 0;
+  p.chunk_fwd = p.chunk_adj = d->chunk > 0 ? d->chunk : 0;
   PupilParams& pp = p.pp;
   std::memset(&pp, 0, sizeof(pp));
   pp.N = p.N;
@@ -270,7 +263,7 @@ struct Layout {
 
 struct Work {
   int n_masks, n_img, n_chunks, chunk, n_items;
-  size_t Xm, RTm, partial, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, peak;
+  size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, peak;
   size_t total;
 };
 
@@ -278,7 +271,7 @@ Work layout(const Plan& p, int n_masks, bool backward) {
   Work w{};
   w.n_masks = n_masks;
   w.n_img = n_masks * p.n_planes;
-  int chunk = p.chunk_override;
+  int chunk = backward ? p.chunk_adj : p.chunk_fwd;
   if (chunk <= 0) {
     const long long total = (long long)w.n_img * p.spp;
     const long long target = 4LL * p.cus;
@@ -296,6 +289,16 @@ Work layout(const Plan& p, int n_masks, bool backward) {
   w.RTm = L.take(C * (size_t)n_masks * p.Wbm * p.N);
   w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
   w.peak = L.take(R * (size_t)w.n_img);
+  {
+    // split-sum scratch: coarse partials (forward) or band gradients (backward)
+    const long long mm = (long long)p.M * p.M, bb = 2LL * p.Wbm * p.Wbm;
+    size_t seg = R * (size_t)w.n_img * segment_splits(w.n_img, w.n_chunks, mm) * mm;
+    if (backward)
+      seg = std::max<size_t>(seg, R * (size_t)n_masks *
+                                      segment_splits(n_masks, p.n_planes * w.n_chunks, bb) * bb);
+    w.seg = L.take(seg);
+  }
+  if (!p.direct) w.IM = L.take(R * (size_t)w.n_img * p.M * p.M);
   if (!p.direct) {
     w.RTc = L.take(C * (size_t)w.n_img * p.WbI * p.M);
     w.Y = L.take(C * (size_t)w.n_img * p.WbI * p.WbI);
@@ -370,12 +373,15 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
   T* pk = peak ? (T*)peak : (T*)(ws + w.peak);
   LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
   if (p.direct) {
-    LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.N * p.N, (T*)images, pk, st));
+    LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.N * p.N, (T*)images, pk,
+                          (T*)(ws + w.seg), st));
   } else {
     const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
     cx<T>* Y = (cx<T>*)(ws + w.Y);
-    LB_HIP(band_analysis<T>(p.M, partial, 0, (long long)w.n_chunks * p.M * p.M,
-                            (long long)p.M * p.M, w.n_chunks, w.n_img, -p.QI, p.WbI, invM2,
+    T* IM = (T*)(ws + w.IM);
+    LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.M * p.M, IM, (T*)nullptr,
+                          (T*)(ws + w.seg), st));
+    LB_HIP(band_analysis<T>(p.M, IM, 0, (long long)p.M * p.M, 0, 1, w.n_img, -p.QI, p.WbI, invM2,
                             (cx<T>*)(ws + w.RTc), Y, st));
     LB_HIP(band_synthesis<T>(p.N, Y, w.n_img, -p.QI, p.WbI, 1, (T)1, 0, (cx<T>*)(ws + w.TT),
                              (T*)images, pk, st));
@@ -407,7 +413,7 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
   LB_HIP(field_adjoint(p.precision, fc, a, gM, gX, st));
   cx<T>* GX = (cx<T>*)(ws + w.GX);
   LB_HIP(segment_sum<T>((const T*)gX, n_masks, p.n_planes * w.n_chunks,
-                        2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, st));
+                        2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, (T*)(ws + w.seg), st));
   LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, 1, invN2, 0, (cx<T>*)(ws + w.TTm),
                            (T*)dmask, (T*)nullptr, st));
   return LB_OK;
@@ -762,6 +768,44 @@ lb_status lb_socs_weights(lb_plan plan, double* weights, double* energy, int32_t
   if (energy) *energy = p.energy;
   if (n_band) *n_band = p.n_band;
   if (n_source) *n_source = p.S;
+  return LB_OK;
+}
+
+lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads,
+                            int32_t column_splits, int32_t chunk) {
+  if (!plan) return fail(LB_ERR_ARG, "null plan");
+  Plan& p = plan->p;
+  FieldConfig fc{p.M, block_threads, column_splits, 0};
+  if (!field_config_supported(p.precision, fc))
+    return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel of shape M=" +
+                                        std::to_string(p.M) + " threads=" +
+                                        std::to_string(block_threads) + " splits=" +
+                                        std::to_string(column_splits));
+  if (field_smem_bytes(p.precision, fc, p.W, adjoint != 0) > kLdsLimit)
+    return fail(LB_ERR_UNSUPPORTED, "shape does not fit in LDS");
+  if (adjoint)
+    p.adj = fc;
+  else
+    p.fwd = fc;
+  if (chunk >= 0) (adjoint ? p.chunk_adj : p.chunk_fwd) = chunk;
+  return LB_OK;
+}
+
+lb_status lb_plan_shapes(lb_plan plan, int32_t adjoint, int32_t* shapes, int32_t max_shapes,
+                         int32_t* n_shapes) {
+  if (!plan || !n_shapes) return fail(LB_ERR_ARG, "null argument");
+  Plan& p = plan->p;
+  int n = 0;
+  for (const FieldConfig& c : field_configs()) {
+    if (c.M != p.M) continue;
+    if (field_smem_bytes(p.precision, c, p.W, adjoint != 0) > kLdsLimit) continue;
+    if (shapes && n < max_shapes) {
+      shapes[2 * n] = c.nt;
+      shapes[2 * n + 1] = c.csplit;
+    }
+    ++n;
+  }
+  *n_shapes = n;
   return LB_OK;
 }
 

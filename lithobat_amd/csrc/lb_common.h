@@ -131,9 +131,66 @@ struct DFT2k<4, DIR, T> {
   }
 };
 
+// e^{DIR * 2 pi i k / R} for compile-time R | 12 (multiples of 30 degrees).
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cx<T> root12(int k) {
+  const int q = (k * (12 / R)) % 12;
+  const T h = (T)0.86602540378443865;  // sqrt(3)/2
+  T c, s;
+  switch (q) {
+    case 0: c = 1; s = 0; break;
+    case 1: c = h; s = (T)0.5; break;
+    case 2: c = (T)0.5; s = h; break;
+    case 3: c = 0; s = 1; break;
+    case 4: c = (T)-0.5; s = h; break;
+    case 5: c = -h; s = (T)0.5; break;
+    case 6: c = -1; s = 0; break;
+    case 7: c = -h; s = (T)-0.5; break;
+    case 8: c = (T)-0.5; s = -h; break;
+    case 9: c = 0; s = -1; break;
+    case 10: c = (T)0.5; s = -h; break;
+    default: c = h; s = (T)-0.5; break;
+  }
+  return cx<T>{c, DIR > 0 ? s : -s};
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft(cx<T>* v);
+
+// Composite R = A * B in registers (Cooley-Tukey): input n = B n1 + n2,
+// output k = k1 + A k2; A-point DFTs, twiddles e^{DIR 2 pi i n2 k1 / R},
+// then B-point DFTs.  Used for the 3 * 2^k sizes (R = 6, 12).
+template <int A, int B, int DIR, typename T>
+__device__ __forceinline__ void dft_mixed(cx<T>* v) {
+  constexpr int R = A * B;
+  cx<T> y[B][A];
+#pragma unroll
+  for (int n2 = 0; n2 < B; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) y[n2][n1] = v[B * n1 + n2];
+    dft<A, DIR>(y[n2]);
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1)
+      if (n2 > 0) y[n2][k1] = y[n2][k1] * root12<R, DIR, T>(n2 * k1);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) {
+    cx<T> z[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) z[n2] = y[n2][k1];
+    dft<B, DIR>(z);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = z[k2];
+  }
+}
+
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dft(cx<T>* v) {
-  if constexpr (R == 3) {
+  if constexpr (R == 6) {
+    dft_mixed<2, 3, DIR, T>(v);
+  } else if constexpr (R == 12) {
+    dft_mixed<4, 3, DIR, T>(v);
+  } else if constexpr (R == 3) {
     const T h = (T)0.86602540378443865;  // sqrt(3)/2
     cx<T> s = v[1] + v[2];
     cx<T> d = mul_i<DIR>(v[1] - v[2]);

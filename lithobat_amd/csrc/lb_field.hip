@@ -43,6 +43,14 @@ struct FieldRadix<32> {
   static constexpr int R1 = 8, R2 = 4;
 };
 template <>
+struct FieldRadix<48> {
+  static constexpr int R1 = 12, R2 = 4;
+};
+template <>
+struct FieldRadix<96> {
+  static constexpr int R1 = 12, R2 = 8;
+};
+template <>
 struct FieldRadix<64> {
   static constexpr int R1 = 8, R2 = 8;
 };
@@ -63,13 +71,13 @@ struct FieldShape {
   static constexpr int CPR = NT / T1;
   static constexpr int MC = M / CSPLIT;
   static constexpr int ROUNDS = MC / CPR;
-  static constexpr int Q2 = R1 / R2;
+  static constexpr int NT2 = CPR * T2;             // stage-2 tasks per round
+  static constexpr int Q2 = (NT2 + NT - 1) / NT;   // per thread (the last pass may idle)
   static constexpr int NACC = ROUNDS * Q2 * R2;
   static constexpr int LDC = M + 1;  // padded line stride of the round buffer
   static_assert(T2 == R1, "two-stage plan");
   static_assert(CPR * T1 == NT, "stage 1 maps one task per thread");
   static_assert(CPR <= MC && MC % CPR == 0, "rounds tile the split");
-  static_assert(R1 % R2 == 0, "stage-2 tasks per thread");
 };
 
 __device__ __forceinline__ int wrap_idx(int i, int n) {
@@ -113,7 +121,7 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
       const int t = tid + q * NT;
       const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
       const int a = a0 + rl;
-      if (a < W) {
+      if (t < F::NT2 && a < W) {
         cx<T> v[F::R2];
 #pragma unroll
         for (int r = 0; r < F::R2; ++r) {
@@ -181,16 +189,19 @@ __global__ __launch_bounds__(NT) void k_field_fwd(const cx<T>* __restrict__ X, i
 #pragma unroll
           for (int q = 0; q < F::Q2; ++q) {
             const int t = tid + q * NT;
-            const int c2 = t % F::CPR, j2 = t / F::CPR;
-            cx<T> v[F::R2];
+            if (F::NT2 % NT == 0 || t < F::NT2) {
+              const int c2 = t % F::CPR, j2 = t / F::CPR;
+              cx<T> v[F::R2];
 #pragma unroll
-            for (int r = 0; r < F::R2; ++r) {
-              v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
-              if (r > 0) v[r] = v[r] * tw[r * j2];
+              for (int r = 0; r < F::R2; ++r) {
+                v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
+                if (r > 0) v[r] = v[r] * tw[r * j2];
+              }
+              dft<F::R2, +1>(v);
+#pragma unroll
+              for (int r = 0; r < F::R2; ++r)
+                acc[(rd * F::Q2 + q) * F::R2 + r] += w * norm2(v[r]);
             }
-            dft<F::R2, +1>(v);
-#pragma unroll
-            for (int r = 0; r < F::R2; ++r) acc[(rd * F::Q2 + q) * F::R2 + r] += w * norm2(v[r]);
           }
           __syncthreads();
         }
@@ -201,11 +212,13 @@ __global__ __launch_bounds__(NT) void k_field_fwd(const cx<T>* __restrict__ X, i
 #pragma unroll
         for (int q = 0; q < F::Q2; ++q) {
           const int t = tid + q * NT;
-          const int c2 = t % F::CPR, j2 = t / F::CPR;
-          const int x = cs * F::MC + rd * F::CPR + c2;
+          if (F::NT2 % NT == 0 || t < F::NT2) {
+            const int c2 = t % F::CPR, j2 = t / F::CPR;
+            const int x = cs * F::MC + rd * F::CPR + c2;
 #pragma unroll
-          for (int r = 0; r < F::R2; ++r)
-            out[(j2 + r * F::R1) * M + x] = acc[(rd * F::Q2 + q) * F::R2 + r];
+            for (int r = 0; r < F::R2; ++r)
+              out[(j2 + r * F::R1) * M + x] = acc[(rd * F::Q2 + q) * F::R2 + r];
+          }
         }
     }
   }
@@ -247,11 +260,12 @@ __global__ __launch_bounds__(NT) void k_field_adj(const cx<T>* __restrict__ X, i
 #pragma unroll
         for (int q = 0; q < F::Q2; ++q) {
           const int t = tid + q * NT;
+          const bool ok = F::NT2 % NT == 0 || t < F::NT2;
           const int c2 = t % F::CPR, j2 = t / F::CPR;
           const int x = cs * F::MC + rd * F::CPR + c2;
 #pragma unroll
           for (int r = 0; r < F::R2; ++r)
-            gm[(rd * F::Q2 + q) * F::R2 + r] = g[(j2 + r * F::R1) * M + x];
+            gm[(rd * F::Q2 + q) * F::R2 + r] = ok ? g[(j2 + r * F::R1) * M + x] : (T)0;
         }
       for (int j = s0; j < s1; ++j) {
         const int2 bx = box[j];
@@ -276,6 +290,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(const cx<T>* __restrict__ X, i
 #pragma unroll
           for (int q = 0; q < F::Q2; ++q) {
             const int t = tid + q * NT;
+            if (!(F::NT2 % NT == 0 || t < F::NT2)) continue;
             const int c2 = t % F::CPR, j2 = t / F::CPR;
             cx<T> v[F::R2];
 #pragma unroll
@@ -323,6 +338,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(const cx<T>* __restrict__ X, i
 #pragma unroll
           for (int q = 0; q < F::Q2; ++q) {
             const int t = tid + q * NT;
+            if (!(F::NT2 % NT == 0 || t < F::NT2)) continue;
             const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
             cx<T> v[F::R2];
 #pragma unroll
@@ -363,8 +379,18 @@ __global__ __launch_bounds__(NT) void k_field_adj(const cx<T>* __restrict__ X, i
 
 // ---------------------------------------------------------------- dispatch
 #define LB_FIELD_CONFIGS(X)                                                                  \
-  X(16, 64, 1) X(32, 128, 1) X(64, 256, 1) X(64, 128, 1) X(64, 256, 2) X(128, 256, 1)        \
-      X(128, 512, 1) X(128, 256, 2) X(128, 256, 4) X(256, 256, 2) X(256, 256, 4) X(256, 256, 8)
+  X(16, 64, 1) X(32, 128, 1) X(48, 64, 1) X(64, 256, 1) X(64, 128, 1) X(64, 256, 2)          \
+      X(96, 256, 1) X(96, 128, 1) X(96, 256, 3) X(128, 256, 1) X(128, 512, 1) X(128, 256, 2) \
+      X(128, 256, 4) X(256, 256, 2) X(256, 256, 4) X(256, 256, 8)
+
+const std::vector<FieldConfig>& field_configs() {
+  static const std::vector<FieldConfig> all = {
+#define LB_CASE(M_, NT_, CS_) FieldConfig{M_, NT_, CS_, 0},
+      LB_FIELD_CONFIGS(LB_CASE)
+#undef LB_CASE
+  };
+  return all;
+}
 
 bool field_config_supported(int precision, const FieldConfig& c) {
   (void)precision;

@@ -52,6 +52,9 @@ struct FieldConfig {
   int grid;    // blocks (persistent)
 };
 
+#include <vector>
+// every compiled coherent-system kernel shape, default-preference order
+const std::vector<FieldConfig>& field_configs();
 bool field_config_supported(int precision, const FieldConfig& c);
 size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint);
 
@@ -65,10 +68,13 @@ hipError_t field_adjoint(int precision, const FieldConfig& c, const FieldArgs& a
 // ---- small helpers (lb_util.hip) ------------------------------------------
 template <typename T>
 hipError_t fill(T* p, T v, long long n, hipStream_t st);
-// out[g][e] = sum_{k < cnt} in[(g*cnt + k)][e], e < len (deterministic order)
+// out[g][e] = sum_{k < cnt} in[(g*cnt + k)][e], e < len (deterministic order).
+// scratch (nullable) holds groups * kSegmentSplits * len values for the split path.
+constexpr int kSegmentSplits = 16;
+int segment_splits(int groups, int cnt, long long len);  // scratch slices the split path uses
 template <typename T>
 hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, T* peak,
-                       hipStream_t st);
+                       T* scratch, hipStream_t st);
 
 template <typename T>
 hipError_t scatter_band(const cx<T>* in, int K, int Wb, int nu0, int N, cx<T>* out,

@@ -34,6 +34,20 @@ __device__ __forceinline__ void amax(double* a, double v) {
             (unsigned long long)__double_as_longlong(v > 0.0 ? v : 0.0));
 }
 
+// scratch[g][k][e] = sum of slices [k*per, min((k+1)*per, cnt)) of group g
+template <typename T>
+__global__ void k_segment_split(const T* __restrict__ in, int cnt, int per, int ns, long long len,
+                                T* __restrict__ scratch) {
+  const int g = blockIdx.y / ns, k = blockIdx.y % ns;
+  const int c0 = k * per, c1 = min(cnt, c0 + per);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+       e += (long long)gridDim.x * blockDim.x) {
+    T s = 0;
+    for (int c = c0; c < c1; ++c) s += in[((long long)g * cnt + c) * len + e];
+    scratch[((long long)g * ns + k) * len + e] = s;
+  }
+}
+
 template <typename T>
 __global__ void k_segment_sum(const T* __restrict__ in, int cnt, long long len, T* __restrict__ out,
                               T* __restrict__ peak) {
@@ -59,23 +73,47 @@ __global__ void k_segment_sum(const T* __restrict__ in, int cnt, long long len, 
   }
 }
 
+int segment_splits(int groups, int cnt, long long len) {
+  long long bx = (len + 255) / 256;
+  if (bx > 4096) bx = 4096;
+  int ns = 1;
+  if (cnt >= 8)
+    while (ns < kSegmentSplits && bx * groups * ns < 2048 && ns * 2 <= cnt) ns *= 2;
+  return ns;
+}
+
+// Sums `cnt` slices per group.  When the slices are many and the slices
+// short (few blocks), the slice axis is split first: each of ns blocks sums
+// a contiguous run of slices into scratch, then a second pass sums the ns
+// partials.  Order of summation is fixed, so results are deterministic.
 template <typename T>
 hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, T* peak,
-                       hipStream_t st) {
+                       T* scratch, hipStream_t st) {
   if (groups <= 0 || len <= 0) return hipSuccess;
   long long bx = (len + 255) / 256;
   if (bx > 4096) bx = 4096;
-  hipLaunchKernelGGL(k_segment_sum<T>, dim3((unsigned)bx, groups), dim3(256), 0, st, in, cnt, len,
-                     out, peak);
+  const int ns = scratch ? segment_splits(groups, cnt, len) : 1;
+  if (ns == 1) {
+    hipLaunchKernelGGL(k_segment_sum<T>, dim3((unsigned)bx, groups), dim3(256), 0, st, in, cnt,
+                       len, out, peak);
+    return hipGetLastError();
+  }
+  const int per = (cnt + ns - 1) / ns;
+  hipLaunchKernelGGL(k_segment_split<T>, dim3((unsigned)bx, groups * ns), dim3(256), 0, st, in,
+                     cnt, per, ns, len, scratch);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_segment_sum<T>, dim3((unsigned)bx, groups), dim3(256), 0, st,
+                     (const T*)scratch, ns, len, out, peak);
   return hipGetLastError();
 }
 
 template hipError_t fill<float>(float*, float, long long, hipStream_t);
 template hipError_t fill<double>(double*, double, long long, hipStream_t);
-template hipError_t segment_sum<float>(const float*, int, int, long long, float*, float*,
+template hipError_t segment_sum<float>(const float*, int, int, long long, float*, float*, float*,
                                        hipStream_t);
 template hipError_t segment_sum<double>(const double*, int, int, long long, double*, double*,
-                                        hipStream_t);
+                                        double*, hipStream_t);
 
 }  // namespace lb
 

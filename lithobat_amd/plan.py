@@ -110,6 +110,7 @@ class ImagingPlan:
         self._lib = lib
         self._ws = None
         self._lock = threading.Lock()
+        self._tuned_for = set()
         info = _lib.PlanInfo()
         _lib.check(lib.lb_plan_info_get(self._h, C.byref(info)))
         self.info = {k: getattr(info, k) for k, _ in _lib.PlanInfo._fields_}
@@ -152,6 +153,13 @@ class ImagingPlan:
             self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
         return self._ws
 
+    def _use_tuning(self, n_masks: int) -> None:
+        """Select the autotuned kernel shapes recorded for this batch size, once."""
+        if n_masks not in self._tuned_for:
+            from . import autotune
+            autotune.apply(self, n_masks)
+            self._tuned_for.add(n_masks)
+
     def workspace_bytes(self, n_masks: int, backward: bool = False) -> int:
         return int(self._lib.lb_workspace_bytes(self._h, int(n_masks), 1 if backward else 0))
 
@@ -176,6 +184,7 @@ class ImagingPlan:
         out = torch.empty((B, P, self.n, self.n), dtype=self.real_dtype, device=m.device)
         peak = torch.empty((B, P), dtype=self.real_dtype, device=m.device) if with_peak else None
         with self._lock:
+            self._use_tuning(B)
             nb = self.workspace_bytes(B, False)
             ws = self._workspace(nb, m.device)
             st = torch.cuda.current_stream(m.device).cuda_stream
@@ -194,6 +203,7 @@ class ImagingPlan:
         g = d_images.to(self.real_dtype).reshape(B, P, self.n, self.n).contiguous()
         out = torch.empty((B, self.n, self.n), dtype=self.real_dtype, device=m.device)
         with self._lock:
+            self._use_tuning(B)
             nb = self.workspace_bytes(B, True)
             ws = self._workspace(nb, m.device)
             st = torch.cuda.current_stream(m.device).cuda_stream
