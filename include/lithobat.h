@@ -53,6 +53,7 @@ typedef enum {
 } lb_polarisation;
 
 typedef struct lb_plan_s* lb_plan;
+typedef struct lb_batch_s* lb_batch;
 
 /* Optics, grid and source of one imaging set-up (OpticsConfig + GridConfig +
  * SourcePoints of the reference, core/config.py:383-406, 741-744;
@@ -147,6 +148,32 @@ lb_status lb_backward(lb_plan plan, const void* masks, int32_t n_masks, const vo
 lb_status lb_fields(lb_plan plan, const void* masks, int32_t mask_complex, int32_t n_masks,
                     void* fields, void* workspace, size_t workspace_bytes, void* stream);
 size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks);
+
+/* The reference's own dataflow on the device, for comparison: the raw image
+ * of one real mask on one plane as one dense N x N inverse transform per
+ * coherent system, `batch` systems per transform launch, |E|^2 accumulated
+ * with the source weights (expose/aerial_image.py:421-488 with _Batch
+ * :312-348).  Same result as lb_forward; no coarse grid. */
+size_t lb_dense_workspace_bytes(lb_plan plan, int32_t batch);
+lb_status lb_forward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,
+                           void* image, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Batches of clips (no direct counterpart in the reference, which images one
+ * mask per call of compute_aerial_image / runs one run_ilt per clip): plans of
+ * different grid sizes and optics -- one mask each -- imaged together.  The
+ * band transforms run per clip; all clips' coherent systems run in one
+ * persistent launch.  Plans must share precision and coarse grid side (create
+ * them with the same coarse_m) and must outlive the batch.  Arrays are host
+ * arrays of n_plans device pointers; peaks may be NULL. */
+lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out);
+lb_status lb_batch_destroy(lb_batch batch);
+size_t lb_batch_workspace_bytes(lb_batch batch, int32_t backward);
+lb_status lb_batch_forward(lb_batch batch, const void* const* masks, void* const* images,
+                           void* const* peaks, void* workspace, size_t workspace_bytes,
+                           void* stream);
+lb_status lb_batch_backward(lb_batch batch, const void* const* masks,
+                            const void* const* d_images, void* const* d_masks, void* workspace,
+                            size_t workspace_bytes, void* stream);
 
 /* Kernel-shape control for the autotuner (no counterpart in the reference):
  * list the coherent-system kernel shapes (threads, column splits) that fit

@@ -44,7 +44,8 @@ EXPORTS = [
     "lb_plan_destroy", "lb_plan_info_get", "lb_plan_clear", "lb_socs_weights",
     "lb_socs_kernels", "lb_workspace_bytes", "lb_forward", "lb_backward", "lb_fields",
     "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read", "lb_plan_shapes",
-    "lb_plan_set_shape",
+    "lb_plan_set_shape", "lb_batch_create", "lb_batch_destroy", "lb_batch_workspace_bytes",
+    "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
 ]
 
 _lib = None
@@ -82,6 +83,15 @@ def load() -> C.CDLL:
         "lb_profile_read": ([vp, C.POINTER(i32), C.POINTER(C.c_double)], C.c_int),
         "lb_plan_shapes": ([vp, i32, C.POINTER(i32), i32, C.POINTER(i32)], C.c_int),
         "lb_plan_set_shape": ([vp, i32, i32, i32, i32], C.c_int),
+        "lb_dense_workspace_bytes": ([vp, i32], sz),
+        "lb_forward_dense": ([vp, vp, i32, i32, vp, vp, sz, vp], C.c_int),
+        "lb_batch_create": ([C.POINTER(vp), i32, plan_p], C.c_int),
+        "lb_batch_destroy": ([vp], C.c_int),
+        "lb_batch_workspace_bytes": ([vp, i32], sz),
+        "lb_batch_forward": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, sz, vp],
+                             C.c_int),
+        "lb_batch_backward": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, sz, vp],
+                              C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -214,6 +214,11 @@ lb_status choose_grid(Plan& p, int Wbox, int forced_m) {
   if (!pick_field_configs(p))
     return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel shape fits LDS for M=" +
                                         std::to_string(M) + " W=" + std::to_string(p.W));
+  // roots tables the launches will use (allocated here, never while capturing)
+  for (int L : {p.N, p.M}) {
+    hipError_t e = p.precision == 64 ? ensure_roots<double>(L) : ensure_roots<float>(L);
+    if (e != hipSuccess) return fail(LB_ERR_HIP, std::string("roots table: ") + hipGetErrorString(e));
+  }
   return LB_OK;
 }
 
@@ -267,11 +272,11 @@ struct Work {
   size_t total;
 };
 
-Work layout(const Plan& p, int n_masks, bool backward) {
+Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
   Work w{};
   w.n_masks = n_masks;
   w.n_img = n_masks * p.n_planes;
-  int chunk = backward ? p.chunk_adj : p.chunk_fwd;
+  int chunk = chunk_forced > 0 ? chunk_forced : (backward ? p.chunk_adj : p.chunk_fwd);
   if (chunk <= 0) {
     const long long total = (long long)w.n_img * p.spp;
     const long long target = 4LL * p.cus;
@@ -352,24 +357,21 @@ hipError_t profile_events(Plan& p, int* e0, int* e1) {
   return hipSuccess;
 }
 
+// Forward, phase 1: band spectra of the masks.
 template <typename T>
-lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* images, void* peak,
-                    char* ws, const Work& w, hipStream_t st) {
+lb_status fwd_pre(Plan& p, const void* masks, int cplx, int n_masks, char* ws, const Work& w,
+                  hipStream_t st) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
-  cx<T>* Xm = (cx<T>*)(ws + w.Xm);
   LB_HIP(band_analysis<T>(p.N, masks, cplx, (long long)p.N * p.N * (cplx ? 2 : 1), 0, 1, n_masks,
-                          p.nu0m, p.Wbm, invN2, (cx<T>*)(ws + w.RTm), Xm, st));
-  FieldConfig fc = p.fwd;
-  fc.grid = grid_for(p, fc, w.n_items, false);
-  FieldArgs a = field_args(p, Xm, w);
+                          p.nu0m, p.Wbm, invN2, (cx<T>*)(ws + w.RTm), (cx<T>*)(ws + w.Xm), st));
+  return LB_OK;
+}
+
+// Forward, phase 3: reduce the coarse partials, lift them to the N x N
+// images (or sum them directly when the coarse grid is the image grid).
+template <typename T>
+lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, hipStream_t st) {
   T* partial = (T*)(ws + w.partial);
-  int e0 = -1, e1 = -1;
-  if (p.profile) {
-    LB_HIP(profile_events(p, &e0, &e1));
-    LB_HIP(hipEventRecord(p.ev_pool[e0], st));
-  }
-  LB_HIP(field_forward(p.precision, fc, a, partial, st));
-  if (p.profile) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
   T* pk = peak ? (T*)peak : (T*)(ws + w.peak);
   LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
   if (p.direct) {
@@ -390,13 +392,32 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
 }
 
 template <typename T>
-lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, void* dmask,
-                     char* ws, const Work& w, hipStream_t st) {
+lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* images, void* peak,
+                    char* ws, const Work& w, hipStream_t st) {
+  lb_status rc = fwd_pre<T>(p, masks, cplx, n_masks, ws, w, st);
+  if (rc != LB_OK) return rc;
+  FieldConfig fc = p.fwd;
+  fc.grid = grid_for(p, fc, w.n_items, false);
+  FieldArgs a = field_args(p, ws + w.Xm, w);
+  int e0 = -1, e1 = -1;
+  if (p.profile) {
+    LB_HIP(profile_events(p, &e0, &e1));
+    LB_HIP(hipEventRecord(p.ev_pool[e0], st));
+  }
+  LB_HIP(field_forward(p.precision, fc, a, ws + w.partial, st));
+  if (p.profile) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
+  return fwd_post<T>(p, images, peak, ws, w, st);
+}
+
+// Backward, phase 1: band spectra of the masks and the coarse sensitivity
+// gM (the exact adjoint of the lift from M x M to N x N).
+template <typename T>
+lb_status bwd_pre(Plan& p, const void* masks, int n_masks, const void* dimg, char* ws,
+                  const Work& w, const T** gM, hipStream_t st) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
-  cx<T>* Xm = (cx<T>*)(ws + w.Xm);
   LB_HIP(band_analysis<T>(p.N, masks, 0, (long long)p.N * p.N, 0, 1, n_masks, p.nu0m, p.Wbm,
-                          invN2, (cx<T>*)(ws + w.RTm), Xm, st));
-  const T* gM = (const T*)dimg;
+                          invN2, (cx<T>*)(ws + w.RTm), (cx<T>*)(ws + w.Xm), st));
+  *gM = (const T*)dimg;
   if (!p.direct) {
     const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
     cx<T>* GY = (cx<T>*)(ws + w.GY);
@@ -404,19 +425,35 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
                             (T)1, (cx<T>*)(ws + w.RTg), GY, st));
     LB_HIP(band_synthesis<T>(p.M, GY, w.n_img, -p.QI, p.WbI, 1, invM2, 0, (cx<T>*)(ws + w.TTc),
                              (T*)(ws + w.gM), (T*)nullptr, st));
-    gM = (const T*)(ws + w.gM);
+    *gM = (const T*)(ws + w.gM);
   }
-  FieldConfig fc = p.adj;
-  fc.grid = grid_for(p, fc, w.n_items, true);
-  FieldArgs a = field_args(p, Xm, w);
-  cx<T>* gX = (cx<T>*)(ws + w.gX);
-  LB_HIP(field_adjoint(p.precision, fc, a, gM, gX, st));
+  return LB_OK;
+}
+
+// Backward, phase 3: sum the band-gradient partials per mask and synthesise
+// the real mask gradient.
+template <typename T>
+lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, hipStream_t st) {
+  const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
   cx<T>* GX = (cx<T>*)(ws + w.GX);
-  LB_HIP(segment_sum<T>((const T*)gX, n_masks, p.n_planes * w.n_chunks,
+  LB_HIP(segment_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes * w.n_chunks,
                         2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, (T*)(ws + w.seg), st));
   LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, 1, invN2, 0, (cx<T>*)(ws + w.TTm),
                            (T*)dmask, (T*)nullptr, st));
   return LB_OK;
+}
+
+template <typename T>
+lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, void* dmask,
+                     char* ws, const Work& w, hipStream_t st) {
+  const T* gM = nullptr;
+  lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st);
+  if (rc != LB_OK) return rc;
+  FieldConfig fc = p.adj;
+  fc.grid = grid_for(p, fc, w.n_items, true);
+  FieldArgs a = field_args(p, ws + w.Xm, w);
+  LB_HIP(field_adjoint(p.precision, fc, a, gM, ws + w.gX, st));
+  return bwd_post<T>(p, n_masks, dmask, ws, w, st);
 }
 
 // mask band covering every box: symmetric [-Q, Q], or the whole grid
@@ -436,6 +473,25 @@ void set_mask_band(Plan& p, int Q) {
 
 struct lb_plan_s {
   Plan p;
+};
+
+// Several plans (clips of different sizes / optics) imaged together: their
+// band passes run per clip, their coherent systems in ONE persistent launch
+// of the coherent-system kernel over a descriptor table of work items.
+struct lb_batch_s {
+  std::vector<lb_plan> plans;  // not owned
+  int precision = 32, M = 0, Wmax = 0, cus = 256, chunk = 1;
+  FieldConfig fwd{}, adj{};
+  std::vector<Work> wf, wb;
+  std::vector<size_t> base_f, base_b;
+  size_t ws_f = 0, ws_b = 0;
+  ItemDesc* items_f = nullptr;
+  ItemDesc* items_b = nullptr;
+  int n_items_f = 0, n_items_b = 0;
+  ~lb_batch_s() {
+    if (items_f) (void)hipFree(items_f);
+    if (items_b) (void)hipFree(items_b);
+  }
 };
 
 extern "C" {
@@ -882,6 +938,258 @@ lb_status lb_socs_kernels(lb_plan plan, void* kernels, void* stream) {
   LB_HIP(scatter_band<double>((const cx<double>*)p.phi128, p.K, p.Wbm, p.nu0m, p.N,
                               (cx<double>*)kernels, st));
   return LB_OK;
+}
+
+lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) {
+  g_err.clear();
+  if (!plans || n_plans < 1 || !out) return fail(LB_ERR_ARG, "need >= 1 plan and an output");
+  *out = nullptr;
+  lb_batch b = new lb_batch_s();
+  b->plans.assign(plans, plans + n_plans);
+  const Plan& p0 = plans[0]->p;
+  b->precision = p0.precision;
+  b->M = p0.M;
+  b->cus = p0.cus;
+  long long total_sys = 0;
+  for (lb_plan h : b->plans) {
+    const Plan& p = h->p;
+    if (!h) {
+      delete b;
+      return fail(LB_ERR_ARG, "null plan in batch");
+    }
+    if (p.precision != b->precision || p.M != b->M) {
+      delete b;
+      return fail(LB_ERR_ARG, "batched plans must share precision and coarse grid side "
+                              "(create them with the same coarse_m)");
+    }
+    if (p.direct) {
+      delete b;
+      return fail(LB_ERR_UNSUPPORTED, "batched plans need a coarse grid smaller than the clip");
+    }
+    if (p.spp < 1) {
+      delete b;
+      return fail(LB_ERR_ARG, "plan has no coherent systems");
+    }
+    b->Wmax = std::max(b->Wmax, p.W);
+    total_sys += (long long)p.spp * p.n_planes;
+  }
+  bool got_f = false, got_a = false;
+  for (const FieldConfig& c : field_configs()) {
+    if (c.M != b->M) continue;
+    if (!got_f && field_smem_bytes(b->precision, c, b->Wmax, false) <= kLdsLimit) {
+      b->fwd = c;
+      got_f = true;
+    }
+    if (!got_a && field_smem_bytes(b->precision, c, b->Wmax, true) <= kLdsLimit) {
+      b->adj = c;
+      got_a = true;
+    }
+  }
+  if (!got_f || !got_a) {
+    delete b;
+    return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel shape fits the widest clip");
+  }
+  b->chunk = (int)std::max<long long>(1, (total_sys + 4LL * b->cus - 1) / (4LL * b->cus));
+  std::vector<ItemDesc> itf, itb;
+  size_t off_f = 0, off_b = 0;
+  for (lb_plan h : b->plans) {
+    const Plan& p = h->p;
+    Work wf = layout(p, 1, false, b->chunk), wb = layout(p, 1, true, b->chunk);
+    b->wf.push_back(wf);
+    b->wb.push_back(wb);
+    b->base_f.push_back(off_f);
+    b->base_b.push_back(off_b);
+    const size_t R = p.rsz(), C = p.csz();
+    for (int img = 0; img < wf.n_img; ++img)
+      for (int k = 0; k < wf.n_chunks; ++k) {
+        const int plane = img % p.n_planes;
+        ItemDesc d{};
+        d.Wb = p.Wbm;
+        d.wrapN = p.wrapm;
+        d.H = p.H;
+        d.W = p.W;
+        d.box = p.box;
+        d.wsys = p.wsys;
+        d.s0 = plane * p.spp + k * wf.chunk;
+        d.s1 = std::min(d.s0 + wf.chunk, (plane + 1) * p.spp);
+        const long long item = (long long)img * wf.n_chunks + k;
+        d.X_off = (long long)(off_f + wf.Xm);
+        d.part_off = (long long)(off_f + wf.partial + item * p.M * p.M * R);
+        d.gM_off = -1;
+        d.gX_off = -1;
+        itf.push_back(d);
+        d.X_off = (long long)(off_b + wb.Xm);
+        d.part_off = -1;
+        d.gM_off = (long long)(off_b + wb.gM + (size_t)img * p.M * p.M * R);
+        d.gX_off = (long long)(off_b + wb.gX + item * p.Wbm * p.Wbm * C);
+        itb.push_back(d);
+      }
+    off_f += (wf.total + 255) / 256 * 256;
+    off_b += (wb.total + 255) / 256 * 256;
+  }
+  b->ws_f = off_f;
+  b->ws_b = off_b;
+  b->n_items_f = (int)itf.size();
+  b->n_items_b = (int)itb.size();
+  hipError_t e = hipMalloc(&b->items_f, sizeof(ItemDesc) * itf.size());
+  if (e == hipSuccess) e = hipMalloc(&b->items_b, sizeof(ItemDesc) * itb.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(b->items_f, itf.data(), sizeof(ItemDesc) * itf.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->items_b, itb.data(), sizeof(ItemDesc) * itb.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(e == hipErrorOutOfMemory ? LB_ERR_OOM : LB_ERR_HIP,
+                std::string("batch tables: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return LB_OK;
+}
+
+lb_status lb_batch_destroy(lb_batch batch) {
+  delete batch;
+  return LB_OK;
+}
+
+size_t lb_batch_workspace_bytes(lb_batch batch, int32_t backward) {
+  if (!batch) return 0;
+  return backward ? batch->ws_b : batch->ws_f;
+}
+
+static int batch_grid(const lb_batch_s& b, const FieldConfig& fc, int n_items, bool adjoint) {
+  const size_t sm = field_smem_bytes(b.precision, fc, b.Wmax, adjoint);
+  int occ = (int)std::max<size_t>(1, kLdsLimit / std::max<size_t>(sm, 1));
+  occ = std::min(occ, std::max(1, 2048 / fc.nt));
+  return std::max(1, std::min(n_items, b.cus * occ));
+}
+
+}  // extern "C"
+
+template <typename T>
+static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* const* images,
+                                 void* const* peaks, char* ws, hipStream_t st) {
+  const int n = (int)b.plans.size();
+  for (int c = 0; c < n; ++c) {
+    lb_status rc = fwd_pre<T>(b.plans[c]->p, masks[c], 0, 1, ws + b.base_f[c], b.wf[c], st);
+    if (rc != LB_OK) return rc;
+  }
+  FieldConfig fc = b.fwd;
+  fc.grid = batch_grid(b, fc, b.n_items_f, false);
+  LB_HIP(field_multi(b.precision, fc, b.items_f, b.n_items_f, b.Wmax, ws, false, st));
+  for (int c = 0; c < n; ++c) {
+    lb_status rc = fwd_post<T>(b.plans[c]->p, images[c], peaks ? peaks[c] : nullptr,
+                               ws + b.base_f[c], b.wf[c], st);
+    if (rc != LB_OK) return rc;
+  }
+  return LB_OK;
+}
+
+template <typename T>
+static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
+                                  const void* const* dimg, void* const* dmask, char* ws,
+                                  hipStream_t st) {
+  const int n = (int)b.plans.size();
+  for (int c = 0; c < n; ++c) {
+    const T* gM = nullptr;
+    lb_status rc =
+        bwd_pre<T>(b.plans[c]->p, masks[c], 1, dimg[c], ws + b.base_b[c], b.wb[c], &gM, st);
+    if (rc != LB_OK) return rc;
+  }
+  FieldConfig fc = b.adj;
+  fc.grid = batch_grid(b, fc, b.n_items_b, true);
+  LB_HIP(field_multi(b.precision, fc, b.items_b, b.n_items_b, b.Wmax, ws, true, st));
+  for (int c = 0; c < n; ++c) {
+    lb_status rc = bwd_post<T>(b.plans[c]->p, 1, dmask[c], ws + b.base_b[c], b.wb[c], st);
+    if (rc != LB_OK) return rc;
+  }
+  return LB_OK;
+}
+
+extern "C" {
+
+lb_status lb_batch_forward(lb_batch batch, const void* const* masks, void* const* images,
+                           void* const* peaks, void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!batch || !masks || !images) return fail(LB_ERR_ARG, "null argument");
+  for (size_t c = 0; c < batch->plans.size(); ++c)
+    if (!masks[c] || !images[c]) return fail(LB_ERR_ARG, "null mask or image pointer");
+  if (!workspace || ws_bytes < batch->ws_f)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(batch->ws_f) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return batch->precision == 64
+             ? batch_forward_t<double>(*batch, masks, images, peaks, (char*)workspace, st)
+             : batch_forward_t<float>(*batch, masks, images, peaks, (char*)workspace, st);
+}
+
+lb_status lb_batch_backward(lb_batch batch, const void* const* masks,
+                            const void* const* d_images, void* const* d_masks, void* workspace,
+                            size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!batch || !masks || !d_images || !d_masks) return fail(LB_ERR_ARG, "null argument");
+  for (size_t c = 0; c < batch->plans.size(); ++c)
+    if (!masks[c] || !d_images[c] || !d_masks[c]) return fail(LB_ERR_ARG, "null pointer");
+  if (!workspace || ws_bytes < batch->ws_b)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(batch->ws_b) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return batch->precision == 64
+             ? batch_backward_t<double>(*batch, masks, d_images, d_masks, (char*)workspace, st)
+             : batch_backward_t<float>(*batch, masks, d_images, d_masks, (char*)workspace, st);
+}
+
+size_t lb_dense_workspace_bytes(lb_plan plan, int32_t batch) {
+  if (!plan || batch < 1) return 0;
+  const Plan& p = plan->p;
+  Layout L;
+  L.take(p.csz() * (size_t)p.N * p.N);      // whole-grid mask spectrum
+  L.take(p.csz() * (size_t)p.N * p.N);      // analysis scratch
+  L.take(p.csz() * (size_t)batch * p.N * p.N);  // dense spectra
+  L.take(p.csz() * (size_t)batch * p.N * p.N);  // synthesis scratch
+  L.take(p.csz() * (size_t)batch * p.N * p.N);  // fields
+  return L.off;
+}
+
+}  // extern "C"
+
+template <typename T>
+static lb_status dense_t(Plan& p, const void* mask, int plane, int batch, void* image, char* ws,
+                         hipStream_t st) {
+  const int N = p.N;
+  Layout L;
+  cx<T>* Xf = (cx<T>*)(ws + L.take(p.csz() * (size_t)N * N));
+  cx<T>* RT = (cx<T>*)(ws + L.take(p.csz() * (size_t)N * N));
+  cx<T>* spec = (cx<T>*)(ws + L.take(p.csz() * (size_t)batch * N * N));
+  cx<T>* TT = (cx<T>*)(ws + L.take(p.csz() * (size_t)batch * N * N));
+  cx<T>* E = (cx<T>*)(ws + L.take(p.csz() * (size_t)batch * N * N));
+  const T invN2 = (T)(1.0 / ((double)N * (double)N));
+  // the mask spectrum on the whole grid (provider.py:177-178), scaled by 1/N^2
+  LB_HIP(band_analysis<T>(N, mask, 0, (long long)N * N, 0, 1, 1, -(N / 2), N, invN2, RT, Xf, st));
+  LB_HIP(fill<T>((T*)image, (T)0, (long long)N * N, st));
+  const int s_begin = plane * p.spp, s_end = (plane + 1) * p.spp;
+  for (int j0 = s_begin; j0 < s_end; j0 += batch) {
+    const int ns = std::min(batch, s_end - j0);
+    LB_HIP(dense_spectra<T>(Xf, N, (const cx<T>*)p.H, p.W, p.box, j0, ns, p.nu0m, spec, st));
+    LB_HIP(band_synthesis<T>(N, spec, ns, -(N / 2), N, 0, (T)1, 1, TT, (T*)E, (T*)nullptr, st));
+    LB_HIP(accum_norm2<T>(E, ns, (long long)N * N, (const T*)p.wsys, j0, (T*)image, st));
+  }
+  return LB_OK;
+}
+
+extern "C" {
+
+lb_status lb_forward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,
+                           void* image, void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !mask || !image) return fail(LB_ERR_ARG, "null argument");
+  Plan& p = plan->p;
+  if (plane < 0 || plane >= p.n_planes) return fail(LB_ERR_ARG, "plane out of range");
+  if (batch < 1) return fail(LB_ERR_ARG, "batch must be >= 1");
+  const size_t need = lb_dense_workspace_bytes(plan, batch);
+  if (!workspace || ws_bytes < need)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? dense_t<double>(p, mask, plane, batch, image, (char*)workspace, st)
+             : dense_t<float>(p, mask, plane, batch, image, (char*)workspace, st);
 }
 
 size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks) {

@@ -276,6 +276,46 @@ __device__ cx<T>* fft_rows_lds(cx<T>* a, cx<T>* b, int ntr, int ld, const cx<T>*
   return src;
 }
 
+// The stages R_a, R_b, R_c (1 = unused) of a length-L Stockham transform whose
+// earlier stages (span Ns0 so far) are already done and held in `a`.
+// Same conventions as fft_rows_lds.
+template <typename T, int DIR, int L, int Ra, int Rb, int Rc>
+__device__ cx<T>* fft_tail_lds(cx<T>* a, cx<T>* b, int ntr, int ld, const cx<T>* tw, int tid,
+                               int nt, int Ns0) {
+  cx<T>* src = a;
+  cx<T>* dst = b;
+  int Ns = Ns0;
+  auto stage = [&](auto rc) {
+    constexpr int R = decltype(rc)::value;
+    if constexpr (R > 1) {
+      const int tasks = L / R;
+      for (int t = tid; t < ntr * tasks; t += nt) {
+        const int row = t / tasks, j = t - row * tasks;
+        stockham_stage<R, DIR, T>(src + row * ld, dst + row * ld, L, Ns, j, tw);
+      }
+      __syncthreads();
+      cx<T>* x = src;
+      src = dst;
+      dst = x;
+      Ns *= R;
+    }
+  };
+  stage(std::integral_constant<int, Ra>{});
+  stage(std::integral_constant<int, Rb>{});
+  stage(std::integral_constant<int, Rc>{});
+  return src;
+}
+
+// Copy a precomputed roots table (e^{+2 pi i t / L}, t < L) into LDS.
+template <typename T>
+__device__ __forceinline__ void load_roots(cx<T>* tw, const cx<T>* __restrict__ twg, int L,
+                                           int tid, int nt) {
+  for (int t = tid; t < L; t += nt) tw[t] = twg[t];
+}
+
+// Canonical frequency of FFT bin b (numpy fftfreq order).
+__device__ __forceinline__ int nu_of_bin(int b, int n) { return b <= (n - 1) / 2 ? b : b - n; }
+
 // Frequency index nu (any integer) -> FFT bin in [0, n).
 __device__ __forceinline__ int bin_of(int nu, int n) {
   int b = nu % n;

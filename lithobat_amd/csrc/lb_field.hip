@@ -140,28 +140,104 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
   }
 }
 
-template <typename T, int M, int NT, int CSPLIT>
-__global__ __launch_bounds__(NT) void k_field_fwd(const cx<T>* __restrict__ X, int Wb, int wrapN,
-                                                  const cx<T>* __restrict__ H, int W,
-                                                  const int2* __restrict__ box,
-                                                  const T* __restrict__ wsys, ItemMap map,
-                                                  T* __restrict__ partial) {
+// ---------------------------------------------------------------- work items
+// What one block needs for one work item: a mask's band spectrum, the
+// systems [s0, s1) of one image, and where its partial goes.
+template <typename T>
+struct ItemView {
+  const cx<T>* X;    // band spectrum of the item's mask, Wb x Wb
+  int Wb, wrapN;
+  const cx<T>* H;    // transfer tables (system j at H + j W W)
+  int W;
+  const int2* box;
+  const T* wsys;
+  int s0, s1;
+  T* part;           // forward: M x M coarse partial
+  const T* gM;       // adjoint: M x M coarse sensitivity of the item's image
+  cx<T>* gX;         // adjoint: Wb x Wb band-gradient partial
+};
+
+// Items of one plan, by arithmetic (ItemMap).
+template <typename T, int M>
+struct MapSource {
+  const cx<T>* X;
+  int Wb, wrapN;
+  const cx<T>* H;
+  int W;
+  const int2* box;
+  const T* wsys;
+  ItemMap map;
+  T* partial;
+  const T* gM;
+  cx<T>* gX;
+  __device__ int count() const { return map.n_items; }
+  __device__ ItemView<T> get(int item) const {
+    const int img = item / map.n_chunks, chunk = item - img * map.n_chunks;
+    const int plane = img % map.n_planes, mask = img / map.n_planes;
+    ItemView<T> v;
+    v.X = X + (long long)mask * Wb * Wb;
+    v.Wb = Wb;
+    v.wrapN = wrapN;
+    v.H = H;
+    v.W = W;
+    v.box = box;
+    v.wsys = wsys;
+    v.s0 = plane * map.sys_per_plane + chunk * map.chunk_sz;
+    v.s1 = min(v.s0 + map.chunk_sz, (plane + 1) * map.sys_per_plane);
+    v.part = partial ? partial + (long long)item * M * M : nullptr;
+    v.gM = gM ? gM + (long long)img * M * M : nullptr;
+    v.gX = gX ? gX + (long long)item * Wb * Wb : nullptr;
+    return v;
+  }
+};
+
+// Items of several plans (uneven clips) from a descriptor table; buffer
+// offsets are relative to the workspace base passed at launch.
+template <typename T>
+struct TableSource {
+  const ItemDesc* items;
+  int n;
+  char* ws;
+  __device__ int count() const { return n; }
+  __device__ ItemView<T> get(int i) const {
+    const ItemDesc d = items[i];
+    ItemView<T> v;
+    v.X = (const cx<T>*)(ws + d.X_off);
+    v.Wb = d.Wb;
+    v.wrapN = d.wrapN;
+    v.H = (const cx<T>*)d.H;
+    v.W = d.W;
+    v.box = d.box;
+    v.wsys = (const T*)d.wsys;
+    v.s0 = d.s0;
+    v.s1 = d.s1;
+    v.part = d.part_off >= 0 ? (T*)(ws + d.part_off) : nullptr;
+    v.gM = d.gM_off >= 0 ? (const T*)(ws + d.gM_off) : nullptr;
+    v.gX = d.gX_off >= 0 ? (cx<T>*)(ws + d.gX_off) : nullptr;
+    return v;
+  }
+};
+
+template <typename T, int M, int NT, int CSPLIT, typename Src>
+__global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restrict__ twg) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
   cx<T>* Cb = tw + M;                              // CPR x LDC (row scratch too)
   cx<T>* Tb = Cb + F::CPR * F::LDC;                // W x MC
   const int tid = threadIdx.x;
-  fill_roots(tw, M, tid, NT);
+  load_roots(tw, twg, M, tid, NT);
   __syncthreads();
   const int c1 = tid % F::CPR, jc1 = tid / F::CPR;  // column stage-1 mapping
 
-  for (int item = blockIdx.x; item < map.n_items; item += gridDim.x) {
-    const int img = item / map.n_chunks, chunk = item - img * map.n_chunks;
-    const int plane = img % map.n_planes, mask = img / map.n_planes;
-    const int s0 = plane * map.sys_per_plane + chunk * map.chunk_sz;
-    const int s1 = min(s0 + map.chunk_sz, (plane + 1) * map.sys_per_plane);
-    const cx<T>* Xm = X + (long long)mask * Wb * Wb;
+  const int n_items = src.count();
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const ItemView<T> iv = src.get(item);
+    const cx<T>* Xm = iv.X;
+    const int Wb = iv.Wb, wrapN = iv.wrapN, W = iv.W, s0 = iv.s0, s1 = iv.s1;
+    const cx<T>* H = iv.H;
+    const int2* box = iv.box;
+    const T* wsys = iv.wsys;
     for (int cs = 0; cs < CSPLIT; ++cs) {
       T acc[F::NACC];
 #pragma unroll
@@ -206,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(const cx<T>* __restrict__ X, i
           __syncthreads();
         }
       }
-      T* out = partial + (long long)item * M * M;
+      T* out = iv.part;
 #pragma unroll
       for (int rd = 0; rd < F::ROUNDS; ++rd)
 #pragma unroll
@@ -224,33 +300,31 @@ __global__ __launch_bounds__(NT) void k_field_fwd(const cx<T>* __restrict__ X, i
   }
 }
 
-template <typename T, int M, int NT, int CSPLIT>
-__global__ __launch_bounds__(NT) void k_field_adj(const cx<T>* __restrict__ X, int Wb, int wrapN,
-                                                  const cx<T>* __restrict__ H, int W,
-                                                  const int2* __restrict__ box,
-                                                  const T* __restrict__ wsys, ItemMap map,
-                                                  const T* __restrict__ gM,
-                                                  cx<T>* __restrict__ gX) {
+template <typename T, int M, int NT, int CSPLIT, typename Src>
+__global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
+                                                  const cx<T>* __restrict__ twg) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
   cx<T>* Cb = tw + M;                              // CPR x LDC
   cx<T>* Tb = Cb + F::CPR * F::LDC;                // W x MC
-  cx<T>* Tp = Tb + W * F::MC;                      // W x MC
+  cx<T>* Tp = Tb + Wmax * F::MC;                   // W x MC
   const int tid = threadIdx.x;
-  fill_roots(tw, M, tid, NT);
+  load_roots(tw, twg, M, tid, NT);
   __syncthreads();
   const int c1 = tid % F::CPR, jc1 = tid / F::CPR;
   const int rl1 = tid / F::T1, j1r = tid - (tid / F::T1) * F::T1;
 
-  for (int item = blockIdx.x; item < map.n_items; item += gridDim.x) {
-    const int img = item / map.n_chunks, chunk = item - img * map.n_chunks;
-    const int plane = img % map.n_planes, mask = img / map.n_planes;
-    const int s0 = plane * map.sys_per_plane + chunk * map.chunk_sz;
-    const int s1 = min(s0 + map.chunk_sz, (plane + 1) * map.sys_per_plane);
-    const cx<T>* Xm = X + (long long)mask * Wb * Wb;
-    const T* g = gM + (long long)img * M * M;
-    cx<T>* gout = gX + (long long)item * Wb * Wb;
+  const int n_items = src.count();
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const ItemView<T> iv = src.get(item);
+    const cx<T>* Xm = iv.X;
+    const int Wb = iv.Wb, wrapN = iv.wrapN, W = iv.W, s0 = iv.s0, s1 = iv.s1;
+    const cx<T>* H = iv.H;
+    const int2* box = iv.box;
+    const T* wsys = iv.wsys;
+    const T* g = iv.gM;
+    cx<T>* gout = iv.gX;
     for (int e = tid; e < Wb * Wb; e += NT) gout[e] = cx<T>{0, 0};
     __syncthreads();
     for (int cs = 0; cs < CSPLIT; ++cs) {
@@ -418,31 +492,89 @@ size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint
   return 0;
 }
 
+template <typename T, int M>
+static MapSource<T, M> map_source(const FieldArgs& a, void* partial, const void* gM, void* gX) {
+  MapSource<T, M> m;
+  m.X = (const cx<T>*)a.X;
+  m.Wb = a.Wb;
+  m.wrapN = a.wrapN;
+  m.H = (const cx<T>*)a.H;
+  m.W = a.W;
+  m.box = a.box;
+  m.wsys = (const T*)a.wsys;
+  m.map = a.map;
+  m.partial = (T*)partial;
+  m.gM = (const T*)gM;
+  m.gX = (cx<T>*)gX;
+  return m;
+}
+
 template <typename T, int M, int NT, int CS>
 static hipError_t launch_fwd(const FieldConfig& c, const FieldArgs& a, void* partial,
                              hipStream_t st) {
-  auto k = k_field_fwd<T, M, NT, CS>;
+  auto k = k_field_fwd<T, M, NT, CS, MapSource<T, M>>;
   const size_t sm = smem_for<T, M, NT, CS>(a.W, false);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  const cx<T>* twg = nullptr;
+  hipError_t e = table_for<T>(M, &twg);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st, (const cx<T>*)a.X, a.Wb, a.wrapN,
-                     (const cx<T>*)a.H, a.W, a.box, (const T*)a.wsys, a.map, (T*)partial);
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st,
+                     map_source<T, M>(a, partial, nullptr, nullptr), twg);
   return hipGetLastError();
 }
 
 template <typename T, int M, int NT, int CS>
 static hipError_t launch_adj(const FieldConfig& c, const FieldArgs& a, const void* gM, void* gX,
                              hipStream_t st) {
-  auto k = k_field_adj<T, M, NT, CS>;
+  auto k = k_field_adj<T, M, NT, CS, MapSource<T, M>>;
   const size_t sm = smem_for<T, M, NT, CS>(a.W, true);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  const cx<T>* twg = nullptr;
+  hipError_t e = table_for<T>(M, &twg);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st, (const cx<T>*)a.X, a.Wb, a.wrapN,
-                     (const cx<T>*)a.H, a.W, a.box, (const T*)a.wsys, a.map, (const T*)gM,
-                     (cx<T>*)gX);
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st,
+                     map_source<T, M>(a, nullptr, gM, gX), a.W, twg);
   return hipGetLastError();
+}
+
+template <typename T, int M, int NT, int CS>
+static hipError_t launch_multi(const FieldConfig& c, const ItemDesc* items, int n_items, int Wmax,
+                               char* ws, bool adjoint, hipStream_t st) {
+  const size_t sm = smem_for<T, M, NT, CS>(Wmax, adjoint);
+  const cx<T>* twg = nullptr;
+  hipError_t e = table_for<T>(M, &twg);
+  if (e != hipSuccess) return e;
+  TableSource<T> src{items, n_items, ws};
+  if (adjoint) {
+    auto k = k_field_adj<T, M, NT, CS, TableSource<T>>;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st, src, Wmax, twg);
+  } else {
+    auto k = k_field_fwd<T, M, NT, CS, TableSource<T>>;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st, src, twg);
+  }
+  return hipGetLastError();
+}
+
+hipError_t field_multi(int precision, const FieldConfig& c, const ItemDesc* items, int n_items,
+                       int Wmax, char* ws, bool adjoint, hipStream_t st) {
+#define LB_CASE(M_, NT_, CS_)                                                                 \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == CS_)                                            \
+    return precision == 64                                                                    \
+               ? launch_multi<double, M_, NT_, CS_>(c, items, n_items, Wmax, ws, adjoint, st) \
+               : launch_multi<float, M_, NT_, CS_>(c, items, n_items, Wmax, ws, adjoint, st);
+  LB_FIELD_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return hipErrorInvalidValue;
 }
 
 hipError_t field_forward(int precision, const FieldConfig& c, const FieldArgs& a, void* partial,

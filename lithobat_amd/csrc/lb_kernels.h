@@ -4,6 +4,26 @@
 
 namespace lb {
 
+// ---- roots-of-unity tables (lb_util.hip) -----------------------------------
+// e^{+2 pi i t / L} for t < L on the current device.  ensure_roots builds a
+// table once (it allocates: call it at plan creation, not while capturing);
+// roots_table only looks one up (nullptr when absent).
+template <typename T>
+hipError_t ensure_roots(int L);
+template <typename T>
+const cx<T>* roots_table(int L);
+
+// the table for L, built on first use
+template <typename T>
+inline hipError_t table_for(int L, const cx<T>** tw) {
+  *tw = roots_table<T>(L);
+  if (*tw) return hipSuccess;
+  hipError_t e = ensure_roots<T>(L);
+  if (e != hipSuccess) return e;
+  *tw = roots_table<T>(L);
+  return *tw ? hipSuccess : hipErrorInvalidValue;
+}
+
 // ---- band transforms (lb_band.hip) ---------------------------------------
 bool band_size_supported(int n);
 
@@ -56,6 +76,22 @@ struct FieldConfig {
 // every compiled coherent-system kernel shape, default-preference order
 const std::vector<FieldConfig>& field_configs();
 bool field_config_supported(int precision, const FieldConfig& c);
+
+// One work item of a multi-plan (uneven clip) launch; *_off are byte offsets
+// into the workspace (-1: unused), the rest are plan-owned device tables.
+struct ItemDesc {
+  long long X_off;
+  int Wb, wrapN;
+  const void* H;
+  int W;
+  const int2* box;
+  const void* wsys;
+  int s0, s1;
+  long long part_off, gM_off, gX_off;
+};
+// items: device array; Wmax sizes LDS; forward writes partials, adjoint gX partials
+hipError_t field_multi(int precision, const FieldConfig& c, const ItemDesc* items, int n_items,
+                       int Wmax, char* ws, bool adjoint, hipStream_t st);
 size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint);
 
 // partial[item][M][M] = sum over the item's systems of w |E|^2 on the coarse grid
@@ -82,6 +118,15 @@ hipError_t scatter_band(const cx<T>* in, int K, int Wb, int nu0, int N, cx<T>* o
 template <typename T>
 hipError_t band_mul(const cx<T>* X, const cx<T>* Hk, int n_masks, int K, long long len, cx<T>* Y,
                     hipStream_t st);
+
+// Per-system dense spectra on the whole grid (band order, nu0 = -N/2) for the
+// dense comparison form, and the weighted |E|^2 accumulation over them.
+template <typename T>
+hipError_t dense_spectra(const cx<T>* X, int N, const cx<T>* H, int W, const int2* box, int j0,
+                         int nsys, int nu0m, cx<T>* spec, hipStream_t st);
+template <typename T>
+hipError_t accum_norm2(const cx<T>* E, int nsys, long long len, const T* w, int j0, T* img,
+                       hipStream_t st);
 
 // ---- pupil tables (lb_pupil.hip) ------------------------------------------
 struct PupilSys {

@@ -1,3 +1,7 @@
+#include <map>
+#include <mutex>
+#include <tuple>
+
 // Fill and deterministic segment sums (partials -> images, band gradients ->
 // masks).  A fill kernel is used instead of hipMemsetAsync so the whole
 // forward/backward chain can be captured into a graph and replayed.
@@ -171,5 +175,128 @@ template hipError_t band_mul<float>(const cx<float>*, const cx<float>*, int, int
                                     cx<float>*, hipStream_t);
 template hipError_t band_mul<double>(const cx<double>*, const cx<double>*, int, int, long long,
                                      cx<double>*, hipStream_t);
+
+}  // namespace lb
+
+namespace lb {
+
+template <typename T>
+__global__ void k_roots(cx<T>* tw, int L) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < L; t += gridDim.x * blockDim.x) {
+    double s, c;
+    sincospi(2.0 * (double)t / (double)L, &s, &c);
+    tw[t] = cx<T>{(T)c, (T)s};
+  }
+}
+
+namespace {
+std::mutex g_roots_mu;
+std::map<std::tuple<int, int, int>, void*> g_roots;  // (device, sizeof(T), L) -> table
+}  // namespace
+
+template <typename T>
+const cx<T>* roots_table(int L) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(g_roots_mu);
+  auto it = g_roots.find(std::make_tuple(dev, (int)sizeof(T), L));
+  return it == g_roots.end() ? nullptr : (const cx<T>*)it->second;
+}
+
+template <typename T>
+hipError_t ensure_roots(int L) {
+  if (roots_table<T>(L)) return hipSuccess;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  void* p = nullptr;
+  e = hipMalloc(&p, sizeof(cx<T>) * (size_t)L);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_roots<T>, dim3((L + 255) / 256), dim3(256), 0, 0, (cx<T>*)p, L);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return e;
+  }
+  std::lock_guard<std::mutex> g(g_roots_mu);
+  auto key = std::make_tuple(dev, (int)sizeof(T), L);
+  if (g_roots.count(key)) {
+    (void)hipFree(p);  // another thread won the race
+  } else {
+    g_roots[key] = p;  // kept for the life of the process
+  }
+  return hipSuccess;
+}
+
+template hipError_t ensure_roots<float>(int);
+template hipError_t ensure_roots<double>(int);
+template const cx<float>* roots_table<float>(int);
+template const cx<double>* roots_table<double>(int);
+
+}  // namespace lb
+
+namespace lb {
+
+// spec[k][u][v] (band order over the whole grid, nu0 = -N/2) = X[u][v] * H_j
+// on system j = j0 + k's box, zero elsewhere (zeroed by the caller).
+template <typename T>
+__global__ void k_dense_spectra(const cx<T>* __restrict__ X, int N, const cx<T>* __restrict__ H,
+                                int W, const int2* __restrict__ box, int j0, int nsys,
+                                int nu0m, cx<T>* __restrict__ spec) {
+  const int k = blockIdx.y;
+  if (k >= nsys) return;
+  const int j = j0 + k;
+  const int2 b = box[j];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < W * W; e += gridDim.x * blockDim.x) {
+    const int a = e / W, c = e - (e / W) * W;
+    // box frequency (b.x + a + nu0m, b.y + c + nu0m) -> whole-grid band position
+    const int uy = (((b.x + a + nu0m) + N / 2) % N + N) % N;
+    const int ux = (((b.y + c + nu0m) + N / 2) % N + N) % N;
+    const cx<T> h = H[(long long)j * W * W + e];
+    spec[((long long)k * N + uy) * N + ux] = X[(long long)uy * N + ux] * h;
+  }
+}
+
+// img[e] += sum_k w[j0 + k] |E_k[e]|^2
+template <typename T>
+__global__ void k_accum_norm2(const cx<T>* __restrict__ E, int nsys, long long len,
+                              const T* __restrict__ w, int j0, T* __restrict__ img) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+       e += (long long)gridDim.x * blockDim.x) {
+    T s = img[e];
+    for (int k = 0; k < nsys; ++k) s += w[j0 + k] * norm2(E[(long long)k * len + e]);
+    img[e] = s;
+  }
+}
+
+template <typename T>
+hipError_t dense_spectra(const cx<T>* X, int N, const cx<T>* H, int W, const int2* box, int j0,
+                         int nsys, int nu0m, cx<T>* spec, hipStream_t st) {
+  hipError_t e = fill<T>((T*)spec, (T)0, 2LL * nsys * N * N, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_dense_spectra<T>, dim3((W * W + 255) / 256, nsys), dim3(256), 0, st, X, N,
+                     H, W, box, j0, nsys, nu0m, spec);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t accum_norm2(const cx<T>* E, int nsys, long long len, const T* w, int j0, T* img,
+                       hipStream_t st) {
+  long long b = (len + 255) / 256;
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_accum_norm2<T>, dim3((unsigned)b), dim3(256), 0, st, E, nsys, len, w, j0,
+                     img);
+  return hipGetLastError();
+}
+
+template hipError_t dense_spectra<float>(const cx<float>*, int, const cx<float>*, int,
+                                         const int2*, int, int, int, cx<float>*, hipStream_t);
+template hipError_t dense_spectra<double>(const cx<double>*, int, const cx<double>*, int,
+                                          const int2*, int, int, int, cx<double>*, hipStream_t);
+template hipError_t accum_norm2<float>(const cx<float>*, int, long long, const float*, int,
+                                       float*, hipStream_t);
+template hipError_t accum_norm2<double>(const cx<double>*, int, long long, const double*, int,
+                                        double*, hipStream_t);
 
 }  // namespace lb

@@ -225,6 +225,21 @@ class ImagingPlan:
                                            ws.data_ptr(), ws.numel(), st))
         return out
 
+    def forward_dense(self, mask: torch.Tensor, plane: int = 0, batch: int = 8) -> torch.Tensor:
+        """The reference's dataflow on the device (one dense N x N transform per
+        coherent system); for comparison -- same image as ``forward``."""
+        m, cplx = self._check_masks(mask)
+        if cplx or m.shape[0] != 1:
+            raise ValueError("forward_dense takes one real mask")
+        out = torch.empty((self.n, self.n), dtype=self.real_dtype, device=m.device)
+        nb = int(self._lib.lb_dense_workspace_bytes(self._h, batch))
+        with self._lock:
+            ws = self._workspace(nb, m.device)
+            st = torch.cuda.current_stream(m.device).cuda_stream
+            _lib.check(self._lib.lb_forward_dense(self._h, m.data_ptr(), plane, batch,
+                                                  out.data_ptr(), ws.data_ptr(), ws.numel(), st))
+        return out
+
     def profile(self, on: bool) -> None:
         """Record HIP events around every coherent-system kernel launch (resets)."""
         _lib.check(self._lib.lb_profile_enable(self._h, 1 if on else 0))
