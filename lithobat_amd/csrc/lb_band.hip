@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kBandThreads) void k_analysis_rows(const T* __restr
 // columns of RT[img][y][v] (contiguous runs of a row) into padded LDS; only
 // the band rows of each result are stored, X[img][u][v] = s * Z[bin(nu0 + u)].
 template <typename T, int N>
-__global__ __launch_bounds__(64 * col_waves<T, N>()) void k_analysis_cols(const cx<T>* __restrict__ RT, int nu0, int Wb, T s,
+__global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(const cx<T>* __restrict__ RT, int nu0, int Wb, T s,
                                 const cx<T>* __restrict__ twg, cx<T>* __restrict__ X) {
   constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = N + 1;
   constexpr int NT = 64 * col_waves<T, N>();
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64 * col_waves<T, N>()) void k_analysis_cols(const 
 // The CPB columns of a block are written out cooperatively as row runs,
 // TT[img][y][v].
 template <typename T, int N>
-__global__ __launch_bounds__(64 * col_waves<T, N>()) void k_synth_cols(const cx<T>* __restrict__ Y, int nu0, int Wb, int herm,
+__global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const cx<T>* __restrict__ Y, int nu0, int Wb, int herm,
                              const cx<T>* __restrict__ twg, cx<T>* __restrict__ TT) {
   constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = N + 1;
   constexpr int NT = 64 * col_waves<T, N>();
