@@ -56,6 +56,24 @@ def optics_cfg4():
                         normalisation="clear")
 
 
+def clips_cfg5():
+    """16 clips, sizes from {512, 768, 1024} (seed 11), Zernike 4-11 ~ U(+-0.02) waves
+    (seed 12), defocus ~ U(+-50 nm) per clip (seed 13); NA 1.35 TE vector, 4 nm."""
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    sizes = np.random.default_rng(11).choice([512, 768, 1024], 16)
+    zr = np.random.default_rng(12)
+    zern = {j: float(zr.uniform(-0.02, 0.02)) for j in range(4, 12)}
+    dz = np.random.default_rng(13).uniform(-50e-9, 50e-9, 16)
+    clips = []
+    for n, d in zip(sizes, dz):
+        o = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, imaging_model="vector",
+                         polarisation="te", zernike_coeffs=zern, defocus=float(d),
+                         source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55,
+                         source_grid=47, normalisation="clear")
+        clips.append((o, GridConfig(n_pixels=int(n), pixel_size=PIXEL)))
+    return clips
+
+
 def field_flops(M: int, W: int) -> float:
     """Algorithmic flops of one coherent system on the coarse grid.
 
@@ -63,6 +81,10 @@ def field_flops(M: int, W: int) -> float:
     (6 W^2) and the weighted |E|^2 accumulation (4 M^2).
     """
     return 5.0 * M * math.log2(M) * (W + M) + 6.0 * W * W + 4.0 * M * M
+
+
+def ratio(a, b):
+    return None if a is None or b is None else a / b
 
 
 def timed(fn, steps, warmup, sync):
@@ -85,6 +107,82 @@ def cpu_baseline(mask, optics, grid, plane):
     return raw[0], dt
 
 
+def ilt_leg(args, ilt, dev, sync):
+    """Seconds per ILT forward + backward step at 1024^2 (autograd, eager)."""
+    import torch
+
+    from lithobat_amd.ilt import abbe_intensity, mask_of, relaxed_resist_loss
+    from lithobat_amd.masks import manhattan
+    scale = 1.0 / float(ilt.clear[0])
+    rng = np.random.default_rng(3000)
+    theta = torch.tensor(rng.standard_normal((1, N, N)), device=dev, dtype=torch.float32,
+                         requires_grad=True)
+    target = torch.tensor(manhattan(N, PIXEL, seed=3001)[None], device=dev, dtype=torch.float32)
+
+    def ilt_step():
+        theta.grad = None
+        raw = abbe_intensity(mask_of(theta, 8.0), ilt)[:, 0]
+        loss = relaxed_resist_loss(raw, target, scale, 0.3, 50.0)
+        loss.backward()
+
+    return timed(ilt_step, args.ilt_steps, 2, sync)
+
+
+def clips_leg(args, dev, sync):
+    """Seconds per batched vector ILT step over 16 uneven clips (graph replay), and per
+    plain step (one clip per call, eager, synchronised after every clip)."""
+    import torch
+
+    from lithobat_amd.batch import ILTStep, ImagingBatch
+    from lithobat_amd.masks import manhattan
+    clips = clips_cfg5()
+    batch = ImagingBatch(clips, precision=32)
+    thetas = [torch.tensor(np.random.default_rng(5000 + c).standard_normal((g.n_pixels,) * 2),
+                           device=dev, dtype=torch.float32) for c, (_, g) in enumerate(clips)]
+    targets = [torch.tensor(manhattan(g.n_pixels, PIXEL, seed=6000 + c), device=dev,
+                            dtype=torch.float32) for c, (_, g) in enumerate(clips)]
+    bstep = ILTStep(batch, thetas, targets)
+    bstep.run()
+    sync()
+    bgraph = None
+    if not args.no_graph:
+        s2 = torch.cuda.Stream()
+        s2.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s2):
+            bstep.run()
+        torch.cuda.current_stream().wait_stream(s2)
+        sync()
+        bgraph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(bgraph):
+            bstep.run()
+        sync()
+    clip_ilt_s = timed(bgraph.replay if bgraph is not None else bstep.run,
+                       max(3, args.ilt_steps // 2), 1, sync)
+    singles = [ILTStep(ImagingBatch([c], precision=32), [t], [tg])
+               for c, t, tg in zip(clips, thetas, targets)]
+
+    def plain_clips():
+        for one in singles:
+            one.run()
+            torch.cuda.synchronize()
+
+    return clip_ilt_s, timed(plain_clips, 2, 1, sync)
+
+
+def dense_leg(args, plan, masks, sync):
+    """Images/s of the reference's dataflow on the device: one dense N x N inverse
+    transform per source point (8 per launch), host synchronisation per image."""
+    import torch
+    kd = max(1, args.dense_images)
+    plan.forward_dense(masks[0], batch=8)
+    sync()
+    t0 = time.perf_counter()
+    for i in range(kd):
+        plan.forward_dense(masks[i % N_MASKS], batch=8)
+        torch.cuda.synchronize()
+    return kd / (time.perf_counter() - t0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +193,13 @@ def main():
     ap.add_argument("--plain-images", type=int, default=20,
                     help="images timed for the plain single-image form")
     ap.add_argument("--ilt-steps", type=int, default=10)
+    ap.add_argument("--dense-images", type=int, default=3,
+                    help="images timed for the dense per-source-point form")
+    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,cpu",
+                    help="comma list of measurements to run (batched always runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the legs, copy a MIB-sized buffer once (a plain copy kernel of "
+                         "known size, to scale hardware byte counters)")
     ap.add_argument("--tune", metavar="TABLE", default=None,
                     help="autotune the kernel shapes of this workload and write the table")
     args = ap.parse_args()
@@ -104,10 +209,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    legs = set(args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
+
     import torch
 
     from lithobat_amd.config import GridConfig
-    from lithobat_amd.ilt import abbe_intensity, mask_of, relaxed_resist_loss
     from lithobat_amd.masks import manhattan
     from lithobat_amd.plan import ImagingPlan
 
@@ -188,40 +296,43 @@ def main():
 
     # ---------------- plain: one (mask, plane) per call, sync per image ---------
     plain_plans = [ImagingPlan(optics, grid, [pl], precision=32) for pl in planes]
-    k = max(1, args.plain_images)
+    plain_images_per_s = None
+    if "plain" in legs:
+        k = max(1, args.plain_images)
 
-    def plain_image(i):
-        p = plain_plans[i % len(planes)]
-        p.forward(masks[(i // len(planes)) % N_MASKS])
-        torch.cuda.synchronize()
+        def plain_image(i):
+            p = plain_plans[i % len(planes)]
+            p.forward(masks[(i // len(planes)) % N_MASKS])
+            torch.cuda.synchronize()
 
-    for i in range(3):
-        plain_image(i)
-    t0 = time.perf_counter()
-    for i in range(k):
-        plain_image(i)
-    plain_s = (time.perf_counter() - t0) / k
-    plain_images_per_s = 1.0 / plain_s
+        for i in range(3):
+            plain_image(i)
+        t0 = time.perf_counter()
+        for i in range(k):
+            plain_image(i)
+        plain_images_per_s = k / (time.perf_counter() - t0)
 
     # ---------------- ILT forward + backward at 1024^2 ---------------------------
-    scale = 1.0 / float(ilt.clear[0])
-    rng = np.random.default_rng(3000)
-    theta = torch.tensor(rng.standard_normal((1, N, N)), device=dev, dtype=torch.float32,
-                         requires_grad=True)
-    target = torch.tensor(manhattan(N, PIXEL, seed=3001)[None], device=dev, dtype=torch.float32)
-
-    def ilt_step():
-        theta.grad = None
-        raw = abbe_intensity(mask_of(theta, 8.0), ilt)[:, 0]
-        loss = relaxed_resist_loss(raw, target, scale, 0.3, 50.0)
-        loss.backward()
-
-    ilt_s = timed(ilt_step, args.ilt_steps, 2, sync)
+    ilt_s = clip_ilt_s = clip_plain_s = dense_images_per_s = None
+    if "ilt" in legs:
+        ilt_s = ilt_leg(args, ilt, dev, sync)
+    if "clips" in legs:
+        clip_ilt_s, clip_plain_s = clips_leg(args, dev, sync)
+    if "dense" in legs:
+        dense_images_per_s = dense_leg(args, plain_plans[0], masks, sync)
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        a = torch.empty(nb, dtype=torch.uint8, device=dev).fill_(1)
+        b = torch.empty_like(a)
+        sync()
+        b.copy_(a)
+        sync()
+        del a, b
 
     # ---------------- error vs float64 and the CPU baseline ---------------------
     err = None
     cpu = None
-    if not args.no_cpu:
+    if "cpu" in legs:
         ref, cpu_s = cpu_baseline(masks_np[0], optics, grid, planes[0])
         gpu0 = out[0, 0].double().cpu().numpy()
         err = float(np.abs(gpu0 - ref).max() / np.abs(ref).max())
@@ -249,8 +360,22 @@ def main():
                    "images_per_step": n_img, "n": N, "coarse_grid": M, "pupil_box": W,
                    "graph": graph is not None, "parallelism": "single GPU"},
         "plain_images_per_s": plain_images_per_s,
-        "batched_vs_plain": images_per_s / plain_images_per_s,
-        "ilt_steps_per_s": 1.0 / ilt_s,
+        "batched_vs_plain": ratio(images_per_s, plain_images_per_s),
+        "plain_dense_images_per_s": dense_images_per_s,
+        "batched_vs_plain_dense": ratio(images_per_s, dense_images_per_s),
+        "plain_definitions": {
+            "plain": "this library, one (mask, plane) per call, host sync per image, no graph",
+            "plain_dense": "the reference's dataflow on the device: one dense 1024^2 inverse "
+                           "transform per source point (8 per launch), |E|^2 accumulated, host "
+                           "sync per image"},
+        "clip_ilt_steps_per_s": ratio(1.0, clip_ilt_s),
+        "clip_ilt_plain_steps_per_s": ratio(1.0, clip_plain_s),
+        "clip_ilt_batched_vs_plain": ratio(clip_plain_s, clip_ilt_s),
+        "clip_ilt_config": "16 clips {512,768,1024}^2 @ 4 nm, NA 1.35 TE vector, Zernike 4-11, "
+                           "defocus +-50 nm, forward + adjoint, one persistent coherent-system "
+                           "launch per pass, graph-captured step; plain = one clip per call, "
+                           "eager, sync per clip",
+        "ilt_steps_per_s": ratio(1.0, ilt_s),
         "ilt_config": "1024^2 @ 4 nm, NA 1.35 quadrupole (148 pts), sigmoid(8 theta), "
                       "threshold 0.3 steepness 50, FP32 forward + adjoint",
         "fp32_rel_err_vs_fp64": err,

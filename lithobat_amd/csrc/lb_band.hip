@@ -10,6 +10,8 @@
 // planes are packed two rows per complex transform.  The intermediate
 // (band x N) is stored transposed so both passes read and write contiguous
 // runs.  Grid: (lines / lines-per-block, batch).
+#include <algorithm>
+
 #include "lb_kernels.h"
 
 namespace lb {
@@ -49,8 +51,41 @@ template <int N>
 constexpr int tpw() {
   return N >= 1024 ? 1 : pow2_floor(1024 / N);
 }
-constexpr int kBandThreads = 256;
-constexpr int kWaves = kBandThreads / 64;
+
+// LDS layout: one pad slot after every 256 bytes of complex values, so the
+// strided writes of a Stockham stage (stride R) and the strided twiddle reads
+// (stride r * step) fall in distinct banks.  Transforms, the roots table and
+// the transposition buffers all use this indexing.
+template <typename T>
+constexpr int kPadShift = sizeof(cx<T>) == 8 ? 5 : 4;
+template <typename T>
+__device__ __forceinline__ int pidx(int i) {
+  return i + (i >> kPadShift<T>);
+}
+template <typename T, int L>
+constexpr int padded() {
+  return L + (L >> kPadShift<T>);
+}
+
+// Wavefronts per block of the row kernels: the most resident waves per CU
+// (160 KB of LDS, each block also holding the N-point roots table), ties
+// going to the larger block; at most 8 waves so two blocks can overlap.
+template <typename T, int N>
+constexpr int row_waves() {
+  constexpr int cap = (160 * 1024) / (int)sizeof(cx<T>);
+  int best = 1, best_waves = 0;
+  for (int w = 1; w <= 8; w *= 2) {
+    const int per_block = padded<T, N>() + w * tpw<N>() * padded<T, N>();
+    int blocks = cap / per_block;
+    int waves = blocks * w;
+    if (waves > 32) waves = 32;
+    if (blocks >= 1 && waves >= best_waves) {
+      best_waves = waves;
+      best = w;
+    }
+  }
+  return best;
+}
 
 // One Stockham stage over the TPW transforms of one wavefront, in place in
 // wave-private LDS `wb` (transform tr at wb + tr * L).  Every lane reads all
@@ -58,8 +93,13 @@ constexpr int kWaves = kBandThreads / 64;
 // operations complete in issue order, so no barrier is needed.  The first
 // stage gathers its inputs through `load(tr, index)`; the last stage hands
 // its outputs to `store(tr, index, value)` instead of writing LDS.
+//
+// SPARSE (first stage only): the inputs are known to vanish outside the
+// first and the last stride-S run of the line (a band of frequencies
+// |nu| < S, S = L / R), so each radix-R DFT has two inputs and reduces to
+// X_k = v_0 + v_{R-1} e^{DIR 2 pi i (R-1) k / R}.
 template <typename T, int DIR, int L, int LD, int TPW, int R, bool FIRST, bool LAST, class Load,
-          class Store>
+          class Store, bool SPARSE = false>
 __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane, int ntr, int Ns,
                                            Load& load, Store& store) {
   constexpr int S = L / R;
@@ -71,13 +111,20 @@ __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane,
     const int t = lane + 64 * i;
     const int tr = t / S, j = t - (t / S) * S;
     if (t < TASKS && tr < ntr) {
+      if constexpr (FIRST && SPARSE) {
+        const cx<T> a = load(tr, j), b = load(tr, j + (R - 1) * S);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = FIRST ? load(tr, j + r * S) : wb[tr * LD + j + r * S];
+        for (int k = 0; k < R; ++k) v[i][k] = a + mul_root<R, DIR, T>(b, (R - 1) * k);
+        continue;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        v[i][r] = FIRST ? load(tr, j + r * S) : wb[tr * LD + pidx<T>(j + r * S)];
       if (!FIRST) {
         const int k = j % Ns;
         const int step = L / (Ns * R);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[i][r] = v[i][r] * tw_get<DIR>(tw, r * k * step);
+        for (int r = 1; r < R; ++r) v[i][r] = v[i][r] * tw_get<DIR>(tw, pidx<T>(r * k * step));
       }
       dft<R, DIR>(v[i]);
     }
@@ -95,7 +142,7 @@ __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane,
         if (LAST)
           store(tr, base + r * Ns, v[i][r]);
         else
-          wb[tr * LD + base + r * Ns] = v[i][r];
+          wb[tr * LD + pidx<T>(base + r * Ns)] = v[i][r];
       }
     }
   }
@@ -103,14 +150,17 @@ __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane,
 }
 
 // A length-L transform of the wavefront's `ntr` (<= TPW) transforms, held at
-// wb + tr * LD.
-template <typename T, int DIR, int L, int TPW, int LD = L, class Load, class Store>
+// wb + tr * LD in padded indexing; tw is the padded roots table.
+// SPARSE: the input vanishes outside the first and last L / R0 points.
+template <typename T, int DIR, int L, int TPW, int LD = padded<T, L>(), bool SPARSE = false,
+          class Load, class Store>
 __device__ __forceinline__ void wave_fft(cx<T>* wb, const cx<T>* tw, int lane, int ntr, Load& load,
                                          Store& store) {
   using P = Plan<L>;
   constexpr int NST = (P::R0 > 1) + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
   int Ns = 1;
-  wave_stage<T, DIR, L, LD, TPW, P::R0, true, NST == 1>(wb, tw, lane, ntr, Ns, load, store);
+  wave_stage<T, DIR, L, LD, TPW, P::R0, true, NST == 1, Load, Store, SPARSE>(wb, tw, lane, ntr, Ns,
+                                                                             load, store);
   Ns *= P::R0;
   if constexpr (P::R1 > 1) {
     wave_stage<T, DIR, L, LD, TPW, P::R1, false, NST == 2>(wb, tw, lane, ntr, Ns, load, store);
@@ -125,14 +175,42 @@ __device__ __forceinline__ void wave_fft(cx<T>* wb, const cx<T>* tw, int lane, i
   }
 }
 
+// Stages 2.. of a length-L transform whose first stage the caller has
+// written into wb (Stockham order, span R0).
+template <typename T, int DIR, int L, int TPW, int LD, class Store>
+__device__ __forceinline__ void wave_fft_tail(cx<T>* wb, const cx<T>* tw, int lane, int ntr,
+                                              Store& store) {
+  using P = Plan<L>;
+  constexpr int NST = (P::R0 > 1) + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
+  static_assert(NST >= 2, "the tail needs a second stage");
+  auto none = [](int, int) { return cx<T>{0, 0}; };
+  int Ns = P::R0;
+  wave_stage<T, DIR, L, LD, TPW, P::R1, false, NST == 2>(wb, tw, lane, ntr, Ns, none, store);
+  Ns *= P::R1;
+  if constexpr (P::R2 > 1) {
+    wave_stage<T, DIR, L, LD, TPW, P::R2, false, NST == 3>(wb, tw, lane, ntr, Ns, none, store);
+    Ns *= P::R2;
+  }
+  if constexpr (P::R3 > 1) {
+    wave_stage<T, DIR, L, LD, TPW, P::R3, false, NST == 4>(wb, tw, lane, ntr, Ns, none, store);
+  }
+}
+
 // Column kernels: CPB columns per block (block-cooperative transposition
 // through LDS, so the band x N intermediate is read and written in
-// contiguous runs of a row), TPW per wavefront, padded LDS stride.
+// contiguous runs of a row), TPW per wavefront.  Column c sits at c * LDC:
+// LDC = padded length + 64 / CPB puts the 64 lanes of a cooperative copy
+// (CPB columns x 64 / CPB rows) in distinct banks.
 template <typename T, int N>
 constexpr int cpb() {
-  constexpr int room = (160 * 1024) / (int)sizeof(cx<T>) - N;  // LDS after the roots table
-  constexpr int by_lds = room < N + 1 ? 1 : pow2_floor(room / (N + 1));
+  constexpr int room = (160 * 1024) / (int)sizeof(cx<T>) - padded<T, N>();  // after the roots
+  constexpr int per = padded<T, N>() + 64;
+  constexpr int by_lds = room < per ? 1 : pow2_floor(room / per);
   return by_lds < 1 ? 1 : (by_lds > 16 ? 16 : by_lds);
+}
+template <typename T, int N>
+constexpr int col_ld() {
+  return padded<T, N>() + 64 / cpb<T, N>();
 }
 template <typename T, int N>
 constexpr int col_waves() {
@@ -141,6 +219,12 @@ constexpr int col_waves() {
 template <typename T, int N>
 constexpr int col_tpw() {  // transforms per wave in the column kernels
   return cpb<T, N>() < tpw<N>() ? cpb<T, N>() : tpw<N>();
+}
+
+template <typename T>
+__device__ __forceinline__ void load_roots_padded(cx<T>* tw, const cx<T>* __restrict__ twg, int L,
+                                                  int tid, int nt) {
+  for (int t = tid; t < L; t += nt) tw[pidx<T>(t)] = twg[t];
 }
 
 // Band position of FFT bin b, or -1 when the bin is outside the band.
@@ -158,13 +242,15 @@ __device__ __forceinline__ int mirror_of(int u, int nu0, int Wb) {
 
 #define LB_BAND_PROLOGUE(T, N)                                       \
   constexpr int TPW = tpw<N>();                                      \
-  constexpr int TPB = kWaves * TPW;                                  \
+  constexpr int NT = 64 * row_waves<T, N>();                         \
+  constexpr int TPB = row_waves<T, N>() * TPW;                       \
+  constexpr int PL = padded<T, N>();                                 \
   extern __shared__ __align__(16) unsigned char smem_raw[];          \
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);                    \
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;     \
-  cx<T>* wb = tw + N + wave * TPW * N;                               \
+  cx<T>* wb = tw + PL + wave * TPW * PL;                             \
   const int img = blockIdx.y;                                        \
-  load_roots(tw, twg, N, tid, kBandThreads);                         \
+  load_roots_padded(tw, twg, N, tid, NT);                            \
   __syncthreads();
 
 // ---------------------------------------------------------------- analysis
@@ -172,7 +258,7 @@ __device__ __forceinline__ int mirror_of(int u, int nu0, int Wb) {
 // (real planes, packed as row 2p + i row 2p+1); the band columns are pulled
 // out of the packed spectrum, RT[img][y][v] (rows of Wb, contiguous).
 template <typename T, int N, bool CPLX>
-__global__ __launch_bounds__(kBandThreads) void k_analysis_rows(const T* __restrict__ in,
+__global__ __launch_bounds__((64 * row_waves<T, N>())) void k_analysis_rows(const T* __restrict__ in,
                                                                 long long img_stride,
                                                                 long long slice_stride, int nsum,
                                                                 int nu0, int Wb,
@@ -202,7 +288,7 @@ __global__ __launch_bounds__(kBandThreads) void k_analysis_rows(const T* __restr
     }
     return cx<T>{re, im};
   };
-  auto store = [&](int tr, int x, cx<T> v) { wb[tr * N + x] = v; };
+  auto store = [&](int tr, int x, cx<T> v) { wb[tr * PL + pidx<T>(x)] = v; };
   wave_fft<T, -1, N, TPW>(wb, tw, lane, ntr, load, store);
   __syncthreads();
   // block-cooperative extraction of the band columns
@@ -210,17 +296,17 @@ __global__ __launch_bounds__(kBandThreads) void k_analysis_rows(const T* __restr
   const int y0 = CPLX ? tr_blk : 2 * tr_blk;
   const int nrows = min(ROWS, N - y0);
   cx<T>* out = RT + (long long)img * Wb * N;
-  const cx<T>* Zb = tw + N;
-  for (int t = tid; t < Wb * ROWS; t += kBandThreads) {
+  const cx<T>* Zb = tw + PL;
+  for (int t = tid; t < Wb * ROWS; t += NT) {
     const int yy = t / Wb, v = t - yy * Wb;
     if (yy >= nrows) continue;
     const int nu = nu0 + v;
     if (CPLX) {
-      out[(long long)(y0 + yy) * Wb + v] = Zb[yy * N + bin_of(nu, N)];
+      out[(long long)(y0 + yy) * Wb + v] = Zb[yy * PL + pidx<T>(bin_of(nu, N))];
     } else {
       const int tr = yy >> 1;
-      const cx<T> a = Zb[tr * N + bin_of(nu, N)];
-      const cx<T> b = conj(Zb[tr * N + bin_of(-nu, N)]);
+      const cx<T> a = Zb[tr * PL + pidx<T>(bin_of(nu, N))];
+      const cx<T> b = conj(Zb[tr * PL + pidx<T>(bin_of(-nu, N))]);
       cx<T> r;
       if ((yy & 1) == 0) {
         r = scale(a + b, (T)0.5);  // transform of the even row
@@ -239,27 +325,27 @@ __global__ __launch_bounds__(kBandThreads) void k_analysis_rows(const T* __restr
 template <typename T, int N>
 __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(const cx<T>* __restrict__ RT, int nu0, int Wb, T s,
                                 const cx<T>* __restrict__ twg, cx<T>* __restrict__ X) {
-  constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = N + 1;
+  constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = col_ld<T, N>();
   constexpr int NT = 64 * col_waves<T, N>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* buf = tw + N;  // CPB x LD
+  cx<T>* buf = tw + padded<T, N>();  // CPB x LD
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int img = blockIdx.y;
   const int vb = blockIdx.x * CPB;
   const int ncol = max(0, min(CPB, Wb - vb));
-  load_roots(tw, twg, N, tid, NT);
+  load_roots_padded(tw, twg, N, tid, NT);
   const cx<T>* src = RT + (long long)img * N * Wb;
   for (int t = tid; t < N * CPB; t += NT) {
     const int y = t / CPB, c = t - y * CPB;
-    if (c < ncol) buf[c * LD + y] = src[(long long)y * Wb + vb + c];
+    if (c < ncol) buf[c * LD + pidx<T>(y)] = src[(long long)y * Wb + vb + c];
   }
   __syncthreads();
   const int c0 = wave * TPW;
   const int ntr = max(0, min(TPW, ncol - c0));
   cx<T>* wb = buf + c0 * LD;
   cx<T>* out = X + (long long)img * Wb * Wb;
-  auto load = [&](int tr, int y) -> cx<T> { return wb[tr * LD + y]; };
+  auto load = [&](int tr, int y) -> cx<T> { return wb[tr * LD + pidx<T>(y)]; };
   auto store = [&](int tr, int b, cx<T> val) {
     const int u = band_pos(b, N, nu0, Wb);
     if (u >= 0) out[(long long)u * Wb + vb + c0 + tr] = scale(val, s);
@@ -272,19 +358,19 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(cons
 // gathers only the in-band inputs from Y (Hermitianised on load when herm).
 // The CPB columns of a block are written out cooperatively as row runs,
 // TT[img][y][v].
-template <typename T, int N>
+template <typename T, int N, bool SP>
 __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const cx<T>* __restrict__ Y, int nu0, int Wb, int herm,
                              const cx<T>* __restrict__ twg, cx<T>* __restrict__ TT) {
-  constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = N + 1;
+  constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = col_ld<T, N>();
   constexpr int NT = 64 * col_waves<T, N>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-  cx<T>* buf = tw + N;  // CPB x LD
+  cx<T>* buf = tw + padded<T, N>();  // CPB x LD
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int img = blockIdx.y;
   const int vb = blockIdx.x * CPB;
   const int ncol = max(0, min(CPB, Wb - vb));
-  load_roots(tw, twg, N, tid, NT);
+  load_roots_padded(tw, twg, N, tid, NT);
   __syncthreads();
   const int c0 = wave * TPW;
   const int ntr = max(0, min(TPW, ncol - c0));
@@ -301,13 +387,13 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const c
     }
     return val;
   };
-  auto store = [&](int tr, int y, cx<T> val) { wb[tr * LD + y] = val; };
-  wave_fft<T, +1, N, TPW, LD>(wb, tw, lane, ntr, load, store);
+  auto store = [&](int tr, int y, cx<T> val) { wb[tr * LD + pidx<T>(y)] = val; };
+  wave_fft<T, +1, N, TPW, LD, SP>(wb, tw, lane, ntr, load, store);
   __syncthreads();
   cx<T>* out = TT + (long long)img * N * Wb;
   for (int t = tid; t < N * CPB; t += NT) {
     const int y = t / CPB, c = t - y * CPB;
-    if (c < ncol) out[(long long)y * Wb + vb + c] = buf[c * LD + y];
+    if (c < ncol) out[(long long)y * Wb + vb + c] = buf[c * LD + pidx<T>(y)];
   }
 }
 
@@ -327,8 +413,8 @@ __device__ __forceinline__ void atomic_max_nonneg<double>(double* addr, double v
 // (real output: row y + i row y+1, valid because the band is Hermitian);
 // sparse first stage as above, outputs scaled and stored straight to HBM,
 // per-image running maximum for the peak.
-template <typename T, int N, bool COUT>
-__global__ __launch_bounds__(kBandThreads) void k_synth_rows(const cx<T>* __restrict__ TT,
+template <typename T, int N, bool COUT, bool SP>
+__global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const cx<T>* __restrict__ TT,
                                                              int nu0, int Wb, T s,
                                                              const cx<T>* __restrict__ twg,
                                                              T* __restrict__ out,
@@ -360,7 +446,114 @@ __global__ __launch_bounds__(kBandThreads) void k_synth_rows(const cx<T>* __rest
       lmax = fmax(lmax, fmax(a, b));
     }
   };
-  wave_fft<T, +1, N, TPW>(wb, tw, lane, ntr, load, store);
+  wave_fft<T, +1, N, TPW, PL, SP>(wb, tw, lane, ntr, load, store);
+  if (!COUT && peak) {
+    for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
+    if (lane == 0) atomic_max_nonneg(peak + img, lmax);
+  }
+}
+
+// Waves per SIMD that the LDS of the row kernels allows, at most 4 (the
+// register budget is held to it, so registers never cut the residency).
+template <typename T, int N>
+constexpr int row_waves_per_simd() {
+  constexpr int cap = (160 * 1024) / (int)sizeof(cx<T>);
+  constexpr int per_block = padded<T, N>() * (1 + row_waves<T, N>() * tpw<N>());
+  constexpr int w = (cap / per_block) * row_waves<T, N>() / 4;
+  return w < 1 ? 1 : (w > 4 ? 4 : w);
+}
+
+// Row pass for a band inside |nu| < S = N / R0 (the usual case): only two
+// inputs per first-stage task are non-zero.  Wavefronts are persistent over
+// transform units (TPW transforms each, `stride` apart): the next unit's band
+// values are loaded while the current one is transformed, the roots table is
+// loaded once per block, and the peak is one atomic per wavefront.
+template <typename T, int N, bool COUT>
+__global__ __launch_bounds__((64 * row_waves<T, N>()),
+                             (row_waves_per_simd<T, N>())) void k_synth_rows_sp(
+    const cx<T>* __restrict__ TT, int nu0, int Wb, T s, const cx<T>* __restrict__ twg,
+    T* __restrict__ out, T* __restrict__ peak) {
+  LB_BAND_PROLOGUE(T, N)
+  constexpr int R = Plan<N>::R0, S = N / R;
+  constexpr int NTR = COUT ? N : N / 2;
+  constexpr int NU = (NTR + TPW - 1) / TPW;
+  constexpr int TASKS = TPW * S, TPL = (TASKS + 63) / 64;
+  const cx<T>* src = TT + (long long)img * N * Wb;
+  // per task slot: transform within the unit, and the band positions of its
+  // two inputs (nu = j and nu = j - S), -1 outside the band
+  int trs[TPL], plo[TPL], phi[TPL], jj[TPL];
+#pragma unroll
+  for (int i = 0; i < TPL; ++i) {
+    const int t = lane + 64 * i;
+    const int tr = t / S, j = t - (t / S) * S;
+    const bool ok = t < TASKS;
+    trs[i] = ok ? tr : TPW;
+    jj[i] = j;
+    const int lo = j - nu0, hi = j - S - nu0;
+    plo[i] = (ok && lo >= 0 && lo < Wb) ? lo : -1;
+    phi[i] = (ok && hi >= 0 && hi < Wb) ? hi : -1;
+  }
+  // raw band values of one unit: even / odd rows (real output) at both inputs
+  cx<T> a0[TPL], a1[TPL], b0[TPL], b1[TPL];
+  auto fetch = [&](int u) {
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      a0[i] = a1[i] = b0[i] = b1[i] = cx<T>{0, 0};
+      const int p = u * TPW + trs[i];
+      if (trs[i] < TPW && p < NTR) {
+        const long long r0 = COUT ? (long long)p * Wb : (long long)(2 * p) * Wb;
+        if (plo[i] >= 0) {
+          a0[i] = src[r0 + plo[i]];
+          if (!COUT) a1[i] = src[r0 + Wb + plo[i]];
+        }
+        if (phi[i] >= 0) {
+          b0[i] = src[r0 + phi[i]];
+          if (!COUT) b1[i] = src[r0 + Wb + phi[i]];
+        }
+      }
+    }
+  };
+  const int nw = row_waves<T, N>();
+  const int stride = gridDim.x * nw;
+  int u = blockIdx.x * nw + wave;
+  if (u < NU) fetch(u);
+  T lmax = 0;
+  for (; u < NU; u += stride) {
+    cx<T> lo[TPL], hi[TPL];
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      // real output: row 2p + i row 2p+1 (the band is Hermitian per row pair)
+      lo[i] = COUT ? a0[i] : cx<T>{a0[i].re - a1[i].im, a0[i].im + a1[i].re};
+      hi[i] = COUT ? b0[i] : cx<T>{b0[i].re - b1[i].im, b0[i].im + b1[i].re};
+      lo[i] = scale(lo[i], s);
+      hi[i] = scale(hi[i], s);
+    }
+    if (u + stride < NU) fetch(u + stride);
+    const int ntr = min(TPW, NTR - u * TPW);
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      if (trs[i] < ntr) {
+        cx<T>* w = wb + trs[i] * PL;
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          w[pidx<T>(jj[i] * R + k)] = lo[i] + mul_root<R, +1, T>(hi[i], (R - 1) * k);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int p0 = u * TPW;
+    auto store = [&](int tr, int x, cx<T> val) {
+      const int p = p0 + tr;
+      if (COUT) {
+        reinterpret_cast<cx<T>*>(out)[((long long)img * N + p) * N + x] = val;
+      } else {
+        T* o = out + (long long)img * N * N + (long long)(2 * p) * N + x;
+        o[0] = val.re;
+        o[N] = val.im;
+        lmax = fmax(lmax, fmax(val.re, val.im));
+      }
+    };
+    wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store);
+  }
   if (!COUT && peak) {
     for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
     if (lane == 0) atomic_max_nonneg(peak + img, lmax);
@@ -370,12 +563,14 @@ __global__ __launch_bounds__(kBandThreads) void k_synth_rows(const cx<T>* __rest
 // ---------------------------------------------------------------- launchers
 template <typename T, int N>
 static size_t smem_bytes() {
-  return sizeof(cx<T>) * (size_t)(N + kWaves * tpw<N>() * N);
+  return sizeof(cx<T>) * (size_t)(padded<T, N>() * (1 + row_waves<T, N>() * tpw<N>()));
 }
 
 template <typename T, int N>
 static size_t col_smem_bytes() {
-  return sizeof(cx<T>) * (size_t)(N + cpb<T, N>() * (N + 1));
+  static_assert(sizeof(cx<T>) * (padded<T, N>() + cpb<T, N>() * col_ld<T, N>()) <= 160 * 1024,
+                "column block exceeds the LDS");
+  return sizeof(cx<T>) * (size_t)(padded<T, N>() + cpb<T, N>() * col_ld<T, N>());
 }
 
 template <typename K>
@@ -383,6 +578,12 @@ static hipError_t set_smem(K kern, size_t bytes) {
   return hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
+
+// Persistent row kernels: units per wavefront, and the wavefront count
+// (about 8 resident per SIMD over 256 CUs) below which fewer units per
+// wavefront are used.
+constexpr int kUnitsPerWave = 4;
+constexpr int kRowWavesTarget = 8192;
 
 static unsigned blocks_for(int lines, int per_block) {
   return (unsigned)((lines + per_block - 1) / per_block);
@@ -392,7 +593,8 @@ template <typename T, int N>
 static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
                              long long slice_stride, int nsum, int nimg, int nu0, int Wb, T s,
                              cx<T>* RT, cx<T>* X, hipStream_t st) {
-  constexpr int TPB = kWaves * tpw<N>();
+  constexpr int TPB = row_waves<T, N>() * tpw<N>();
+  constexpr int NT = 64 * row_waves<T, N>();
   const size_t sm = smem_bytes<T, N>();
   const cx<T>* twg = nullptr;
   hipError_t e = table_for<T>(N, &twg);
@@ -400,12 +602,12 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
   if (cplx) {
     auto k = k_analysis_rows<T, N, true>;
     if ((e = set_smem(k, sm)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(blocks_for(N, TPB), nimg), dim3(kBandThreads), sm, st,
+    hipLaunchKernelGGL(k, dim3(blocks_for(N, TPB), nimg), dim3(NT), sm, st,
                        (const T*)in, img_stride, slice_stride, nsum, nu0, Wb, twg, RT);
   } else {
     auto k = k_analysis_rows<T, N, false>;
     if ((e = set_smem(k, sm)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(blocks_for(N / 2, TPB), nimg), dim3(kBandThreads), sm, st,
+    hipLaunchKernelGGL(k, dim3(blocks_for(N / 2, TPB), nimg), dim3(NT), sm, st,
                        (const T*)in, img_stride, slice_stride, nsum, nu0, Wb, twg, RT);
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -420,26 +622,46 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
 template <typename T, int N>
 static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int herm, T s, int cout,
                               cx<T>* TT, T* out, T* peak, hipStream_t st) {
-  constexpr int TPB = kWaves * tpw<N>();
+  constexpr int TPB = row_waves<T, N>() * tpw<N>();
+  constexpr int NT = 64 * row_waves<T, N>();
   const size_t sm = smem_bytes<T, N>();
   const cx<T>* twg = nullptr;
   hipError_t e = table_for<T>(N, &twg);
   if (e != hipSuccess) return e;
-  auto k1 = k_synth_cols<T, N>;
+  // the band lies within |nu| < N / R0: sparse first radix stage
+  constexpr int S0 = N / Plan<N>::R0;
+  const bool sp = nu0 > -S0 && nu0 + Wb - 1 < S0;
+  auto k1 = sp ? k_synth_cols<T, N, true> : k_synth_cols<T, N, false>;
   const size_t smc = col_smem_bytes<T, N>();
   if ((e = set_smem(k1, smc)) != hipSuccess) return e;
   hipLaunchKernelGGL(k1, dim3(blocks_for(Wb, cpb<T, N>()), nimg), dim3(64 * col_waves<T, N>()),
                      smc, st, Y, nu0, Wb, herm, twg, TT);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  if constexpr (Plan<N>::R1 > 1) {
+    if (sp) {
+      // persistent rows: about kUnitsPerWave units per wavefront, fewer when
+      // the batch is too small to fill the device
+      const int units = ((cout ? N : N / 2) + tpw<N>() - 1) / tpw<N>();
+      const int nw = row_waves<T, N>();
+      int upw = (int)std::min<long long>(kUnitsPerWave, std::max<long long>(
+                                             1, (long long)units * nimg / (nw * kRowWavesTarget)));
+      const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
+      auto k2 = cout ? k_synth_rows_sp<T, N, true> : k_synth_rows_sp<T, N, false>;
+      if ((e = set_smem(k2, sm)) != hipSuccess) return e;
+      hipLaunchKernelGGL(k2, dim3(gx, nimg), dim3(NT), sm, st, (const cx<T>*)TT, nu0, Wb, s, twg,
+                         out, cout ? (T*)nullptr : peak);
+      return hipGetLastError();
+    }
+  }
   if (cout) {
-    auto k2 = k_synth_rows<T, N, true>;
+    auto k2 = sp ? k_synth_rows<T, N, true, true> : k_synth_rows<T, N, true, false>;
     if ((e = set_smem(k2, sm)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k2, dim3(blocks_for(N, TPB), nimg), dim3(kBandThreads), sm, st,
+    hipLaunchKernelGGL(k2, dim3(blocks_for(N, TPB), nimg), dim3(NT), sm, st,
                        (const cx<T>*)TT, nu0, Wb, s, twg, out, (T*)nullptr);
   } else {
-    auto k2 = k_synth_rows<T, N, false>;
+    auto k2 = sp ? k_synth_rows<T, N, false, true> : k_synth_rows<T, N, false, false>;
     if ((e = set_smem(k2, sm)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k2, dim3(blocks_for(N / 2, TPB), nimg), dim3(kBandThreads), sm, st,
+    hipLaunchKernelGGL(k2, dim3(blocks_for(N / 2, TPB), nimg), dim3(NT), sm, st,
                        (const cx<T>*)TT, nu0, Wb, s, twg, out, peak);
   }
   return hipGetLastError();

@@ -132,3 +132,25 @@ def test_1024_quadrupole_defocus_matches_oracle(device, precision):
     raw = compute_aerial_image(mask, optics, grid, return_raw=True, precision=precision)[0]
     tol = 1e-12 if precision == 64 else 1e-5
     assert _rel(raw, ref) <= tol, _rel(raw, ref)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_dense_reference_dataflow_equals_coarse_grid(device, precision):
+    """One dense N x N transform per source point gives the coarse-grid image."""
+    import torch
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=256, pixel_size=4e-9)
+    optics = OpticsConfig(NA=0.93, source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55,
+                          source_grid=15, defocus=50e-9, zernike_coeffs={8: 0.02},
+                          normalisation="none")
+    plan = ImagingPlan(optics, grid, precision=precision)
+    dt = torch.float64 if precision == 64 else torch.float32
+    m = torch.tensor(manhattan(256, 4e-9, seed=5, n_rect=(5, 12)), device=device, dtype=dt)
+    coarse = plan.forward(m)[0][0, 0].double().cpu().numpy()
+    dense = plan.forward_dense(m, batch=5).double().cpu().numpy()
+    ref = O.compute_aerial_image(m.double().cpu().numpy(), optics, grid, return_raw=True)[0]
+    tol = 1e-12 if precision == 64 else 1e-5
+    assert _rel(dense, ref) <= tol
+    assert _rel(coarse, ref) <= tol
