@@ -140,6 +140,90 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
   }
 }
 
+// Row-pass inputs of one system held in registers, so the next system's
+// mask-band and transfer-table values can be loaded while the current one
+// runs its column transforms.  Thread (rl1, j1) of the stage-1 mapping needs
+// rows a = c * CPR + rl1 and columns b = j1 + r * T1 (a, b < W <= (M+1)/2).
+// Used when that is at most 16 pairs per thread (4 in double precision).
+template <typename T, int M, int NT, int CSPLIT>
+struct RowPre {
+  using F = FieldShape<T, M, NT, CSPLIT>;
+  static constexpr int MAXW = (M + 1) / 2;
+  static constexpr int NCH = (MAXW + F::CPR - 1) / F::CPR;
+  static constexpr int NR = (MAXW + F::T1 - 1) / F::T1 < F::R1 ? (MAXW + F::T1 - 1) / F::T1 : F::R1;
+  static constexpr bool ON = NCH * NR <= (sizeof(T) == 4 ? 16 : 4);
+  cx<T> x[NCH][NR], h[NCH][NR];
+
+  __device__ __forceinline__ void fetch(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
+                                        const cx<T>* __restrict__ Hj, int W, int2 bx, int rl1,
+                                        int j1) {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int a = c * F::CPR + rl1;
+      const bool ra = a < W;
+      const int xr = ra ? wrap_idx(bx.x + a, wrapN) : 0;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int b = j1 + r * F::T1;
+        if (ra && b < W) {
+          x[c][r] = Xm[(long long)xr * Wb + wrap_idx(bx.y + b, wrapN)];
+          h[c][r] = Hj[a * W + b];
+        } else {
+          x[c][r] = cx<T>{0, 0};
+          h[c][r] = cx<T>{0, 0};
+        }
+      }
+    }
+  }
+};
+
+// row_pass with the stage-1 inputs taken from a RowPre.
+template <typename T, int M, int NT, int CSPLIT>
+__device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre, int W,
+                                             const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs,
+                                             int tid) {
+  using F = FieldShape<T, M, NT, CSPLIT>;
+  using P = RowPre<T, M, NT, CSPLIT>;
+  const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
+#pragma unroll
+  for (int c = 0; c < P::NCH; ++c) {
+    const int a0 = c * F::CPR;
+    if (a0 >= W) break;  // uniform over the block
+    {
+      if (a0 + rl1 < W) {
+        cx<T> v[F::R1];
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) v[r] = r < P::NR ? pre.x[c][r] * pre.h[c][r] : cx<T>{0, 0};
+        dft<F::R1, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) S[rl1 * F::LDC + j1 * F::R1 + r] = v[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < F::Q2; ++q) {
+      const int t = tid + q * NT;
+      const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+      const int a = a0 + rl;
+      if (t < F::NT2 && a < W) {
+        cx<T> v[F::R2];
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          v[r] = S[rl * F::LDC + j2 + r * F::R1];
+          if (r > 0) v[r] = v[r] * tw[r * j2];
+        }
+        dft<F::R2, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          const int x = j2 + r * F::R1 - cs * F::MC;
+          if (CSPLIT == 1 || (x >= 0 && x < F::MC)) Tb[a * F::MC + x] = v[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- work items
 // What one block needs for one work item: a mask's band spectrum, the
 // systems [s0, s1) of one image, and where its partial goes.
@@ -238,15 +322,26 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
     const cx<T>* H = iv.H;
     const int2* box = iv.box;
     const T* wsys = iv.wsys;
+    using Pre = RowPre<T, M, NT, CSPLIT>;
+    Pre pre;
+    const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
+    if (Pre::ON && s0 < s1)
+      pre.fetch(Xm, Wb, wrapN, H + (long long)s0 * W * W, W, box[s0], rl1, j1);
     for (int cs = 0; cs < CSPLIT; ++cs) {
       T acc[F::NACC];
 #pragma unroll
       for (int i = 0; i < F::NACC; ++i) acc[i] = 0;
       for (int j = s0; j < s1; ++j) {
-        const int2 bx = box[j];
         const T w = wsys[j];
-        row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, H + (long long)j * W * W, W, bx, tw, Cb, Tb, cs,
-                                   tid);
+        if constexpr (Pre::ON) {
+          row_pass_pre<T, M, NT, CSPLIT>(pre, W, tw, Cb, Tb, cs, tid);
+          // the next system's inputs load while this one runs its columns
+          const int jn = j + 1 < s1 ? j + 1 : (cs + 1 < CSPLIT ? s0 : -1);
+          if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1);
+        } else {
+          row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, H + (long long)j * W * W, W, box[j], tw, Cb,
+                                     Tb, cs, tid);
+        }
 #pragma unroll
         for (int rd = 0; rd < F::ROUNDS; ++rd) {
           {
@@ -327,6 +422,10 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
     cx<T>* gout = iv.gX;
     for (int e = tid; e < Wb * Wb; e += NT) gout[e] = cx<T>{0, 0};
     __syncthreads();
+    using Pre = RowPre<T, M, NT, CSPLIT>;
+    Pre pre;
+    if (Pre::ON && s0 < s1)
+      pre.fetch(Xm, Wb, wrapN, H + (long long)s0 * W * W, W, box[s0], rl1, j1r);
     for (int cs = 0; cs < CSPLIT; ++cs) {
       T gm[F::NACC];
 #pragma unroll
@@ -345,7 +444,13 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
         const int2 bx = box[j];
         const T w2 = (T)2 * wsys[j];
         const cx<T>* Hj = H + (long long)j * W * W;
-        row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, Hj, W, bx, tw, Cb, Tb, cs, tid);
+        if constexpr (Pre::ON) {
+          row_pass_pre<T, M, NT, CSPLIT>(pre, W, tw, Cb, Tb, cs, tid);
+          const int jn = j + 1 < s1 ? j + 1 : (cs + 1 < CSPLIT ? s0 : -1);
+          if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1r);
+        } else {
+          row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, Hj, W, bx, tw, Cb, Tb, cs, tid);
+        }
 #pragma unroll
         for (int rd = 0; rd < F::ROUNDS; ++rd) {
           const int xl = rd * F::CPR + c1;
