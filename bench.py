@@ -11,14 +11,19 @@ Also reported on the same line:
 * the plain single-image form (the same library called once per
   (mask, plane) with a host synchronisation after each image) and the
   batched / plain speed-up;
+* the dense per-source-point form (the reference's dataflow on the device:
+  one dense 1024^2 inverse transform per source point) and batched / dense;
 * ILT forward + backward steps/s at 1024^2 (theta -> sigmoid(8 theta) -> Abbe
   at NA 1.35 quadrupole -> relaxed resist L2 vs a Manhattan target);
+* the 16-clip vector TE ILT step (clips of 512/768/1024^2 in one persistent
+  launch per pass, graph-captured) against one clip per call;
 * the FP32 image error against the float64 oracle on a sample image;
-* roofline of the coherent-system kernel (HIP events on its stream);
+* roofline of the coherent-system kernel (HIP events on its stream; HBM
+  traffic from the committed counter record when it matches the shape);
 * the oracle (the reference algorithm in float64 NumPy/SciPy) timed on the
   host cores on a bounded sample, as the CPU baseline.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--legs batched,plain,...]
 """
 from __future__ import annotations
 
@@ -26,6 +31,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -81,6 +87,27 @@ def field_flops(M: int, W: int) -> float:
     (6 W^2) and the weighted |E|^2 accumulation (4 M^2).
     """
     return 5.0 * M * math.log2(M) * (W + M) + 6.0 * W * W + 4.0 * M * M
+
+
+COUNTERS = os.path.join(ROOT, "profiles", "k_field_fwd_abbe_1024_64x5_counters.md")
+
+
+def recorded_traffic(plan):
+    """HBM bytes per launch of the coherent-system kernel, as the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by a 1 GiB copy) recorded them, when that
+    record is for the kernel shape this run uses; else None."""
+    i = plan.info
+    shape = (f"M={i['m']} W={i['box']} threads={i['block_threads']} "
+             f"splits={i['column_splits']} chunk={i['chunk']}")
+    try:
+        with open(COUNTERS) as f:
+            text = f.read()
+    except OSError:
+        return None, shape
+    rec = dict(re.findall(r"^(traffic_bytes_per_launch|shape): (.+)$", text, re.M))
+    if rec.get("shape", "").strip() != shape or "traffic_bytes_per_launch" not in rec:
+        return None, shape
+    return int(rec["traffic_bytes_per_launch"]), shape
 
 
 def ratio(a, b):
@@ -293,6 +320,7 @@ def main():
     n_sys = plan.info["n_systems"] * n_img
     flops = n_sys * field_flops(M, W)
     achieved_tflops = flops / kern_s / 1e12
+    traffic, shape = recorded_traffic(plan)
 
     # ---------------- plain: one (mask, plane) per call, sync per image ---------
     plain_plans = [ImagingPlan(optics, grid, [pl], precision=32) for pl in planes]
@@ -381,7 +409,10 @@ def main():
         "fp32_rel_err_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-                     "traffic": None,
+                     "traffic": traffic,
+                     "traffic_source": ("profiles/k_field_fwd_abbe_1024_64x5_counters.md"
+                                        if traffic is not None else None),
+                     "kernel_shape": shape,
                      "kernel": "k_field_fwd (coherent systems, FP32 VALU radix FFTs)",
                      "kernel_ms": kern_s * 1e3, "flops_per_launch": flops},
         "cpu_baseline": cpu,

@@ -38,6 +38,7 @@ LB_PLAN(192, 16, 4, 3, 1)
 LB_PLAN(384, 16, 8, 3, 1)
 LB_PLAN(768, 16, 16, 3, 1)
 LB_PLAN(1536, 16, 16, 2, 3)
+LB_PLAN(80, 16, 5, 1, 1)
 #undef LB_PLAN
 
 constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
@@ -668,7 +669,8 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
 }
 
 #define LB_BAND_SIZES(X) \
-  X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(48) X(96) X(192) X(384) X(768) X(1536)
+  X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(48) X(96) X(192) X(384) X(768) \
+      X(1536) X(80)
 
 bool band_size_supported(int n) {
   switch (n) {

@@ -1,5 +1,5 @@
 // Device building blocks shared by every kernel of the imaging path:
-// a complex type, in-register radix-2/3/4/8/16 DFTs, and one Stockham
+// a complex type, in-register radix-2/3/4/5/8/10/12/16 DFTs, and one Stockham
 // autosort stage over LDS (Govindaraju et al., "High performance discrete
 // Fourier transforms on graphics processors", SC 2008).
 //
@@ -154,12 +154,43 @@ __device__ __forceinline__ cx<T> root12(int k) {
   return cx<T>{c, DIR > 0 ? s : -s};
 }
 
+// e^{DIR * 2 pi i k / R} for compile-time R | 10 (multiples of 36 degrees).
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cx<T> root10(int k) {
+  const int q = (k * (10 / R)) % 10;
+  const T c1 = (T)0.80901699437494742, s1 = (T)0.58778525229247313;  // 36 deg
+  const T c2 = (T)0.30901699437494742, s2 = (T)0.95105651629515357;  // 72 deg
+  T c, s;
+  switch (q) {
+    case 0: c = 1; s = 0; break;
+    case 1: c = c1; s = s1; break;
+    case 2: c = c2; s = s2; break;
+    case 3: c = -c2; s = s2; break;
+    case 4: c = -c1; s = s1; break;
+    case 5: c = -1; s = 0; break;
+    case 6: c = -c1; s = -s1; break;
+    case 7: c = -c2; s = -s2; break;
+    case 8: c = c2; s = -s2; break;
+    default: c = c1; s = -s1; break;
+  }
+  return cx<T>{c, DIR > 0 ? s : -s};
+}
+
+// Twiddle of a composite in-register DFT of size R (R | 12 or R | 10).
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cx<T> root_of(int k) {
+  if constexpr (12 % R == 0)
+    return root12<R, DIR, T>(k);
+  else
+    return root10<R, DIR, T>(k);
+}
+
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dft(cx<T>* v);
 
 // Composite R = A * B in registers (Cooley-Tukey): input n = B n1 + n2,
 // output k = k1 + A k2; A-point DFTs, twiddles e^{DIR 2 pi i n2 k1 / R},
-// then B-point DFTs.  Used for the 3 * 2^k sizes (R = 6, 12).
+// then B-point DFTs.  Used for R = 6, 12 (3 * 2^k) and R = 10.
 template <int A, int B, int DIR, typename T>
 __device__ __forceinline__ void dft_mixed(cx<T>* v) {
   constexpr int R = A * B;
@@ -171,7 +202,7 @@ __device__ __forceinline__ void dft_mixed(cx<T>* v) {
     dft<A, DIR>(y[n2]);
 #pragma unroll
     for (int k1 = 1; k1 < A; ++k1)
-      if (n2 > 0) y[n2][k1] = y[n2][k1] * root12<R, DIR, T>(n2 * k1);
+      if (n2 > 0) y[n2][k1] = y[n2][k1] * root_of<R, DIR, T>(n2 * k1);
   }
 #pragma unroll
   for (int k1 = 0; k1 < A; ++k1) {
@@ -190,6 +221,23 @@ __device__ __forceinline__ void dft(cx<T>* v) {
     dft_mixed<2, 3, DIR, T>(v);
   } else if constexpr (R == 12) {
     dft_mixed<4, 3, DIR, T>(v);
+  } else if constexpr (R == 10) {
+    dft_mixed<2, 5, DIR, T>(v);
+  } else if constexpr (R == 5) {
+    // X_k = sum_n v_n w^{nk}, w = e^{DIR 2 pi i / 5}; pairs (1,4), (2,3)
+    const T c1 = (T)0.30901699437494742, s1 = (T)0.95105651629515357;   // 72 deg
+    const T c2 = (T)-0.80901699437494742, s2 = (T)0.58778525229247313;  // 144 deg
+    const cx<T> p14 = v[1] + v[4], m14 = v[1] - v[4];
+    const cx<T> p23 = v[2] + v[3], m23 = v[2] - v[3];
+    const cx<T> a1 = v[0] + scale(p14, c1) + scale(p23, c2);
+    const cx<T> a2 = v[0] + scale(p14, c2) + scale(p23, c1);
+    const cx<T> b1 = mul_i<DIR>(scale(m14, s1) + scale(m23, s2));
+    const cx<T> b2 = mul_i<DIR>(scale(m14, s2) - scale(m23, s1));
+    v[0] = v[0] + p14 + p23;
+    v[1] = a1 + b1;
+    v[4] = a1 - b1;
+    v[2] = a2 + b2;
+    v[3] = a2 - b2;
   } else if constexpr (R == 3) {
     const T h = (T)0.86602540378443865;  // sqrt(3)/2
     cx<T> s = v[1] + v[2];

@@ -47,6 +47,10 @@ struct FieldRadix<48> {
   static constexpr int R1 = 12, R2 = 4;
 };
 template <>
+struct FieldRadix<80> {
+  static constexpr int R1 = 10, R2 = 8;
+};
+template <>
 struct FieldRadix<96> {
   static constexpr int R1 = 12, R2 = 8;
 };
@@ -559,7 +563,8 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
 // ---------------------------------------------------------------- dispatch
 #define LB_FIELD_CONFIGS(X)                                                                  \
   X(16, 64, 1) X(32, 128, 1) X(48, 64, 1) X(64, 256, 1) X(64, 128, 1) X(64, 256, 2)          \
-      X(96, 256, 1) X(96, 128, 1) X(96, 256, 3) X(128, 256, 1) X(128, 512, 1) X(128, 256, 2) \
+      X(80, 320, 1) X(80, 320, 2) X(80, 160, 1)                                              \
+      X(96, 256, 1) X(96, 128, 1) X(96, 256, 3) X(96, 384, 1) X(128, 256, 1) X(128, 512, 1) X(128, 256, 2) \
       X(128, 256, 4) X(256, 256, 2) X(256, 256, 4) X(256, 256, 8)
 
 const std::vector<FieldConfig>& field_configs() {

@@ -118,6 +118,23 @@ def test_batched_masks_and_planes_equal_single_calls(device):
             assert _rel(batch[b, p], one) <= 1e-6
 
 
+def test_boxes_tightened_to_pupil_support(device):
+    """NA 0.93 at 1024 x 4 nm: pupil radius 19.74 bins covers at most 40 bins per axis, so the
+    boxes shrink from floor(2r) + 3 = 42 to 40 (coarse grid 96; 80 on request)."""
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.plan import ImagingPlan
+    optics = OpticsConfig(NA=0.93, source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55,
+                          source_grid=47, normalisation="none")
+    plan = ImagingPlan(optics, GridConfig(n_pixels=1024, pixel_size=4e-9), precision=32)
+    assert plan.info["box"] == 40 and plan.info["m"] == 96
+    small = ImagingPlan(optics, GridConfig(n_pixels=1024, pixel_size=4e-9), precision=32,
+                        coarse_m=80)
+    assert small.info["box"] == 40 and small.info["m"] == 80
+    forced = ImagingPlan(optics, GridConfig(n_pixels=1024, pixel_size=4e-9), precision=32,
+                         coarse_m=128)
+    assert forced.info["box"] == 40 and forced.info["m"] == 128
+
+
 @pytest.mark.parametrize("precision", [32, 64])
 def test_1024_quadrupole_defocus_matches_oracle(device, precision):
     """The batched-benchmark geometry at one mask and one plane."""
@@ -132,6 +149,14 @@ def test_1024_quadrupole_defocus_matches_oracle(device, precision):
     raw = compute_aerial_image(mask, optics, grid, return_raw=True, precision=precision)[0]
     tol = 1e-12 if precision == 64 else 1e-5
     assert _rel(raw, ref) <= tol, _rel(raw, ref)
+    from lithobat_amd.plan import ImagingPlan
+    for m in (80, 128):  # the same image on the radix-5 and on a larger coarse grid
+        plan = ImagingPlan(optics, grid, precision=precision, coarse_m=m)
+        import torch
+        t = torch.tensor(mask, device=device,
+                         dtype=torch.float64 if precision == 64 else torch.float32)
+        img = plan.forward(t)[0][0, 0].double().cpu().numpy()
+        assert _rel(img, ref) <= tol, (m, _rel(img, ref))
 
 
 @pytest.mark.parametrize("precision", [64, 32])

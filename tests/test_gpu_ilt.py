@@ -16,6 +16,10 @@ GRAD_CASES = {
                                 sigma_outer=0.85, sigma_inner=0.55, source_grid=11,
                                 defocus=30e-9, zernike_coeffs={8: 0.02}), 64, 4e-9),
     "scalar_direct": (dict(NA=0.93, sigma_outer=0.8, source_grid=7), 32, 8e-9),
+    # pupil radius 19.7 bins: 40-bin boxes (coarse grid 96; 80 in the forced case below)
+    "scalar_quad_w40": (dict(NA=0.93, source_type="quadrupole", sigma_outer=0.85,
+                             sigma_inner=0.55, source_grid=15, defocus=-40e-9,
+                             zernike_coeffs={7: 0.02}), 512, 8e-9),
     "vector_te": (dict(NA=1.35, n_immersion=1.44, imaging_model="vector", polarisation="te",
                        source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55,
                        source_grid=9, defocus=-50e-9, zernike_coeffs={5: 0.02, 9: -0.01}),
@@ -41,6 +45,28 @@ def test_backward_matches_oracle_adjoint(device, name, precision):
     sens = rng.standard_normal((n, n))
     ref = O.abbe_gradient(mask, optics, grid, sens)
     plan = ImagingPlan(optics, grid, precision=precision)
+    dt = torch.float64 if precision == 64 else torch.float32
+    g = plan.backward(torch.tensor(mask[None], device=device, dtype=dt),
+                      torch.tensor(sens[None, None], device=device, dtype=dt))
+    tol = 1e-10 if precision == 64 else 2e-5
+    assert _rel(g[0].cpu().numpy(), ref) <= tol, _rel(g[0].cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_backward_on_radix5_coarse_grid(device, precision):
+    """The adjoint on the 80-point (radix 10 x 8) coarse grid equals the oracle's."""
+    import torch
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.plan import ImagingPlan
+    kw, n, px = GRAD_CASES["scalar_quad_w40"]
+    optics = OpticsConfig(**kw)
+    grid = GridConfig(n_pixels=n, pixel_size=px)
+    rng = np.random.default_rng(8)
+    mask = rng.random((n, n))
+    sens = rng.standard_normal((n, n))
+    ref = O.abbe_gradient(mask, optics, grid, sens)
+    plan = ImagingPlan(optics, grid, precision=precision, coarse_m=80)
+    assert plan.info["m"] == 80
     dt = torch.float64 if precision == 64 else torch.float32
     g = plan.backward(torch.tensor(mask[None], device=device, dtype=dt),
                       torch.tensor(sens[None, None], device=device, dtype=dt))
