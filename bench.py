@@ -17,6 +17,8 @@ Also reported on the same line:
   at NA 1.35 quadrupole -> relaxed resist L2 vs a Manhattan target);
 * the 16-clip vector TE ILT step (clips of 512/768/1024^2 in one persistent
   launch per pass, graph-captured) against one clip per call;
+* Hopkins / SOCS: the FP64 device decomposition (K = 24) and 256 masks at
+  512^2 imaged through it, with the truncation error against full Abbe;
 * the FP32 image error against the float64 oracle on a sample image;
 * roofline of the coherent-system kernel (HIP events on its stream; HBM
   traffic from the committed counter record when it matches the shape);
@@ -134,8 +136,14 @@ def cpu_baseline(mask, optics, grid, plane):
     return raw[0], dt
 
 
+ILT_BATCH = 16
+
+
 def ilt_leg(args, ilt, dev, sync):
-    """Seconds per ILT forward + backward step at 1024^2 (autograd, eager)."""
+    """ILT forward + backward at 1024^2 (autograd, eager): seconds per step for one mask,
+    per step of a batch of ILT_BATCH masks (the step count is ILT_BATCH), and per step of
+    the dense per-source form (one dense fft2 / ifft2 per source point, host sync per
+    step)."""
     import torch
 
     from lithobat_amd.ilt import abbe_intensity, mask_of, relaxed_resist_loss
@@ -152,7 +160,32 @@ def ilt_leg(args, ilt, dev, sync):
         loss = relaxed_resist_loss(raw, target, scale, 0.3, 50.0)
         loss.backward()
 
-    return timed(ilt_step, args.ilt_steps, 2, sync)
+    one = timed(ilt_step, args.ilt_steps, 2, sync)
+    thetas = torch.tensor(np.random.default_rng(3100).standard_normal((ILT_BATCH, N, N)),
+                          device=dev, dtype=torch.float32, requires_grad=True)
+    targets = torch.tensor(np.stack([manhattan(N, PIXEL, seed=3200 + b)
+                                     for b in range(ILT_BATCH)]), device=dev,
+                           dtype=torch.float32)
+
+    def batch_step():
+        thetas.grad = None
+        raw = abbe_intensity(mask_of(thetas, 8.0), ilt)[:, 0]
+        loss = relaxed_resist_loss(raw, targets, scale, 0.3, 50.0)
+        loss.backward()
+
+    batched = timed(batch_step, max(2, args.ilt_steps // 2), 1, sync) / ILT_BATCH
+    th = theta.detach()[0]
+
+    def dense_step():
+        m = torch.sigmoid(torch.clamp(8.0 * th, -60.0, 60.0))
+        raw = ilt.forward_dense(m, batch=8).requires_grad_(True)
+        loss = relaxed_resist_loss(raw[None], target, scale, 0.3, 50.0)
+        loss.backward()
+        gm = ilt.backward_dense(m, raw.grad, batch=8)
+        (gm * 8.0 * m * (1.0 - m)).sum().item()  # host sync, as a per-step loop has
+
+    dense = timed(dense_step, max(1, args.dense_images), 1, sync)
+    return one, batched, dense
 
 
 def clips_leg(args, dev, sync):
@@ -196,6 +229,45 @@ def clips_leg(args, dev, sync):
     return clip_ilt_s, timed(plain_clips, 2, 1, sync)
 
 
+def socs_leg(args, dev, sync):
+    """Hopkins / SOCS at 512^2, 8 nm: FP64 TCC decomposition on the device (Gram route,
+    Jacobi) truncated to 24 kernels, then 256 masks (seeds 2000-2255) imaged through the
+    kernels in complex64; error of the K = 24 image against the full Abbe sum (FP32
+    device path, itself within 1e-5 of the float64 oracle) on mask 2000."""
+    import torch
+
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.hopkins import SOCSKernels
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=512, pixel_size=8e-9)
+    optics = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="annular",
+                          sigma_outer=0.9, sigma_inner=0.6, source_grid=21,
+                          normalisation="none")
+    sync()
+    t0 = time.perf_counter()
+    k24 = SOCSKernels.build(optics, grid, tol=0.0, max_kernels=24, precision=32)
+    sync()
+    build_s = time.perf_counter() - t0
+    masks = torch.tensor(np.stack([manhattan(512, 8e-9, seed=s) for s in range(2000, 2256)]),
+                         device=dev, dtype=torch.float32)
+    plan = k24.plan
+
+    def step():
+        plan.forward(masks, with_peak=False)
+
+    sec = timed(step, max(2, args.steps // 2), 1, sync)
+    abbe = ImagingPlan(optics, grid, precision=32)
+    ref = abbe.forward(masks[:1])[0][0, 0]
+    img = plan.forward(masks[:1])[0][0, 0]
+    err = float((img - ref).abs().max() / ref.abs().max())
+    return {"images_per_s": 256 / sec, "build_s": build_s, "kernels": len(k24),
+            "n_source": k24.n_source, "n_band": k24.n_band,
+            "energy_kept": float(k24.energy_kept), "rel_err_vs_abbe": err,
+            "config": "256 masks 512^2 @ 8 nm, NA 1.35 annular 0.6-0.9 (142 points), "
+                      "K = 24 of the FP64 Gram/Jacobi decomposition, complex64 imaging"}
+
+
 def dense_leg(args, plan, masks, sync):
     """Images/s of the reference's dataflow on the device: one dense N x N inverse
     transform per source point (8 per launch), host synchronisation per image."""
@@ -222,7 +294,7 @@ def main():
     ap.add_argument("--ilt-steps", type=int, default=10)
     ap.add_argument("--dense-images", type=int, default=3,
                     help="images timed for the dense per-source-point form")
-    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,cpu",
+    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,cpu",
                     help="comma list of measurements to run (batched always runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the legs, copy a MIB-sized buffer once (a plain copy kernel of "
@@ -341,13 +413,15 @@ def main():
         plain_images_per_s = k / (time.perf_counter() - t0)
 
     # ---------------- ILT forward + backward at 1024^2 ---------------------------
-    ilt_s = clip_ilt_s = clip_plain_s = dense_images_per_s = None
+    ilt_s = ilt_batched_s = ilt_dense_s = None
+    clip_ilt_s = clip_plain_s = dense_images_per_s = None
     if "ilt" in legs:
-        ilt_s = ilt_leg(args, ilt, dev, sync)
+        ilt_s, ilt_batched_s, ilt_dense_s = ilt_leg(args, ilt, dev, sync)
     if "clips" in legs:
         clip_ilt_s, clip_plain_s = clips_leg(args, dev, sync)
     if "dense" in legs:
         dense_images_per_s = dense_leg(args, plain_plans[0], masks, sync)
+    socs = socs_leg(args, dev, sync) if "socs" in legs else None
     if args.copy_calibration > 0:
         nb = args.copy_calibration << 20
         a = torch.empty(nb, dtype=torch.uint8, device=dev).fill_(1)
@@ -404,8 +478,15 @@ def main():
                            "launch per pass, graph-captured step; plain = one clip per call, "
                            "eager, sync per clip",
         "ilt_steps_per_s": ratio(1.0, ilt_s),
+        "ilt_batched_steps_per_s": ratio(1.0, ilt_batched_s),
+        "ilt_plain_dense_steps_per_s": ratio(1.0, ilt_dense_s),
+        "ilt_batched_vs_plain_dense": ratio(ilt_dense_s, ilt_batched_s),
+        "ilt_single_vs_plain_dense": ratio(ilt_dense_s, ilt_s),
         "ilt_config": "1024^2 @ 4 nm, NA 1.35 quadrupole (148 pts), sigmoid(8 theta), "
-                      "threshold 0.3 steepness 50, FP32 forward + adjoint",
+                      "threshold 0.3 steepness 50, FP32 forward + adjoint; batched = "
+                      f"{ILT_BATCH} masks per call (steps/s counts masks); plain dense = "
+                      "one dense fft2/ifft2 pair per source point, host sync per step",
+        "socs": socs,
         "fp32_rel_err_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,

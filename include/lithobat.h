@@ -157,6 +157,14 @@ size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks);
 size_t lb_dense_workspace_bytes(lb_plan plan, int32_t batch);
 lb_status lb_forward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,
                            void* image, void* workspace, size_t workspace_bytes, void* stream);
+/* The matching dense adjoint (Abbe plans, real masks): d_mask = d/dmask of
+ * sum(d_image * I) for one mask and plane, per coherent system as the
+ * reference's adjoint runs it (opc/ilt.py:235-266: fft2 of the sensitivity
+ * times the field, conj(pupil), one inverse transform at the end).  Same
+ * result as lb_backward; same workspace size as lb_forward_dense. */
+lb_status lb_backward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,
+                            const void* d_image, void* d_mask, void* workspace,
+                            size_t workspace_bytes, void* stream);
 
 /* Batches of clips (no direct counterpart in the reference, which images one
  * mask per call of compute_aerial_image / runs one run_ilt per clip): plans of

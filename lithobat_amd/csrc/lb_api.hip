@@ -1223,6 +1223,7 @@ size_t lb_dense_workspace_bytes(lb_plan plan, int32_t batch) {
   L.take(p.csz() * (size_t)batch * p.N * p.N);  // dense spectra
   L.take(p.csz() * (size_t)batch * p.N * p.N);  // synthesis scratch
   L.take(p.csz() * (size_t)batch * p.N * p.N);  // fields
+  L.take(p.csz() * (size_t)p.N * p.N);      // adjoint: spectrum-gradient accumulator
   return L.off;
 }
 
@@ -1252,7 +1253,59 @@ static lb_status dense_t(Plan& p, const void* mask, int plane, int batch, void* 
   return LB_OK;
 }
 
+// The dense adjoint, per source point as the reference's adjoint computes it
+// (opc/ilt.py:235-266 restated for Abbe systems): E_s on the whole grid,
+// G_s = fft2(g E_s), A += w_s conj(H_s) G_s, d mask = 2 Re ifft2(A).
+template <typename T>
+static lb_status dense_bwd_t(Plan& p, const void* mask, int plane, int batch, const void* dimg,
+                             void* dmask, char* ws, hipStream_t st) {
+  const int N = p.N;
+  const long long NN = (long long)N * N;
+  Layout L;
+  cx<T>* Xf = (cx<T>*)(ws + L.take(p.csz() * (size_t)NN));
+  cx<T>* RT = (cx<T>*)(ws + L.take(p.csz() * (size_t)NN));
+  cx<T>* spec = (cx<T>*)(ws + L.take(p.csz() * (size_t)batch * NN));
+  cx<T>* TT = (cx<T>*)(ws + L.take(p.csz() * (size_t)batch * NN));
+  cx<T>* E = (cx<T>*)(ws + L.take(p.csz() * (size_t)batch * NN));
+  cx<T>* A = (cx<T>*)(ws + L.take(p.csz() * (size_t)NN));
+  const T invN2 = (T)(1.0 / ((double)N * (double)N));
+  LB_HIP(band_analysis<T>(N, mask, 0, NN, 0, 1, 1, -(N / 2), N, invN2, RT, Xf, st));
+  LB_HIP(fill<T>((T*)A, (T)0, 2 * NN, st));
+  const int s_begin = plane * p.spp, s_end = (plane + 1) * p.spp;
+  for (int j0 = s_begin; j0 < s_end; j0 += batch) {
+    const int ns = std::min(batch, s_end - j0);
+    LB_HIP(dense_spectra<T>(Xf, N, (const cx<T>*)p.H, p.W, p.box, j0, ns, p.nu0m, spec, st));
+    LB_HIP(band_synthesis<T>(N, spec, ns, -(N / 2), N, 0, (T)1, 1, TT, (T*)E, (T*)nullptr, st));
+    LB_HIP(mul_real<T>(E, ns, NN, (const T*)dimg, st));
+    LB_HIP(band_analysis<T>(N, E, 1, 2 * NN, 0, 1, ns, -(N / 2), N, (T)1, TT, spec, st));
+    LB_HIP(dense_adjoint_accum<T>(spec, N, (const cx<T>*)p.H, p.W, p.box, (const T*)p.wsys, j0,
+                                  ns, p.nu0m, A, st));
+  }
+  LB_HIP(band_synthesis<T>(N, A, 1, -(N / 2), N, 1, (T)2 * invN2, 0, TT, (T*)dmask, (T*)nullptr,
+                           st));
+  return LB_OK;
+}
+
 extern "C" {
+
+lb_status lb_backward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,
+                            const void* d_image, void* d_mask, void* workspace, size_t ws_bytes,
+                            void* stream) {
+  g_err.clear();
+  if (!plan || !mask || !d_image || !d_mask) return fail(LB_ERR_ARG, "null argument");
+  Plan& p = plan->p;
+  if (p.socs) return fail(LB_ERR_UNSUPPORTED, "the dense adjoint is for Abbe plans");
+  if (plane < 0 || plane >= p.n_planes) return fail(LB_ERR_ARG, "plane out of range");
+  if (batch < 1) return fail(LB_ERR_ARG, "batch must be >= 1");
+  const size_t need = lb_dense_workspace_bytes(plan, batch);
+  if (!workspace || ws_bytes < need)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64 ? dense_bwd_t<double>(p, mask, plane, batch, d_image, d_mask,
+                                                 (char*)workspace, st)
+                           : dense_bwd_t<float>(p, mask, plane, batch, d_image, d_mask,
+                                                (char*)workspace, st);
+}
 
 lb_status lb_forward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,
                            void* image, void* workspace, size_t ws_bytes, void* stream) {

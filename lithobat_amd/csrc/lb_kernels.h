@@ -127,6 +127,12 @@ hipError_t dense_spectra(const cx<T>* X, int N, const cx<T>* H, int W, const int
 template <typename T>
 hipError_t accum_norm2(const cx<T>* E, int nsys, long long len, const T* w, int j0, T* img,
                        hipStream_t st);
+// E_k *= g (real) for a batch of fields; A += w conj(H) G over each box (atomic)
+template <typename T>
+hipError_t mul_real(cx<T>* E, int nsys, long long len, const T* g, hipStream_t st);
+template <typename T>
+hipError_t dense_adjoint_accum(const cx<T>* G, int N, const cx<T>* H, int W, const int2* box,
+                               const T* w, int j0, int nsys, int nu0m, cx<T>* A, hipStream_t st);
 
 // ---- pupil tables (lb_pupil.hip) ------------------------------------------
 struct PupilSys {

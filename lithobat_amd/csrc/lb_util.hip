@@ -270,6 +270,66 @@ __global__ void k_accum_norm2(const cx<T>* __restrict__ E, int nsys, long long l
   }
 }
 
+// E_k[e] *= g[e] for the nsys fields of a batch (the sensitivity times the field).
+template <typename T>
+__global__ void k_mul_real(cx<T>* __restrict__ E, int nsys, long long len,
+                           const T* __restrict__ g) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+       e += (long long)gridDim.x * blockDim.x) {
+    const T gv = g[e];
+    for (int k = 0; k < nsys; ++k) E[(long long)k * len + e] = scale(E[(long long)k * len + e], gv);
+  }
+}
+
+// A[f] += w_j conj(H_j[f]) G_k[f] over each system's box (whole-grid band
+// layout, as k_dense_spectra); boxes of one batch overlap, so the sums are
+// atomic.  Used only by the dense comparison adjoint.
+template <typename T>
+__global__ void k_dense_adjoint_accum(const cx<T>* __restrict__ G, int N,
+                                      const cx<T>* __restrict__ H, int W,
+                                      const int2* __restrict__ box, const T* __restrict__ w,
+                                      int j0, int nsys, int nu0m, cx<T>* __restrict__ A) {
+  const int k = blockIdx.y;
+  if (k >= nsys) return;
+  const int j = j0 + k;
+  const int2 b = box[j];
+  const T wj = w[j];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < W * W; e += gridDim.x * blockDim.x) {
+    const int a = e / W, c = e - (e / W) * W;
+    const int uy = (((b.x + a + nu0m) + N / 2) % N + N) % N;
+    const int ux = (((b.y + c + nu0m) + N / 2) % N + N) % N;
+    const long long f = (long long)uy * N + ux;
+    const cx<T> v = scale(mulc(G[(long long)k * N * N + f], H[(long long)j * W * W + e]), wj);
+    atomicAdd(&A[f].re, v.re);
+    atomicAdd(&A[f].im, v.im);
+  }
+}
+
+template <typename T>
+hipError_t mul_real(cx<T>* E, int nsys, long long len, const T* g, hipStream_t st) {
+  long long b = (len + 255) / 256;
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_mul_real<T>, dim3((unsigned)b), dim3(256), 0, st, E, nsys, len, g);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t dense_adjoint_accum(const cx<T>* G, int N, const cx<T>* H, int W, const int2* box,
+                               const T* w, int j0, int nsys, int nu0m, cx<T>* A, hipStream_t st) {
+  hipLaunchKernelGGL(k_dense_adjoint_accum<T>, dim3((W * W + 255) / 256, nsys), dim3(256), 0, st,
+                     G, N, H, W, box, w, j0, nsys, nu0m, A);
+  return hipGetLastError();
+}
+
+template hipError_t mul_real<float>(cx<float>*, int, long long, const float*, hipStream_t);
+template hipError_t mul_real<double>(cx<double>*, int, long long, const double*, hipStream_t);
+template hipError_t dense_adjoint_accum<float>(const cx<float>*, int, const cx<float>*, int,
+                                               const int2*, const float*, int, int, int,
+                                               cx<float>*, hipStream_t);
+template hipError_t dense_adjoint_accum<double>(const cx<double>*, int, const cx<double>*, int,
+                                                const int2*, const double*, int, int, int,
+                                                cx<double>*, hipStream_t);
+
 template <typename T>
 hipError_t dense_spectra(const cx<T>* X, int N, const cx<T>* H, int W, const int2* box, int j0,
                          int nsys, int nu0m, cx<T>* spec, hipStream_t st) {

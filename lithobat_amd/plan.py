@@ -111,9 +111,6 @@ class ImagingPlan:
         self._ws = None
         self._lock = threading.Lock()
         self._tuned_for = set()
-        info = _lib.PlanInfo()
-        _lib.check(lib.lb_plan_info_get(self._h, C.byref(info)))
-        self.info = {k: getattr(info, k) for k, _ in _lib.PlanInfo._fields_}
         clear = np.zeros(len(self.planes))
         _lib.check(lib.lb_plan_clear(self._h, _dptr(clear, C.c_double)))
         self.clear = clear
@@ -152,6 +149,13 @@ class ImagingPlan:
         if self._ws is None or self._ws.numel() < nbytes or self._ws.device != device:
             self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
         return self._ws
+
+    @property
+    def info(self) -> dict:
+        """Geometry and the current coherent-system kernel shape (lb_plan_info_get)."""
+        info = _lib.PlanInfo()
+        _lib.check(self._lib.lb_plan_info_get(self._h, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in _lib.PlanInfo._fields_}
 
     def _use_tuning(self, n_masks: int) -> None:
         """Select the autotuned kernel shapes recorded for this batch size, once."""
@@ -238,6 +242,24 @@ class ImagingPlan:
             st = torch.cuda.current_stream(m.device).cuda_stream
             _lib.check(self._lib.lb_forward_dense(self._h, m.data_ptr(), plane, batch,
                                                   out.data_ptr(), ws.data_ptr(), ws.numel(), st))
+        return out
+
+    def backward_dense(self, mask: torch.Tensor, d_image: torch.Tensor, plane: int = 0,
+                       batch: int = 8) -> torch.Tensor:
+        """The matching dense adjoint (one dense fft2 / ifft2 pair per coherent system);
+        for comparison -- same gradient as ``backward``."""
+        m, cplx = self._check_masks(mask)
+        if cplx or m.shape[0] != 1:
+            raise ValueError("backward_dense takes one real mask")
+        g = d_image.to(self.real_dtype).reshape(self.n, self.n).contiguous()
+        out = torch.empty((self.n, self.n), dtype=self.real_dtype, device=m.device)
+        nb = int(self._lib.lb_dense_workspace_bytes(self._h, batch))
+        with self._lock:
+            ws = self._workspace(nb, m.device)
+            st = torch.cuda.current_stream(m.device).cuda_stream
+            _lib.check(self._lib.lb_backward_dense(self._h, m.data_ptr(), plane, batch,
+                                                   g.data_ptr(), out.data_ptr(), ws.data_ptr(),
+                                                   ws.numel(), st))
         return out
 
     def profile(self, on: bool) -> None:

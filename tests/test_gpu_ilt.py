@@ -74,6 +74,31 @@ def test_backward_on_radix5_coarse_grid(device, precision):
     assert _rel(g[0].cpu().numpy(), ref) <= tol, _rel(g[0].cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_dense_adjoint_equals_coarse_adjoint_and_oracle(device, precision):
+    """The per-system dense adjoint (the reference's dataflow) gives the same gradient."""
+    import torch
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.plan import ImagingPlan
+    optics = OpticsConfig(NA=1.35, n_immersion=1.44, source_type="quadrupole", sigma_outer=0.85,
+                          sigma_inner=0.55, source_grid=11, defocus=25e-9,
+                          zernike_coeffs={6: 0.02})
+    grid = GridConfig(n_pixels=128, pixel_size=4e-9)
+    rng = np.random.default_rng(17)
+    mask = rng.random((128, 128))
+    sens = rng.standard_normal((128, 128))
+    ref = O.abbe_gradient(mask, optics, grid, sens)
+    plan = ImagingPlan(optics, grid, precision=precision)
+    dt = torch.float64 if precision == 64 else torch.float32
+    m = torch.tensor(mask, device=device, dtype=dt)
+    g = torch.tensor(sens, device=device, dtype=dt)
+    dense = plan.backward_dense(m, g, batch=3).double().cpu().numpy()
+    coarse = plan.backward(m[None], g[None, None])[0].double().cpu().numpy()
+    tol = 1e-10 if precision == 64 else 2e-5
+    assert _rel(dense, ref) <= tol, _rel(dense, ref)
+    assert _rel(coarse, ref) <= tol, _rel(coarse, ref)
+
+
 def test_autograd_ilt_step_vs_central_differences(device):
     """sigmoid(8 theta) -> Abbe (FP64 on the device) -> relaxed resist L2; FD on 16 pixels."""
     import torch
