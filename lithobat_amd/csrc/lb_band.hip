@@ -320,6 +320,144 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_analysis_rows(cons
   }
 }
 
+// Waves per SIMD that the LDS of the row kernels allows, at most 4 (the
+// register budget is held to it, so registers never cut the residency).
+template <typename T, int N>
+constexpr int row_waves_per_simd() {
+  constexpr int cap = (160 * 1024) / (int)sizeof(cx<T>);
+  constexpr int per_block = padded<T, N>() * (1 + row_waves<T, N>() * tpw<N>());
+  constexpr int w = (cap / per_block) * row_waves<T, N>() / 4;
+  return w < 1 ? 1 : (w > 4 ? 4 : w);
+}
+
+// Row pass of the analysis for one input slice, persistent per wavefront:
+// each wavefront loops over transform units (TPW rows, or row pairs of a
+// real plane), loading the next unit's inputs while it transforms the
+// current one, and extracts its own units' band columns (no block barrier
+// after the roots table).
+// Wavefronts per block of the persistent analysis rows (at most 4), and the
+// blocks per CU the LDS allows, at most 3 (the register budget is held to
+// it; more would spill).
+template <typename T, int N>
+constexpr int ap_waves() {
+  return row_waves<T, N>() < 4 ? row_waves<T, N>() : 4;
+}
+template <typename T, int N>
+constexpr int ap_blocks_per_cu() {
+  constexpr int cap = (160 * 1024) / (int)sizeof(cx<T>);
+  constexpr int b = cap / (padded<T, N>() * (1 + ap_waves<T, N>() * tpw<N>()));
+  return b < 1 ? 1 : (b > 3 ? 3 : b);
+}
+template <typename T, int N>
+static size_t ap_smem_bytes() {
+  return sizeof(cx<T>) * (size_t)(padded<T, N>() * (1 + ap_waves<T, N>() * tpw<N>()));
+}
+
+template <typename T, int N, bool CPLX>
+__global__ __launch_bounds__((64 * ap_waves<T, N>()),
+                             (ap_blocks_per_cu<T, N>() * ap_waves<T, N>() / 4 < 1
+                                  ? 1
+                                  : ap_blocks_per_cu<T, N>() * ap_waves<T, N>() /
+                                        4)) void k_analysis_rows_p(
+    const T* __restrict__ in, long long img_stride, int nu0, int Wb,
+    const cx<T>* __restrict__ twg, cx<T>* __restrict__ RT) {
+  constexpr int TPW = tpw<N>();
+  constexpr int NW = ap_waves<T, N>();
+  constexpr int PL = padded<T, N>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  cx<T>* wb = tw + PL + wave * TPW * PL;
+  const int img = blockIdx.y;
+  load_roots_padded(tw, twg, N, tid, 64 * NW);
+  __syncthreads();
+  constexpr int R = Plan<N>::R0, S = N / R;
+  constexpr int NTR = CPLX ? N : N / 2;
+  constexpr int NU = (NTR + TPW - 1) / TPW;
+  constexpr int TASKS = TPW * S, TPL = (TASKS + 63) / 64;
+  constexpr int ROWS_U = CPLX ? TPW : 2 * TPW;  // image rows per unit
+  const T* base = in + (long long)img * img_stride;
+  int trs[TPL], jj[TPL];
+#pragma unroll
+  for (int i = 0; i < TPL; ++i) {
+    const int t = lane + 64 * i;
+    trs[i] = t < TASKS ? t / S : TPW;
+    jj[i] = t - (t / S) * S;
+  }
+  // the unit's inputs x = j + r S: real part / imaginary part (complex plane)
+  // or even row / odd row (real plane, packed as even + i odd)
+  T ra[TPL][R], rb[TPL][R];
+  auto fetch = [&](int u) {
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      const int p = u * TPW + trs[i];
+      const bool ok = trs[i] < TPW && p < NTR;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int x = jj[i] + r * S;
+        if (CPLX) {
+          const cx<T> c = ok ? reinterpret_cast<const cx<T>*>(base)[(long long)p * N + x]
+                             : cx<T>{0, 0};
+          ra[i][r] = c.re;
+          rb[i][r] = c.im;
+        } else {
+          const long long off = (long long)(2 * p) * N + x;
+          ra[i][r] = ok ? base[off] : (T)0;
+          rb[i][r] = ok ? base[off + N] : (T)0;
+        }
+      }
+    }
+  };
+  const int stride = gridDim.x * NW;
+  int u = blockIdx.x * NW + wave;
+  if (u < NU) fetch(u);
+  cx<T>* out = RT + (long long)img * Wb * N;
+  for (; u < NU; u += stride) {
+    cx<T> v[TPL][R];
+#pragma unroll
+    for (int i = 0; i < TPL; ++i)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = cx<T>{ra[i][r], rb[i][r]};
+    if (u + stride < NU) fetch(u + stride);
+    const int ntr = min(TPW, NTR - u * TPW);
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      if (trs[i] < ntr) {
+        dft<R, -1>(v[i]);
+        cx<T>* w = wb + trs[i] * PL;
+#pragma unroll
+        for (int k = 0; k < R; ++k) w[pidx<T>(jj[i] * R + k)] = v[i][k];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    auto store = [&](int tr, int x, cx<T> val) { wb[tr * PL + pidx<T>(x)] = val; };
+    wave_fft_tail<T, -1, N, TPW, PL>(wb, tw, lane, ntr, store);
+    // band columns of this unit's rows, RT[img][y][v]
+    const int y0 = u * ROWS_U;
+    const int nrows = min(ROWS_U, N - y0);
+    for (int t = lane; t < Wb * nrows; t += 64) {
+      const int yy = t / Wb, vv = t - yy * Wb;
+      const int nu = nu0 + vv;
+      cx<T> r;
+      if (CPLX) {
+        r = wb[yy * PL + pidx<T>(bin_of(nu, N))];
+      } else {
+        const int tr = yy >> 1;
+        const cx<T> a = wb[tr * PL + pidx<T>(bin_of(nu, N))];
+        const cx<T> b = conj(wb[tr * PL + pidx<T>(bin_of(-nu, N))]);
+        if ((yy & 1) == 0) {
+          r = scale(a + b, (T)0.5);  // transform of the even row
+        } else {
+          const cx<T> d = a - b;     // (a - b) / (2i)
+          r = cx<T>{d.im * (T)0.5, -d.re * (T)0.5};
+        }
+      }
+      out[(long long)(y0 + yy) * Wb + vv] = r;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // Pass 2: one forward transform per band column.  The block loads CPB
 // columns of RT[img][y][v] (contiguous runs of a row) into padded LDS; only
 // the band rows of each result are stored, X[img][u][v] = s * Z[bin(nu0 + u)].
@@ -452,16 +590,6 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
     for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
     if (lane == 0) atomic_max_nonneg(peak + img, lmax);
   }
-}
-
-// Waves per SIMD that the LDS of the row kernels allows, at most 4 (the
-// register budget is held to it, so registers never cut the residency).
-template <typename T, int N>
-constexpr int row_waves_per_simd() {
-  constexpr int cap = (160 * 1024) / (int)sizeof(cx<T>);
-  constexpr int per_block = padded<T, N>() * (1 + row_waves<T, N>() * tpw<N>());
-  constexpr int w = (cap / per_block) * row_waves<T, N>() / 4;
-  return w < 1 ? 1 : (w > 4 ? 4 : w);
 }
 
 // Row pass for a band inside |nu| < S = N / R0 (the usual case): only two
@@ -600,7 +728,24 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
   const cx<T>* twg = nullptr;
   hipError_t e = table_for<T>(N, &twg);
   if (e != hipSuccess) return e;
-  if (cplx) {
+  bool done = false;
+  if constexpr (Plan<N>::R1 > 1) {
+    if (nsum == 1) {
+      const int units = ((cplx ? N : N / 2) + tpw<N>() - 1) / tpw<N>();
+      const int nw = ap_waves<T, N>();
+      const int upw = (int)std::min<long long>(
+          kUnitsPerWave, std::max<long long>(1, (long long)units * nimg / (nw * kRowWavesTarget)));
+      const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
+      auto k = cplx ? k_analysis_rows_p<T, N, true> : k_analysis_rows_p<T, N, false>;
+      const size_t smp = ap_smem_bytes<T, N>();
+      if ((e = set_smem(k, smp)) != hipSuccess) return e;
+      hipLaunchKernelGGL(k, dim3(gx, nimg), dim3(64 * nw), smp, st, (const T*)in, img_stride,
+                         nu0, Wb, twg, RT);
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (cplx) {
     auto k = k_analysis_rows<T, N, true>;
     if ((e = set_smem(k, sm)) != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(blocks_for(N, TPB), nimg), dim3(NT), sm, st,

@@ -79,6 +79,8 @@ struct FieldShape {
   static constexpr int Q2 = (NT2 + NT - 1) / NT;   // per thread (the last pass may idle)
   static constexpr int NACC = ROUNDS * Q2 * R2;
   static constexpr int LDC = M + 1;  // padded line stride of the round buffer
+  // stage-1 slots r >= NRW address box rows / columns >= (M+1)/2 >= W: always zero
+  static constexpr int NRW = ((M + 1) / 2 + T1 - 1) / T1 < R1 ? ((M + 1) / 2 + T1 - 1) / T1 : R1;
   static_assert(T2 == R1, "two-stage plan");
   static_assert(CPR * T1 == NT, "stage 1 maps one task per thread");
   static_assert(CPR <= MC && MC % CPR == 0, "rounds tile the split");
@@ -107,7 +109,7 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
 #pragma unroll
         for (int r = 0; r < F::R1; ++r) {
           const int b = j1 + r * F::T1;
-          if (b < W) {
+          if (r < F::NRW && b < W) {
             const cx<T> x = Xm[(long long)xr * Wb + wrap_idx(bx.y + b, wrapN)];
             v[r] = x * Hj[a * W + b];
           } else {
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
 #pragma unroll
             for (int r = 0; r < F::R1; ++r) {
               const int a = jc1 + r * F::T1;
-              v[r] = (a < W) ? Tb[a * F::MC + xl] : cx<T>{0, 0};
+              v[r] = (r < F::NRW && a < W) ? Tb[a * F::MC + xl] : cx<T>{0, 0};
             }
             dft<F::R1, +1>(v);
 #pragma unroll
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
 #pragma unroll
             for (int r = 0; r < F::R1; ++r) {
               const int a = jc1 + r * F::T1;
-              v[r] = (a < W) ? Tb[a * F::MC + xl] : cx<T>{0, 0};
+              v[r] = (r < F::NRW && a < W) ? Tb[a * F::MC + xl] : cx<T>{0, 0};
             }
             dft<F::R1, +1>(v);
 #pragma unroll
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
 #pragma unroll
             for (int r = 0; r < F::R1; ++r) {
               const int a = jc1 + r * F::T1;
-              if (a < W) Tp[a * F::MC + xl] = v[r];
+              if (r < F::NRW && a < W) Tp[a * F::MC + xl] = v[r];
             }
           }
           __syncthreads();
@@ -545,7 +547,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
 #pragma unroll
               for (int r = 0; r < F::R1; ++r) {
                 const int b = j1r + r * F::T1;
-                if (b < W) {
+                if (r < F::NRW && b < W) {
                   const cx<T> gc = scale(mulc(v[r], Hj[a * W + b]), w2);
                   cx<T>* dst = gout + (long long)xr * Wb + wrap_idx(bx.y + b, wrapN);
                   *dst = *dst + gc;
