@@ -100,7 +100,7 @@ def _time(fn, reps: int) -> float:
 
 
 def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
-         chunks=(0, 1, 2, 4, 8, 16, 32), save: bool = True) -> dict:
+         chunks=(0, 1, 2, 4, 8, 12, 16, 24, 32), save: bool = True) -> dict:
     """Time every candidate for ``masks`` (the real batch) and record the winners."""
     n_masks = masks.shape[0] if masks.dim() == 3 else 1
     plan._tuned_for.add(n_masks)  # keep the table from overriding the candidates

@@ -38,7 +38,7 @@ lb_status fail(lb_status s, const std::string& msg) {
                   std::string(#call) + ": " + hipGetErrorString(e_));                   \
   } while (0)
 
-constexpr size_t kLdsLimit = 160 * 1024;
+constexpr size_t kLdsLimit = 160 * 1024 - 64;  // dynamic LDS; the kernels' static slots take the rest
 const int kFieldSizes[] = {16, 32, 48, 64, 80, 96, 128, 256};
 // Sizes the automatic choice picks from: the 80-point grid (radix 10 x 8)
 // runs slower per image than the 96-point one on the same boxes, so it is
@@ -305,7 +305,7 @@ struct Layout {
 
 struct Work {
   int n_masks, n_img, n_chunks, chunk, n_items;
-  size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, peak;
+  size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, peak, queue;
   size_t total;
 };
 
@@ -327,6 +327,7 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
   w.n_items = w.n_img * w.n_chunks;
   const size_t C = p.csz(), R = p.rsz();
   Layout L;
+  w.queue = L.take(256);  // work-item counter of the coherent-system launch
   w.Xm = L.take(C * n_masks * p.Wbm * p.Wbm);
   w.RTm = L.take(C * (size_t)n_masks * p.Wbm * p.N);
   w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
@@ -436,6 +437,8 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
   FieldConfig fc = p.fwd;
   fc.grid = grid_for(p, fc, w.n_items, false);
   FieldArgs a = field_args(p, ws + w.Xm, w);
+  a.queue = (int*)(ws + w.queue);
+  LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
   int e0 = -1, e1 = -1;
   if (p.profile) {
     LB_HIP(profile_events(p, &e0, &e1));
@@ -489,6 +492,8 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
   FieldConfig fc = p.adj;
   fc.grid = grid_for(p, fc, w.n_items, true);
   FieldArgs a = field_args(p, ws + w.Xm, w);
+  a.queue = (int*)(ws + w.queue);
+  LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
   LB_HIP(field_adjoint(p.precision, fc, a, gM, ws + w.gX, st));
   return bwd_post<T>(p, n_masks, dmask, ws, w, st);
 }
@@ -1153,7 +1158,9 @@ static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* 
   }
   FieldConfig fc = b.fwd;
   fc.grid = batch_grid(b, fc, b.n_items_f, false);
-  LB_HIP(field_multi(b.precision, fc, b.items_f, b.n_items_f, b.Wmax, ws, false, st));
+  int* q = (int*)(ws + b.base_f[0] + b.wf[0].queue);
+  LB_HIP(fill<float>((float*)q, 0.0f, 1, st));
+  LB_HIP(field_multi(b.precision, fc, b.items_f, b.n_items_f, b.Wmax, ws, q, false, st));
   for (int c = 0; c < n; ++c) {
     lb_status rc = fwd_post<T>(b.plans[c]->p, images[c], peaks ? peaks[c] : nullptr,
                                ws + b.base_f[c], b.wf[c], st);
@@ -1175,7 +1182,9 @@ static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
   }
   FieldConfig fc = b.adj;
   fc.grid = batch_grid(b, fc, b.n_items_b, true);
-  LB_HIP(field_multi(b.precision, fc, b.items_b, b.n_items_b, b.Wmax, ws, true, st));
+  int* q = (int*)(ws + b.base_b[0] + b.wb[0].queue);
+  LB_HIP(fill<float>((float*)q, 0.0f, 1, st));
+  LB_HIP(field_multi(b.precision, fc, b.items_b, b.n_items_b, b.Wmax, ws, q, true, st));
   for (int c = 0; c < n; ++c) {
     lb_status rc = bwd_post<T>(b.plans[c]->p, 1, dmask[c], ws + b.base_b[c], b.wb[c], st);
     if (rc != LB_OK) return rc;

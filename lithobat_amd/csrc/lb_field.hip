@@ -260,6 +260,7 @@ struct MapSource {
   T* partial;
   const T* gM;
   cx<T>* gX;
+  int* queue;
   __device__ int count() const { return map.n_items; }
   __device__ ItemView<T> get(int item) const {
     const int img = item / map.n_chunks, chunk = item - img * map.n_chunks;
@@ -288,6 +289,7 @@ struct TableSource {
   const ItemDesc* items;
   int n;
   char* ws;
+  int* queue;
   __device__ int count() const { return n; }
   __device__ ItemView<T> get(int i) const {
     const ItemDesc d = items[i];
@@ -308,6 +310,18 @@ struct TableSource {
   }
 };
 
+// The block's next work item: the first is blockIdx.x, later ones come from
+// an atomic counter (items differ in cost -- the last chunk of an image, clips
+// of different boxes -- so blocks that finish early take more), or in a fixed
+// stride when there is no counter.  Called by every thread of the block.
+__device__ __forceinline__ int next_item(int* queue, int item, int* s_next) {
+  if (!queue) return item + (int)gridDim.x;
+  __syncthreads();
+  if (threadIdx.x == 0) *s_next = (int)gridDim.x + atomicAdd(queue, 1);
+  __syncthreads();
+  return *s_next;
+}
+
 template <typename T, int M, int NT, int CSPLIT, typename Src>
 __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restrict__ twg) {
   using F = FieldShape<T, M, NT, CSPLIT>;
@@ -321,7 +335,8 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
   const int c1 = tid % F::CPR, jc1 = tid / F::CPR;  // column stage-1 mapping
 
   const int n_items = src.count();
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+  __shared__ int s_next;
+  for (int item = blockIdx.x; item < n_items; item = next_item(src.queue, item, &s_next)) {
     const ItemView<T> iv = src.get(item);
     const cx<T>* Xm = iv.X;
     const int Wb = iv.Wb, wrapN = iv.wrapN, W = iv.W, s0 = iv.s0, s1 = iv.s1;
@@ -417,7 +432,8 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
   const int rl1 = tid / F::T1, j1r = tid - (tid / F::T1) * F::T1;
 
   const int n_items = src.count();
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+  __shared__ int s_next;
+  for (int item = blockIdx.x; item < n_items; item = next_item(src.queue, item, &s_next)) {
     const ItemView<T> iv = src.get(item);
     const cx<T>* Xm = iv.X;
     const int Wb = iv.Wb, wrapN = iv.wrapN, W = iv.W, s0 = iv.s0, s1 = iv.s1;
@@ -618,6 +634,7 @@ static MapSource<T, M> map_source(const FieldArgs& a, void* partial, const void*
   m.partial = (T*)partial;
   m.gM = (const T*)gM;
   m.gX = (cx<T>*)gX;
+  m.queue = a.queue;
   return m;
 }
 
@@ -655,12 +672,12 @@ static hipError_t launch_adj(const FieldConfig& c, const FieldArgs& a, const voi
 
 template <typename T, int M, int NT, int CS>
 static hipError_t launch_multi(const FieldConfig& c, const ItemDesc* items, int n_items, int Wmax,
-                               char* ws, bool adjoint, hipStream_t st) {
+                               char* ws, int* queue, bool adjoint, hipStream_t st) {
   const size_t sm = smem_for<T, M, NT, CS>(Wmax, adjoint);
   const cx<T>* twg = nullptr;
   hipError_t e = table_for<T>(M, &twg);
   if (e != hipSuccess) return e;
-  TableSource<T> src{items, n_items, ws};
+  TableSource<T> src{items, n_items, ws, queue};
   if (adjoint) {
     auto k = k_field_adj<T, M, NT, CS, TableSource<T>>;
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
@@ -678,12 +695,14 @@ static hipError_t launch_multi(const FieldConfig& c, const ItemDesc* items, int 
 }
 
 hipError_t field_multi(int precision, const FieldConfig& c, const ItemDesc* items, int n_items,
-                       int Wmax, char* ws, bool adjoint, hipStream_t st) {
-#define LB_CASE(M_, NT_, CS_)                                                                 \
-  if (c.M == M_ && c.nt == NT_ && c.csplit == CS_)                                            \
-    return precision == 64                                                                    \
-               ? launch_multi<double, M_, NT_, CS_>(c, items, n_items, Wmax, ws, adjoint, st) \
-               : launch_multi<float, M_, NT_, CS_>(c, items, n_items, Wmax, ws, adjoint, st);
+                       int Wmax, char* ws, int* queue, bool adjoint, hipStream_t st) {
+#define LB_CASE(M_, NT_, CS_)                                                                   \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == CS_)                                              \
+    return precision == 64                                                                      \
+               ? launch_multi<double, M_, NT_, CS_>(c, items, n_items, Wmax, ws, queue, adjoint, \
+                                                    st)                                         \
+               : launch_multi<float, M_, NT_, CS_>(c, items, n_items, Wmax, ws, queue, adjoint, \
+                                                   st);
   LB_FIELD_CONFIGS(LB_CASE)
 #undef LB_CASE
   return hipErrorInvalidValue;

@@ -63,6 +63,7 @@ struct FieldArgs {
   const int2* box;    // [n_sys] band index (row, col) of each box origin
   const void* wsys;   // [n_sys] real weights
   ItemMap map;
+  int* queue;         // work-item counter, zero at launch (null: static item order)
 };
 
 struct FieldConfig {
@@ -91,7 +92,7 @@ struct ItemDesc {
 };
 // items: device array; Wmax sizes LDS; forward writes partials, adjoint gX partials
 hipError_t field_multi(int precision, const FieldConfig& c, const ItemDesc* items, int n_items,
-                       int Wmax, char* ws, bool adjoint, hipStream_t st);
+                       int Wmax, char* ws, int* queue, bool adjoint, hipStream_t st);
 size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint);
 
 // partial[item][M][M] = sum over the item's systems of w |E|^2 on the coarse grid
