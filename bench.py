@@ -185,7 +185,15 @@ def ilt_leg(args, ilt, dev, sync):
         (gm * 8.0 * m * (1.0 - m)).sum().item()  # host sync, as a per-step loop has
 
     dense = timed(dense_step, max(1, args.dense_images), 1, sync)
-    return one, batched, dense
+
+    def blur_step():  # the reference's objective with its blurs on: PEB 2 px, mask smoothing 1 px
+        theta.grad = None
+        raw = abbe_intensity(mask_of(theta, 8.0, 1.0), ilt)[:, 0]
+        loss = relaxed_resist_loss(raw, target, scale, 0.3, 50.0, sigma_px=2.0)
+        loss.backward()
+
+    blurred = timed(blur_step, args.ilt_steps, 2, sync)
+    return one, batched, dense, blurred
 
 
 def clips_leg(args, dev, sync):
@@ -268,6 +276,19 @@ def socs_leg(args, dev, sync):
                       "K = 24 of the FP64 Gram/Jacobi decomposition, complex64 imaging"}
 
 
+def volume_leg(args, masks, sync):
+    """Seconds per depth-resolved image through a 100 nm resist (21 optical planes in one
+    launch, linear interpolation onto 50 voxels of 2 nm), one 1024^2 mask."""
+    from lithobat_amd.config import GridConfig, ResistConfig
+    from lithobat_amd.resist3d import exposure_volume
+    grid = GridConfig(n_pixels=N, pixel_size=PIXEL, dz=2e-9, n_z_slices=21)
+    optics = optics_cfg2()
+    resist = ResistConfig(thickness=100e-9, n_resist=1.70, focus_reference="mid")
+    m = masks[0]
+    return timed(lambda: exposure_volume(m, optics, grid, resist), max(2, args.steps // 2), 1,
+                 sync)
+
+
 def dense_leg(args, plan, masks, sync):
     """Images/s of the reference's dataflow on the device: one dense N x N inverse
     transform per source point (8 per launch), host synchronisation per image."""
@@ -294,7 +315,7 @@ def main():
     ap.add_argument("--ilt-steps", type=int, default=10)
     ap.add_argument("--dense-images", type=int, default=3,
                     help="images timed for the dense per-source-point form")
-    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,cpu",
+    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,volume,cpu",
                     help="comma list of measurements to run (batched always runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the legs, copy a MIB-sized buffer once (a plain copy kernel of "
@@ -413,10 +434,12 @@ def main():
         plain_images_per_s = k / (time.perf_counter() - t0)
 
     # ---------------- ILT forward + backward at 1024^2 ---------------------------
-    ilt_s = ilt_batched_s = ilt_dense_s = None
+    ilt_s = ilt_batched_s = ilt_dense_s = ilt_blur_s = vol_s = None
     clip_ilt_s = clip_plain_s = dense_images_per_s = None
     if "ilt" in legs:
-        ilt_s, ilt_batched_s, ilt_dense_s = ilt_leg(args, ilt, dev, sync)
+        ilt_s, ilt_batched_s, ilt_dense_s, ilt_blur_s = ilt_leg(args, ilt, dev, sync)
+    if "volume" in legs:
+        vol_s = volume_leg(args, masks, sync)
     if "clips" in legs:
         clip_ilt_s, clip_plain_s = clips_leg(args, dev, sync)
     if "dense" in legs:
@@ -479,13 +502,19 @@ def main():
                            "eager, sync per clip",
         "ilt_steps_per_s": ratio(1.0, ilt_s),
         "ilt_batched_steps_per_s": ratio(1.0, ilt_batched_s),
+        "ilt_blur_steps_per_s": ratio(1.0, ilt_blur_s),
+        "exposure_volumes_per_s": ratio(1.0, vol_s),
+        "exposure_volume_config": "1 mask 1024^2 @ 4 nm, NA 0.93 quadrupole (148 pts), resist "
+                                  "100 nm / n 1.70, focus mid, 21 optical planes -> 50 voxels "
+                                  "of 2 nm, FP32",
         "ilt_plain_dense_steps_per_s": ratio(1.0, ilt_dense_s),
         "ilt_batched_vs_plain_dense": ratio(ilt_dense_s, ilt_batched_s),
         "ilt_single_vs_plain_dense": ratio(ilt_dense_s, ilt_s),
         "ilt_config": "1024^2 @ 4 nm, NA 1.35 quadrupole (148 pts), sigmoid(8 theta), "
                       "threshold 0.3 steepness 50, FP32 forward + adjoint; batched = "
                       f"{ILT_BATCH} masks per call (steps/s counts masks); plain dense = "
-                      "one dense fft2/ifft2 pair per source point, host sync per step",
+                      "one dense fft2/ifft2 pair per source point, host sync per step; "
+                      "blur = PEB sigma 2 px and mask smoothing 1 px on",
         "socs": socs,
         "fp32_rel_err_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,

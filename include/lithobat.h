@@ -166,6 +166,17 @@ lb_status lb_backward_dense(lb_plan plan, const void* mask, int32_t plane, int32
                             const void* d_image, void* d_mask, void* workspace,
                             size_t workspace_bytes, void* stream);
 
+/* Gaussian blur of n_img n x n planes (FP32 or FP64, in and out may alias):
+ * the PEB / ILT blur of opc/ilt.py:274-289 (scipy.ndimage.gaussian_filter with
+ * mode "reflect", truncate 4.0), separable, weights computed on the device.
+ * sigma < 1e-3 copies the input, as the reference returns it unblurred.  It is
+ * its own adjoint (symmetric taps, symmetric extension), which is how the
+ * reference's chain rule (ilt.py:336, 381-389) uses it. */
+size_t lb_gaussian_blur_workspace_bytes(int32_t n_img, int32_t n, double sigma, int32_t precision);
+lb_status lb_gaussian_blur(const void* in, void* out, int32_t n_img, int32_t n, double sigma,
+                           int32_t precision, void* workspace, size_t workspace_bytes,
+                           void* stream);
+
 /* Batches of clips (no direct counterpart in the reference, which images one
  * mask per call of compute_aerial_image / runs one run_ilt per clip): plans of
  * different grid sizes and optics -- one mask each -- imaged together.  The

@@ -7,12 +7,13 @@ Drop-in entry points with the reference's names and semantics:
 * ``SOCSKernels`` (expose/hopkins.py)
 * ``socs_fields``, ``socs_gradient`` and the differentiable
   ``AbbeIntensity`` (opc/ilt.py)
+* ``exposure_volume``, ``effective_defocus`` (develop/resist3d.py)
 
 plus the batched device forms ``compute_aerial_batch`` and ``ImagingPlan``.
 The heavy lifting is the C-ABI library ``liblithobat_amd.so`` (HIP, gfx950);
 there is no CPU fallback.
 """
-from .config import GridConfig, OpticsConfig  # noqa: F401
+from .config import GridConfig, OpticsConfig, ResistConfig  # noqa: F401
 from .masks import lines_and_spaces, manhattan  # noqa: F401
 from .source import SourcePoints, build_source, source_points  # noqa: F401
 
@@ -25,6 +26,9 @@ def __getattr__(name):
                 "normalisation_scale"):
         from . import expose
         return getattr(expose, name)
+    if name in ("exposure_volume", "effective_defocus", "focus_reference_depth"):
+        from . import resist3d
+        return getattr(resist3d, name)
     if name == "SOCSKernels":
         from .hopkins import SOCSKernels
         return SOCSKernels

@@ -46,7 +46,7 @@ EXPORTS = [
     "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read", "lb_plan_shapes",
     "lb_plan_set_shape", "lb_batch_create", "lb_batch_destroy", "lb_batch_workspace_bytes",
     "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
-    "lb_backward_dense",
+    "lb_backward_dense", "lb_gaussian_blur_workspace_bytes", "lb_gaussian_blur",
 ]
 
 _lib = None
@@ -87,6 +87,8 @@ def load() -> C.CDLL:
         "lb_dense_workspace_bytes": ([vp, i32], sz),
         "lb_forward_dense": ([vp, vp, i32, i32, vp, vp, sz, vp], C.c_int),
         "lb_backward_dense": ([vp, vp, i32, i32, vp, vp, vp, sz, vp], C.c_int),
+        "lb_gaussian_blur_workspace_bytes": ([i32, i32, C.c_double, i32], sz),
+        "lb_gaussian_blur": ([vp, vp, i32, i32, C.c_double, i32, vp, sz, vp], C.c_int),
         "lb_batch_create": ([C.POINTER(vp), i32, plan_p], C.c_int),
         "lb_batch_destroy": ([vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),

@@ -132,3 +132,24 @@ class GridConfig:
     @property
     def grid_size(self) -> float:
         return self.n_pixels * self.pixel_size
+
+
+FOCUS_REFERENCES = ("top", "mid", "bottom")
+
+
+@dataclass
+class ResistConfig:
+    """The resist-film fields the depth-resolved image reads (core/config.py:667-672, checks
+    :698-709): film thickness [m], film index, and the film plane ``OpticsConfig.defocus``
+    refers to.  The reference's full ``ResistConfig`` (chemistry, develop) is accepted by
+    duck typing wherever this one is."""
+
+    thickness: float = 100e-9
+    n_resist: float = 1.70
+    focus_reference: str = "mid"
+
+    def __post_init__(self) -> None:
+        r = "ResistConfig"
+        _bounded(r, "thickness", self.thickness, 0.0)
+        _bounded(r, "n_resist", self.n_resist, 0.0, lo_open=True)
+        _choice(r, "focus_reference", self.focus_reference, FOCUS_REFERENCES)

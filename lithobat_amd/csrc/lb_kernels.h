@@ -135,6 +135,14 @@ template <typename T>
 hipError_t dense_adjoint_accum(const cx<T>* G, int N, const cx<T>* H, int W, const int2* box,
                                const T* w, int j0, int nsys, int nu0m, cx<T>* A, hipStream_t st);
 
+// ---- Gaussian blur (lb_blur.hip) ---------------------------------------------
+// scipy.ndimage.gaussian_filter(mode="reflect", truncate=4.0) of n_img n x n
+// planes; tmp holds n_img n^2 values, w 2R+1 doubles (R = gauss_radius(sigma)).
+int gauss_radius(double sigma);
+template <typename T>
+hipError_t gaussian_blur(const T* in, T* out, int n_img, int n, double sigma, T* tmp, double* w,
+                         hipStream_t st);
+
 // ---- pupil tables (lb_pupil.hip) ------------------------------------------
 struct PupilSys {
   double fsx, fsy;       // source-point frequency offset [1/m]

@@ -15,6 +15,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from .blur import blur
 from .expose import _to_device
 from .plan import ImagingPlan
 
@@ -45,24 +46,42 @@ def abbe_intensity(mask: torch.Tensor, plan: ImagingPlan) -> torch.Tensor:
     return AbbeIntensity.apply(mask, plan)
 
 
-def mask_of(theta: torch.Tensor, beta: float) -> torch.Tensor:
-    """ilt.py:368-378 with no curvature smoothing: sigmoid(clip(beta * theta, +-60))."""
-    return torch.sigmoid(torch.clamp(beta * theta, -60.0, 60.0))
+def mask_of(theta: torch.Tensor, beta: float, smooth_px: float = 0.0) -> torch.Tensor:
+    """ilt.py:368-378: sigmoid(clip(beta * G(theta), +-60)), G the Gaussian of width
+    ``smooth_px`` (the identity below 1e-3 px); the backward is ilt.py:381-389."""
+    return torch.sigmoid(torch.clamp(beta * blur(theta, smooth_px), -60.0, 60.0))
 
 
 def relaxed_resist_loss(raw: torch.Tensor, target: torch.Tensor, scale: float, level: float,
-                        steepness: float, sign: float = 1.0) -> torch.Tensor:
-    """ilt.py:311-330 with sigma_px = 0 and no penalties: mean (sigmoid(a s) - t)^2.
+                        steepness: float, sign: float = 1.0, sigma_px: float = 0.0
+                        ) -> torch.Tensor:
+    """ilt.py:311-330 without penalties: mean (sigmoid(a s) - t)^2.
 
-    a = steepness / |level| and s = sign * (raw * scale - level), the reference's
-    relaxed resist.  The mean runs over the last two (image) axes and is
+    a = steepness / |level| and s = sign * (G(raw * scale) - level), the
+    reference's relaxed resist, G the PEB blur of width ``sigma_px`` (identity
+    below 1e-3 px).  The mean runs over the last two (image) axes and is
     summed over any leading axes.
     """
-    signed = sign * (raw * scale - level)
+    signed = sign * (blur(raw * scale, sigma_px) - level)
     a = steepness / max(abs(level), 1e-12)
     z = torch.sigmoid(torch.clamp(a * signed, -60.0, 60.0))
     resid = z - target
     return (resid * resid).mean(dim=(-2, -1)).sum()
+
+
+def mask_penalties(mask: torch.Tensor, weight_tv: float = 0.0,
+                   weight_discrete: float = 0.0) -> torch.Tensor:
+    """ilt.py:339-355: weight_tv * mean |grad m|^2 (periodic differences) +
+    weight_discrete * mean m (1 - m), per image, summed over leading axes."""
+    npix = float(mask.shape[-1] * mask.shape[-2])
+    cost = mask.new_zeros(())
+    if weight_tv > 0.0:
+        gx = mask - torch.roll(mask, 1, -1)
+        gy = mask - torch.roll(mask, 1, -2)
+        cost = cost + weight_tv * (gx * gx + gy * gy).sum() / npix
+    if weight_discrete > 0.0:
+        cost = cost + weight_discrete * (mask * (1.0 - mask)).sum() / npix
+    return cost
 
 
 class FieldStack(np.ndarray):

@@ -577,32 +577,83 @@ def sigmoid_clipped(z):
     return 1.0 / (1.0 + np.exp(-np.clip(z, -60.0, 60.0)))
 
 
-def relaxed_objective(raw, target, scale, level, steepness, sign=1.0):
-    """ilt.py:311-337 with sigma_px = 0 and no penalties.
+def blur(a, sigma_px):
+    """ilt.py:274-289: scipy.ndimage.gaussian_filter(a, sigma) (mode "reflect",
+    truncate 4.0, scipy 1.15.3 here), the input itself below 1e-3 px."""
+    from scipy.ndimage import gaussian_filter
+    if sigma_px < 1e-3:
+        return a
+    return gaussian_filter(a, sigma=sigma_px)
+
+
+def blur_direct(a, sigma):
+    """Independent restatement of that filter: taps exp(-k^2 / 2 sigma^2) / sum for
+    |k| <= int(4 sigma + 0.5), half-sample symmetric extension, rows then columns,
+    explicit index arithmetic (no scipy)."""
+    a = np.asarray(a, dtype=np.float64)
+    n0, n1 = a.shape
+    R = int(4.0 * sigma + 0.5)
+    k = np.arange(-R, R + 1)
+    w = np.exp(-0.5 / (sigma * sigma) * k.astype(np.float64) ** 2)
+    w /= w.sum()
+
+    def refl(i, n):
+        r = np.mod(i, 2 * n)
+        return np.where(r < n, r, 2 * n - 1 - r)
+
+    out = np.zeros_like(a)
+    for t, wk in zip(k, w):
+        out += wk * a[refl(np.arange(n0) + t, n0), :]
+    res = np.zeros_like(a)
+    for t, wk in zip(k, w):
+        res += wk * out[:, refl(np.arange(n1) + t, n1)]
+    return res
+
+
+def relaxed_objective(raw, target, scale, level, steepness, sign=1.0, sigma_px=0.0):
+    """ilt.py:311-337 without penalties.
 
     Returns (cost, dcost/draw, signed, pattern_error).
     """
     aerial = raw * scale
-    signed = sign * (aerial - level)
+    fld = blur(aerial, sigma_px)
+    signed = sign * (fld - level)
     a = steepness / max(abs(level), 1e-12)
     z = sigmoid_clipped(a * signed)
     resid = z - target
     npix = float(resid.size)
     cost = float(np.sum(resid ** 2) / npix)
     dz = (2.0 / npix) * resid * (a * z * (1.0 - z))
-    d_raw = sign * dz * scale
+    d_raw = blur(sign * dz, sigma_px) * scale
     pe = int(np.count_nonzero((signed > 0.0) != (target > 0.5)))
     return cost, d_raw, signed, pe
 
 
-def mask_of(theta, beta):
-    """ilt.py:368-378 with smooth_px = 0."""
-    return sigmoid_clipped(beta * theta)
+def mask_penalties(mask, weight_tv=0.0, weight_discrete=0.0):
+    """ilt.py:339-355: (cost, gradient) of the TV and discreteness penalties."""
+    npix = float(mask.size)
+    cost, grad = 0.0, np.zeros_like(mask)
+    if weight_tv > 0.0:
+        lap = (np.roll(mask, 1, 0) + np.roll(mask, -1, 0) + np.roll(mask, 1, 1)
+               + np.roll(mask, -1, 1) - 4.0 * mask)
+        gx = mask - np.roll(mask, 1, 1)
+        gy = mask - np.roll(mask, 1, 0)
+        cost += weight_tv * float(np.sum(gx ** 2 + gy ** 2) / npix)
+        grad += weight_tv * (-2.0 / npix) * lap
+    if weight_discrete > 0.0:
+        cost += weight_discrete * float(np.sum(mask * (1.0 - mask)) / npix)
+        grad += weight_discrete * (1.0 - 2.0 * mask) / npix
+    return cost, grad
 
 
-def grad_theta(grad_mask, mask, beta):
-    """ilt.py:381-389 with smooth_px = 0."""
-    return grad_mask * (beta * mask * (1.0 - mask))
+def mask_of(theta, beta, smooth_px=0.0):
+    """ilt.py:368-378."""
+    return sigmoid_clipped(beta * blur(theta, smooth_px))
+
+
+def grad_theta(grad_mask, mask, beta, smooth_px=0.0):
+    """ilt.py:381-389."""
+    return blur(grad_mask * (beta * mask * (1.0 - mask)), smooth_px)
 
 
 # --------------------------------------------------------------------------
@@ -619,3 +670,45 @@ def lines_and_spaces(n, pixel_size, pitch, cd, orientation="vertical"):
     if orientation == "horizontal":
         return np.tile(line[:, None], (1, n))
     raise ValueError(orientation)
+
+
+# --------------------------------------------------------------------------
+# Depth-resolved image through the resist film (develop/resist3d.py:66-185)
+# --------------------------------------------------------------------------
+
+
+def focus_reference_depth(resist):
+    """resist3d.py:66-77."""
+    ref = resist.focus_reference
+    if ref == "top":
+        return 0.0
+    if ref == "mid":
+        return resist.thickness / 2.0
+    if ref == "bottom":
+        return resist.thickness
+    raise ValueError(f"focus_reference must be 'top', 'mid', or 'bottom', got '{ref}'")
+
+
+def effective_defocus(depth, optics, resist):
+    """resist3d.py:80-110."""
+    d_ref = focus_reference_depth(resist)
+    return optics.defocus - (depth - d_ref) * (optics.n_immersion / resist.n_resist)
+
+
+def exposure_volume(mask, optics, grid, resist, dose=1.0):
+    """resist3d.py:113-185: planes through the film, per-pixel np.interp onto the voxel
+    grid, bottom-up."""
+    nz = max(int(round(resist.thickness / grid.dz)), 1)
+    n_slices = int(np.clip(grid.n_z_slices, 2, nz))
+    depths = np.linspace(0.0, resist.thickness, n_slices)
+    ratio = resist.n_resist / optics.n_immersion
+    planes = compute_aerial_planes(
+        mask, optics, grid,
+        [(effective_defocus(float(d), optics, resist) * ratio, resist.n_resist) for d in depths],
+        dose=dose)
+    z_vox_depth = ((np.arange(nz) + 0.5) * grid.dz)[::-1]
+    flat = planes.reshape(n_slices, -1)
+    out = np.empty((nz, flat.shape[1]), dtype=np.float64)
+    for k in range(flat.shape[1]):
+        out[:, k] = np.interp(z_vox_depth, depths, flat[:, k])
+    return out.reshape(nz, grid.n_pixels, grid.n_pixels), (np.arange(nz) + 0.5) * grid.dz

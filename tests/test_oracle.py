@@ -352,3 +352,84 @@ def test_lines_and_spaces_period_and_duty():
     assert np.mean(row[:150] == row[50:200]) > 0.97
     assert abs(row.mean() - 0.5) < 0.02                 # 100 nm CD
     assert np.all(m == row[None, :])                    # vertical lines
+
+
+def test_effective_defocus_closed_forms():
+    """resist3d.py:80-110: defocus - (depth - d_ref) n_imm / n_resist, d_ref by reference plane."""
+    from lithobat_amd.config import OpticsConfig, ResistConfig
+    o = OpticsConfig(NA=1.35, n_immersion=1.44, defocus=30e-9)
+    for ref, d_ref in (("top", 0.0), ("mid", 50e-9), ("bottom", 100e-9)):
+        r = ResistConfig(thickness=100e-9, n_resist=1.7, focus_reference=ref)
+        assert O.focus_reference_depth(r) == d_ref
+        for depth in (0.0, 20e-9, 100e-9):
+            want = 30e-9 - (depth - d_ref) * 1.44 / 1.7
+            assert O.effective_defocus(depth, o, r) == pytest.approx(want, rel=1e-15, abs=1e-24)
+    with pytest.raises(ValueError, match="focus_reference"):
+        ResistConfig(focus_reference="centre")
+
+
+def test_exposure_volume_interpolates_between_the_right_planes():
+    """Each voxel lies between the two film planes around its depth (bottom-up order), and the
+    linear interpolation of the smooth through-focus image stays close to the image evaluated
+    directly at the voxel's own depth."""
+    from lithobat_amd.config import GridConfig, OpticsConfig, ResistConfig
+    grid = GridConfig(n_pixels=32, pixel_size=8e-9, dz=4e-9, n_z_slices=6)
+    optics = OpticsConfig(NA=0.93, source_type="annular", sigma_outer=0.8, sigma_inner=0.5,
+                          source_grid=7, defocus=20e-9, normalisation="none")
+    resist = ResistConfig(thickness=100e-9, n_resist=1.7, focus_reference="top")
+    mask = O.lines_and_spaces(32, 8e-9, 128e-9, 64e-9)
+    vol, z = O.exposure_volume(mask, optics, grid, resist)
+    nz = 25
+    assert vol.shape == (nz, 32, 32) and np.allclose(z, (np.arange(nz) + 0.5) * 4e-9)
+    ratio = 1.7 / optics.n_immersion
+    for iz in (0, 7, nz - 1):  # iz = 0 is the bottom of the film
+        depth = 100e-9 - (iz + 0.5) * 4e-9
+        direct = O.compute_aerial_planes(
+            mask, optics, grid, [(O.effective_defocus(depth, optics, resist) * ratio, 1.7)])[0]
+        assert np.abs(vol[iz] - direct).max() <= 2e-3 * np.abs(direct).max(), iz
+    # planted mistake: a top-down (unflipped) volume misses the direct image badly
+    flipped = vol[::-1]
+    depth0 = 100e-9 - 0.5 * 4e-9
+    direct0 = O.compute_aerial_planes(
+        mask, optics, grid, [(O.effective_defocus(depth0, optics, resist) * ratio, 1.7)])[0]
+    assert np.abs(flipped[0] - direct0).max() > 5 * np.abs(vol[0] - direct0).max()
+
+
+@pytest.mark.parametrize("sigma", [0.6, 2.3, 7.0])
+def test_blur_is_the_direct_gaussian_with_reflect_edges(sigma):
+    """The oracle's blur (the reference's scipy call) equals an explicit restatement."""
+    rng = np.random.default_rng(int(sigma * 10))
+    a = rng.standard_normal((24, 40))
+    assert np.abs(O.blur(a, sigma) - O.blur_direct(a, sigma)).max() <= 1e-14
+    # a planted mistake: periodic instead of reflected borders differs at the edges
+    import scipy.ndimage as nd
+    wrong = nd.gaussian_filter(a, sigma, mode="wrap")
+    assert np.abs(wrong - O.blur(a, sigma)).max() > 1e-3
+    assert O.blur(a, 5e-4) is a
+
+
+def test_blur_is_self_adjoint_and_objective_gradient_matches_fd():
+    """<B x, y> = <x, B y> (symmetric taps + symmetric extension), so the chain rule of
+    ilt.py:336 is exact; the relaxed objective with a blur against central differences."""
+    rng = np.random.default_rng(5)
+    x, y = rng.standard_normal((2, 20, 20))
+    assert abs(np.sum(O.blur(x, 1.7) * y) - np.sum(x * O.blur(y, 1.7))) <= 1e-12
+    raw = rng.random((16, 16)) * 1.5
+    target = (rng.random((16, 16)) > 0.5).astype(float)
+    cost, d_raw, _, _ = O.relaxed_objective(raw, target, 0.8, 0.3, 5.0, sigma_px=1.3)
+    h = 1e-6
+    for i, j in [(0, 0), (3, 7), (15, 2)]:
+        rp, rm = raw.copy(), raw.copy()
+        rp[i, j] += h
+        rm[i, j] -= h
+        fd = (O.relaxed_objective(rp, target, 0.8, 0.3, 5.0, sigma_px=1.3)[0]
+              - O.relaxed_objective(rm, target, 0.8, 0.3, 5.0, sigma_px=1.3)[0]) / (2 * h)
+        assert d_raw[i, j] == pytest.approx(fd, rel=1e-5, abs=1e-10)
+    m = rng.random((12, 12))
+    c, g = O.mask_penalties(m, 0.3, 0.2)
+    for i, j in [(1, 1), (11, 5)]:
+        mp, mm = m.copy(), m.copy()
+        mp[i, j] += h
+        mm[i, j] -= h
+        fd = (O.mask_penalties(mp, 0.3, 0.2)[0] - O.mask_penalties(mm, 0.3, 0.2)[0]) / (2 * h)
+        assert g[i, j] == pytest.approx(fd, rel=1e-6, abs=1e-12)
