@@ -19,6 +19,8 @@ Also reported on the same line:
   launch per pass, graph-captured) against one clip per call;
 * Hopkins / SOCS: the FP64 device decomposition (K = 24) and 256 masks at
   512^2 imaged through it, with the truncation error against full Abbe;
+* the depth-resolved image through a resist film and the full ILT solve
+  (run_ilt) on the device;
 * the FP32 image error against the float64 oracle on a sample image;
 * roofline of the coherent-system kernel (HIP events on its stream; HBM
   traffic from the committed counter record when it matches the shape);
@@ -289,6 +291,30 @@ def volume_leg(args, masks, sync):
                  sync)
 
 
+def run_ilt_leg(args, sync):
+    """The full inverse-lithography solve on the device: 512^2 @ 8 nm, NA 1.35 annular
+    0.6-0.9 SOCS kernels (FP32), threshold resist with 10 nm PEB, 4 nm mask smoothing,
+    TV and discreteness penalties, 30 Adam iterations (one host scalar per iteration,
+    as the reference's convergence test needs).  Seconds per iteration."""
+    from lithobat_amd.config import GridConfig, OpticsConfig, ResistConfig
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.opc import PrintModel, run_ilt
+    grid = GridConfig(n_pixels=512, pixel_size=8e-9)
+    optics = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="annular",
+                          sigma_outer=0.9, sigma_inner=0.6, source_grid=21,
+                          normalisation="clear")
+    model = PrintModel(optics, ResistConfig(threshold=0.3, diffusion_sigma=10e-9), grid,
+                       precision=32)
+    target = manhattan(512, 8e-9, seed=4000)
+    kw = dict(smooth=4e-9, weight_tv=1e-3, weight_discrete=1e-3, tol=0.0)
+    run_ilt(target, model, max_iter=2, **kw)  # kernels built, shapes warmed
+    sync()
+    t0 = time.perf_counter()
+    res = run_ilt(target, model, max_iter=30, **kw)
+    sync()
+    return (time.perf_counter() - t0) / res.iterations, res.iterations
+
+
 def dense_leg(args, plan, masks, sync):
     """Images/s of the reference's dataflow on the device: one dense N x N inverse
     transform per source point (8 per launch), host synchronisation per image."""
@@ -315,7 +341,7 @@ def main():
     ap.add_argument("--ilt-steps", type=int, default=10)
     ap.add_argument("--dense-images", type=int, default=3,
                     help="images timed for the dense per-source-point form")
-    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,volume,cpu",
+    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,volume,solve,cpu",
                     help="comma list of measurements to run (batched always runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the legs, copy a MIB-sized buffer once (a plain copy kernel of "
@@ -440,6 +466,7 @@ def main():
         ilt_s, ilt_batched_s, ilt_dense_s, ilt_blur_s = ilt_leg(args, ilt, dev, sync)
     if "volume" in legs:
         vol_s = volume_leg(args, masks, sync)
+    solve = run_ilt_leg(args, sync) if "solve" in legs else None
     if "clips" in legs:
         clip_ilt_s, clip_plain_s = clips_leg(args, dev, sync)
     if "dense" in legs:
@@ -504,6 +531,10 @@ def main():
         "ilt_batched_steps_per_s": ratio(1.0, ilt_batched_s),
         "ilt_blur_steps_per_s": ratio(1.0, ilt_blur_s),
         "exposure_volumes_per_s": ratio(1.0, vol_s),
+        "run_ilt_iterations_per_s": None if solve is None else 1.0 / solve[0],
+        "run_ilt_config": "512^2 @ 8 nm, NA 1.35 annular (142 pts) SOCS kernels FP32, "
+                          "threshold resist, PEB 10 nm, smoothing 4 nm, TV + discreteness, "
+                          "30 Adam iterations",
         "exposure_volume_config": "1 mask 1024^2 @ 4 nm, NA 0.93 quadrupole (148 pts), resist "
                                   "100 nm / n 1.70, focus mid, 21 optical planes -> 50 voxels "
                                   "of 2 nm, FP32",

@@ -137,19 +137,29 @@ class GridConfig:
 FOCUS_REFERENCES = ("top", "mid", "bottom")
 
 
+TONES = ("positive", "negative")
+
+
 @dataclass
 class ResistConfig:
-    """The resist-film fields the depth-resolved image reads (core/config.py:667-672, checks
-    :698-709): film thickness [m], film index, and the film plane ``OpticsConfig.defocus``
-    refers to.  The reference's full ``ResistConfig`` (chemistry, develop) is accepted by
-    duck typing wherever this one is."""
+    """The resist fields the imaging path reads (core/config.py:634-672, checks :686-709):
+    tone, develop threshold and PEB diffusion length (the threshold develop model and the
+    ILT objective), film thickness, film index and the film plane ``OpticsConfig.defocus``
+    refers to (the depth-resolved image).  The reference's full ``ResistConfig``
+    (chemistry, develop rates) is accepted by duck typing wherever this one is."""
 
+    tone: str = "positive"
+    threshold: float = 0.30
+    diffusion_sigma: float = 20e-9
     thickness: float = 100e-9
     n_resist: float = 1.70
     focus_reference: str = "mid"
 
     def __post_init__(self) -> None:
         r = "ResistConfig"
+        _choice(r, "tone", self.tone, TONES)
+        _bounded(r, "threshold", self.threshold, 0.0, 1.0, lo_open=True, hi_open=True)
+        _bounded(r, "diffusion_sigma", self.diffusion_sigma, 0.0)
         _bounded(r, "thickness", self.thickness, 0.0)
         _bounded(r, "n_resist", self.n_resist, 0.0, lo_open=True)
         _choice(r, "focus_reference", self.focus_reference, FOCUS_REFERENCES)

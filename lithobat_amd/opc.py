@@ -1,0 +1,271 @@
+"""Print and inverse-lithography solve on the device (opc/model.py, opc/ilt.py,
+develop/resist.py and bake/peb.py of the reference, threshold develop model).
+
+``PrintModel`` carries the process (optics, resist, grid, dose, tone) and builds
+its SOCS kernels once; ``print_mask`` images a mask array and develops it to the
+latent field; ``run_ilt`` is the reference's Adam solve on the latent theta with
+the mask held in [0, 1] by sigmoid(beta G(theta)), beta annealed, the relaxed
+resist objective (PEB blur, TV and discreteness penalties) and the SOCS adjoint
+-- every array stays on the device; the host sees one scalar cost per iteration.
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass, field
+from typing import Callable
+
+import numpy as np
+import torch
+
+from .blur import blur, gaussian_blur
+from .expose import normalisation_scale
+from .hopkins import SOCSKernels
+from .ilt import abbe_intensity, mask_of, mask_penalties, relaxed_resist_loss
+
+
+def apply_peb(pac_map, diffusion_sigma: float, pixel_size: float):
+    """peb.py:24-70: Gaussian of diffusion_sigma / pixel_size px, clipped to [0, 1]
+    (a copy, unblurred, below 1e-3 px).  NumPy in -> FP64 on the device -> NumPy out."""
+    was_np = not isinstance(pac_map, torch.Tensor)
+    a = (torch.as_tensor(np.asarray(pac_map, dtype=np.float64), device="cuda") if was_np
+         else pac_map)
+    if a.dim() != 2:
+        raise ValueError(f"apply_peb expects a 2-D (ny, nx) map, got ndim={a.dim()}; "
+                         "use apply_peb_3d for volumes.")
+    if diffusion_sigma < 0:
+        raise ValueError(f"diffusion_sigma must be >= 0, got {diffusion_sigma!r}")
+    if pixel_size <= 0:
+        raise ValueError(f"pixel_size must be > 0, got {pixel_size!r}")
+    sigma_px = diffusion_sigma / pixel_size
+    out = a.clone() if sigma_px < 1e-3 else torch.clamp(gaussian_blur(a, sigma_px), 0.0, 1.0)
+    return out.cpu().numpy() if was_np else out
+
+
+def develop_field(aerial, cfg, grid, model: str = "threshold"):
+    """resist.py:452-503 for the threshold model: (PEB-diffused aerial, threshold)."""
+    if model != "threshold":
+        raise NotImplementedError(
+            f"develop model {model!r} (resist chemistry) is outside this library; the "
+            "threshold model is implemented")
+    return apply_peb(aerial, cfg.diffusion_sigma, grid.pixel_size), float(cfg.threshold)
+
+
+@dataclass
+class PrintedImage:
+    """model.py:60-96: the mask, its aerial image, the latent field and its level."""
+
+    mask: object
+    aerial: object
+    field: object
+    level: float
+    tone: str
+    grid: object
+
+    @property
+    def signed(self):
+        s = self.field - self.level
+        return s if self.tone == "clear" else -s
+
+    @property
+    def printed(self):
+        return self.signed > 0.0
+
+
+@dataclass
+class PrintModel:
+    """model.py:126-215 (the imaging part): process + SOCS kernels built once per optics.
+
+    ``precision`` is the device precision of the kernels (64 by default, as the
+    reference computes in float64)."""
+
+    optics: object
+    resist: object
+    grid: object
+    dose: float = 1.0
+    tone: str = "clear"
+    model: str = "threshold"
+    normalisation: str | None = "clear"
+    imaging: str = "auto"
+    socs_tol: float = 1e-8
+    precision: int = 64
+    _cache: dict = field(default_factory=dict, repr=False, compare=False)
+
+    def __post_init__(self) -> None:
+        if self.tone not in ("clear", "dark"):
+            raise ValueError(f"tone must be 'clear' or 'dark', got '{self.tone}'")
+        if self.imaging not in ("auto", "socs", "abbe"):
+            raise ValueError(f"imaging must be 'auto', 'socs' or 'abbe', got '{self.imaging}'")
+        if self.normalisation is not None and self.normalisation != self.optics.normalisation:
+            self.optics = dataclasses.replace(self.optics, normalisation=self.normalisation)
+        if self.imaging == "socs" and not SOCSKernels.supports(self.optics):
+            raise ValueError(
+                "imaging='socs' needs the scalar thin-mask process; got "
+                f"imaging_model={self.optics.imaging_model!r}, "
+                f"mask_model={getattr(self.optics, 'mask_model', 'thin')!r}")
+
+    def kernels(self) -> SOCSKernels | None:
+        """model.py:201-221: the kernels, rebuilt only when optics / grid change."""
+        if self.imaging == "abbe":
+            return None
+        if self.imaging == "auto" and not SOCSKernels.supports(self.optics):
+            return None
+        key = (repr(self.optics), self.grid.n_pixels, self.grid.pixel_size, self.socs_tol,
+               self.precision)
+        cached = self._cache.get("kernels")
+        if cached is None or cached[0] != key:
+            cached = (key, SOCSKernels.build(self.optics, self.grid, tol=self.socs_tol,
+                                             precision=self.precision))
+            self._cache["kernels"] = cached
+        return cached[1]
+
+
+def print_mask(mask, model: PrintModel) -> PrintedImage:
+    """ilt.py:624-648: image a mask array through the model and develop it."""
+    from .expose import compute_aerial_image
+    kernels = model.kernels()
+    if kernels is not None:
+        raw, peak, clear = kernels.raw(mask)
+    else:
+        raw, peak, clear = compute_aerial_image(mask, model.optics, model.grid, return_raw=True)
+    aerial = raw * normalisation_scale(model.optics.normalisation, peak, clear, model.dose)
+    fld, level = develop_field(aerial, model.resist, model.grid, model=model.model)
+    return PrintedImage(mask, aerial, fld, level, model.tone, model.grid)
+
+
+@dataclass
+class ILTFrame:
+    """ilt.py:60-78: one progress report."""
+
+    iteration: int
+    mask: np.ndarray
+    signed: np.ndarray
+    cost: float
+    pattern_error: int
+    beta: float
+
+
+@dataclass
+class ILTResult:
+    """ilt.py:81-95."""
+
+    mask: np.ndarray
+    theta: np.ndarray
+    target: np.ndarray
+    printed: PrintedImage
+    history: object
+    iterations: int
+    converged: bool
+
+
+def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, step: float = 0.4,
+            tol: float = 1e-7, beta: float = 4.0, beta_final: float | None = 12.0,
+            initial_grey: float = 0.9, steepness: float = 12.0, weight_tv: float = 0.0,
+            weight_discrete: float = 0.0, smooth: float = 0.0,
+            progress: Callable[[ILTFrame], None] | None = None) -> ILTResult:
+    """ilt.py:397-621 on the device (same parameters, checks, schedule and history)."""
+    kernels = model.kernels()
+    if kernels is None:
+        raise ValueError(
+            "inverse lithography images through the SOCS kernels; this model images through "
+            f"the Abbe sum (imaging={model.imaging!r}, "
+            f"imaging_model={model.optics.imaging_model!r}, "
+            f"mask_model={getattr(model.optics, 'mask_model', 'thin')!r}). The adjoint is "
+            "written against the kernel decomposition and has no thick-mask form.")
+    if model.optics.normalisation.lower() == "peak":
+        raise ValueError(
+            "inverse lithography needs a mask-independent normalisation; 'peak' rescales the "
+            "image by whatever the current mask's brightest point happens to be, which moves "
+            "the objective every step. Use normalisation='clear'.")
+    n = model.grid.n_pixels
+    tgt = np.asarray(target, dtype=np.float64)
+    if tgt.shape != (n, n):
+        raise ValueError(f"target must be {(n, n)} for this grid, got {tgt.shape}")
+    if smooth < 0.0:
+        raise ValueError(f"smooth is a length and cannot be negative, got {smooth:g}")
+    smooth_px = float(smooth / model.grid.pixel_size)
+    scale = normalisation_scale(model.optics.normalisation, 1.0, kernels.clear, model.dose)
+    level = float(model.resist.threshold)
+    sigma_px = float(model.resist.diffusion_sigma / model.grid.pixel_size)
+    sign = 1.0 if model.tone == "clear" else -1.0
+    plan = kernels.plan
+    dt = plan.real_dtype
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t_dev = torch.as_tensor(tgt, dtype=dt, device=dev)
+
+    def objective(m):
+        """ilt.py:311-360 on the device: cost (device scalar), d cost / d mask, signed field."""
+        m = m.detach().requires_grad_(True)
+        raw = abbe_intensity(m[None], plan)[:, 0]
+        cost = (relaxed_resist_loss(raw, t_dev[None], scale, level, steepness, sign, sigma_px)
+                + mask_penalties(m, weight_tv, weight_discrete))
+        cost.backward()
+        with torch.no_grad():
+            signed = sign * (blur(raw * scale, sigma_px) - level)[0]
+        return cost.detach(), m.grad, signed
+
+    def grad_theta(grad_m, m, beta_k):
+        """ilt.py:381-389: back through the sigmoid's slope, then the Gaussian."""
+        return blur(grad_m * (beta_k * m * (1.0 - m)), smooth_px)
+
+    p = float(np.clip(initial_grey, 0.5 + 1e-3, 1.0 - 1e-6))
+    if initial is None:
+        m0 = 0.5 + (np.clip(tgt, 0.0, 1.0) - 0.5) * (2.0 * p - 1.0)
+    else:
+        m0 = np.clip(np.asarray(initial, dtype=np.float64), 1.0 - p, p)
+    theta = torch.as_tensor(np.log(m0 / (1.0 - m0)) / beta, dtype=dt, device=dev)
+    growth = (1.0 if beta_final is None or max_iter <= 1
+              else (beta_final / beta) ** (1.0 / (max_iter - 1)))
+    mom1 = torch.zeros_like(theta)
+    mom2 = torch.zeros_like(theta)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    beta_k = float(beta)
+    with torch.no_grad():
+        mask = mask_of(theta, beta_k, smooth_px)
+    cost, grad_m, signed = objective(mask)
+    lit = int((signed > 0.0).sum())
+    if lit == 0 or lit == signed.numel():
+        peak = float((signed.abs() + level).max())
+        raise ValueError(
+            f"the initial mask prints {'nothing' if lit == 0 else 'everywhere'} — no contour "
+            f"exists, so the objective has no gradient. The resist level is {level:g} and the "
+            f"field reaches {peak:.4g} at dose {model.dose:g}. Size the dose first.")
+    rows = []
+
+    def record(it, cost, grad_m, signed, mask):
+        perr = int(((signed > 0.0) != (t_dev > 0.5)).sum())
+        rows.append({"iteration": it, "cost": float(cost), "pattern_error": perr,
+                     "beta": beta_k, "grad_norm": float(torch.linalg.vector_norm(grad_m))})
+        if progress is not None:
+            progress(ILTFrame(it, mask.cpu().numpy(), signed.cpu().numpy(), float(cost), perr,
+                              beta_k))
+        return float(cost)
+
+    record(0, cost, grad_m, signed, mask)
+    prev_cost, converged, taken = np.inf, False, 0
+    for it in range(1, max_iter + 1):
+        with torch.no_grad():
+            grad_t = grad_theta(grad_m, mask, beta_k)
+        mom1.mul_(b1).add_(grad_t, alpha=1.0 - b1)
+        mom2.mul_(b2).addcmul_(grad_t, grad_t, value=1.0 - b2)
+        m_hat = mom1 / (1.0 - b1 ** it)
+        v_hat = mom2 / (1.0 - b2 ** it)
+        theta = theta - step * m_hat / (torch.sqrt(v_hat) + eps)
+        beta_k *= growth
+        with torch.no_grad():
+            mask = mask_of(theta, beta_k, smooth_px)
+        cost, grad_m, signed = objective(mask)
+        taken = it
+        c = record(it, cost, grad_m, signed, mask)
+        if abs(prev_cost - c) < tol:
+            converged = True
+            break
+        prev_cost = c
+    try:
+        import pandas as pd
+        history = pd.DataFrame(rows)
+    except ImportError:  # pragma: no cover - pandas is a reference dependency
+        history = rows
+    mask_np = mask.double().cpu().numpy()
+    printed = print_mask(mask_np if model.precision == 64 else mask, model)
+    return ILTResult(mask=mask_np, theta=theta.double().cpu().numpy(), target=tgt > 0.5,
+                     printed=printed, history=history, iterations=taken, converged=converged)

@@ -712,3 +712,79 @@ def exposure_volume(mask, optics, grid, resist, dose=1.0):
     for k in range(flat.shape[1]):
         out[:, k] = np.interp(z_vox_depth, depths, flat[:, k])
     return out.reshape(nz, grid.n_pixels, grid.n_pixels), (np.arange(nz) + 0.5) * grid.dz
+
+
+# --------------------------------------------------------------------------
+# Print and solve (bake/peb.py:24-70, develop/resist.py:452-510,
+# opc/ilt.py:397-648)
+# --------------------------------------------------------------------------
+
+
+def apply_peb(pac_map, diffusion_sigma, pixel_size):
+    """peb.py:24-70: Gaussian of diffusion_sigma / pixel_size px, clipped to [0, 1]."""
+    sigma_px = diffusion_sigma / pixel_size
+    if sigma_px < 1e-3:
+        return np.array(pac_map, dtype=np.float64, copy=True)
+    from scipy.ndimage import gaussian_filter
+    return np.clip(gaussian_filter(np.asarray(pac_map, dtype=np.float64), sigma=sigma_px), 0.0, 1.0)
+
+
+def develop_field(aerial, resist, grid):
+    """resist.py:452-503, model "threshold": (PEB-diffused aerial, threshold)."""
+    return apply_peb(aerial, resist.diffusion_sigma, grid.pixel_size), float(resist.threshold)
+
+
+def run_ilt(target, optics, grid, resist, dose=1.0, tone="clear", *, max_iter=120, step=0.4,
+            tol=1e-7, beta=4.0, beta_final=12.0, initial_grey=0.9, steepness=12.0,
+            weight_tv=0.0, weight_discrete=0.0, smooth=0.0, socs_tol=1e-8):
+    """ilt.py:397-621 through SOCS kernels (tol socs_tol): Adam on theta, mask =
+    sigmoid(beta G(theta)), beta annealed geometrically, stop when the cost moves by
+    less than tol.  Returns (mask, theta, history rows, iterations, converged)."""
+    k = socs_build(optics, grid, tol=socs_tol)
+    tgt = np.asarray(target, dtype=np.float64)
+    smooth_px = float(smooth / grid.pixel_size)
+    scale = normalisation_scale(optics.normalisation, 1.0, k.clear, dose)
+    level = float(resist.threshold)
+    sigma_px = float(resist.diffusion_sigma / grid.pixel_size)
+    sign = 1.0 if tone == "clear" else -1.0
+
+    def objective(mask):
+        fields, raw = socs_fields(k, mask)
+        cost, d_raw, signed, perr = relaxed_objective(raw, tgt, scale, level, steepness, sign,
+                                                      sigma_px)
+        grad = socs_gradient(k, fields, d_raw)
+        c2, g2 = mask_penalties(mask, weight_tv, weight_discrete)
+        return cost + c2, grad + g2, signed, perr
+
+    p = float(np.clip(initial_grey, 0.5 + 1e-3, 1.0 - 1e-6))
+    m0 = 0.5 + (np.clip(tgt, 0.0, 1.0) - 0.5) * (2.0 * p - 1.0)
+    theta = np.log(m0 / (1.0 - m0)) / beta
+    growth = (1.0 if beta_final is None or max_iter <= 1
+              else (beta_final / beta) ** (1.0 / (max_iter - 1)))
+    mom1 = np.zeros_like(theta)
+    mom2 = np.zeros_like(theta)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    beta_k = float(beta)
+    mask = mask_of(theta, beta_k, smooth_px)
+    cost, grad_m, signed, perr = objective(mask)
+    rows = [{"iteration": 0, "cost": cost, "pattern_error": perr, "beta": beta_k,
+             "grad_norm": float(np.linalg.norm(grad_m))}]
+    prev_cost, converged, taken = np.inf, False, 0
+    for it in range(1, max_iter + 1):
+        grad_t = grad_theta(grad_m, mask, beta_k, smooth_px)
+        mom1 = b1 * mom1 + (1.0 - b1) * grad_t
+        mom2 = b2 * mom2 + (1.0 - b2) * grad_t ** 2
+        m_hat = mom1 / (1.0 - b1 ** it)
+        v_hat = mom2 / (1.0 - b2 ** it)
+        theta = theta - step * m_hat / (np.sqrt(v_hat) + eps)
+        beta_k *= growth
+        mask = mask_of(theta, beta_k, smooth_px)
+        cost, grad_m, signed, perr = objective(mask)
+        taken = it
+        rows.append({"iteration": it, "cost": cost, "pattern_error": perr, "beta": beta_k,
+                     "grad_norm": float(np.linalg.norm(grad_m))})
+        if abs(prev_cost - cost) < tol:
+            converged = True
+            break
+        prev_cost = cost
+    return mask, theta, rows, taken, converged

@@ -433,3 +433,52 @@ def test_blur_is_self_adjoint_and_objective_gradient_matches_fd():
         mm[i, j] -= h
         fd = (O.mask_penalties(mp, 0.3, 0.2)[0] - O.mask_penalties(mm, 0.3, 0.2)[0]) / (2 * h)
         assert g[i, j] == pytest.approx(fd, rel=1e-6, abs=1e-12)
+
+
+def test_apply_peb_and_develop_field():
+    """peb.py:24-70: blur then clip to [0, 1]; no blur below 1e-3 px; threshold model."""
+    from lithobat_amd.config import GridConfig
+    rng = np.random.default_rng(3)
+    a = rng.random((32, 32)) * 1.4
+    out = O.apply_peb(a, 10e-9, 4e-9)
+    assert out.min() >= 0.0 and out.max() <= 1.0
+    assert np.allclose(out, np.clip(O.blur(a, 2.5), 0.0, 1.0), atol=0, rtol=0)
+    same = O.apply_peb(a, 1e-15, 4e-9)
+    assert same is not a and np.array_equal(same, a)
+
+    class R:
+        diffusion_sigma, threshold = 8e-9, 0.3
+    fld, lvl = O.develop_field(a, R(), GridConfig(n_pixels=32, pixel_size=4e-9))
+    assert lvl == 0.3 and np.array_equal(fld, O.apply_peb(a, 8e-9, 4e-9))
+
+
+def test_run_ilt_descends_and_stops():
+    """The restated solve lowers the cost, anneals beta geometrically to beta_final, and its
+    first step equals a hand-computed Adam step (the bias-corrected first step is -step *
+    sign(g) wherever g != 0)."""
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    grid = GridConfig(n_pixels=32, pixel_size=8e-9)
+    optics = OpticsConfig(NA=1.35, n_immersion=1.44, source_type="annular", sigma_outer=0.9,
+                          sigma_inner=0.6, source_grid=9, normalisation="clear")
+
+    class R:
+        threshold, diffusion_sigma = 0.3, 8e-9
+    target = O.lines_and_spaces(32, 8e-9, 128e-9, 64e-9)
+    mask, theta, rows, taken, conv = O.run_ilt(target, optics, grid, R(), max_iter=6,
+                                               smooth=8e-9, weight_tv=1e-3)
+    assert taken == 6 and len(rows) == 7
+    assert rows[-1]["cost"] < rows[0]["cost"]
+    # beta_k = beta growth^k, growth = (beta_final / beta)^(1 / (max_iter - 1)): beta_final is
+    # reached at iteration max_iter - 1 and the last step goes one factor past it (ilt.py:538-541)
+    assert rows[5]["beta"] == pytest.approx(12.0, rel=1e-12)
+    assert rows[6]["beta"] == pytest.approx(12.0 * 3.0 ** 0.2, rel=1e-12)
+    # hand-computed first Adam step from the initial state
+    _, theta1, _, _, _ = O.run_ilt(target, optics, grid, R(), max_iter=1, beta_final=None,
+                                   smooth=8e-9, weight_tv=1e-3)
+    p = 0.9
+    m0 = 0.5 + (target - 0.5) * (2 * p - 1)
+    th0 = np.log(m0 / (1 - m0)) / 4.0
+    # bias-corrected first Adam step: |d theta| = step |g| / (|g| + eps), just under step
+    # (without the bias correction it would be step * 0.1 / sqrt(0.001) = 1.26 step)
+    d = np.abs(theta1 - th0)
+    assert np.all(d <= 0.4) and np.all(d >= 0.399)
