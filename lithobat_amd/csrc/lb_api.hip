@@ -329,7 +329,7 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
   Layout L;
   w.queue = L.take(256);  // work-item counter of the coherent-system launch
   w.Xm = L.take(C * n_masks * p.Wbm * p.Wbm);
-  w.RTm = L.take(C * (size_t)n_masks * p.Wbm * p.N);
+  w.RTm = L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
   w.peak = L.take(R * (size_t)w.n_img);
   {
@@ -343,20 +343,20 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
   }
   if (!p.direct) w.IM = L.take(R * (size_t)w.n_img * p.M * p.M);
   if (!p.direct) {
-    w.RTc = L.take(C * (size_t)w.n_img * p.WbI * p.M);
+    w.RTc = L.take(C * (size_t)w.n_img * band_ld(p.WbI) * p.M);
     w.Y = L.take(C * (size_t)w.n_img * p.WbI * p.WbI);
-    w.TT = L.take(C * (size_t)w.n_img * p.WbI * p.N);
+    w.TT = L.take(C * (size_t)w.n_img * band_ld(p.WbI) * p.N);
   }
   if (backward) {
     if (!p.direct) {
       w.gM = L.take(R * (size_t)w.n_img * p.M * p.M);
-      w.RTg = L.take(C * (size_t)w.n_img * p.WbI * p.N);
+      w.RTg = L.take(C * (size_t)w.n_img * band_ld(p.WbI) * p.N);
       w.GY = L.take(C * (size_t)w.n_img * p.WbI * p.WbI);
-      w.TTc = L.take(C * (size_t)w.n_img * p.WbI * p.M);
+      w.TTc = L.take(C * (size_t)w.n_img * band_ld(p.WbI) * p.M);
     }
     w.gX = L.take(C * (size_t)w.n_items * p.Wbm * p.Wbm);
     w.GX = L.take(C * (size_t)n_masks * p.Wbm * p.Wbm);
-    w.TTm = L.take(C * (size_t)n_masks * p.Wbm * p.N);
+    w.TTm = L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   }
   w.total = L.off;
   return w;
@@ -1372,9 +1372,9 @@ size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks) {
   const Plan& p = plan->p;
   Layout L;
   L.take(p.csz() * (size_t)n_masks * p.Wbm * p.Wbm);
-  L.take(p.csz() * (size_t)n_masks * p.Wbm * p.N);
+  L.take(p.csz() * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   L.take(p.csz() * (size_t)n_masks * p.spp * p.Wbm * p.Wbm);
-  L.take(p.csz() * (size_t)n_masks * p.spp * p.Wbm * p.N);
+  L.take(p.csz() * (size_t)n_masks * p.spp * band_ld(p.Wbm) * p.N);
   return L.off;
 }
 
@@ -1386,9 +1386,9 @@ static lb_status fields_t(Plan& p, const void* masks, int cplx, int n_masks, voi
   Layout L;
   const size_t C = p.csz();
   cx<T>* Xm = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.Wbm * p.Wbm));
-  cx<T>* RT = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.Wbm * p.N));
+  cx<T>* RT = (cx<T>*)(ws + L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N));
   cx<T>* Y = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.spp * p.Wbm * p.Wbm));
-  cx<T>* TT = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.spp * p.Wbm * p.N));
+  cx<T>* TT = (cx<T>*)(ws + L.take(C * (size_t)n_masks * p.spp * band_ld(p.Wbm) * p.N));
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
   LB_HIP(band_analysis<T>(p.N, masks, cplx, (long long)p.N * p.N * (cplx ? 2 : 1), 0, 1, n_masks,
                           p.nu0m, p.Wbm, invN2, RT, Xm, st));

@@ -296,14 +296,15 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_analysis_rows(cons
   constexpr int ROWS = CPLX ? TPB : 2 * TPB;
   const int y0 = CPLX ? tr_blk : 2 * tr_blk;
   const int nrows = min(ROWS, N - y0);
-  cx<T>* out = RT + (long long)img * Wb * N;
+  const int ld = band_ld(Wb);
+  cx<T>* out = RT + (long long)img * ld * N;
   const cx<T>* Zb = tw + PL;
   for (int t = tid; t < Wb * ROWS; t += NT) {
     const int yy = t / Wb, v = t - yy * Wb;
     if (yy >= nrows) continue;
     const int nu = nu0 + v;
     if (CPLX) {
-      out[(long long)(y0 + yy) * Wb + v] = Zb[yy * PL + pidx<T>(bin_of(nu, N))];
+      out[(long long)(y0 + yy) * ld + v] = Zb[yy * PL + pidx<T>(bin_of(nu, N))];
     } else {
       const int tr = yy >> 1;
       const cx<T> a = Zb[tr * PL + pidx<T>(bin_of(nu, N))];
@@ -315,7 +316,7 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_analysis_rows(cons
         const cx<T> d = a - b;     // (a - b) / (2i)
         r = cx<T>{d.im * (T)0.5, -d.re * (T)0.5};
       }
-      out[(long long)(y0 + yy) * Wb + v] = r;
+      out[(long long)(y0 + yy) * ld + v] = r;
     }
   }
 }
@@ -411,7 +412,8 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
   const int stride = gridDim.x * NW;
   int u = blockIdx.x * NW + wave;
   if (u < NU) fetch(u);
-  cx<T>* out = RT + (long long)img * Wb * N;
+  const int ld = band_ld(Wb);
+  cx<T>* out = RT + (long long)img * ld * N;
   for (; u < NU; u += stride) {
     cx<T> v[TPL][R];
 #pragma unroll
@@ -452,7 +454,7 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
           r = cx<T>{d.im * (T)0.5, -d.re * (T)0.5};
         }
       }
-      out[(long long)(y0 + yy) * Wb + vv] = r;
+      out[(long long)(y0 + yy) * ld + vv] = r;
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -474,10 +476,11 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(cons
   const int vb = blockIdx.x * CPB;
   const int ncol = max(0, min(CPB, Wb - vb));
   load_roots_padded(tw, twg, N, tid, NT);
-  const cx<T>* src = RT + (long long)img * N * Wb;
+  const int ld = band_ld(Wb);
+  const cx<T>* src = RT + (long long)img * N * ld;
   for (int t = tid; t < N * CPB; t += NT) {
     const int y = t / CPB, c = t - y * CPB;
-    if (c < ncol) buf[c * LD + pidx<T>(y)] = src[(long long)y * Wb + vb + c];
+    if (c < ncol) buf[c * LD + pidx<T>(y)] = src[(long long)y * ld + vb + c];
   }
   __syncthreads();
   const int c0 = wave * TPW;
@@ -529,10 +532,11 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const c
   auto store = [&](int tr, int y, cx<T> val) { wb[tr * LD + pidx<T>(y)] = val; };
   wave_fft<T, +1, N, TPW, LD, SP>(wb, tw, lane, ntr, load, store);
   __syncthreads();
-  cx<T>* out = TT + (long long)img * N * Wb;
+  const int ld = band_ld(Wb);
+  cx<T>* out = TT + (long long)img * N * ld;
   for (int t = tid; t < N * CPB; t += NT) {
     const int y = t / CPB, c = t - y * CPB;
-    if (c < ncol) out[(long long)y * Wb + vb + c] = buf[c * LD + pidx<T>(y)];
+    if (c < ncol) out[(long long)y * ld + vb + c] = buf[c * LD + pidx<T>(y)];
   }
 }
 
@@ -562,14 +566,15 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
   constexpr int NTR = COUT ? N : N / 2;
   const int tr0 = blockIdx.x * TPB + wave * TPW;
   const int ntr = max(0, min(TPW, NTR - tr0));
-  const cx<T>* src = TT + (long long)img * N * Wb;
+  const int ld = band_ld(Wb);
+  const cx<T>* src = TT + (long long)img * N * ld;
   auto load = [&](int tr, int b) -> cx<T> {
     const int v = band_pos(b, N, nu0, Wb);
     if (v < 0) return cx<T>{0, 0};
     const int p = tr0 + tr;
-    if (COUT) return src[(long long)p * Wb + v];
-    const cx<T> a = src[(long long)(2 * p) * Wb + v];
-    const cx<T> c = src[(long long)(2 * p + 1) * Wb + v];
+    if (COUT) return src[(long long)p * ld + v];
+    const cx<T> a = src[(long long)(2 * p) * ld + v];
+    const cx<T> c = src[(long long)(2 * p + 1) * ld + v];
     return cx<T>{a.re - c.im, a.im + c.re};  // a + i c
   };
   T lmax = 0;
@@ -607,7 +612,8 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
   constexpr int NTR = COUT ? N : N / 2;
   constexpr int NU = (NTR + TPW - 1) / TPW;
   constexpr int TASKS = TPW * S, TPL = (TASKS + 63) / 64;
-  const cx<T>* src = TT + (long long)img * N * Wb;
+  const int ld = band_ld(Wb);
+  const cx<T>* src = TT + (long long)img * N * ld;
   // per task slot: transform within the unit, and the band positions of its
   // two inputs (nu = j and nu = j - S), -1 outside the band
   int trs[TPL], plo[TPL], phi[TPL], jj[TPL];
@@ -630,14 +636,14 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
       a0[i] = a1[i] = b0[i] = b1[i] = cx<T>{0, 0};
       const int p = u * TPW + trs[i];
       if (trs[i] < TPW && p < NTR) {
-        const long long r0 = COUT ? (long long)p * Wb : (long long)(2 * p) * Wb;
+        const long long r0 = COUT ? (long long)p * ld : (long long)(2 * p) * ld;
         if (plo[i] >= 0) {
           a0[i] = src[r0 + plo[i]];
-          if (!COUT) a1[i] = src[r0 + Wb + plo[i]];
+          if (!COUT) a1[i] = src[r0 + ld + plo[i]];
         }
         if (phi[i] >= 0) {
           b0[i] = src[r0 + phi[i]];
-          if (!COUT) b1[i] = src[r0 + Wb + phi[i]];
+          if (!COUT) b1[i] = src[r0 + ld + phi[i]];
         }
       }
     }

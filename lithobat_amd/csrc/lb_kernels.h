@@ -27,15 +27,20 @@ inline hipError_t table_for(int L, const cx<T>** tw) {
 // ---- band transforms (lb_band.hip) ---------------------------------------
 bool band_size_supported(int n);
 
+// Row stride of the band x N intermediates (RT / TT, [img][y][v]): the band
+// width rounded up to 16 complex values, so rows start on 128-byte lines
+// (FP32) and the row-major copies move whole lines.
+__host__ __device__ inline int band_ld(int Wb) { return Wb < 16 ? Wb : (Wb + 15) / 16 * 16; }
+
 // X[img][u][v] = s * fft2(f_img)[nu0+u][nu0+v]; f_img = sum over nsum slices
-// (f real, or complex interleaved when cplx).  RT: scratch img x Wb x N.
+// (f real, or complex interleaved when cplx).  RT: scratch img x N x band_ld(Wb).
 template <typename T>
 hipError_t band_analysis(int N, const void* in, int cplx, long long img_stride,
                          long long slice_stride, int nsum, int nimg, int nu0, int Wb, T s,
                          cx<T>* RT, cx<T>* X, hipStream_t st);
 
 // out[img] = s * Re sum X e^{+...} (cout = 0; X Hermitianised when herm) or the
-// complex sum (cout = 1, out holds complex).  TT: scratch img x Wb x N.
+// complex sum (cout = 1, out holds complex).  TT: scratch img x N x band_ld(Wb).
 // peak (nullable, real output only): running max per image, must start at 0.
 template <typename T>
 hipError_t band_synthesis(int N, const cx<T>* Y, int nimg, int nu0, int Wb, int herm, T s,
