@@ -99,13 +99,22 @@ constexpr int row_waves() {
 // first and the last stride-S run of the line (a band of frequencies
 // |nu| < S, S = L / R), so each radix-R DFT has two inputs and reduces to
 // X_k = v_0 + v_{R-1} e^{DIR 2 pi i (R-1) k / R}.
-template <typename T, int DIR, int L, int LD, int TPW, int R, bool FIRST, bool LAST, class Load,
-          class Store, bool SPARSE = false>
-__device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane, int ntr, int Ns,
+template <typename T, int DIR, int L, int LD, int TPW, int R, int NS, bool FIRST, bool LAST,
+          class Load, class Store, bool SPARSE = false>
+__device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane, int ntr,
                                            Load& load, Store& store) {
   constexpr int S = L / R;
   constexpr int TASKS = TPW * S;
   constexpr int TPL = (TASKS + 63) / 64;
+  // Padded LDS offsets that are linear in r, so they fold into the
+  // instruction's immediate offset: reads j + r S when S is a whole number of
+  // 256-byte rows (BANK values); writes base + r NS when the task's R outputs
+  // stay in one BANK block (NS R divides BANK) or NS is a power of two and
+  // NS R a multiple of BANK (then the base's low part plus r NS never carries).
+  constexpr int SH = kPadShift<T>, BANK = 1 << SH;
+  constexpr bool RD_LIN = S % BANK == 0;
+  constexpr bool NS_POW2 = (NS & (NS - 1)) == 0;
+  constexpr bool WR_LIN = NS_POW2 && ((NS * R) % BANK == 0 || BANK % (NS * R) == 0);
   cx<T> v[TPL][R];
 #pragma unroll
   for (int i = 0; i < TPL; ++i) {
@@ -118,12 +127,19 @@ __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane,
         for (int k = 0; k < R; ++k) v[i][k] = a + mul_root<R, DIR, T>(b, (R - 1) * k);
         continue;
       }
+      const cx<T>* rd = wb + tr * LD + pidx<T>(j);
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        v[i][r] = FIRST ? load(tr, j + r * S) : wb[tr * LD + pidx<T>(j + r * S)];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (FIRST)
+          v[i][r] = load(tr, j + r * S);
+        else if constexpr (RD_LIN)
+          v[i][r] = rd[r * (S + (S >> SH))];
+        else
+          v[i][r] = wb[tr * LD + pidx<T>(j + r * S)];
+      }
       if (!FIRST) {
-        const int k = j % Ns;
-        const int step = L / (Ns * R);
+        const int k = j % NS;
+        constexpr int step = L / (NS * R);
 #pragma unroll
         for (int r = 1; r < R; ++r) v[i][r] = v[i][r] * tw_get<DIR>(tw, pidx<T>(r * k * step));
       }
@@ -136,14 +152,17 @@ __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane,
     const int t = lane + 64 * i;
     const int tr = t / S, j = t - (t / S) * S;
     if (t < TASKS && tr < ntr) {
-      const int k = j % Ns;
-      const int base = (j / Ns) * Ns * R + k;
+      const int k = j % NS;
+      const int base = (j / NS) * NS * R + k;
+      cx<T>* wr = wb + tr * LD + pidx<T>(base);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if (LAST)
-          store(tr, base + r * Ns, v[i][r]);
+        if constexpr (LAST)
+          store(tr, base + r * NS, v[i][r]);
+        else if constexpr (WR_LIN)
+          wr[r * NS + ((r * NS) >> SH)] = v[i][r];
         else
-          wb[tr * LD + pidx<T>(base + r * Ns)] = v[i][r];
+          wb[tr * LD + pidx<T>(base + r * NS)] = v[i][r];
       }
     }
   }
@@ -159,21 +178,16 @@ __device__ __forceinline__ void wave_fft(cx<T>* wb, const cx<T>* tw, int lane, i
                                          Store& store) {
   using P = Plan<L>;
   constexpr int NST = (P::R0 > 1) + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
-  int Ns = 1;
-  wave_stage<T, DIR, L, LD, TPW, P::R0, true, NST == 1, Load, Store, SPARSE>(wb, tw, lane, ntr, Ns,
-                                                                             load, store);
-  Ns *= P::R0;
-  if constexpr (P::R1 > 1) {
-    wave_stage<T, DIR, L, LD, TPW, P::R1, false, NST == 2>(wb, tw, lane, ntr, Ns, load, store);
-    Ns *= P::R1;
-  }
-  if constexpr (P::R2 > 1) {
-    wave_stage<T, DIR, L, LD, TPW, P::R2, false, NST == 3>(wb, tw, lane, ntr, Ns, load, store);
-    Ns *= P::R2;
-  }
-  if constexpr (P::R3 > 1) {
-    wave_stage<T, DIR, L, LD, TPW, P::R3, false, NST == 4>(wb, tw, lane, ntr, Ns, load, store);
-  }
+  wave_stage<T, DIR, L, LD, TPW, P::R0, 1, true, NST == 1, Load, Store, SPARSE>(wb, tw, lane, ntr,
+                                                                                load, store);
+  if constexpr (P::R1 > 1)
+    wave_stage<T, DIR, L, LD, TPW, P::R1, P::R0, false, NST == 2>(wb, tw, lane, ntr, load, store);
+  if constexpr (P::R2 > 1)
+    wave_stage<T, DIR, L, LD, TPW, P::R2, P::R0 * P::R1, false, NST == 3>(wb, tw, lane, ntr, load,
+                                                                          store);
+  if constexpr (P::R3 > 1)
+    wave_stage<T, DIR, L, LD, TPW, P::R3, P::R0 * P::R1 * P::R2, false, NST == 4>(
+        wb, tw, lane, ntr, load, store);
 }
 
 // Stages 2.. of a length-L transform whose first stage the caller has
@@ -185,16 +199,13 @@ __device__ __forceinline__ void wave_fft_tail(cx<T>* wb, const cx<T>* tw, int la
   constexpr int NST = (P::R0 > 1) + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
   static_assert(NST >= 2, "the tail needs a second stage");
   auto none = [](int, int) { return cx<T>{0, 0}; };
-  int Ns = P::R0;
-  wave_stage<T, DIR, L, LD, TPW, P::R1, false, NST == 2>(wb, tw, lane, ntr, Ns, none, store);
-  Ns *= P::R1;
-  if constexpr (P::R2 > 1) {
-    wave_stage<T, DIR, L, LD, TPW, P::R2, false, NST == 3>(wb, tw, lane, ntr, Ns, none, store);
-    Ns *= P::R2;
-  }
-  if constexpr (P::R3 > 1) {
-    wave_stage<T, DIR, L, LD, TPW, P::R3, false, NST == 4>(wb, tw, lane, ntr, Ns, none, store);
-  }
+  wave_stage<T, DIR, L, LD, TPW, P::R1, P::R0, false, NST == 2>(wb, tw, lane, ntr, none, store);
+  if constexpr (P::R2 > 1)
+    wave_stage<T, DIR, L, LD, TPW, P::R2, P::R0 * P::R1, false, NST == 3>(wb, tw, lane, ntr, none,
+                                                                          store);
+  if constexpr (P::R3 > 1)
+    wave_stage<T, DIR, L, LD, TPW, P::R3, P::R0 * P::R1 * P::R2, false, NST == 4>(
+        wb, tw, lane, ntr, none, store);
 }
 
 // Column kernels: CPB columns per block (block-cooperative transposition
@@ -427,8 +438,15 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
       if (trs[i] < ntr) {
         dft<R, -1>(v[i]);
         cx<T>* w = wb + trs[i] * PL;
+        cx<T>* w0 = w + pidx<T>(jj[i] * R);
+        constexpr bool LIN = (1 << kPadShift<T>) % R == 0;  // the R outputs share a bank row
 #pragma unroll
-        for (int k = 0; k < R; ++k) w[pidx<T>(jj[i] * R + k)] = v[i][k];
+        for (int k = 0; k < R; ++k) {
+          if constexpr (LIN)
+            w0[k] = v[i][k];
+          else
+            w[pidx<T>(jj[i] * R + k)] = v[i][k];
+        }
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -669,9 +687,16 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
     for (int i = 0; i < TPL; ++i) {
       if (trs[i] < ntr) {
         cx<T>* w = wb + trs[i] * PL;
+        cx<T>* w0 = w + pidx<T>(jj[i] * R);
+        constexpr bool LIN = (1 << kPadShift<T>) % R == 0;  // the R outputs share a bank row
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-          w[pidx<T>(jj[i] * R + k)] = lo[i] + mul_root<R, +1, T>(hi[i], (R - 1) * k);
+        for (int k = 0; k < R; ++k) {
+          const cx<T> val = lo[i] + mul_root<R, +1, T>(hi[i], (R - 1) * k);
+          if constexpr (LIN)
+            w0[k] = val;
+          else
+            w[pidx<T>(jj[i] * R + k)] = val;
+        }
       }
     }
     __builtin_amdgcn_wave_barrier();
