@@ -99,10 +99,14 @@ constexpr int row_waves() {
 // first and the last stride-S run of the line (a band of frequencies
 // |nu| < S, S = L / R), so each radix-R DFT has two inputs and reduces to
 // X_k = v_0 + v_{R-1} e^{DIR 2 pi i (R-1) k / R}.
+//
+// RTW: the stage's twiddles come from registers, rtw[i * (R - 1) + r - 1] for
+// task slot i and input r (see load_stage_tw), instead of the roots table.
 template <typename T, int DIR, int L, int LD, int TPW, int R, int NS, bool FIRST, bool LAST,
-          class Load, class Store, bool SPARSE = false>
+          class Load, class Store, bool SPARSE = false, bool RTW = false>
 __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane, int ntr,
-                                           Load& load, Store& store) {
+                                           Load& load, Store& store,
+                                           const cx<T>* rtw = nullptr) {
   constexpr int S = L / R;
   constexpr int TASKS = TPW * S;
   constexpr int TPL = (TASKS + 63) / 64;
@@ -141,7 +145,12 @@ __device__ __forceinline__ void wave_stage(cx<T>* wb, const cx<T>* tw, int lane,
         const int k = j % NS;
         constexpr int step = L / (NS * R);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[i][r] = v[i][r] * tw_get<DIR>(tw, pidx<T>(r * k * step));
+        for (int r = 1; r < R; ++r) {
+          if constexpr (RTW)
+            v[i][r] = v[i][r] * rtw[i * (R - 1) + r - 1];
+          else
+            v[i][r] = v[i][r] * tw_get<DIR>(tw, pidx<T>(r * k * step));
+        }
       }
       dft<R, DIR>(v[i]);
     }
@@ -190,22 +199,66 @@ __device__ __forceinline__ void wave_fft(cx<T>* wb, const cx<T>* tw, int lane, i
         wb, tw, lane, ntr, load, store);
 }
 
-// Stages 2.. of a length-L transform whose first stage the caller has
-// written into wb (Stockham order, span R0).
-template <typename T, int DIR, int L, int TPW, int LD, class Store>
-__device__ __forceinline__ void wave_fft_tail(cx<T>* wb, const cx<T>* tw, int lane, int ntr,
-                                              Store& store) {
+// Twiddle count of one non-first stage for one lane: task slots x (R - 1).
+template <int L, int TPW, int R>
+constexpr int stage_ntw() {
+  return R > 1 ? (TPW * (L / R) + 63) / 64 * (R - 1) : 0;
+}
+
+// The twiddles of one non-first stage for this lane's task slots, in the
+// order wave_stage<..., RTW = true> reads them.  They depend on the lane
+// only, so a persistent kernel loads them once and reuses them per unit.
+template <typename T, int DIR, int L, int TPW, int R, int NS>
+__device__ __forceinline__ void load_stage_tw(cx<T>* w, const cx<T>* tw, int lane) {
+  constexpr int S = L / R, TPL = (TPW * S + 63) / 64, step = L / (NS * R);
+#pragma unroll
+  for (int i = 0; i < TPL; ++i) {
+    const int t = lane + 64 * i;
+    const int k = (t - (t / S) * S) % NS;  // r * k * step < L: inside the table
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[i * (R - 1) + r - 1] = tw_get<DIR>(tw, pidx<T>(r * k * step));
+  }
+}
+
+// Register copy of the twiddles of stages 2.. of a length-L transform, in
+// single precision: stage after stage while the running count stays within
+// BUDGET complex values per lane; the other stages read the roots table.
+template <typename T, int DIR, int L, int TPW, int BUDGET>
+struct TailTw {
   using P = Plan<L>;
+  static constexpr int N1 = stage_ntw<L, TPW, P::R1>(), N2 = stage_ntw<L, TPW, P::R2>(),
+                       N3 = stage_ntw<L, TPW, P::R3>();
+  static constexpr bool ON1 = sizeof(T) == 4 && N1 > 0 && N1 <= BUDGET;
+  static constexpr bool ON2 = ON1 && N2 > 0 && N1 + N2 <= BUDGET;
+  static constexpr bool ON3 = ON2 && N3 > 0 && N1 + N2 + N3 <= BUDGET;
+  cx<T> w1[ON1 ? N1 : 1], w2[ON2 ? N2 : 1], w3[ON3 ? N3 : 1];
+  __device__ __forceinline__ void load(const cx<T>* tw, int lane) {
+    if constexpr (ON1) load_stage_tw<T, DIR, L, TPW, P::R1, P::R0>(w1, tw, lane);
+    if constexpr (ON2) load_stage_tw<T, DIR, L, TPW, P::R2, P::R0 * P::R1>(w2, tw, lane);
+    if constexpr (ON3) load_stage_tw<T, DIR, L, TPW, P::R3, P::R0 * P::R1 * P::R2>(w3, tw, lane);
+  }
+};
+
+// Stages 2.. of a length-L transform whose first stage the caller has
+// written into wb (Stockham order, span R0); twiddles from `rt` when it holds
+// them, else from the roots table.
+template <typename T, int DIR, int L, int TPW, int LD, class Store, int BUDGET>
+__device__ __forceinline__ void wave_fft_tail(cx<T>* wb, const cx<T>* tw, int lane, int ntr,
+                                              Store& store,
+                                              const TailTw<T, DIR, L, TPW, BUDGET>& rt) {
+  using P = Plan<L>;
+  using Q = TailTw<T, DIR, L, TPW, BUDGET>;
   constexpr int NST = (P::R0 > 1) + (P::R1 > 1) + (P::R2 > 1) + (P::R3 > 1);
   static_assert(NST >= 2, "the tail needs a second stage");
   auto none = [](int, int) { return cx<T>{0, 0}; };
-  wave_stage<T, DIR, L, LD, TPW, P::R1, P::R0, false, NST == 2>(wb, tw, lane, ntr, none, store);
+  wave_stage<T, DIR, L, LD, TPW, P::R1, P::R0, false, NST == 2, decltype(none), Store, false,
+             Q::ON1>(wb, tw, lane, ntr, none, store, rt.w1);
   if constexpr (P::R2 > 1)
-    wave_stage<T, DIR, L, LD, TPW, P::R2, P::R0 * P::R1, false, NST == 3>(wb, tw, lane, ntr, none,
-                                                                          store);
+    wave_stage<T, DIR, L, LD, TPW, P::R2, P::R0 * P::R1, false, NST == 3, decltype(none), Store,
+               false, Q::ON2>(wb, tw, lane, ntr, none, store, rt.w2);
   if constexpr (P::R3 > 1)
-    wave_stage<T, DIR, L, LD, TPW, P::R3, P::R0 * P::R1 * P::R2, false, NST == 4>(
-        wb, tw, lane, ntr, none, store);
+    wave_stage<T, DIR, L, LD, TPW, P::R3, P::R0 * P::R1 * P::R2, false, NST == 4,
+               decltype(none), Store, false, Q::ON3>(wb, tw, lane, ntr, none, store, rt.w3);
 }
 
 // Column kernels: CPB columns per block (block-cooperative transposition
@@ -423,6 +476,9 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
   const int stride = gridDim.x * NW;
   int u = blockIdx.x * NW + wave;
   if (u < NU) fetch(u);
+  wait_vmem_all();  // no loads in flight at the loop head (see wait_vmem_all)
+  TailTw<T, -1, N, TPW, 32> rt;  // 3 waves per SIMD: room for 64 more VGPRs
+  rt.load(tw, lane);
   const int ld = band_ld(Wb);
   cx<T>* out = RT + (long long)img * ld * N;
   for (; u < NU; u += stride) {
@@ -451,7 +507,7 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
     }
     __builtin_amdgcn_wave_barrier();
     auto store = [&](int tr, int x, cx<T> val) { wb[tr * PL + pidx<T>(x)] = val; };
-    wave_fft_tail<T, -1, N, TPW, PL>(wb, tw, lane, ntr, store);
+    wave_fft_tail<T, -1, N, TPW, PL>(wb, tw, lane, ntr, store, rt);
     // band columns of this unit's rows, RT[img][y][v]
     const int y0 = u * ROWS_U;
     const int nrows = min(ROWS_U, N - y0);
@@ -670,6 +726,9 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
   const int stride = gridDim.x * nw;
   int u = blockIdx.x * nw + wave;
   if (u < NU) fetch(u);
+  wait_vmem_all();  // no loads in flight at the loop head (see wait_vmem_all)
+  TailTw<T, +1, N, TPW, 16> rt;  // 4 waves per SIMD: 128 VGPRs in all
+  rt.load(tw, lane);
   T lmax = 0;
   for (; u < NU; u += stride) {
     cx<T> lo[TPL], hi[TPL];
@@ -706,13 +765,14 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
       if (COUT) {
         reinterpret_cast<cx<T>*>(out)[((long long)img * N + p) * N + x] = val;
       } else {
+        // the image is written once and not read back here: streaming stores
         T* o = out + (long long)img * N * N + (long long)(2 * p) * N + x;
-        o[0] = val.re;
-        o[N] = val.im;
+        __builtin_nontemporal_store(val.re, o);
+        __builtin_nontemporal_store(val.im, o + N);
         lmax = fmax(lmax, fmax(val.re, val.im));
       }
     };
-    wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store);
+    wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store, rt);
   }
   if (!COUT && peak) {
     for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));

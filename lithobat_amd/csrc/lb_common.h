@@ -38,6 +38,14 @@ __device__ __forceinline__ cx<T> mulc(cx<T> a, cx<T> b) {
 }
 template <typename T>
 __device__ __forceinline__ T norm2(cx<T> a) { return a.re * a.re + a.im * a.im; }
+
+// Wait until every vector-memory access of this wave has completed
+// (s_waitcnt vmcnt(0), other counters unconstrained).  Placed after the
+// prologue prefetch of a persistent loop: the compiler's wait analysis merges
+// the loop-entry state (prefetch loads still in flight) with the back edge
+// (the previous iteration's stores in flight) and would otherwise wait for
+// all of those stores at the loop head of every iteration.
+__device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 // multiply by (DIR * i)
 template <int DIR, typename T>
 __device__ __forceinline__ cx<T> mul_i(cx<T> a) {
