@@ -158,6 +158,10 @@ struct RowPre {
   static constexpr int NCH = (MAXW + F::CPR - 1) / F::CPR;
   static constexpr int NR = (MAXW + F::T1 - 1) / F::T1 < F::R1 ? (MAXW + F::T1 - 1) / F::T1 : F::R1;
   static constexpr bool ON = NCH * NR <= (sizeof(T) == 4 ? 16 : 4);
+  // With the inputs in registers the forward row pass transforms all W rows
+  // at once, so its scratch buffer Cb holds up to MAXW rows; the adjoint (and
+  // the row pass without RowPre) runs in chunks of CPR rows.
+  static constexpr int CB_ROWS_FWD = ON && MAXW > F::CPR ? MAXW : F::CPR;
   cx<T> x[NCH][NR], h[NCH][NR];
 
   __device__ __forceinline__ void fetch(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
@@ -183,19 +187,22 @@ struct RowPre {
   }
 };
 
-// row_pass with the stage-1 inputs taken from a RowPre.
-template <typename T, int M, int NT, int CSPLIT>
+// row_pass with the stage-1 inputs taken from a RowPre.  ALL: stage 1 of all
+// W rows (thread (rl1, j1) takes rows c * CPR + rl1), one barrier, stage 2 of
+// all W rows (W * T2 tasks over the block), one barrier, with S holding
+// CB_ROWS_FWD rows; else chunks of CPR rows, two barriers each.
+template <bool ALL, typename T, int M, int NT, int CSPLIT>
 __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre, int W,
                                              const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs,
                                              int tid) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   using P = RowPre<T, M, NT, CSPLIT>;
   const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
+  if constexpr (!ALL) {
 #pragma unroll
-  for (int c = 0; c < P::NCH; ++c) {
-    const int a0 = c * F::CPR;
-    if (a0 >= W) break;  // uniform over the block
-    {
+    for (int c = 0; c < P::NCH; ++c) {
+      const int a0 = c * F::CPR;
+      if (a0 >= W) break;  // uniform over the block
       if (a0 + rl1 < W) {
         cx<T> v[F::R1];
 #pragma unroll
@@ -204,30 +211,65 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
 #pragma unroll
         for (int r = 0; r < F::R1; ++r) S[rl1 * F::LDC + j1 * F::R1 + r] = v[r];
       }
-    }
-    __syncthreads();
+      __syncthreads();
 #pragma unroll
-    for (int q = 0; q < F::Q2; ++q) {
-      const int t = tid + q * NT;
-      const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
-      const int a = a0 + rl;
-      if (t < F::NT2 && a < W) {
-        cx<T> v[F::R2];
+      for (int q = 0; q < F::Q2; ++q) {
+        const int t = tid + q * NT;
+        const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+        const int a = a0 + rl;
+        if (t < F::NT2 && a < W) {
+          cx<T> v[F::R2];
 #pragma unroll
-        for (int r = 0; r < F::R2; ++r) {
-          v[r] = S[rl * F::LDC + j2 + r * F::R1];
-          if (r > 0) v[r] = v[r] * tw[r * j2];
-        }
-        dft<F::R2, +1>(v);
+          for (int r = 0; r < F::R2; ++r) {
+            v[r] = S[rl * F::LDC + j2 + r * F::R1];
+            if (r > 0) v[r] = v[r] * tw[r * j2];
+          }
+          dft<F::R2, +1>(v);
 #pragma unroll
-        for (int r = 0; r < F::R2; ++r) {
-          const int x = j2 + r * F::R1 - cs * F::MC;
-          if (CSPLIT == 1 || (x >= 0 && x < F::MC)) Tb[a * F::MC + x] = v[r];
+          for (int r = 0; r < F::R2; ++r) {
+            const int x = j2 + r * F::R1 - cs * F::MC;
+            if (CSPLIT == 1 || (x >= 0 && x < F::MC)) Tb[a * F::MC + x] = v[r];
+          }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
+    return;
   }
+#pragma unroll
+  for (int c = 0; c < P::NCH; ++c) {
+    const int a = c * F::CPR + rl1;
+    if (a < W) {
+      cx<T> v[F::R1];
+#pragma unroll
+      for (int r = 0; r < F::R1; ++r) v[r] = r < P::NR ? pre.x[c][r] * pre.h[c][r] : cx<T>{0, 0};
+      dft<F::R1, +1>(v);
+#pragma unroll
+      for (int r = 0; r < F::R1; ++r) S[a * F::LDC + j1 * F::R1 + r] = v[r];
+    }
+  }
+  __syncthreads();
+  constexpr int Q2R = (P::MAXW * F::T2 + NT - 1) / NT;
+#pragma unroll
+  for (int q = 0; q < Q2R; ++q) {
+    const int t = tid + q * NT;
+    const int a = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+    if (a < W) {
+      cx<T> v[F::R2];
+#pragma unroll
+      for (int r = 0; r < F::R2; ++r) {
+        v[r] = S[a * F::LDC + j2 + r * F::R1];
+        if (r > 0) v[r] = v[r] * tw[r * j2];
+      }
+      dft<F::R2, +1>(v);
+#pragma unroll
+      for (int r = 0; r < F::R2; ++r) {
+        const int x = j2 + r * F::R1 - cs * F::MC;
+        if (CSPLIT == 1 || (x >= 0 && x < F::MC)) Tb[a * F::MC + x] = v[r];
+      }
+    }
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------- work items
@@ -327,8 +369,8 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
   using F = FieldShape<T, M, NT, CSPLIT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
-  cx<T>* Cb = tw + M;                              // CPR x LDC (row scratch too)
-  cx<T>* Tb = Cb + F::CPR * F::LDC;                // W x MC
+  cx<T>* Cb = tw + M;                              // CB_ROWS_FWD x LDC (row scratch too)
+  cx<T>* Tb = Cb + RowPre<T, M, NT, CSPLIT>::CB_ROWS_FWD * F::LDC;  // W x MC
   const int tid = threadIdx.x;
   load_roots(tw, twg, M, tid, NT);
   __syncthreads();
@@ -355,7 +397,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
       for (int j = s0; j < s1; ++j) {
         const T w = wsys[j];
         if constexpr (Pre::ON) {
-          row_pass_pre<T, M, NT, CSPLIT>(pre, W, tw, Cb, Tb, cs, tid);
+          row_pass_pre<true>(pre, W, tw, Cb, Tb, cs, tid);
           // the next system's inputs load while this one runs its columns
           const int jn = j + 1 < s1 ? j + 1 : (cs + 1 < CSPLIT ? s0 : -1);
           if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1);
@@ -467,7 +509,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
         const T w2 = (T)2 * wsys[j];
         const cx<T>* Hj = H + (long long)j * W * W;
         if constexpr (Pre::ON) {
-          row_pass_pre<T, M, NT, CSPLIT>(pre, W, tw, Cb, Tb, cs, tid);
+          row_pass_pre<false>(pre, W, tw, Cb, Tb, cs, tid);
           const int jn = j + 1 < s1 ? j + 1 : (cs + 1 < CSPLIT ? s0 : -1);
           if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1r);
         } else {
@@ -606,7 +648,8 @@ bool field_config_supported(int precision, const FieldConfig& c) {
 template <typename T, int M, int NT, int CS>
 static size_t smem_for(int W, bool adjoint) {
   using F = FieldShape<T, M, NT, CS>;
-  size_t n = M + F::CPR * F::LDC + (size_t)W * F::MC * (adjoint ? 2 : 1);
+  const int cb_rows = adjoint ? F::CPR : RowPre<T, M, NT, CS>::CB_ROWS_FWD;
+  size_t n = M + (size_t)cb_rows * F::LDC + (size_t)W * F::MC * (adjoint ? 2 : 1);
   return n * sizeof(cx<T>);
 }
 
