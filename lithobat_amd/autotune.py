@@ -100,16 +100,23 @@ def _time(fn, reps: int) -> float:
 
 
 def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
-         chunks=(0, 1, 2, 4, 8, 12, 16, 24, 32), save: bool = True) -> dict:
-    """Time every candidate for ``masks`` (the real batch) and record the winners."""
+         chunks=(0, 1, 2, 4, 8, 12, 16, 24, 32), save: bool = True,
+         candidates: bool = False) -> dict:
+    """Time every candidate for ``masks`` (the real batch) and record the winners.
+
+    ``candidates``: also return every candidate's time under ``"candidates"``
+    (the saved table keeps the winners only).
+    """
     n_masks = masks.shape[0] if masks.dim() == 3 else 1
     plan._tuned_for.add(n_masks)  # keep the table from overriding the candidates
     P = len(plan.planes)
     results = {}
     spp = plan.info["n_systems"]
     chunk_cands = sorted({c for c in chunks if c <= max(spp, 1)})
+    timings = {}
     for adj in ([False, True] if adjoint else [False]):
         best = None
+        cands = timings.setdefault(key_of(plan, n_masks, adj), [])
         d_img = torch.randn((n_masks, P, plan.n, plan.n), device=masks.device,
                             dtype=plan.real_dtype) if adj else None
         for nt, cs in shapes(plan, adj):
@@ -119,6 +126,7 @@ def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
                     ms = _time(lambda: plan.backward(masks, d_img), reps)
                 else:
                     ms = _time(lambda: plan.forward(masks, with_peak=False), reps)
+                cands.append({"threads": nt, "splits": cs, "chunk": ch, "ms": round(ms, 4)})
                 if best is None or ms < best[0]:
                     best = (ms, nt, cs, ch)
         ms, nt, cs, ch = best
@@ -130,4 +138,7 @@ def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
         with _lock:
             table.update(results)
         save_table(table)
+    if candidates:
+        for k, v in timings.items():
+            results[k] = dict(results[k], candidates=v)
     return results
