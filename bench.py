@@ -195,7 +195,44 @@ def ilt_leg(args, ilt, dev, sync):
         loss.backward()
 
     blurred = timed(blur_step, args.ilt_steps, 2, sync)
-    return one, batched, dense, blurred
+
+    # the same autograd steps (forward, loss, backward) captured once into a graph and
+    # replayed: the single-mask step is bound by host launches when run eagerly
+    g_one = graph_step_seconds(ilt_step, lambda: theta.grad, args, sync)
+    g_batch = graph_step_seconds(batch_step, lambda: thetas.grad, args, sync)
+    return one, batched, dense, blurred, g_one, None if g_batch is None else g_batch / ILT_BATCH
+
+
+def graph_step_seconds(fn, result, args, sync):
+    """Seconds per call of ``fn`` captured into one graph and replayed, or None when graphs
+    are off or a replay's ``result()`` differs from an eager call's (relative 1e-5)."""
+    import torch
+    if args.no_graph:
+        return None
+    fn()
+    sync()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fn()
+    torch.cuda.current_stream().wait_stream(side)
+    sync()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        fn()
+    sync()
+    sec = timed(graph.replay, max(3, args.ilt_steps), 2, sync)
+    graph.replay()
+    sync()
+    replayed = result().clone()
+    fn()
+    sync()
+    eager = result()
+    err = float((replayed - eager).abs().max() / eager.abs().max().clamp_min(1e-30))
+    if not err <= 1e-5:
+        print(f"graph replay differs from eager (rel {err:.3g}); not reported", file=sys.stderr)
+        return None
+    return sec
 
 
 def clips_leg(args, dev, sync):
@@ -461,9 +498,11 @@ def main():
 
     # ---------------- ILT forward + backward at 1024^2 ---------------------------
     ilt_s = ilt_batched_s = ilt_dense_s = ilt_blur_s = vol_s = None
+    ilt_graph_s = ilt_graph_batched_s = None
     clip_ilt_s = clip_plain_s = dense_images_per_s = None
     if "ilt" in legs:
-        ilt_s, ilt_batched_s, ilt_dense_s, ilt_blur_s = ilt_leg(args, ilt, dev, sync)
+        (ilt_s, ilt_batched_s, ilt_dense_s, ilt_blur_s, ilt_graph_s,
+         ilt_graph_batched_s) = ilt_leg(args, ilt, dev, sync)
     if "volume" in legs:
         vol_s = volume_leg(args, masks, sync)
     solve = run_ilt_leg(args, sync) if "solve" in legs else None
@@ -530,6 +569,8 @@ def main():
         "ilt_steps_per_s": ratio(1.0, ilt_s),
         "ilt_batched_steps_per_s": ratio(1.0, ilt_batched_s),
         "ilt_blur_steps_per_s": ratio(1.0, ilt_blur_s),
+        "ilt_graph_steps_per_s": ratio(1.0, ilt_graph_s),
+        "ilt_graph_batched_steps_per_s": ratio(1.0, ilt_graph_batched_s),
         "exposure_volumes_per_s": ratio(1.0, vol_s),
         "run_ilt_iterations_per_s": None if solve is None else 1.0 / solve[0],
         "run_ilt_config": "512^2 @ 8 nm, NA 1.35 annular (142 pts) SOCS kernels FP32, "
@@ -545,7 +586,9 @@ def main():
                       "threshold 0.3 steepness 50, FP32 forward + adjoint; batched = "
                       f"{ILT_BATCH} masks per call (steps/s counts masks); plain dense = "
                       "one dense fft2/ifft2 pair per source point, host sync per step; "
-                      "blur = PEB sigma 2 px and mask smoothing 1 px on",
+                      "blur = PEB sigma 2 px and mask smoothing 1 px on; graph = the "
+                      "same autograd step captured once into a graph and replayed, 1 mask "
+                      f"and {ILT_BATCH} masks per step",
         "socs": socs,
         "fp32_rel_err_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,

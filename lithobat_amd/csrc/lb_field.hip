@@ -161,7 +161,13 @@ struct RowPre {
   // With the inputs in registers the forward row pass transforms all W rows
   // at once, so its scratch buffer Cb holds up to MAXW rows; the adjoint (and
   // the row pass without RowPre) runs in chunks of CPR rows.
-  static constexpr int CB_ROWS_FWD = ON && MAXW > F::CPR ? MAXW : F::CPR;
+  // DB: the forward column rounds alternate between two CPR-row halves of Cb,
+  // so a round needs one barrier (between its two radix stages) instead of
+  // two; used when the second half costs at most 16 rows beyond the row pass's
+  // own scratch.
+  static constexpr bool DB = ON && F::ROUNDS >= 2 && 2 * F::CPR <= MAXW + 16;
+  static constexpr int CB_ROWS_FWD =
+      DB ? (MAXW > 2 * F::CPR ? MAXW : 2 * F::CPR) : (ON && MAXW > F::CPR ? MAXW : F::CPR);
   cx<T> x[NCH][NR], h[NCH][NR];
 
   __device__ __forceinline__ void fetch(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
@@ -405,8 +411,13 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
           row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, H + (long long)j * W * W, W, box[j], tw, Cb,
                                      Tb, cs, tid);
         }
+        // Column rounds.  With Pre::DB round rd uses half (rd & 1) of Cb: a
+        // round's stage 1 may overwrite the other half while slower threads
+        // still read this one, and every thread has left round rd - 2 when it
+        // passes round rd - 1's barrier.
 #pragma unroll
         for (int rd = 0; rd < F::ROUNDS; ++rd) {
+          cx<T>* Cr = Pre::DB ? Cb + (rd & 1) * F::CPR * F::LDC : Cb;
           {
             cx<T> v[F::R1];
             const int xl = rd * F::CPR + c1;
@@ -417,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
             }
             dft<F::R1, +1>(v);
 #pragma unroll
-            for (int r = 0; r < F::R1; ++r) Cb[c1 * F::LDC + jc1 * F::R1 + r] = v[r];
+            for (int r = 0; r < F::R1; ++r) Cr[c1 * F::LDC + jc1 * F::R1 + r] = v[r];
           }
           __syncthreads();
 #pragma unroll
@@ -428,7 +439,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
               cx<T> v[F::R2];
 #pragma unroll
               for (int r = 0; r < F::R2; ++r) {
-                v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
+                v[r] = Cr[c2 * F::LDC + j2 + r * F::R1];
                 if (r > 0) v[r] = v[r] * tw[r * j2];
               }
               dft<F::R2, +1>(v);
@@ -437,8 +448,9 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
                 acc[(rd * F::Q2 + q) * F::R2 + r] += w * norm2(v[r]);
             }
           }
-          __syncthreads();
+          if (!Pre::DB) __syncthreads();
         }
+        if (Pre::DB) __syncthreads();  // the next row pass writes over both halves
       }
       T* out = iv.part;
 #pragma unroll
