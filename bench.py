@@ -158,8 +158,8 @@ def ilt_leg(args, ilt, dev, sync):
 
     def ilt_step():
         theta.grad = None
-        raw = abbe_intensity(mask_of(theta, 8.0), ilt)[:, 0]
-        loss = relaxed_resist_loss(raw, target, scale, 0.3, 50.0)
+        raw = abbe_intensity(mask_of(theta, 8.0), ilt)  # [1, 1, N, N]: one plane
+        loss = relaxed_resist_loss(raw, target[:, None], scale, 0.3, 50.0)
         loss.backward()
 
     one = timed(ilt_step, args.ilt_steps, 2, sync)
@@ -171,8 +171,8 @@ def ilt_leg(args, ilt, dev, sync):
 
     def batch_step():
         thetas.grad = None
-        raw = abbe_intensity(mask_of(thetas, 8.0), ilt)[:, 0]
-        loss = relaxed_resist_loss(raw, targets, scale, 0.3, 50.0)
+        raw = abbe_intensity(mask_of(thetas, 8.0), ilt)
+        loss = relaxed_resist_loss(raw, targets[:, None], scale, 0.3, 50.0)
         loss.backward()
 
     batched = timed(batch_step, max(2, args.ilt_steps // 2), 1, sync) / ILT_BATCH
@@ -190,8 +190,8 @@ def ilt_leg(args, ilt, dev, sync):
 
     def blur_step():  # the reference's objective with its blurs on: PEB 2 px, mask smoothing 1 px
         theta.grad = None
-        raw = abbe_intensity(mask_of(theta, 8.0, 1.0), ilt)[:, 0]
-        loss = relaxed_resist_loss(raw, target, scale, 0.3, 50.0, sigma_px=2.0)
+        raw = abbe_intensity(mask_of(theta, 8.0, 1.0), ilt)
+        loss = relaxed_resist_loss(raw, target[:, None], scale, 0.3, 50.0, sigma_px=2.0)
         loss.backward()
 
     blurred = timed(blur_step, args.ilt_steps, 2, sync)
@@ -418,6 +418,8 @@ def main():
         res = autotune.tune(plan, masks, adjoint=False, save=False)
         th = torch.randn((1, N, N), device=dev)
         res.update(autotune.tune(ilt, torch.sigmoid(8 * th), adjoint=True, save=False))
+        thb = torch.randn((ILT_BATCH, N, N), device=dev)
+        res.update(autotune.tune(ilt, torch.sigmoid(8 * thb), adjoint=True, save=False))
         one = ImagingPlan(optics, grid, planes[:1], precision=32)
         res.update(autotune.tune(one, masks[:1], adjoint=False, save=False))
         table = autotune.load_table()

@@ -177,6 +177,34 @@ lb_status lb_gaussian_blur(const void* in, void* out, int32_t n_img, int32_t n, 
                            int32_t precision, void* workspace, size_t workspace_bytes,
                            void* stream);
 
+/* The relaxed-resist objective of opc/ilt.py:311-337 (_Objective.__call__,
+ * without the TV / discreteness penalties) over n_img images of npix pixels
+ * each, in one pass: f = scale * intensity (pass the PEB-blurred field with
+ * scale 1 when sigma_px > 0), signed = sign * (f - level),
+ * z = 1 / (1 + exp(-clip(a * signed, -60, 60))), a = steepness / max(|level|,
+ * 1e-12); cost[b] = sum (z - target)^2 / npix and total = sum_b cost[b];
+ * d_intensity = sign * (2 / npix) * (z - target) * a * z (1 - z) * scale (the
+ * reference's d_intensity before its blur); mismatch[b] =
+ * count_nonzero((signed > 0) != (target > 0.5)) (ilt.py:357).  intensity and
+ * target are [n_img][npix] in the given precision; cost and total are
+ * device scalars of that precision; every output may be NULL.  Costs are
+ * reduced in a fixed order (deterministic). */
+size_t lb_resist_objective_workspace_bytes(int32_t n_img, int64_t npix);
+lb_status lb_resist_objective(const void* intensity, const void* target, int32_t n_img,
+                              int64_t npix, double scale, double level, double steepness,
+                              double sign, void* cost, void* total, void* d_intensity,
+                              void* signed_field, int32_t* mismatch, int32_t precision,
+                              void* workspace, size_t workspace_bytes, void* stream);
+
+/* The ILT mask parameterisation (opc/ilt.py:368-389, _mask_of / _grad_theta
+ * without their Gaussian, which lb_gaussian_blur applies): mask =
+ * 1 / (1 + exp(-clip(beta * x, -60, 60))) and grad_x = grad_mask * beta *
+ * mask * (1 - mask), over n values. */
+lb_status lb_mask_sigmoid(const void* x, int64_t n, double beta, void* mask, int32_t precision,
+                          void* stream);
+lb_status lb_mask_sigmoid_grad(const void* grad_mask, const void* mask, int64_t n, double beta,
+                               void* grad_x, int32_t precision, void* stream);
+
 /* Batches of clips (no direct counterpart in the reference, which images one
  * mask per call of compute_aerial_image / runs one run_ilt per clip): plans of
  * different grid sizes and optics -- one mask each -- imaged together.  The

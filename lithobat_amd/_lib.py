@@ -47,6 +47,8 @@ EXPORTS = [
     "lb_plan_set_shape", "lb_batch_create", "lb_batch_destroy", "lb_batch_workspace_bytes",
     "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
     "lb_backward_dense", "lb_gaussian_blur_workspace_bytes", "lb_gaussian_blur",
+    "lb_resist_objective_workspace_bytes", "lb_resist_objective", "lb_mask_sigmoid",
+    "lb_mask_sigmoid_grad",
 ]
 
 _lib = None
@@ -89,6 +91,11 @@ def load() -> C.CDLL:
         "lb_backward_dense": ([vp, vp, i32, i32, vp, vp, vp, sz, vp], C.c_int),
         "lb_gaussian_blur_workspace_bytes": ([i32, i32, C.c_double, i32], sz),
         "lb_gaussian_blur": ([vp, vp, i32, i32, C.c_double, i32, vp, sz, vp], C.c_int),
+        "lb_resist_objective_workspace_bytes": ([i32, C.c_int64], sz),
+        "lb_resist_objective": ([vp, vp, i32, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, vp, vp, vp, vp, vp, i32, vp, sz, vp], C.c_int),
+        "lb_mask_sigmoid": ([vp, C.c_int64, C.c_double, vp, i32, vp], C.c_int),
+        "lb_mask_sigmoid_grad": ([vp, vp, C.c_int64, C.c_double, vp, i32, vp], C.c_int),
         "lb_batch_create": ([C.POINTER(vp), i32, plan_p], C.c_int),
         "lb_batch_destroy": ([vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),

@@ -13,7 +13,7 @@ import ctypes as C
 
 import torch
 
-from . import _lib
+from . import _lib, objective
 from .plan import ImagingPlan
 
 
@@ -135,9 +135,12 @@ class ILTStep:
     def __init__(self, batch: ImagingBatch, thetas, targets, beta=8.0, level=0.3,
                  steepness=50.0):
         self.batch = batch
-        self.beta, self.level, self.a = beta, level, steepness / max(abs(level), 1e-12)
+        self.beta, self.level, self.steep = beta, level, steepness
         self.theta = [t.to(batch.real_dtype).contiguous() for t in thetas]
         self.target = [t.to(batch.real_dtype).contiguous() for t in targets]
+        # the target of every plane of the clip, as the objective reads it
+        self.target_planes = [t.expand(len(p.planes), *t.shape[-2:]).contiguous()
+                              for p, t in zip(batch.plans, self.target)]
         self.scale = [1.0 / float(p.clear[0]) if p.clear[0] > 0 else 1.0 for p in batch.plans]
         self.mask = [torch.empty_like(t) for t in self.theta]
         self.raw = [torch.empty((len(p.planes), p.n, p.n), dtype=batch.real_dtype,
@@ -146,22 +149,22 @@ class ILTStep:
         self.d_mask = [torch.empty_like(t) for t in self.theta]
         self.grad = [torch.empty_like(t) for t in self.theta]
         self.loss = torch.zeros((), dtype=batch.real_dtype, device=self.theta[0].device)
+        self.costs = torch.zeros(len(self.theta), dtype=batch.real_dtype,
+                                 device=self.theta[0].device)
+        self.obj_ws = [torch.empty(max(objective.workspace_bytes(len(r), r.shape[-1] ** 2), 256),
+                                   dtype=torch.uint8, device=r.device) for r in self.raw]
         batch.workspace(False, self.theta[0].device)
         batch.workspace(True, self.theta[0].device)
 
     def run(self) -> None:
         for th, m in zip(self.theta, self.mask):
-            torch.sigmoid(torch.clamp(self.beta * th, -60.0, 60.0), out=m)
+            objective.mask_sigmoid(th, self.beta, out=m)
         self.batch.forward_into(self.mask, self.raw)
-        total = None
-        for raw, t, s, d in zip(self.raw, self.target, self.scale, self.d_raw):
-            z = torch.sigmoid(torch.clamp(self.a * (raw * s - self.level), -60.0, 60.0))
-            resid = z - t
-            npix = float(raw.shape[-1] * raw.shape[-2])
-            lc = (resid * resid).sum() / npix
-            total = lc if total is None else total + lc
-            d.copy_((2.0 / npix) * resid * (self.a * z * (1.0 - z)) * s)
-        self.loss.copy_(total)
+        for c, (raw, t, s, d, ws) in enumerate(zip(self.raw, self.target_planes, self.scale,
+                                                    self.d_raw, self.obj_ws)):
+            objective.resist_objective_into(raw, t, s, self.level, self.steep, 1.0, None,
+                                            self.costs[c], d, None, None, ws)
+        torch.sum(self.costs, 0, out=self.loss)
         self.batch.backward_into(self.mask, self.d_raw, self.d_mask)
         for g, dm, m in zip(self.grad, self.d_mask, self.mask):
-            torch.mul(dm, self.beta * m * (1.0 - m), out=g)
+            objective.mask_sigmoid_grad(dm, m, self.beta, out=g)

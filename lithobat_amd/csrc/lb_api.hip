@@ -305,7 +305,7 @@ struct Layout {
 
 struct Work {
   int n_masks, n_img, n_chunks, chunk, n_items;
-  size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, peak, queue;
+  size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, queue;
   size_t total;
 };
 
@@ -331,7 +331,6 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
   w.Xm = L.take(C * n_masks * p.Wbm * p.Wbm);
   w.RTm = L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
-  w.peak = L.take(R * (size_t)w.n_img);
   {
     // split-sum scratch: coarse partials (forward) or band gradients (backward)
     const long long mm = (long long)p.M * p.M, bb = 2LL * p.Wbm * p.Wbm;
@@ -410,8 +409,10 @@ lb_status fwd_pre(Plan& p, const void* masks, int cplx, int n_masks, char* ws, c
 template <typename T>
 lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, hipStream_t st) {
   T* partial = (T*)(ws + w.partial);
-  T* pk = peak ? (T*)peak : (T*)(ws + w.peak);
-  LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
+  // the per-image peak is a running maximum (atomics) in the final pass:
+  // computed only when the caller asks for it
+  T* pk = (T*)peak;
+  if (pk) LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
   if (p.direct) {
     LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.N * p.N, (T*)images, pk,
                           (T*)(ws + w.seg), st));
@@ -1348,6 +1349,73 @@ lb_status lb_gaussian_blur(const void* in, void* out, int32_t n_img, int32_t n, 
   else
     LB_HIP(gaussian_blur<float>((const float*)in, (float*)out, n_img, n, sigma, (float*)ws, w,
                                 st));
+  return LB_OK;
+}
+
+size_t lb_resist_objective_workspace_bytes(int32_t n_img, int64_t npix) {
+  if (n_img < 1 || npix < 1) return 0;
+  const size_t k = (size_t)n_img * resist_blocks(npix);
+  return ((k * sizeof(double) + n_img * sizeof(double) + 255) / 256) * 256 + k * sizeof(int);
+}
+
+lb_status lb_resist_objective(const void* intensity, const void* target, int32_t n_img,
+                              int64_t npix, double scale, double level, double steepness,
+                              double sign, void* cost, void* total, void* d_intensity,
+                              void* signed_field, int32_t* mismatch, int32_t precision,
+                              void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!intensity || !target) return fail(LB_ERR_ARG, "null argument");
+  if (n_img < 1 || npix < 1) return fail(LB_ERR_ARG, "n_img and npix must be >= 1");
+  if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
+  const size_t need = lb_resist_objective_workspace_bytes(n_img, npix);
+  if (!workspace || ws_bytes < need)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  const double a = steepness / std::max(std::fabs(level), 1e-12);
+  const size_t k = (size_t)n_img * resist_blocks(npix);
+  double* pc = (double*)workspace;  // k block partials, then n_img image costs
+  int* pe = (int*)((char*)workspace + ((k * sizeof(double) + n_img * sizeof(double) + 255) / 256) *
+                                          256);
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 64)
+    LB_HIP(resist_objective<double>((const double*)intensity, (const double*)target, n_img, npix,
+                                    scale, level, a, sign, (double*)cost, (double*)total,
+                                    (double*)d_intensity, (double*)signed_field, mismatch, pc, pe,
+                                    st));
+  else
+    LB_HIP(resist_objective<float>((const float*)intensity, (const float*)target, n_img, npix,
+                                   (float)scale, (float)level, (float)a, (float)sign,
+                                   (float*)cost, (float*)total, (float*)d_intensity,
+                                   (float*)signed_field, mismatch, pc, pe, st));
+  return LB_OK;
+}
+
+lb_status lb_mask_sigmoid(const void* x, int64_t n, double beta, void* mask, int32_t precision,
+                          void* stream) {
+  g_err.clear();
+  if (!x || !mask) return fail(LB_ERR_ARG, "null argument");
+  if (n < 0) return fail(LB_ERR_ARG, "n must be >= 0");
+  if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 64)
+    LB_HIP(mask_sigmoid<double>((const double*)x, n, beta, (double*)mask, st));
+  else
+    LB_HIP(mask_sigmoid<float>((const float*)x, n, (float)beta, (float*)mask, st));
+  return LB_OK;
+}
+
+lb_status lb_mask_sigmoid_grad(const void* grad_mask, const void* mask, int64_t n, double beta,
+                               void* grad_x, int32_t precision, void* stream) {
+  g_err.clear();
+  if (!grad_mask || !mask || !grad_x) return fail(LB_ERR_ARG, "null argument");
+  if (n < 0) return fail(LB_ERR_ARG, "n must be >= 0");
+  if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 64)
+    LB_HIP(mask_sigmoid_grad<double>((const double*)grad_mask, (const double*)mask, n, beta,
+                                     (double*)grad_x, st));
+  else
+    LB_HIP(mask_sigmoid_grad<float>((const float*)grad_mask, (const float*)mask, n, (float)beta,
+                                    (float*)grad_x, st));
   return LB_OK;
 }
 

@@ -148,6 +148,21 @@ template <typename T>
 hipError_t gaussian_blur(const T* in, T* out, int n_img, int n, double sigma, T* tmp, double* w,
                          hipStream_t st);
 
+// ---- ILT objective (lb_objective.hip) ------------------------------------------
+// Relaxed-resist cost, its derivative, the signed field and the pattern
+// mismatch of n_img images of npix pixels (d_f, signed_f, cost, total and
+// mismatch may be null); part_cost / part_err hold n_img * resist_blocks(npix)
+// partials.  Mask sigmoid m = sigmoid(clip(beta x, +-60)) and its adjoint.
+int resist_blocks(long long npix);
+template <typename T>
+hipError_t resist_objective(const T* f, const T* target, int n_img, long long npix, T scale,
+                            T level, T a, T sign, T* cost, T* total, T* d_f, T* signed_f,
+                            int* mismatch, double* part_cost, int* part_err, hipStream_t st);
+template <typename T>
+hipError_t mask_sigmoid(const T* x, long long n, T beta, T* m, hipStream_t st);
+template <typename T>
+hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, T* gx, hipStream_t st);
+
 // ---- pupil tables (lb_pupil.hip) ------------------------------------------
 struct PupilSys {
   double fsx, fsy;       // source-point frequency offset [1/m]

@@ -1,0 +1,215 @@
+// The inverse-lithography objective around the imaging adjoint, fused into
+// single passes over the images (opc/ilt.py:311-337, 368-389):
+//
+//   relaxed resist   s = sign (f - level), f = scale * I (or its PEB blur),
+//                    z = sigmoid(clip(a s, +-60)), a = steepness / |level|,
+//                    cost_b = sum_pixels (z - t)^2 / npix,
+//                    d f   = sign (2 / npix) (z - t) a z (1 - z),
+//                    mismatch_b = #{(s > 0) != (t > 0.5)}
+//   mask             m = sigmoid(clip(beta x, +-60)),  d x = d m beta m (1 - m)
+//
+// The derivatives are the reference's closed forms (they ignore the clip,
+// whose effect is below e^-60).  Costs are reduced in two deterministic
+// levels (fixed block partials, then a fixed-order sum per image), so the
+// same inputs give bit-identical costs, and nothing here is a memset, so the
+// chain can be captured into a graph.
+#include <cstdint>
+
+#include "lb_kernels.h"
+
+namespace lb {
+
+constexpr int kObjThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ T sigmoid_clipped(T v) {
+  v = v < (T)-60 ? (T)-60 : (v > (T)60 ? (T)60 : v);
+  return (T)1 / ((T)1 + exp(-v));
+}
+
+// Block (blockIdx.x, image blockIdx.y) reduces its share of one image into
+// part_cost / part_err [image][block]; d_f and signed_f are optional.
+template <typename T>
+__global__ __launch_bounds__(kObjThreads) void k_resist_partial(
+    const T* __restrict__ f, const T* __restrict__ target, long long npix, T scale, T level, T a,
+    T sign, T* __restrict__ d_f, T* __restrict__ signed_f, double* __restrict__ part_cost,
+    int* __restrict__ part_err) {
+  const long long base = (long long)blockIdx.y * npix;
+  const T dscale = sign * (T)2 / (T)npix * a * scale;
+  double c = 0;
+  int err = 0;
+  for (long long e = blockIdx.x * (long long)kObjThreads + threadIdx.x; e < npix;
+       e += (long long)gridDim.x * kObjThreads) {
+    const T s = sign * (f[base + e] * scale - level);
+    const T z = sigmoid_clipped(a * s);
+    const T t = target[base + e];
+    const T r = z - t;
+    c += (double)(r * r);
+    err += (s > (T)0) != (t > (T)0.5);
+    if (d_f) d_f[base + e] = dscale * r * z * ((T)1 - z);
+    if (signed_f) signed_f[base + e] = s;
+  }
+  __shared__ double sc[kObjThreads / 64];
+  __shared__ int se[kObjThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    c += __shfl_xor(c, o, 64);
+    err += __shfl_xor(err, o, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sc[wave] = c;
+    se[wave] = err;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double cs = 0;
+    int es = 0;
+    for (int w = 0; w < kObjThreads / 64; ++w) {
+      cs += sc[w];
+      es += se[w];
+    }
+    const long long k = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    part_cost[k] = cs;
+    part_err[k] = es;
+  }
+}
+
+// One block of kFinalWaves wavefronts: wavefront w sums the block partials of
+// images w, w + kFinalWaves, ... (lane-strided, then a fixed shuffle tree);
+// thread 0 then adds the per-image costs in image order for the total.
+constexpr int kFinalWaves = 16;
+
+template <typename T>
+__global__ __launch_bounds__(64 * kFinalWaves) void k_resist_final(
+    const double* __restrict__ part_cost, const int* __restrict__ part_err, int n_img, int nblk,
+    long long npix, double* __restrict__ img_cost, T* __restrict__ cost, T* __restrict__ total,
+    int* __restrict__ mismatch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int b = wave; b < n_img; b += kFinalWaves) {
+    double c = 0;
+    int e = 0;
+    for (int k = lane; k < nblk; k += 64) {
+      c += part_cost[(long long)b * nblk + k];
+      e += part_err[(long long)b * nblk + k];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      c += __shfl_xor(c, o, 64);
+      e += __shfl_xor(e, o, 64);
+    }
+    if (lane == 0) {
+      const double cb = c / (double)npix;
+      img_cost[b] = cb;
+      if (cost) cost[b] = (T)cb;
+      if (mismatch) mismatch[b] = e;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && total) {
+    double tot = 0;
+    for (int b = 0; b < n_img; ++b) tot += img_cost[b];
+    *total = (T)tot;
+  }
+}
+
+int resist_blocks(long long npix) {
+  long long b = (npix + 16LL * kObjThreads - 1) / (16LL * kObjThreads);  // ~16 pixels a thread
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return (int)b;
+}
+
+// part_cost holds n_img * (nblk + 1) doubles: the block partials, then the
+// per-image costs.
+template <typename T>
+hipError_t resist_objective(const T* f, const T* target, int n_img, long long npix, T scale,
+                            T level, T a, T sign, T* cost, T* total, T* d_f, T* signed_f,
+                            int* mismatch, double* part_cost, int* part_err, hipStream_t st) {
+  const int nblk = resist_blocks(npix);
+  hipLaunchKernelGGL(k_resist_partial<T>, dim3(nblk, n_img), dim3(kObjThreads), 0, st, f, target,
+                     npix, scale, level, a, sign, d_f, signed_f, part_cost, part_err);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_resist_final<T>, dim3(1), dim3(64 * kFinalWaves), 0, st, part_cost,
+                     part_err, n_img, nblk, npix, part_cost + (long long)n_img * nblk, cost, total,
+                     mismatch);
+  return hipGetLastError();
+}
+
+// Elementwise passes: each thread handles 16-byte vectors (4 floats or 2
+// doubles) when every pointer is 16-byte aligned (VEC), then a scalar tail.
+template <typename T>
+struct Vec16 {
+  static constexpr int K = 16 / sizeof(T);
+  T v[K];
+};
+
+template <typename T, bool VEC>
+__global__ void k_mask_sigmoid(const T* __restrict__ x, long long n, T beta, T* __restrict__ m) {
+  using V = Vec16<T>;
+  const long long nv = VEC ? n / V::K : 0;
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (long long i = t0; i < nv; i += st) {
+    V a = reinterpret_cast<const V*>(x)[i];
+#pragma unroll
+    for (int k = 0; k < V::K; ++k) a.v[k] = sigmoid_clipped(beta * a.v[k]);
+    reinterpret_cast<V*>(m)[i] = a;
+  }
+  for (long long i = nv * V::K + t0; i < n; i += st) m[i] = sigmoid_clipped(beta * x[i]);
+}
+
+template <typename T, bool VEC>
+__global__ void k_mask_sigmoid_grad(const T* __restrict__ gm, const T* __restrict__ m, long long n,
+                                    T beta, T* __restrict__ gx) {
+  using V = Vec16<T>;
+  const long long nv = VEC ? n / V::K : 0;
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (long long i = t0; i < nv; i += st) {
+    V g = reinterpret_cast<const V*>(gm)[i];
+    const V a = reinterpret_cast<const V*>(m)[i];
+#pragma unroll
+    for (int k = 0; k < V::K; ++k) g.v[k] *= beta * a.v[k] * ((T)1 - a.v[k]);
+    reinterpret_cast<V*>(gx)[i] = g;
+  }
+  for (long long i = nv * V::K + t0; i < n; i += st) {
+    const T v = m[i];
+    gx[i] = gm[i] * (beta * v * ((T)1 - v));
+  }
+}
+
+static unsigned elementwise_blocks(long long n) {
+  long long b = (n + 4 * 256 - 1) / (4 * 256);  // a few vectors per thread
+  if (b > 16384) b = 16384;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+template <typename T>
+hipError_t mask_sigmoid(const T* x, long long n, T beta, T* m, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  auto k = aligned16(x) && aligned16(m) ? k_mask_sigmoid<T, true> : k_mask_sigmoid<T, false>;
+  hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, x, n, beta, m);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, T* gx, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  auto k = aligned16(gm) && aligned16(m) && aligned16(gx) ? k_mask_sigmoid_grad<T, true>
+                                                          : k_mask_sigmoid_grad<T, false>;
+  hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, gm, m, n, beta, gx);
+  return hipGetLastError();
+}
+
+#define LB_OBJ_INST(T)                                                                          \
+  template hipError_t resist_objective<T>(const T*, const T*, int, long long, T, T, T, T, T*, T*, \
+                                          T*, T*, int*, double*, int*, hipStream_t);           \
+  template hipError_t mask_sigmoid<T>(const T*, long long, T, T*, hipStream_t);                 \
+  template hipError_t mask_sigmoid_grad<T>(const T*, const T*, long long, T, T*, hipStream_t);
+LB_OBJ_INST(float)
+LB_OBJ_INST(double)
+#undef LB_OBJ_INST
+
+}  // namespace lb

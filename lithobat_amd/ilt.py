@@ -17,6 +17,7 @@ import torch
 
 from .blur import blur
 from .expose import _to_device
+from .objective import MaskSigmoid, RelaxedResist
 from .plan import ImagingPlan
 
 __all__ = ["AbbeIntensity", "abbe_intensity", "socs_fields", "socs_gradient",
@@ -48,8 +49,9 @@ def abbe_intensity(mask: torch.Tensor, plan: ImagingPlan) -> torch.Tensor:
 
 def mask_of(theta: torch.Tensor, beta: float, smooth_px: float = 0.0) -> torch.Tensor:
     """ilt.py:368-378: sigmoid(clip(beta * G(theta), +-60)), G the Gaussian of width
-    ``smooth_px`` (the identity below 1e-3 px); the backward is ilt.py:381-389."""
-    return torch.sigmoid(torch.clamp(beta * blur(theta, smooth_px), -60.0, 60.0))
+    ``smooth_px`` (the identity below 1e-3 px); the backward is ilt.py:381-389
+    (d theta = G(d m * beta m (1 - m)))."""
+    return MaskSigmoid.apply(blur(theta, smooth_px), beta)
 
 
 def relaxed_resist_loss(raw: torch.Tensor, target: torch.Tensor, scale: float, level: float,
@@ -62,11 +64,9 @@ def relaxed_resist_loss(raw: torch.Tensor, target: torch.Tensor, scale: float, l
     below 1e-3 px).  The mean runs over the last two (image) axes and is
     summed over any leading axes.
     """
-    signed = sign * (blur(raw * scale, sigma_px) - level)
-    a = steepness / max(abs(level), 1e-12)
-    z = torch.sigmoid(torch.clamp(a * signed, -60.0, 60.0))
-    resid = z - target
-    return (resid * resid).mean(dim=(-2, -1)).sum()
+    if sigma_px < 1e-3:
+        return RelaxedResist.apply(raw, target, scale, level, steepness, sign)
+    return RelaxedResist.apply(blur(raw * scale, sigma_px), target, 1.0, level, steepness, sign)
 
 
 def mask_penalties(mask: torch.Tensor, weight_tv: float = 0.0,

@@ -1,0 +1,139 @@
+"""The ILT objective around the imaging adjoint, as fused device passes.
+
+``resist_objective`` is the relaxed resist of opc/ilt.py:311-337
+(``_Objective.__call__`` without its penalties): for intensities I and
+targets t over the last two axes,
+
+    f = scale * I,  s = sign * (f - level),  z = 1 / (1 + exp(-clip(a s, +-60))),
+    cost = mean (z - t)^2 per image,  a = steepness / max(|level|, 1e-12),
+
+with d cost / d I = sign * (2 / npix) * (z - t) * a * z (1 - z) * scale (the
+reference's closed form), the signed field s and the pattern mismatch
+count_nonzero((s > 0) != (t > 0.5)) (ilt.py:357), all from one pass over the
+images.  ``mask_sigmoid`` is the mask parameterisation of ilt.py:368-389
+without its Gaussian: m = 1 / (1 + exp(-clip(beta x, +-60))), d x = d m beta
+m (1 - m).  ``RelaxedResist`` and ``MaskSigmoid`` are their autograd forms.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+__all__ = ["resist_objective", "resist_objective_into", "mask_sigmoid", "mask_sigmoid_grad",
+           "RelaxedResist", "MaskSigmoid"]
+
+
+def _prec(t: torch.Tensor) -> int:
+    if t.dtype == torch.float64:
+        return 64
+    if t.dtype == torch.float32:
+        return 32
+    raise TypeError("float32 or float64 tensors expected")
+
+
+def _on_device(t, name):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a tensor on the HIP device")
+
+
+def resist_objective(intensity: torch.Tensor, target: torch.Tensor, scale: float, level: float,
+                     steepness: float, sign: float = 1.0, grad: bool = True,
+                     signed: bool = False, mismatch: bool = False) -> dict:
+    """Costs per image ``cost`` [...], their sum ``total`` (0-d), and on request
+    ``d_intensity`` (like ``intensity``), ``signed`` and ``mismatch`` (int32 [...])."""
+    _on_device(intensity, "intensity")
+    _on_device(target, "target")
+    if intensity.dim() < 2:
+        raise ValueError("intensity must have (..., ny, nx) image axes")
+    prec = _prec(intensity)
+    f = intensity.contiguous()
+    t = target.to(f.dtype)
+    if t.shape != f.shape:
+        t = t.expand(f.shape)
+    t = t.contiguous()
+    lead = tuple(f.shape[:-2])
+    npix = f.shape[-1] * f.shape[-2]
+    cost = torch.empty(lead, dtype=f.dtype, device=f.device)
+    total = torch.empty((), dtype=f.dtype, device=f.device)
+    d = torch.empty_like(f) if grad else None
+    s = torch.empty_like(f) if signed else None
+    mm = torch.empty(lead, dtype=torch.int32, device=f.device) if mismatch else None
+    ws = torch.empty(max(workspace_bytes(f.numel() // npix, npix), 256), dtype=torch.uint8,
+                     device=f.device)
+    resist_objective_into(f, t, scale, level, steepness, sign, cost, total, d, s, mm, ws)
+    return {"cost": cost, "total": total, "d_intensity": d, "signed": s, "mismatch": mm}
+
+
+def workspace_bytes(n_img: int, npix: int) -> int:
+    """Scratch bytes of ``resist_objective_into`` for n_img images of npix pixels."""
+    return int(_lib.load().lb_resist_objective_workspace_bytes(int(n_img), int(npix)))
+
+
+def resist_objective_into(f, t, scale, level, steepness, sign, cost, total, d, signed, mismatch,
+                          ws) -> None:
+    """``resist_objective`` into preallocated, contiguous tensors (any output may be None):
+    no allocation, so it can be captured into a graph."""
+    npix = f.shape[-1] * f.shape[-2]
+    st = torch.cuda.current_stream(f.device).cuda_stream
+
+    def ptr(x):
+        return x.data_ptr() if x is not None else None
+
+    _lib.check(_lib.load().lb_resist_objective(
+        f.data_ptr(), t.data_ptr(), f.numel() // npix, npix, float(scale), float(level),
+        float(steepness), float(sign), ptr(cost), ptr(total), ptr(d), ptr(signed), ptr(mismatch),
+        _prec(f), ws.data_ptr(), ws.numel(), st))
+
+
+def mask_sigmoid(x: torch.Tensor, beta: float, out: torch.Tensor | None = None) -> torch.Tensor:
+    _on_device(x, "x")
+    prec = _prec(x)
+    xc = x.contiguous()
+    m = torch.empty_like(xc) if out is None else out
+    st = torch.cuda.current_stream(xc.device).cuda_stream
+    _lib.check(_lib.load().lb_mask_sigmoid(xc.data_ptr(), xc.numel(), float(beta), m.data_ptr(),
+                                           prec, st))
+    return m
+
+
+def mask_sigmoid_grad(grad_mask: torch.Tensor, mask: torch.Tensor, beta: float,
+                      out: torch.Tensor | None = None) -> torch.Tensor:
+    g = grad_mask.to(mask.dtype).contiguous()
+    res = torch.empty_like(mask) if out is None else out
+    st = torch.cuda.current_stream(mask.device).cuda_stream
+    _lib.check(_lib.load().lb_mask_sigmoid_grad(g.data_ptr(), mask.data_ptr(), mask.numel(),
+                                                float(beta), res.data_ptr(), _prec(mask), st))
+    return res
+
+
+class RelaxedResist(torch.autograd.Function):
+    """Sum over images of the per-image relaxed-resist cost; the derivative is
+    computed in the same pass as the cost and scaled by the incoming gradient."""
+
+    @staticmethod
+    def forward(ctx, intensity, target, scale, level, steepness, sign):
+        r = resist_objective(intensity, target, scale, level, steepness, sign,
+                             grad=intensity.requires_grad)
+        ctx.save_for_backward(r["d_intensity"] if r["d_intensity"] is not None
+                              else torch.empty(0, device=intensity.device))
+        return r["total"]
+
+    @staticmethod
+    def backward(ctx, g):
+        (d,) = ctx.saved_tensors
+        return d * g, None, None, None, None, None
+
+
+class MaskSigmoid(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, beta):
+        m = mask_sigmoid(x, beta)
+        ctx.beta = beta
+        ctx.save_for_backward(m)
+        return m
+
+    @staticmethod
+    def backward(ctx, g):
+        (m,) = ctx.saved_tensors
+        return mask_sigmoid_grad(g, m, ctx.beta), None

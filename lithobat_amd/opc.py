@@ -195,12 +195,13 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
     def objective(m):
         """ilt.py:311-360 on the device: cost (device scalar), d cost / d mask, signed field."""
         m = m.detach().requires_grad_(True)
-        raw = abbe_intensity(m[None], plan)[:, 0]
-        cost = (relaxed_resist_loss(raw, t_dev[None], scale, level, steepness, sign, sigma_px)
+        raw = abbe_intensity(m[None], plan)  # [1, 1, n, n]
+        cost = (relaxed_resist_loss(raw, t_dev[None, None], scale, level, steepness, sign,
+                                    sigma_px)
                 + mask_penalties(m, weight_tv, weight_discrete))
         cost.backward()
         with torch.no_grad():
-            signed = sign * (blur(raw * scale, sigma_px) - level)[0]
+            signed = sign * (blur(raw[0, 0] * scale, sigma_px) - level)
         return cost.detach(), m.grad, signed
 
     def grad_theta(grad_m, m, beta_k):
