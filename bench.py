@@ -276,28 +276,39 @@ def clips_leg(args, dev, sync):
     return clip_ilt_s, timed(plain_clips, 2, 1, sync)
 
 
+def socs_cfg3():
+    """Hopkins / SOCS configuration: 512^2 @ 8 nm, NA 1.35 (water), annular 0.6-0.9 on a
+    21-point grid (142 source points), raw intensities."""
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    grid = GridConfig(n_pixels=512, pixel_size=8e-9)
+    optics = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="annular",
+                          sigma_outer=0.9, sigma_inner=0.6, source_grid=21,
+                          normalisation="none")
+    return optics, grid
+
+
+def socs_masks(dev):
+    import torch
+
+    from lithobat_amd.masks import manhattan
+    return torch.tensor(np.stack([manhattan(512, 8e-9, seed=s) for s in range(2000, 2256)]),
+                        device=dev, dtype=torch.float32)
+
+
 def socs_leg(args, dev, sync):
     """Hopkins / SOCS at 512^2, 8 nm: FP64 TCC decomposition on the device (Gram route,
     Jacobi) truncated to 24 kernels, then 256 masks (seeds 2000-2255) imaged through the
     kernels in complex64; error of the K = 24 image against the full Abbe sum (FP32
     device path, itself within 1e-5 of the float64 oracle) on mask 2000."""
-    import torch
-
-    from lithobat_amd.config import GridConfig, OpticsConfig
     from lithobat_amd.hopkins import SOCSKernels
-    from lithobat_amd.masks import manhattan
     from lithobat_amd.plan import ImagingPlan
-    grid = GridConfig(n_pixels=512, pixel_size=8e-9)
-    optics = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="annular",
-                          sigma_outer=0.9, sigma_inner=0.6, source_grid=21,
-                          normalisation="none")
+    optics, grid = socs_cfg3()
     sync()
     t0 = time.perf_counter()
     k24 = SOCSKernels.build(optics, grid, tol=0.0, max_kernels=24, precision=32)
     sync()
     build_s = time.perf_counter() - t0
-    masks = torch.tensor(np.stack([manhattan(512, 8e-9, seed=s) for s in range(2000, 2256)]),
-                         device=dev, dtype=torch.float32)
+    masks = socs_masks(dev)
     plan = k24.plan
 
     def step():
@@ -420,6 +431,12 @@ def main():
         res.update(autotune.tune(ilt, torch.sigmoid(8 * th), adjoint=True, save=False))
         thb = torch.randn((ILT_BATCH, N, N), device=dev)
         res.update(autotune.tune(ilt, torch.sigmoid(8 * thb), adjoint=True, save=False))
+        from lithobat_amd.hopkins import SOCSKernels
+        so, sg = socs_cfg3()
+        k24 = SOCSKernels.build(so, sg, tol=0.0, max_kernels=24, precision=32)
+        sm = socs_masks(dev)
+        res.update(autotune.tune(k24.plan, sm, adjoint=False, save=False))
+        res.update(autotune.tune(k24.plan, sm[:1], adjoint=True, save=False))
         one = ImagingPlan(optics, grid, planes[:1], precision=32)
         res.update(autotune.tune(one, masks[:1], adjoint=False, save=False))
         table = autotune.load_table()
