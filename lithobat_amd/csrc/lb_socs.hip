@@ -130,36 +130,86 @@ __global__ __launch_bounds__(kJT) void k_jacobi(c64d* __restrict__ G, c64d* __re
         re_[i] = ea;
       }
       __syncthreads();
-      // rows: G <- U^H G
-      for (long long w = tid; w < (long long)half * S; w += kJT) {
-        const int i = (int)(w / S), k = (int)(w % S);
-        const int p = rp[i], q = rq[i];
-        if (q >= S || rs[i] == 0.0) continue;
-        const double c = rc[i], s = rs[i];
-        const c64d ea = re_[i];
-        const c64d gp = G[(long long)p * S + k], gq = G[(long long)q * S + k];
-        const c64d sq = scale(ea * gq, s);           // s e^{ia} g_q
-        const c64d sp = scale(mulc(gp, ea), s);      // s e^{-ia} g_p
-        G[(long long)p * S + k] = scale(gp, c) - sq;
-        G[(long long)q * S + k] = sp + scale(gq, c);
-      }
-      __syncthreads();
-      // columns: G <- G U, V <- V U
-      for (long long w = tid; w < (long long)half * S; w += kJT) {
-        const int i = (int)(w / S), k = (int)(w % S);
-        const int p = rp[i], q = rq[i];
-        if (q >= S || rs[i] == 0.0) continue;
-        const double c = rc[i], s = rs[i];
-        const c64d ea = re_[i];
-        {
-          const c64d gp = G[(long long)k * S + p], gq = G[(long long)k * S + q];
-          G[(long long)k * S + p] = scale(gp, c) - scale(mulc(gq, ea), s);
-          G[(long long)k * S + q] = scale(ea * gp, s) + scale(gq, c);
-        }
-        {
-          const c64d vp = V[(long long)k * S + p], vq = V[(long long)k * S + q];
-          V[(long long)k * S + p] = scale(vp, c) - scale(mulc(vq, ea), s);
-          V[(long long)k * S + q] = scale(ea * vp, s) + scale(vq, c);
+      // G <- U^H G U and V <- V U in one pass: the rotation of pair a acts on
+      // rows {p_a, q_a} and that of pair b on columns {p_b, q_b}, so every 2 x 2
+      // block (a, b) of G is updated on its own (rows first, then columns, the
+      // same arithmetic as two separate passes), and so is every row segment
+      // (k, b) of V.  Loads of a group of tasks are issued before any store.
+      {
+        const long long nG = (long long)half * half, nV = (long long)S * half;
+        constexpr int U = 2;
+        for (long long w0 = tid; w0 < nG + nV; w0 += U * kJT) {
+          c64d g[U][4];
+          int ra[U], rb[U], ca[U], cb[U];  // rows p_a, q_a; columns p_b, q_b (-1: absent)
+          int pa[U], pb[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const long long w = w0 + (long long)u * kJT;
+            ra[u] = rb[u] = ca[u] = cb[u] = -1;
+            pa[u] = pb[u] = -1;
+            if (w < nG) {
+              const int a = (int)(w / half), b = (int)(w - (long long)(w / half) * half);
+              pa[u] = a;
+              pb[u] = b;
+              ra[u] = rp[a];
+              rb[u] = rq[a] < S ? rq[a] : -1;
+              ca[u] = rp[b];
+              cb[u] = rq[b] < S ? rq[b] : -1;
+              g[u][0] = G[(long long)ra[u] * S + ca[u]];
+              g[u][1] = cb[u] >= 0 ? G[(long long)ra[u] * S + cb[u]] : c64d{0, 0};
+              g[u][2] = rb[u] >= 0 ? G[(long long)rb[u] * S + ca[u]] : c64d{0, 0};
+              g[u][3] = (rb[u] >= 0 && cb[u] >= 0) ? G[(long long)rb[u] * S + cb[u]] : c64d{0, 0};
+            } else if (w < nG + nV) {
+              const long long wv = w - nG;
+              const int k = (int)(wv / half), b = (int)(wv - (long long)(wv / half) * half);
+              pb[u] = b;
+              ra[u] = k;
+              ca[u] = rp[b];
+              cb[u] = rq[b] < S ? rq[b] : -1;
+              g[u][0] = V[(long long)k * S + ca[u]];
+              g[u][1] = cb[u] >= 0 ? V[(long long)k * S + cb[u]] : c64d{0, 0};
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const long long w = w0 + (long long)u * kJT;
+            if (w < nG) {
+              const int a = pa[u], b = pb[u];
+              if (rb[u] >= 0 && rs[a] != 0.0) {  // rows: G <- U_a^H G
+                const double c = rc[a], s_ = rs[a];
+                const c64d ea = re_[a];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                  const c64d gp = g[u][j], gq = g[u][2 + j];
+                  g[u][j] = scale(gp, c) - scale(ea * gq, s_);
+                  g[u][2 + j] = scale(mulc(gp, ea), s_) + scale(gq, c);
+                }
+              }
+              if (cb[u] >= 0 && rs[b] != 0.0) {  // columns: G <- G U_b
+                const double c = rc[b], s_ = rs[b];
+                const c64d eb = re_[b];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                  const c64d gp = g[u][2 * i], gq = g[u][2 * i + 1];
+                  g[u][2 * i] = scale(gp, c) - scale(mulc(gq, eb), s_);
+                  g[u][2 * i + 1] = scale(eb * gp, s_) + scale(gq, c);
+                }
+              }
+              G[(long long)ra[u] * S + ca[u]] = g[u][0];
+              if (cb[u] >= 0) G[(long long)ra[u] * S + cb[u]] = g[u][1];
+              if (rb[u] >= 0) G[(long long)rb[u] * S + ca[u]] = g[u][2];
+              if (rb[u] >= 0 && cb[u] >= 0) G[(long long)rb[u] * S + cb[u]] = g[u][3];
+            } else if (w < nG + nV) {
+              const int b = pb[u];
+              if (cb[u] >= 0 && rs[b] != 0.0) {  // V <- V U_b
+                const double c = rc[b], s_ = rs[b];
+                const c64d eb = re_[b];
+                const c64d vp = g[u][0], vq = g[u][1];
+                V[(long long)ra[u] * S + ca[u]] = scale(vp, c) - scale(mulc(vq, eb), s_);
+                V[(long long)ra[u] * S + cb[u]] = scale(eb * vp, s_) + scale(vq, c);
+              }
+            }
+          }
         }
       }
       __syncthreads();
