@@ -158,6 +158,9 @@ struct RowPre {
   static constexpr int NCH = (MAXW + F::CPR - 1) / F::CPR;
   static constexpr int NR = (MAXW + F::T1 - 1) / F::T1 < F::R1 ? (MAXW + F::T1 - 1) / F::T1 : F::R1;
   static constexpr bool ON = NCH * NR <= (sizeof(T) == 4 ? 16 : 4);
+  // the adjoint keeps its coarse sensitivity in registers as well: prefetch
+  // there only when it costs at most half as much
+  static constexpr bool ON_ADJ = NCH * NR <= (sizeof(T) == 4 ? 8 : 2);
   // With the inputs in registers the forward row pass transforms all W rows
   // at once, so its scratch buffer Cb holds up to MAXW rows; the adjoint (and
   // the row pass without RowPre) runs in chunks of CPR rows.
@@ -478,7 +481,12 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
   cx<T>* Cb = tw + M;                              // CPR x LDC
   cx<T>* Tb = Cb + F::CPR * F::LDC;                // W x MC
-  cx<T>* Tp = Tb + Wmax * F::MC;                   // W x MC
+  // The transposed column stage writes its box rows over the row table: round
+  // rd's stage 1 has read Tb at exactly those (row, column) places two
+  // barriers earlier, later rounds read other columns, and the transposed
+  // row pass ends with a barrier before the next system's row pass.
+  cx<T>* Tp = Tb;
+  (void)Wmax;  // the table extent is the launch's LDS size
   const int tid = threadIdx.x;
   load_roots(tw, twg, M, tid, NT);
   __syncthreads();
@@ -500,7 +508,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
     __syncthreads();
     using Pre = RowPre<T, M, NT, CSPLIT>;
     Pre pre;
-    if (Pre::ON && s0 < s1)
+    if (Pre::ON_ADJ && s0 < s1)
       pre.fetch(Xm, Wb, wrapN, H + (long long)s0 * W * W, W, box[s0], rl1, j1r);
     for (int cs = 0; cs < CSPLIT; ++cs) {
       T gm[F::NACC];
@@ -520,7 +528,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
         const int2 bx = box[j];
         const T w2 = (T)2 * wsys[j];
         const cx<T>* Hj = H + (long long)j * W * W;
-        if constexpr (Pre::ON) {
+        if constexpr (Pre::ON_ADJ) {
           row_pass_pre<false>(pre, W, tw, Cb, Tb, cs, tid);
           const int jn = j + 1 < s1 ? j + 1 : (cs + 1 < CSPLIT ? s0 : -1);
           if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1r);
@@ -661,7 +669,7 @@ template <typename T, int M, int NT, int CS>
 static size_t smem_for(int W, bool adjoint) {
   using F = FieldShape<T, M, NT, CS>;
   const int cb_rows = adjoint ? F::CPR : RowPre<T, M, NT, CS>::CB_ROWS_FWD;
-  size_t n = M + (size_t)cb_rows * F::LDC + (size_t)W * F::MC * (adjoint ? 2 : 1);
+  size_t n = M + (size_t)cb_rows * F::LDC + (size_t)W * F::MC;  // the adjoint's Tp aliases Tb
   return n * sizeof(cx<T>);
 }
 
