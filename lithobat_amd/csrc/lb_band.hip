@@ -689,8 +689,11 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
   const int ld = band_ld(Wb);
   const cx<T>* src = TT + (long long)img * N * ld;
   // per task slot: transform within the unit, and the band positions of its
-  // two inputs (nu = j and nu = j - S), -1 outside the band
+  // two inputs (nu = j and nu = j - S).  The loads below always use an
+  // in-range position (clamped) and are masked afterwards, so the prefetch
+  // has no branches and its registers carry straight into the next unit.
   int trs[TPL], plo[TPL], phi[TPL], jj[TPL];
+  bool vlo[TPL], vhi[TPL];
 #pragma unroll
   for (int i = 0; i < TPL; ++i) {
     const int t = lane + 64 * i;
@@ -699,49 +702,54 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
     trs[i] = ok ? tr : TPW;
     jj[i] = j;
     const int lo = j - nu0, hi = j - S - nu0;
-    plo[i] = (ok && lo >= 0 && lo < Wb) ? lo : -1;
-    phi[i] = (ok && hi >= 0 && hi < Wb) ? hi : -1;
+    vlo[i] = ok && lo >= 0 && lo < Wb;
+    vhi[i] = ok && hi >= 0 && hi < Wb;
+    plo[i] = vlo[i] ? lo : 0;
+    phi[i] = vhi[i] ? hi : 0;
   }
   // raw band values of one unit: even / odd rows (real output) at both inputs
   cx<T> a0[TPL], a1[TPL], b0[TPL], b1[TPL];
-  auto fetch = [&](int u) {
+  auto fetch = [&](int u) {  // any u < NU
 #pragma unroll
     for (int i = 0; i < TPL; ++i) {
-      a0[i] = a1[i] = b0[i] = b1[i] = cx<T>{0, 0};
-      const int p = u * TPW + trs[i];
-      if (trs[i] < TPW && p < NTR) {
-        const long long r0 = COUT ? (long long)p * ld : (long long)(2 * p) * ld;
-        if (plo[i] >= 0) {
-          a0[i] = src[r0 + plo[i]];
-          if (!COUT) a1[i] = src[r0 + ld + plo[i]];
-        }
-        if (phi[i] >= 0) {
-          b0[i] = src[r0 + phi[i]];
-          if (!COUT) b1[i] = src[r0 + ld + phi[i]];
-        }
+      const int p = min(u * TPW + min(trs[i], TPW - 1), NTR - 1);
+      const long long r0 = COUT ? (long long)p * ld : (long long)(2 * p) * ld;
+      a0[i] = src[r0 + plo[i]];
+      b0[i] = src[r0 + phi[i]];
+      if (!COUT) {
+        a1[i] = src[r0 + ld + plo[i]];
+        b1[i] = src[r0 + ld + phi[i]];
       }
+    }
+  };
+  // the two first-stage inputs of every task slot of a unit, scaled
+  cx<T> lo[TPL], hi[TPL];
+  auto take = [&]() {
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      // real output: row 2p + i row 2p+1 (the band is Hermitian per row pair)
+      const cx<T> l = COUT ? a0[i] : cx<T>{a0[i].re - a1[i].im, a0[i].im + a1[i].re};
+      const cx<T> h = COUT ? b0[i] : cx<T>{b0[i].re - b1[i].im, b0[i].im + b1[i].re};
+      lo[i] = vlo[i] ? scale(l, s) : cx<T>{0, 0};
+      hi[i] = vhi[i] ? scale(h, s) : cx<T>{0, 0};
     }
   };
   const int nw = row_waves<T, N>();
   const int stride = gridDim.x * nw;
   int u = blockIdx.x * nw + wave;
-  if (u < NU) fetch(u);
-  wait_vmem_all();  // no loads in flight at the loop head (see wait_vmem_all)
+  // The next unit's values are loaded at the top of an iteration and taken
+  // into lo / hi at its end, after the unit's image stores: a wave then waits
+  // for those loads only, never for its stores, and nothing is in flight at
+  // the loop head.
+  fetch(min(u, NU - 1));
+  take();
   TailTw<T, +1, N, TPW, 16> rt;  // 4 waves per SIMD: 128 VGPRs in all
   rt.load(tw, lane);
   T lmax = 0;
   for (; u < NU; u += stride) {
-    cx<T> lo[TPL], hi[TPL];
-#pragma unroll
-    for (int i = 0; i < TPL; ++i) {
-      // real output: row 2p + i row 2p+1 (the band is Hermitian per row pair)
-      lo[i] = COUT ? a0[i] : cx<T>{a0[i].re - a1[i].im, a0[i].im + a1[i].re};
-      hi[i] = COUT ? b0[i] : cx<T>{b0[i].re - b1[i].im, b0[i].im + b1[i].re};
-      lo[i] = scale(lo[i], s);
-      hi[i] = scale(hi[i], s);
-    }
-    if (u + stride < NU) fetch(u + stride);
-    const int ntr = min(TPW, NTR - u * TPW);
+    fetch(min(u + stride, NU - 1));
+    // every unit is full when TPW divides the transform count (a constant then)
+    const int ntr = NTR % TPW == 0 ? TPW : min(TPW, NTR - u * TPW);
 #pragma unroll
     for (int i = 0; i < TPL; ++i) {
       if (trs[i] < ntr) {
@@ -773,6 +781,7 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
       }
     };
     wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store, rt);
+    take();
   }
   if (!COUT && peak) {
     for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));

@@ -45,7 +45,13 @@ __device__ __forceinline__ T norm2(cx<T> a) { return a.re * a.re + a.im * a.im; 
 // the loop-entry state (prefetch loads still in flight) with the back edge
 // (the previous iteration's stores in flight) and would otherwise wait for
 // all of those stores at the loop head of every iteration.
-__device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// The scheduling barriers keep the compiler from moving the prefetch loads
+// past the wait.
+__device__ __forceinline__ void wait_vmem_all() {
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __builtin_amdgcn_sched_barrier(0);
+}
 // multiply by (DIR * i)
 template <int DIR, typename T>
 __device__ __forceinline__ cx<T> mul_i(cx<T> a) {
