@@ -126,3 +126,33 @@ def test_blurred_objective_matches_oracle(device):
     cost, d_raw, _, _ = O.relaxed_objective(raw[0], target[0], 1.2, 0.3, 50.0, -1.0, 2.0)
     assert float(loss) == pytest.approx(cost, rel=1e-12)
     assert _rel(r.grad[0].cpu(), d_raw) <= 1e-12
+
+
+def test_odd_sizes_unaligned_views_and_many_images(device):
+    """Vector tails (37^2 pixels), 4-byte-offset views (the scalar path) and more images
+    than the final reduction's wavefronts (20)."""
+    import torch
+    from lithobat_amd.objective import mask_sigmoid, mask_sigmoid_grad, resist_objective
+    rng = np.random.default_rng(8)
+    n, b = 37, 20
+    base = rng.standard_normal(b * n * n + 1)
+    x = torch.tensor(base, device=device)[1:].reshape(b, n, n)  # 8-byte offset: not 16-aligned
+    m = mask_sigmoid(x, 8.0)
+    ref_m = O.mask_of(base[1:].reshape(b, n, n), 8.0)
+    assert _rel(m.cpu(), ref_m) <= 1e-12
+    g = rng.standard_normal((b, n, n))
+    out = torch.empty(b * n * n + 1, device=device, dtype=torch.float64)[1:].reshape(b, n, n)
+    gx = mask_sigmoid_grad(torch.tensor(g, device=device), m, 8.0, out=out)
+    assert gx.data_ptr() == out.data_ptr()
+    assert _rel(out.cpu(), O.grad_theta(g, ref_m, 8.0)) <= 1e-12
+    raw, target = _fields(b, n, 12)
+    r = resist_objective(torch.tensor(raw, device=device), torch.tensor(target, device=device),
+                         0.9, 0.3, 50.0, mismatch=True)
+    tot = 0.0
+    for i in range(b):
+        cost, d_raw, _, pe = O.relaxed_objective(raw[i], target[i], 0.9, 0.3, 50.0)
+        assert float(r["cost"][i]) == pytest.approx(cost, rel=1e-12)
+        assert _rel(r["d_intensity"][i].cpu(), d_raw) <= 1e-12
+        assert int(r["mismatch"][i]) == pe
+        tot += cost
+    assert float(r["total"]) == pytest.approx(tot, rel=1e-12)
