@@ -129,7 +129,7 @@ def test_blurred_objective_matches_oracle(device):
 
 
 def test_odd_sizes_unaligned_views_and_many_images(device):
-    """Vector tails (37^2 pixels), 4-byte-offset views (the scalar path) and more images
+    """Vector tails (37^2 pixels), 8-byte-offset views (the scalar path) and more images
     than the final reduction's wavefronts (20)."""
     import torch
     from lithobat_amd.objective import mask_sigmoid, mask_sigmoid_grad, resist_objective
