@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from .blur import blur, gaussian_blur
-from .expose import normalisation_scale
+from .expose import _to_device, normalisation_scale
 from .hopkins import SOCSKernels
 from .ilt import abbe_intensity, mask_of, mask_penalties, relaxed_resist_loss
 
@@ -130,6 +130,62 @@ def print_mask(mask, model: PrintModel) -> PrintedImage:
     aerial = raw * normalisation_scale(model.optics.normalisation, peak, clear, model.dose)
     fld, level = develop_field(aerial, model.resist, model.grid, model=model.model)
     return PrintedImage(mask, aerial, fld, level, model.tone, model.grid)
+
+
+def dose_for_target(target, model: PrintModel, mask=None) -> float:
+    """ilt.py:651-721: the dose at which ``mask`` (default: the target) prints the target's
+    area, ``model.dose`` when the target is empty or full.
+
+    The unit-dose field blur(scale * raw) is imaged and blurred on the device once and
+    sorted there.  The printed area sign * (dose * unit - level) > 0 is then a suffix
+    (clear tone) or prefix (dark tone) of the sorted values, so each of the reference's 60
+    geometric bisection steps counts it by a binary search that makes the same
+    floating-point comparison: the same counts, hence the same dose.
+    """
+    n = model.grid.n_pixels
+    tgt = np.asarray(target, dtype=np.float64)
+    m = tgt if mask is None else np.asarray(mask, dtype=np.float64)
+    if m.shape != (n, n):
+        raise ValueError(f"mask must be {(n, n)} for this grid, got {m.shape}")
+    want = int(np.count_nonzero(tgt > 0.5))
+    if want == 0 or want == tgt.size:
+        return float(model.dose)
+    kernels = model.kernels()
+    if kernels is None:
+        raise ValueError("dose_for_target needs a model that images through SOCS kernels")
+    md, _, _ = _to_device(m, kernels.plan.precision)
+    raw, _peak, clear = kernels.raw(md)
+    sigma_px = float(model.resist.diffusion_sigma / model.grid.pixel_size)
+    unit = blur(raw * normalisation_scale(model.optics.normalisation, 1.0, clear, 1.0), sigma_px)
+    u = torch.sort(unit.reshape(-1).double()).values.cpu().numpy()
+    level = float(model.resist.threshold)
+    clear_tone = model.tone == "clear"
+    total = u.size
+
+    def area(d: float) -> int:
+        # clear tone: d * u - level > 0 (a suffix); dark tone: level - d * u > 0 (a prefix).
+        # d * u is non-decreasing in u, so the first index where the comparison switches
+        # is found by bisection over the sorted values.
+        lo_i, hi_i = 0, total
+        while lo_i < hi_i:
+            mid_i = (lo_i + hi_i) // 2
+            above = float(d * u[mid_i]) - level > 0.0 if clear_tone else \
+                -(float(d * u[mid_i]) - level) <= 0.0
+            if above:
+                hi_i = mid_i
+            else:
+                lo_i = mid_i + 1
+        return total - lo_i if clear_tone else lo_i
+
+    lo, hi = 1e-3, 1e3
+    rising = area(hi) > area(lo)
+    for _ in range(60):
+        mid = float(np.sqrt(lo * hi))
+        if (area(mid) < want) == rising:
+            lo = mid
+        else:
+            hi = mid
+    return float(np.sqrt(lo * hi))
 
 
 @dataclass

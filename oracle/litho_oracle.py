@@ -734,6 +734,40 @@ def develop_field(aerial, resist, grid):
     return apply_peb(aerial, resist.diffusion_sigma, grid.pixel_size), float(resist.threshold)
 
 
+def dose_for_target(target, optics, grid, resist, dose=1.0, tone="clear", mask=None,
+                    socs_tol=1e-8):
+    """ilt.py:651-721: the dose at which ``mask`` (default: the target) prints as many
+    pixels as the target has, by 60 geometric bisection steps on [1e-3, 1e3] over the
+    unit-dose blurred field (the image is computed once)."""
+    n = grid.n_pixels
+    tgt = np.asarray(target, dtype=np.float64)
+    m = tgt if mask is None else np.asarray(mask, dtype=np.float64)
+    if m.shape != (n, n):
+        raise ValueError(f"mask must be {(n, n)} for this grid, got {m.shape}")
+    want = int(np.count_nonzero(tgt > 0.5))
+    if want == 0 or want == tgt.size:
+        return float(dose)
+    k = socs_build(optics, grid, tol=socs_tol)
+    raw, _peak, clear = socs_raw(k, m)
+    sigma_px = float(resist.diffusion_sigma / grid.pixel_size)
+    unit = blur(raw * normalisation_scale(optics.normalisation, 1.0, clear, 1.0), sigma_px)
+    level = float(resist.threshold)
+    sign = 1.0 if tone == "clear" else -1.0
+
+    def area(d):
+        return int(np.count_nonzero(sign * (d * unit - level) > 0.0))
+
+    lo, hi = 1e-3, 1e3
+    rising = area(hi) > area(lo)
+    for _ in range(60):
+        mid = np.sqrt(lo * hi)
+        if (area(mid) < want) == rising:
+            lo = mid
+        else:
+            hi = mid
+    return float(np.sqrt(lo * hi))
+
+
 def run_ilt(target, optics, grid, resist, dose=1.0, tone="clear", *, max_iter=120, step=0.4,
             tol=1e-7, beta=4.0, beta_final=12.0, initial_grey=0.9, steepness=12.0,
             weight_tv=0.0, weight_discrete=0.0, smooth=0.0, socs_tol=1e-8):

@@ -75,3 +75,25 @@ def test_run_ilt_refuses_what_the_reference_refuses(device):
         run_ilt(target, PrintModel(vec, resist, grid), max_iter=1)
     with pytest.raises(ValueError, match="prints nothing"):
         run_ilt(target, PrintModel(optics, resist, grid, dose=1e-3), max_iter=1)
+
+
+@pytest.mark.parametrize("tone", ["clear", "dark"])
+def test_dose_for_target_matches_oracle(device, tone):
+    """ilt.py:651-721: the same 60 geometric bisection steps over the printed-area count give
+    the same dose as the float64 restatement; empty and full targets return the model dose."""
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.opc import PrintModel, dose_for_target
+    optics, grid, resist = _process()
+    target = manhattan(64, 8e-9, seed=7, n_rect=(3, 8))
+    model = PrintModel(optics, resist, grid, dose=1.3, tone=tone)
+    got = dose_for_target(target, model)
+    want = O.dose_for_target(target, optics, grid, resist, dose=1.3, tone=tone)
+    assert got == pytest.approx(want, rel=1e-9)
+    other = O.lines_and_spaces(64, 8e-9, 160e-9, 80e-9)
+    got2 = dose_for_target(target, model, mask=other)
+    want2 = O.dose_for_target(target, optics, grid, resist, dose=1.3, tone=tone, mask=other)
+    assert got2 == pytest.approx(want2, rel=1e-9)
+    assert dose_for_target(np.zeros((64, 64)), model) == 1.3
+    assert dose_for_target(np.ones((64, 64)), model) == 1.3
+    with pytest.raises(ValueError, match="mask must be"):
+        dose_for_target(target, model, mask=other[:32])

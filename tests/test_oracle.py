@@ -482,3 +482,34 @@ def test_run_ilt_descends_and_stops():
     # (without the bias correction it would be step * 0.1 / sqrt(0.001) = 1.26 step)
     d = np.abs(theta1 - th0)
     assert np.all(d <= 0.4) and np.all(d >= 0.399)
+
+
+@pytest.mark.parametrize("tone", ["clear", "dark"])
+def test_dose_for_target_prints_the_target_area(tone):
+    """ilt.py:651-721 restated: at the returned dose the printed-pixel count crosses the
+    target's count (just below the dose on one side, just above on the other, in the
+    direction the tone makes the area move); empty / full targets keep the given dose."""
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    grid = GridConfig(n_pixels=32, pixel_size=8e-9)
+    optics = OpticsConfig(NA=1.35, n_immersion=1.44, source_type="annular", sigma_outer=0.9,
+                          sigma_inner=0.6, source_grid=9, normalisation="clear")
+
+    class R:
+        threshold, diffusion_sigma = 0.3, 8e-9
+    target = O.lines_and_spaces(32, 8e-9, 128e-9, 64e-9)
+    d = O.dose_for_target(target, optics, grid, R(), tone=tone)
+    assert 1e-3 < d < 1e3
+    k = O.socs_build(optics, grid)
+    raw, _, clear = O.socs_raw(k, target)
+    unit = O.blur(raw / clear, 1.0)
+    sign = 1.0 if tone == "clear" else -1.0
+
+    def area(x):
+        return int(np.count_nonzero(sign * (x * unit - 0.3) > 0.0))
+    want = int(np.count_nonzero(target > 0.5))
+    a_lo, a_hi = area(d * (1 - 1e-6)), area(d * (1 + 1e-6))
+    if tone == "clear":  # more dose prints more
+        assert a_lo <= want <= a_hi + 1 and a_hi >= a_lo
+    else:                # more dose clears more of a dark-tone resist
+        assert a_hi <= want <= a_lo + 1 and a_lo >= a_hi
+    assert O.dose_for_target(np.zeros((32, 32)), optics, grid, R(), dose=1.7) == 1.7
