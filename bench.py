@@ -339,6 +339,43 @@ def volume_leg(args, masks, sync):
                  sync)
 
 
+def euv_optics(mask_model):
+    from lithobat_amd.config import OpticsConfig
+    return OpticsConfig(wavelength=13.5e-9, NA=0.33, source_type="annular", sigma_outer=0.8,
+                        sigma_inner=0.4, source_grid=11, reduction=4.0, chief_ray_deg=6.0,
+                        mask_model=mask_model, normalisation="clear")
+
+
+def euv_leg(args, dev, sync):
+    """Reflective EUV masks: 32 masks 1024^2 @ 2 nm, NA 0.33 annular 0.4-0.8, the default
+    EUV stack (60 nm TaBN on 40 Mo/Si bilayers, Ru cap), 6 deg chief ray, FP32.  Images/s
+    with every order reflected by the blank at its own angle, the same batch through the
+    thin mask, and the plan set-up time (blank solve on the host + device tables)."""
+    import torch
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=1024, pixel_size=2e-9)
+    masks = torch.tensor(np.stack([manhattan(1024, 2e-9, seed=s) for s in range(3000, 3032)]),
+                         device=dev, dtype=torch.float32)
+    res = {}
+    for model in ("multilayer", "thin"):
+        t0 = time.perf_counter()
+        plan = ImagingPlan(euv_optics(model), grid, precision=32)
+        sync()
+        build = time.perf_counter() - t0
+        sec = timed(lambda: plan.forward(masks, with_peak=False), max(3, args.steps // 2), 1,
+                    sync)
+        res[model] = {"images_per_s": 32 / sec, "plan_s": build,
+                      "systems": plan.info["n_systems"]}
+    return {"images_per_s": res["multilayer"]["images_per_s"],
+            "thin_images_per_s": res["thin"]["images_per_s"],
+            "plan_s": res["multilayer"]["plan_s"], "thin_plan_s": res["thin"]["plan_s"],
+            "source_points": res["multilayer"]["systems"],
+            "config": "32 masks 1024^2 @ 2 nm, NA 0.33 annular 0.4-0.8 (grid 11), 60 nm TaBN "
+                      "on 40 Mo/Si bilayers, chief ray 6 deg on x, FP32, eager"}
+
+
 def run_ilt_leg(args, sync):
     """The full inverse-lithography solve on the device: 512^2 @ 8 nm, NA 1.35 annular
     0.6-0.9 SOCS kernels (FP32), threshold resist with 10 nm PEB, 4 nm mask smoothing,
@@ -389,7 +426,7 @@ def main():
     ap.add_argument("--ilt-steps", type=int, default=10)
     ap.add_argument("--dense-images", type=int, default=3,
                     help="images timed for the dense per-source-point form")
-    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,volume,solve,cpu",
+    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,volume,solve,euv,cpu",
                     help="comma list of measurements to run (batched always runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the legs, copy a MIB-sized buffer once (a plain copy kernel of "
@@ -530,6 +567,7 @@ def main():
     if "dense" in legs:
         dense_images_per_s = dense_leg(args, plain_plans[0], masks, sync)
     socs = socs_leg(args, dev, sync) if "socs" in legs else None
+    euv = euv_leg(args, dev, sync) if "euv" in legs else None
     if args.copy_calibration > 0:
         nb = args.copy_calibration << 20
         a = torch.empty(nb, dtype=torch.uint8, device=dev).fill_(1)
@@ -609,6 +647,7 @@ def main():
                       "same autograd step captured once into a graph and replayed, 1 mask "
                       f"and {ILT_BATCH} masks per step",
         "socs": socs,
+        "euv_multilayer": euv,
         "fp32_rel_err_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,

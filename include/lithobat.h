@@ -81,6 +81,18 @@ typedef struct {
   int32_t precision;       /* 32: complex64 / FP32 accumulate; 64: complex128 */
   int32_t coarse_m;        /* 0 = automatic coarse-grid side, else forced */
   int32_t chunk;           /* 0 = automatic systems per work item */
+  /* Reflective-mask model (MultilayerSpectra, expose/m3d/provider.py:190-293):
+   * source point s images the spectrum M(f) r(|sin_s + refl_doff f|), r the
+   * blank reflection interpolated linearly in angle (asin of the sine) from
+   * the table, 0 beyond sine 1, and its clear field is scaled by |r|^2 at the
+   * specular order.  n_refl = 0: thin mask (Abbe plans only). */
+  int32_t n_refl;           /* angle-table points (0 = thin mask) */
+  const double* refl_theta; /* [n_refl] angles [rad], increasing */
+  const double* refl_mag;   /* [n_refl] |r| */
+  const double* refl_phase; /* [n_refl] unwrapped arg r [rad] */
+  double refl_doff;         /* wavelength / reduction: sine per unit frequency */
+  const double* src_sin_x;  /* [n_source] mask-side direction sines of each */
+  const double* src_sin_y;  /*   source point's illumination */
 } lb_imaging_desc;
 
 typedef struct {

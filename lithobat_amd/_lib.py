@@ -30,6 +30,10 @@ class ImagingDesc(C.Structure):
         ("src_weight", C.POINTER(C.c_double)), ("src_fx", C.POINTER(C.c_double)),
         ("src_fy", C.POINTER(C.c_double)), ("src_phi", C.POINTER(C.c_double)),
         ("precision", C.c_int32), ("coarse_m", C.c_int32), ("chunk", C.c_int32),
+        ("n_refl", C.c_int32), ("refl_theta", C.POINTER(C.c_double)),
+        ("refl_mag", C.POINTER(C.c_double)), ("refl_phase", C.POINTER(C.c_double)),
+        ("refl_doff", C.c_double), ("src_sin_x", C.POINTER(C.c_double)),
+        ("src_sin_y", C.POINTER(C.c_double)),
     ]
 
 

@@ -84,6 +84,7 @@ struct Plan {
   double energy = 1.0;
   int n_band = 0;
   void* phi128 = nullptr;  // [K][W][W] complex128
+  double* refl = nullptr;  // reflective mask: [3][n_refl] angle, |r|, arg r
   // optional event timing of the coherent-system kernel (lb_profile_*)
   int profile = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -95,6 +96,7 @@ struct Plan {
     if (box) (void)hipFree(box);
     if (wsys) (void)hipFree(wsys);
     if (phi128) (void)hipFree(phi128);
+    if (refl) (void)hipFree(refl);
   }
   size_t csz() const { return precision == 64 ? 16 : 8; }
   size_t rsz() const { return precision == 64 ? 8 : 4; }
@@ -164,6 +166,26 @@ lb_status set_geometry_common(Plan& p, const lb_imaging_desc* d) {
     ++nz;
   }
   pp.n_zern = nz;
+  if (d->n_refl > 0) {  // reflective mask: the blank's reflection table on the device
+    const int nr = d->n_refl;
+    if (nr < 2 || !d->refl_theta || !d->refl_mag || !d->refl_phase || !d->src_sin_x ||
+        !d->src_sin_y)
+      return fail(LB_ERR_ARG, "reflective mask: need >= 2 table points and the source sines");
+    for (int i = 1; i < nr; ++i)
+      if (!(d->refl_theta[i] > d->refl_theta[i - 1]))
+        return fail(LB_ERR_ARG, "reflective mask: table angles must increase");
+    std::vector<double> tab(3 * (size_t)nr);
+    std::copy(d->refl_theta, d->refl_theta + nr, tab.begin());
+    std::copy(d->refl_mag, d->refl_mag + nr, tab.begin() + nr);
+    std::copy(d->refl_phase, d->refl_phase + nr, tab.begin() + 2 * nr);
+    LB_HIP(hipMalloc(&p.refl, sizeof(double) * tab.size()));
+    LB_HIP(hipMemcpy(p.refl, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    pp.n_refl = nr;
+    pp.refl_doff = d->refl_doff;
+    pp.refl_theta = p.refl;
+    pp.refl_mag = p.refl + nr;
+    pp.refl_phase = p.refl + 2 * nr;
+  }
   int dev = 0;
   LB_HIP(hipGetDevice(&dev));
   LB_HIP(hipDeviceGetAttribute(&p.cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -611,6 +633,10 @@ lb_status lb_abbe_plan_create(const lb_imaging_desc* d, lb_plan* out) {
       base.n_image = p.n_image[pl];
       base.box_y = oy[s];
       base.box_x = ox[s];
+      if (d->n_refl > 0) {
+        base.sin_x = d->src_sin_x[s];
+        base.sin_y = d->src_sin_y[s];
+      }
       const int2 bi = make_int2(oy[s] - p.nu0m, ox[s] - p.nu0m);
       if (!p.vector) {
         base.comp = -1;
@@ -695,7 +721,8 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
   *out = nullptr;
   lb_status rc = check_desc(d);
   if (rc != LB_OK) return rc;
-  if (d->vector) return fail(LB_ERR_ARG, "SOCS kernels apply to the scalar thin-mask process only");
+  if (d->vector || d->n_refl > 0)
+    return fail(LB_ERR_ARG, "SOCS kernels apply to the scalar thin-mask process only");
   if (d->n_planes != 1) return fail(LB_ERR_ARG, "SOCS plans image one plane");
   lb_plan h = new lb_plan_s();
   Plan& p = h->p;

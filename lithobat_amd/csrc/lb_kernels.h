@@ -173,6 +173,7 @@ struct PupilSys {
   int comp;              // -1 scalar, 0..2 = x, y, z component
   int box_y, box_x;      // frequency index (box coordinates) of the box origin
   int pad;
+  double sin_x, sin_y;   // reflective mask: mask-side sines of the illumination
 };
 
 constexpr int kMaxZernike = 21;
@@ -188,6 +189,12 @@ struct PupilParams {
   int n_zern;
   int zn[kMaxZernike], zm[kMaxZernike];
   double znorm[kMaxZernike], zcoef[kMaxZernike], zrad[kMaxZernike][3];
+  // reflective mask (n_refl > 0): blank reflection table on the device
+  int n_refl;
+  double refl_doff;
+  const double* refl_theta;
+  const double* refl_mag;
+  const double* refl_phase;
 };
 
 // H[j][a][b] (complex T) and dc[j] = weight^2-free DC power of system j.

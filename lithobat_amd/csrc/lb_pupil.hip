@@ -64,6 +64,41 @@ __device__ double zernike_sum(double rho, double phi, const PupilParams& pp) {
   return W;
 }
 
+// Linear interpolation of (xp, fp) at x as numpy.interp defines it: the end
+// values outside [xp[0], xp[n-1]], fp[j] at a table point, else
+// slope * (x - xp[j]) + fp[j] on the bracketing interval.
+__device__ double interp_table(double x, const double* xp, const double* fp, int n) {
+  if (x < xp[0]) return fp[0];
+  if (x >= xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n - 1;  // xp[lo] <= x < xp[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  if (xp[lo] == x) return fp[lo];
+  const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+  return slope * (x - xp[lo]) + fp[lo];
+}
+
+// The blank's amplitude reflection for an order leaving the mask at direction
+// sine st (provider.py:252-266): zero for an evanescent order (st > 1), else
+// |r| and arg r interpolated at asin(st).
+__device__ void blank_reflection(double st, const PupilParams& pp, double* rr, double* ri) {
+  *rr = 0.0;
+  *ri = 0.0;
+  if (!(st <= 1.0)) return;
+  const double th = asin(fmin(fmax(st, 0.0), 1.0));
+  const double mag = interp_table(th, pp.refl_theta, pp.refl_mag, pp.n_refl);
+  const double ph = interp_table(th, pp.refl_theta, pp.refl_phase, pp.n_refl);
+  double sp, cp;
+  sincos(ph, &sp, &cp);
+  *rr = mag * cp;
+  *ri = mag * sp;
+}
+
 template <typename T>
 __global__ void k_pupil(const PupilSys* __restrict__ sys, int n_sys, PupilParams pp,
                         cx<T>* __restrict__ H, double* __restrict__ dc) {
@@ -110,6 +145,16 @@ __global__ void k_pupil(const PupilSys* __restrict__ sys, int n_sys, PupilParams
         hr = s.weight * cw * v;
         hi = s.weight * sw * v;
         dcv = v * v;
+      }
+      if (pp.n_refl > 0) {
+        // reflective mask: this order leaves the mask at its own angle
+        const double sx = s.sin_x + pp.refl_doff * FX, sy = s.sin_y + pp.refl_doff * FY;
+        double rr, ri;
+        blank_reflection(sqrt(sx * sx + sy * sy), pp, &rr, &ri);
+        const double nr = hr * rr - hi * ri, ni = hr * ri + hi * rr;
+        hr = nr;
+        hi = ni;
+        dcv *= rr * rr + ri * ri;  // the clear field of this source point (specular order)
       }
       h = cx<T>{(T)hr, (T)hi};
     }

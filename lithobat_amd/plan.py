@@ -47,9 +47,12 @@ class ImagingPlan:
             raise ValueError(f"imaging_model must be 'scalar' or 'vector', got "
                              f"'{optics.imaging_model}'")
         mask_model = getattr(optics, "mask_model", "thin")
-        if mask_model != "thin":
+        if mask_model not in ("thin", "multilayer"):
             raise NotImplementedError(
-                f"mask_model={mask_model!r}: only the thin-mask spectrum is implemented")
+                f"mask_model={mask_model!r}: the thin and multilayer mask spectra are "
+                "implemented")
+        if mask_model == "multilayer" and kind != "abbe":
+            raise ValueError("SOCS kernels apply to the scalar thin-mask process only")
         if planes is None:
             planes = [(optics.defocus, _image_index(optics))]
         self.planes = [(float(d), float(n)) for d, n in planes]
@@ -98,6 +101,25 @@ class ImagingPlan:
         d.precision = precision
         d.coarse_m = int(coarse_m)
         d.chunk = int(chunk)
+        if mask_model == "multilayer":
+            # provider.py:529-545: the stack named by the optics, which must be reflective
+            from . import m3d
+            stack = m3d.stack_for(optics)
+            if not stack.is_reflective:
+                raise ValueError(
+                    "mask_model='multilayer' describes the Bragg mirror under an EUV mask, "
+                    f"but the stack regime is '{stack.regime}'. Use mask_model='thin' or "
+                    "'fdtd' for a transmissive mask.")
+            theta, mag, phase = m3d.reflection_table(stack, optics.wavelength)
+            sines = [m3d.mask_side_sin_theta(fx_, fy_, optics)
+                     for fx_, fy_ in zip(self.src.fx, self.src.fy)]
+            d.n_refl = len(theta)
+            d.refl_theta = _dptr(arr(theta), C.c_double)
+            d.refl_mag = _dptr(arr(mag), C.c_double)
+            d.refl_phase = _dptr(arr(phase), C.c_double)
+            d.refl_doff = float(optics.wavelength / optics.reduction)
+            d.src_sin_x = _dptr(arr([v[0] for v in sines]), C.c_double)
+            d.src_sin_y = _dptr(arr([v[1] for v in sines]), C.c_double)
         handle = C.c_void_p()
         if kind == "abbe":
             _lib.check(lib.lb_abbe_plan_create(C.byref(d), C.byref(handle)))

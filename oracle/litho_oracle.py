@@ -352,8 +352,85 @@ class _Stack:
         self.specs, self.planes, self.weights = [], [], []
 
 
+# --------------------------------------------------------------------------
+# Reflective (EUV multilayer) mask spectra (expose/m3d/provider.py:190-293,
+# multilayer.py, stack.py, materials.py, coat/films.py:88-213)
+# --------------------------------------------------------------------------
+
+_EUV_INDEX = {  # materials.py:45-56, n + ik at 13.5 nm
+    "vacuum": 1.0 + 0.0j, "Si": 0.99931 + 0.00182j, "Mo": 0.92108 + 0.00644j,
+    "Ru": 0.88651 + 0.01710j, "TaBN": 0.94900 + 0.03120j, "TaBO": 0.97000 + 0.02500j,
+    "Ta": 0.94270 + 0.04170j, "SiO2": 0.97800 + 0.01070j,
+}
+
+
+def blank_films(optics):
+    """stack.py:106-125 with multilayer.py:95-120: (index, thickness) from the ambient
+    down to the substrate of the EUV blank the optics name (default 40 x Mo/Si, Ru cap)."""
+    spec = optics.mask_stack or {}
+    if spec.get("regime", "euv_reflective") != "euv_reflective":
+        raise ValueError("multilayer mask model needs a reflective stack")
+    ml = dict(n_bilayers=40, period=6.94e-9, gamma=0.4, cap_material="Ru",
+              cap_thickness=2.5e-9, substrate="Si")
+    ml.update(spec.get("multilayer") or {})
+    t_mo = ml["period"] * ml["gamma"]
+    films = [(_EUV_INDEX["vacuum"], 0.0), (_EUV_INDEX[ml["cap_material"]], ml["cap_thickness"])]
+    for _ in range(ml["n_bilayers"]):
+        films += [(_EUV_INDEX["Si"], ml["period"] - t_mo), (_EUV_INDEX["Mo"], t_mo)]
+    films.append((_EUV_INDEX[ml["substrate"]], 0.0))
+    return films
+
+
+def film_reflection(films, lam, theta0, pol):
+    """films.py:168-213: characteristic-matrix reflection seen from the ambient, one angle."""
+    n = [np.conj(complex(f[0])) for f in films]
+    sin0 = n[0] * np.sin(theta0)
+    cos_t = [np.sqrt(1.0 - (sin0 / nj) ** 2) for nj in n]
+    eta = [nj * c if pol == "s" else nj / c for nj, c in zip(n, cos_t)]
+    Mt = np.eye(2, dtype=complex)
+    for j in range(1, len(films) - 1):
+        dlt = 2.0 * np.pi * n[j] * cos_t[j] * films[j][1] / lam
+        Mt = Mt @ np.array([[np.cos(dlt), 1j * np.sin(dlt) / eta[j]],
+                            [1j * eta[j] * np.sin(dlt), np.cos(dlt)]])
+    b, c = Mt @ np.array([1.0, eta[-1]])
+    y = c / b
+    return complex((eta[0] - y) / (eta[0] + y))
+
+
+def reflection_table(optics):
+    """provider.py:226-250: 501 angles over [0, 25 deg]; |r| and unwrapped arg r of the
+    mean of the s and p reflections."""
+    films = blank_films(optics)
+    theta = np.linspace(0.0, np.deg2rad(25.0), 501)
+    r = np.array([0.5 * (film_reflection(films, optics.wavelength, t, "s")
+                         + film_reflection(films, optics.wavelength, t, "p")) for t in theta])
+    return theta, np.abs(r), np.unwrap(np.angle(r))
+
+
+def blank_reflection_at(sin_theta, table):
+    """provider.py:252-266: r at each order's direction sine, zero when evanescent."""
+    theta, mag, phase = table
+    s = np.asarray(sin_theta, dtype=np.float64)
+    ang = np.arcsin(np.clip(s, 0.0, 1.0))
+    r = np.interp(ang, theta, mag) * np.exp(1j * np.interp(ang, theta, phase))
+    return np.where(s <= 1.0, r, 0.0)
+
+
+def mask_side_sines(fs_x, fs_y, optics):
+    """provider.py:72-122 (SOURCE_TILT_SIGN = -1, aerial_image.py:73)."""
+    sx = -optics.wavelength * fs_x / optics.reduction
+    sy = -optics.wavelength * fs_y / optics.reduction
+    cra = math.sin(math.radians(optics.chief_ray_deg))
+    if optics.chief_ray_axis.lower() == "x":
+        sx += cra
+    else:
+        sy += cra
+    return sx, sy
+
+
 def abbe_sum(mask, optics, grid, planes, src=None):
-    """aerial_image.py:351-494 (thin mask): raw (P,n,n), peak (P,), clear (P,)."""
+    """aerial_image.py:351-494: raw (P,n,n), peak (P,), clear (P,); thin mask, or the
+    reflective multilayer model (per-source spectrum and clear scale, :421-438)."""
     n = grid.n_pixels
     lam, NA = optics.wavelength, optics.NA
     model = optics.imaging_model.lower()
@@ -367,6 +444,11 @@ def abbe_sum(mask, optics, grid, planes, src=None):
         src = source_points(optics)
     M = np.fft.fft2(np.asarray(mask).astype(np.complex128))  # provider.py:177-178
     FX, FY = make_freq_grid(n, grid.pixel_size)
+    multilayer = getattr(optics, "mask_model", "thin") == "multilayer"
+    if multilayer:
+        table = reflection_table(optics)
+        dx = optics.wavelength * FX / optics.reduction
+        dy = optics.wavelength * FY / optics.reduction
     P = len(planes)
     aerial = np.zeros((P, n, n))
     clear = np.zeros(P)
@@ -378,7 +460,15 @@ def abbe_sum(mask, optics, grid, planes, src=None):
         idx = np.nonzero(inside)
         rho_in = rho[idx]
         phi_in = None if phi is None else phi[idx]
-        M_in = M[idx]
+        scale_clear = 1.0
+        if multilayer:
+            sx, sy = mask_side_sines(float(src.fx[s]), float(src.fy[s]), optics)
+            M_s = M * blank_reflection_at(np.sqrt((sx + dx) ** 2 + (sy + dy) ** 2), table)
+            scale_clear = float(abs(blank_reflection_at(np.array([math.hypot(sx, sy)]),
+                                                        table)[0]) ** 2)
+            M_in = M_s[idx]
+        else:
+            M_in = M[idx]
         dc_in = float(inside[0, 0])
         states = jones_states(optics.polarisation, float(src.phi[s])) if vector else None
         for p, (dz, n_img) in enumerate(planes):
@@ -388,7 +478,7 @@ def abbe_sum(mask, optics, grid, planes, src=None):
                 spec = np.zeros((n, n), dtype=np.complex128)
                 spec[idx] = coh
                 stack.push(spec, p, w)
-                clear[p] += w * dc_in
+                clear[p] += w * dc_in * scale_clear
                 continue
             for jx, jy, wp in states:
                 V = vector_coefficients(rho_in, phi_in, jx, jy, NA / n_img,
@@ -399,7 +489,7 @@ def abbe_sum(mask, optics, grid, planes, src=None):
                     stack.push(spec, p, w * wp)
                 if dc_in:
                     at = np.nonzero((idx[0] == 0) & (idx[1] == 0))[0][0]
-                    clear[p] += (w * wp) * sum(abs(c[at]) ** 2 for c in V)
+                    clear[p] += (w * wp) * sum(abs(c[at]) ** 2 for c in V) * scale_clear
     stack.flush()
     return aerial, aerial.reshape(P, -1).max(axis=1), clear
 

@@ -156,5 +156,73 @@ def main():
     print("golden vectors written to", HERE)
 
 
+def main_multilayer():
+    """The reflective-mask (EUV multilayer) images: written to multilayer64.npz by
+    ``make_golden.py multilayer``."""
+    sys.path.insert(0, REF_SRC)
+    from litho_sim.core.config import GridConfig, OpticsConfig
+    from litho_sim.expose import compute_aerial_image, compute_aerial_planes
+    from litho_sim.expose.m3d.provider import MultilayerSpectra
+    from litho_sim.expose.m3d.stack import MaskStack
+
+    meta = {"numpy": np.__version__, "scipy": scipy.__version__}
+    keys = ["wavelength", "NA", "n_immersion", "defocus", "sigma_outer", "sigma_inner",
+            "source_type", "source_grid", "source_kwargs", "zernike_coeffs", "exact_defocus",
+            "n_image", "imaging_model", "polarisation", "obliquity", "normalisation",
+            "mask_model", "reduction", "chief_ray_deg", "chief_ray_axis", "mask_stack"]
+
+    def optics_dict(o):
+        d = {k: getattr(o, k) for k in keys}
+        d["zernike_coeffs"] = {str(k): v for k, v in d["zernike_coeffs"].items()}
+        return d
+
+    grid = GridConfig(n_pixels=64, pixel_size=4e-9)
+    mask = manhattan(64, 4e-9, seed=31, wmin=16e-9, wmax=60e-9)
+    arrs = {"mask": mask, "n": 64, "pixel_size": 4e-9}
+    base = dict(wavelength=13.5e-9, NA=0.33, n_immersion=1.0, source_type="annular",
+                sigma_outer=0.8, sigma_inner=0.4, source_grid=11, reduction=4.0,
+                chief_ray_deg=6.0, mask_model="multilayer", normalisation="clear")
+    thin = OpticsConfig(**{**base, "mask_model": "thin"})
+    cases = {
+        "scalar": OpticsConfig(**base, defocus=20e-9),
+        "scalar_y_30bl": OpticsConfig(**{**base, "chief_ray_axis": "y"},
+                                      mask_stack={"regime": "euv_reflective",
+                                                  "multilayer": {"n_bilayers": 30,
+                                                                 "period": 7.0e-9}}),
+        "vector_te": OpticsConfig(**base, imaging_model="vector", polarisation="te",
+                                  zernike_coeffs={5: 0.01, 7: -0.015}),
+    }
+    names = []
+    for name, o in cases.items():
+        raw, peak, clear = compute_aerial_image(mask, o, grid, return_raw=True)
+        arrs[f"{name}__raw"] = raw
+        arrs[f"{name}__peak"] = peak
+        arrs[f"{name}__clear"] = clear
+        arrs[f"{name}__optics"] = np.array(json.dumps(optics_dict(o)))
+        names.append(name)
+        st = MaskStack.from_spec(o.mask_stack) if o.mask_stack else \
+            MaskStack.for_wavelength(o.wavelength)
+        ms = MultilayerSpectra(mask, o, grid, st)
+        arrs[f"{name}__theta"] = ms._theta
+        arrs[f"{name}__mag"] = ms._mag
+        arrs[f"{name}__phase"] = ms._phase
+    planes = [(-30e-9, 1.0), (0.0, 1.0), (30e-9, 1.0)]
+    arrs["planes"] = np.array(planes)
+    arrs["planes__raw"] = compute_aerial_planes(mask, cases["scalar"], grid, planes)
+    raw, peak, clear = compute_aerial_image(mask, thin, grid, return_raw=True)
+    arrs["thin__raw"] = raw
+    arrs["thin__clear"] = clear
+    # the blank's reflectance at 6 degrees, 40 bilayers (the materials table's check)
+    st = MaskStack.for_wavelength(13.5e-9)
+    arrs["blank_r_s_6deg"] = np.array(st.blank_reflection(13.5e-9, np.deg2rad(6.0), "s"))
+    arrs["blank_r_p_6deg"] = np.array(st.blank_reflection(13.5e-9, np.deg2rad(6.0), "p"))
+    np.savez_compressed(os.path.join(HERE, "multilayer64.npz"), meta=json.dumps(meta),
+                        cases=np.array(names), **arrs)
+    print("multilayer golden vectors written to", HERE)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["multilayer"]:
+        main_multilayer()
+    else:
+        main()

@@ -513,3 +513,82 @@ def test_dose_for_target_prints_the_target_area(tone):
     else:                # more dose clears more of a dark-tone resist
         assert a_hi <= want <= a_lo + 1 and a_lo >= a_hi
     assert O.dose_for_target(np.zeros((32, 32)), optics, grid, R(), dose=1.7) == 1.7
+
+
+def _ml_optics(z, case):
+    from lithobat_amd.config import OpticsConfig
+    d = json.loads(str(z[f"{case}__optics"]))
+    d["zernike_coeffs"] = {int(k): v for k, v in d["zernike_coeffs"].items()}
+    return OpticsConfig(**d)
+
+
+def test_multilayer_oracle_matches_golden_images():
+    """Reflective-mask Abbe sums restated in the oracle vs the reference's images (scalar
+    with defocus, a 30-bilayer blank with the chief ray on y, vector TE with Zernike), their
+    clear fields and angle tables; planes of one call vs the reference's plane stack."""
+    from lithobat_amd.config import GridConfig
+    z = np.load(os.path.join(GOLD, "multilayer64.npz"))
+    grid = GridConfig(n_pixels=64, pixel_size=4e-9)
+    for c in z["cases"]:
+        o = _ml_optics(z, c)
+        th, mag, ph = O.reflection_table(o)
+        assert np.array_equal(th, z[f"{c}__theta"])
+        assert np.abs(mag - z[f"{c}__mag"]).max() <= 1e-13
+        assert np.abs(ph - z[f"{c}__phase"]).max() <= 1e-12
+        raw, pk, cl = O.compute_aerial_image(z["mask"], o, grid, return_raw=True)
+        assert np.abs(raw - z[f"{c}__raw"]).max() <= 1e-12 * z[f"{c}__raw"].max()
+        assert cl == pytest.approx(float(z[f"{c}__clear"]), rel=1e-12)
+    o = _ml_optics(z, "scalar")
+    st = O.compute_aerial_planes(z["mask"], o, grid, [tuple(p) for p in z["planes"]])
+    assert np.abs(st - z["planes__raw"]).max() <= 1e-12 * z["planes__raw"].max()
+
+
+def test_film_reflection_closed_forms():
+    """The characteristic-matrix reflection: a zero-thickness layer leaves the Fresnel
+    coefficients of the ambient / substrate interface (both polarisations, oblique); a
+    lossless quarter-wave layer of index sqrt(n_sub) at normal incidence cancels the
+    reflection; the 40-bilayer Mo/Si blank reflects 74.7 % at 6 deg (the materials table's
+    physical check) as the reference computes it."""
+    n0, n2, th = 1.0, 1.5, 0.4
+    films = [(n0, 0.0), (1.3, 0.0), (n2, 0.0)]
+    c0 = np.cos(th)
+    c2 = np.sqrt(1 - (n0 * np.sin(th) / n2) ** 2)
+    rs = (n0 * c0 - n2 * c2) / (n0 * c0 + n2 * c2)
+    rp = (n0 / c0 - n2 / c2) / (n0 / c0 + n2 / c2)
+    assert O.film_reflection(films, 193e-9, th, "s") == pytest.approx(rs, abs=1e-15)
+    assert O.film_reflection(films, 193e-9, th, "p") == pytest.approx(rp, abs=1e-15)
+    lam = 193e-9
+    n1 = np.sqrt(n2)
+    qw = [(n0, 0.0), (n1, lam / (4 * n1)), (n2, 0.0)]
+    assert abs(O.film_reflection(qw, lam, 0.0, "s")) <= 1e-15
+    z = np.load(os.path.join(GOLD, "multilayer64.npz"))
+    from lithobat_amd.config import OpticsConfig
+    o = OpticsConfig(wavelength=13.5e-9, NA=0.33, mask_model="multilayer", chief_ray_deg=6.0)
+    films = O.blank_films(o)
+    r_s = O.film_reflection(films, 13.5e-9, np.deg2rad(6.0), "s")
+    assert abs(r_s) ** 2 == pytest.approx(0.747, abs=5e-4)
+    assert r_s == pytest.approx(complex(z["blank_r_s_6deg"]), abs=1e-13)
+    assert O.film_reflection(films, 13.5e-9, np.deg2rad(6.0), "p") == pytest.approx(
+        complex(z["blank_r_p_6deg"]), abs=1e-13)
+
+
+def test_host_reflection_table_matches_oracle():
+    """lithobat_amd.m3d (the plan's host-side blank solve, vectorised over angles) against
+    the oracle's per-angle restatement and the reference's tables; the reference's errors."""
+    from lithobat_amd import m3d
+    z = np.load(os.path.join(GOLD, "multilayer64.npz"))
+    for c in z["cases"]:
+        o = _ml_optics(z, c)
+        th, mag, ph = m3d.reflection_table(m3d.stack_for(o), o.wavelength)
+        assert np.array_equal(th, z[f"{c}__theta"])
+        assert np.abs(mag - z[f"{c}__mag"]).max() <= 1e-13
+        assert np.abs(ph - z[f"{c}__phase"]).max() <= 1e-12
+        for fx, fy in ((0.0, 0.0), (3.1e6, -1.7e6)):
+            assert m3d.mask_side_sin_theta(fx, fy, o) == pytest.approx(
+                O.mask_side_sines(fx, fy, o), abs=1e-17)
+    with pytest.raises(KeyError, match="Unknown mask material"):
+        m3d.mask_material("Au", 13.5e-9)
+    with pytest.raises(KeyError, match="No mask-material table"):
+        m3d.mask_material("Si", 248e-9)
+    with pytest.raises(ValueError, match="needs a multilayer"):
+        m3d.MaskStack(multilayer=None)
