@@ -39,6 +39,7 @@ lb_status fail(lb_status s, const std::string& msg) {
   } while (0)
 
 constexpr size_t kLdsLimit = 160 * 1024 - 64;  // dynamic LDS; the kernels' static slots take the rest
+constexpr size_t kStagedPassBytes = 160u << 20;  // row tables per staged pass (last-level cache 256 MB)
 const int kFieldSizes[] = {16, 32, 48, 64, 80, 96, 128, 256};
 // Sizes the automatic choice picks from: the 80-point grid (radix 10 x 8)
 // runs slower per image than the 96-point one on the same boxes, so it is
@@ -108,8 +109,9 @@ bool field_size(int m) {
   return false;
 }
 
-// Coherent-system kernel shape for (M, W): prefer no column split, then the
-// smallest split whose LDS fits (the adjoint keeps two W x M/CS tables).
+// Coherent-system kernel shape for (M, W): the first of field_configs() (the
+// staged shapes on M = 256, else no column split, then the smallest split)
+// whose LDS fits; the adjoint takes single-launch shapes only.
 bool pick_field_configs(Plan& p) {
   bool got_f = false, got_a = false;
   for (const FieldConfig& c : field_configs()) {
@@ -327,7 +329,8 @@ struct Layout {
 
 struct Work {
   int n_masks, n_img, n_chunks, chunk, n_items;
-  size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, queue;
+  int pass_img;  // staged forward: images whose row tables are held at once
+  size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, queue, G;
   size_t total;
 };
 
@@ -353,6 +356,14 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
   w.Xm = L.take(C * n_masks * p.Wbm * p.Wbm);
   w.RTm = L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
+  if (!backward && p.fwd.csplit == 0) {
+    // row tables of the staged forward, a pass of images at a time, sized to
+    // stay in the last-level cache between its two launches
+    const size_t per_img = staged_table_bytes(p.precision, p.M, p.W) * (size_t)std::max(p.spp, 1);
+    w.pass_img = (int)std::max<size_t>(1, std::min<size_t>((size_t)w.n_img,
+                                                           kStagedPassBytes / per_img));
+    w.G = L.take(per_img * (size_t)w.pass_img);
+  }
   {
     // split-sum scratch: coarse partials (forward) or band gradients (backward)
     const long long mm = (long long)p.M * p.M, bb = 2LL * p.Wbm * p.Wbm;
@@ -467,7 +478,13 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
     LB_HIP(profile_events(p, &e0, &e1));
     LB_HIP(hipEventRecord(p.ev_pool[e0], st));
   }
-  LB_HIP(field_forward(p.precision, fc, a, ws + w.partial, st));
+  if (fc.csplit == 0) {
+    for (int i0 = 0; i0 < w.n_img; i0 += w.pass_img)
+      LB_HIP(field_forward_staged(p.precision, fc, a, ws + w.partial, ws + w.G, i0,
+                                  std::min(w.pass_img, w.n_img - i0), p.cus, st));
+  } else {
+    LB_HIP(field_forward(p.precision, fc, a, ws + w.partial, st));
+  }
   if (p.profile) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
   return fwd_post<T>(p, images, peak, ws, w, st);
 }
@@ -943,6 +960,8 @@ lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads
   if (!plan) return fail(LB_ERR_ARG, "null plan");
   Plan& p = plan->p;
   FieldConfig fc{p.M, block_threads, column_splits, 0};
+  if (adjoint && column_splits == 0)
+    return fail(LB_ERR_UNSUPPORTED, "the staged coherent-system path (splits 0) is forward only");
   if (!field_config_supported(p.precision, fc))
     return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel of shape M=" +
                                         std::to_string(p.M) + " threads=" +
@@ -1086,7 +1105,7 @@ lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) 
   }
   bool got_f = false, got_a = false;
   for (const FieldConfig& c : field_configs()) {
-    if (c.M != b->M) continue;
+    if (c.M != b->M || c.csplit == 0) continue;  // batches take the single-launch kernels
     if (!got_f && field_smem_bytes(b->precision, c, b->Wmax, false) <= kLdsLimit) {
       b->fwd = c;
       got_f = true;

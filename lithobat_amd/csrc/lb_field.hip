@@ -640,19 +640,197 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
   }
 }
 
+// ------------------------------------------- staged path (row tables in memory)
+// On large coarse grids a system's W x M row table does not fit in LDS beside
+// the round buffer (W = 117, M = 256: 240 KB in FP32), and column splits
+// recompute the row pass once per split.  The staged path instead writes the
+// row table of every (image, system) of a pass to device memory once
+// (k_rows_staged: one block task = CPR rows of one table), then
+// k_cols_staged runs the column transforms of CPR columns of one work item
+// over the item's systems, accumulating w |E|^2 in registers; the next
+// system's column inputs load while the current one runs its second stage.
+// A pass's tables are sized to stay in the last-level cache between the two
+// launches.  Same arithmetic as the single-launch kernel, stage by stage.
+template <typename T, int M, int NT>
+__global__ __launch_bounds__(NT) void k_rows_staged(MapSource<T, M> src, int img0, int n_img,
+                                                    cx<T>* __restrict__ G,
+                                                    const cx<T>* __restrict__ twg) {
+  using F = FieldShape<T, M, NT, 1>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
+  cx<T>* S = tw + M;                               // CPR x LDC
+  const int tid = threadIdx.x;
+  load_roots(tw, twg, M, tid, NT);
+  __syncthreads();
+  const int W = src.W, spp = src.map.sys_per_plane, n_planes = src.map.n_planes;
+  const int nrc = (W + F::CPR - 1) / F::CPR;
+  const long long n_tasks = (long long)n_img * spp * nrc;
+  const int rl1 = tid / F::T1, j1 = tid - rl1 * F::T1;
+  for (long long task = blockIdx.x; task < n_tasks; task += gridDim.x) {
+    const int rc = (int)(task % nrc);
+    const long long slot = task / nrc;  // (image in pass) * spp + system in plane
+    const int jl = (int)(slot % spp);
+    const int img = img0 + (int)(slot / spp);
+    const int plane = img % n_planes, mask = img / n_planes;
+    const int j = plane * spp + jl;
+    const cx<T>* Xm = src.X + (long long)mask * src.Wb * src.Wb;
+    const cx<T>* Hj = src.H + (long long)j * W * W;
+    const int2 bx = src.box[j];
+    cx<T>* Gt = G + slot * W * M;
+    const int a0 = rc * F::CPR;
+    {
+      const int a = a0 + rl1;
+      if (a < W) {
+        cx<T> v[F::R1];
+        const int xr = wrap_idx(bx.x + a, src.wrapN);
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) {
+          const int b = j1 + r * F::T1;
+          if (r < F::NRW && b < W)
+            v[r] = Xm[(long long)xr * src.Wb + wrap_idx(bx.y + b, src.wrapN)] * Hj[a * W + b];
+          else
+            v[r] = cx<T>{0, 0};
+        }
+        dft<F::R1, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) S[rl1 * F::LDC + j1 * F::R1 + r] = v[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < F::Q2; ++q) {
+      const int t = tid + q * NT;
+      const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+      const int a = a0 + rl;
+      if (t < F::CPR * F::T2 && a < W) {
+        cx<T> v[F::R2];
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          v[r] = S[rl * F::LDC + j2 + r * F::R1];
+          if (r > 0) v[r] = v[r] * tw[r * j2];
+        }
+        dft<F::R2, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) Gt[(long long)a * M + j2 + r * F::R1] = v[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int M, int NT>
+__global__ __launch_bounds__(NT) void k_cols_staged(MapSource<T, M> src, int img0, int n_img,
+                                                    const cx<T>* __restrict__ G,
+                                                    const cx<T>* __restrict__ twg) {
+  using F = FieldShape<T, M, NT, 1>;
+  constexpr int NG = M / F::CPR;  // column groups of a coarse grid
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
+  cx<T>* Cb = tw + M;                              // CPR x LDC
+  const int tid = threadIdx.x;
+  load_roots(tw, twg, M, tid, NT);
+  __syncthreads();
+  const int W = src.W, spp = src.map.sys_per_plane, n_planes = src.map.n_planes;
+  const int n_chunks = src.map.n_chunks;
+  const int c1 = tid % F::CPR, jc1 = tid / F::CPR;
+  const long long n_tasks = (long long)n_img * n_chunks * NG;
+  for (long long task = blockIdx.x; task < n_tasks; task += gridDim.x) {
+    const int g = (int)(task % NG);
+    const int item = img0 * n_chunks + (int)(task / NG);
+    const ItemView<T> iv = src.get(item);
+    const int img = item / n_chunks, plane = img % n_planes;
+    // row table of system j (of this image's plane): Gi + j * W * M
+    const cx<T>* Gi = G + ((long long)(img - img0) * spp - (long long)plane * spp) * W * M;
+    const int x = g * F::CPR + c1;
+    cx<T> nx[F::NRW];
+    auto fetch = [&](int j) {
+      const cx<T>* Gt = Gi + (long long)j * W * M;
+#pragma unroll
+      for (int r = 0; r < F::NRW; ++r) {
+        const int a = jc1 + r * F::T1;
+        nx[r] = a < W ? Gt[(long long)a * M + x] : cx<T>{0, 0};
+      }
+    };
+    T acc[F::Q2 * F::R2];
+#pragma unroll
+    for (int i = 0; i < F::Q2 * F::R2; ++i) acc[i] = 0;
+    if (iv.s0 < iv.s1) fetch(iv.s0);
+    for (int j = iv.s0; j < iv.s1; ++j) {
+      const T w = iv.wsys[j];
+      {
+        cx<T> v[F::R1];
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) v[r] = r < F::NRW ? nx[r] : cx<T>{0, 0};
+        dft<F::R1, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) Cb[c1 * F::LDC + jc1 * F::R1 + r] = v[r];
+      }
+      if (j + 1 < iv.s1) fetch(j + 1);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < F::Q2; ++q) {
+        const int t = tid + q * NT;
+        if (F::NT2 % NT == 0 || t < F::NT2) {
+          const int c2 = t % F::CPR, j2 = t / F::CPR;
+          cx<T> v[F::R2];
+#pragma unroll
+          for (int r = 0; r < F::R2; ++r) {
+            v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
+            if (r > 0) v[r] = v[r] * tw[r * j2];
+          }
+          dft<F::R2, +1>(v);
+#pragma unroll
+          for (int r = 0; r < F::R2; ++r) acc[q * F::R2 + r] += w * norm2(v[r]);
+        }
+      }
+      __syncthreads();
+    }
+    T* out = iv.part;
+#pragma unroll
+    for (int q = 0; q < F::Q2; ++q) {
+      const int t = tid + q * NT;
+      if (F::NT2 % NT == 0 || t < F::NT2) {
+        const int c2 = t % F::CPR, j2 = t / F::CPR;
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r)
+          out[(j2 + r * F::R1) * M + g * F::CPR + c2] = acc[q * F::R2 + r];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- dispatch
-#define LB_FIELD_CONFIGS(X)                                                                  \
+#define LB_FIELD_CONFIGS(X)                                                                \
   X(16, 64, 1) X(32, 128, 1) X(48, 64, 1) X(64, 256, 1) X(64, 128, 1) X(64, 256, 2)          \
       X(80, 320, 1) X(80, 320, 2) X(80, 160, 1)                                              \
       X(96, 256, 1) X(96, 128, 1) X(96, 256, 3) X(96, 384, 1) X(128, 256, 1) X(128, 512, 1) X(128, 256, 2) \
       X(128, 256, 4) X(256, 256, 2) X(256, 256, 4) X(256, 256, 8)
 
+// staged shapes (column splits 0): coarse grid, threads per block
+#define LB_STAGED_CONFIGS(X) X(128, 256) X(256, 256) X(256, 512)
+
+// Default preference per coarse grid: the list order, except that on M = 256
+// (where the row table outgrows LDS and splits recompute the row pass) the
+// staged shapes come first.
 const std::vector<FieldConfig>& field_configs() {
-  static const std::vector<FieldConfig> all = {
+  static const std::vector<FieldConfig> all = [] {
+    std::vector<FieldConfig> v, staged = {
+#define LB_CASE(M_, NT_) FieldConfig{M_, NT_, 0, 0},
+      LB_STAGED_CONFIGS(LB_CASE)
+#undef LB_CASE
+    };
+    for (const FieldConfig& c : staged)
+      if (c.M == 256) v.push_back(c);
+    const FieldConfig single[] = {
 #define LB_CASE(M_, NT_, CS_) FieldConfig{M_, NT_, CS_, 0},
       LB_FIELD_CONFIGS(LB_CASE)
 #undef LB_CASE
-  };
+    };
+    for (const FieldConfig& c : single) v.push_back(c);
+    for (const FieldConfig& c : staged)
+      if (c.M != 256) v.push_back(c);
+    return v;
+  }();
   return all;
 }
 
@@ -662,7 +840,17 @@ bool field_config_supported(int precision, const FieldConfig& c) {
   if (c.M == M_ && c.nt == NT_ && c.csplit == CS_) return true;
   LB_FIELD_CONFIGS(LB_CASE)
 #undef LB_CASE
+#define LB_CASE(M_, NT_) \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == 0) return true;
+  LB_STAGED_CONFIGS(LB_CASE)
+#undef LB_CASE
   return false;
+}
+
+template <typename T, int M, int NT>
+static size_t staged_smem() {
+  using F = FieldShape<T, M, NT, 1>;
+  return (M + (size_t)F::CPR * F::LDC) * sizeof(cx<T>);
 }
 
 template <typename T, int M, int NT, int CS>
@@ -674,6 +862,15 @@ static size_t smem_for(int W, bool adjoint) {
 }
 
 size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint) {
+  if (c.csplit == 0) {
+    if (adjoint) return (size_t)-1;  // the staged path is forward only
+#define LB_CASE(M_, NT_)                                                          \
+  if (c.M == M_ && c.nt == NT_)                                                   \
+    return precision == 64 ? staged_smem<double, M_, NT_>() : staged_smem<float, M_, NT_>();
+    LB_STAGED_CONFIGS(LB_CASE)
+#undef LB_CASE
+    return 0;
+  }
 #define LB_CASE(M_, NT_, CS_)                                                   \
   if (c.M == M_ && c.nt == NT_ && c.csplit == CS_)                              \
     return precision == 64 ? smem_for<double, M_, NT_, CS_>(W, adjoint)         \
@@ -755,6 +952,52 @@ static hipError_t launch_multi(const FieldConfig& c, const ItemDesc* items, int 
     hipLaunchKernelGGL(k, dim3(c.grid), dim3(NT), sm, st, src, twg);
   }
   return hipGetLastError();
+}
+
+template <typename T, int M, int NT>
+static hipError_t launch_staged(const FieldArgs& a, void* partial, void* G, int img0, int n_img,
+                                int cus, hipStream_t st) {
+  using F = FieldShape<T, M, NT, 1>;
+  const size_t sm = staged_smem<T, M, NT>();
+  const cx<T>* twg = nullptr;
+  hipError_t e = table_for<T>(M, &twg);
+  if (e != hipSuccess) return e;
+  const int occ = (int)std::min<size_t>(2048 / NT, (160 * 1024 - 64) / sm);
+  const long long max_blocks = (long long)cus * std::max(1, occ);
+  auto kr = k_rows_staged<T, M, NT>;
+  auto kc = k_cols_staged<T, M, NT>;
+  for (const void* k : {reinterpret_cast<const void*>(kr), reinterpret_cast<const void*>(kc)}) {
+    e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  const MapSource<T, M> src = map_source<T, M>(a, partial, nullptr, nullptr);
+  const long long row_tasks =
+      (long long)n_img * a.map.sys_per_plane * ((a.W + F::CPR - 1) / F::CPR);
+  hipLaunchKernelGGL(kr, dim3((unsigned)std::max(1LL, std::min(row_tasks, max_blocks))), dim3(NT),
+                     sm, st, src, img0, n_img, (cx<T>*)G, twg);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long col_tasks = (long long)n_img * a.map.n_chunks * (M / F::CPR);
+  hipLaunchKernelGGL(kc, dim3((unsigned)std::max(1LL, std::min(col_tasks, max_blocks))), dim3(NT),
+                     sm, st, src, img0, n_img, (const cx<T>*)G, twg);
+  return hipGetLastError();
+}
+
+size_t staged_table_bytes(int precision, int M, int W) {
+  return (size_t)W * M * (precision == 64 ? 16 : 8);
+}
+
+hipError_t field_forward_staged(int precision, const FieldConfig& c, const FieldArgs& a,
+                                void* partial, void* G, int img0, int n_img, int cus,
+                                hipStream_t st) {
+#define LB_CASE(M_, NT_)                                                                     \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == 0)                                             \
+    return precision == 64                                                                   \
+               ? launch_staged<double, M_, NT_>(a, partial, G, img0, n_img, cus, st)        \
+               : launch_staged<float, M_, NT_>(a, partial, G, img0, n_img, cus, st);
+  LB_STAGED_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return hipErrorInvalidValue;
 }
 
 hipError_t field_multi(int precision, const FieldConfig& c, const ItemDesc* items, int n_items,

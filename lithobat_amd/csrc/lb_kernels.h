@@ -74,7 +74,7 @@ struct FieldArgs {
 struct FieldConfig {
   int M;       // coarse grid side
   int nt;      // threads per block
-  int csplit;  // column splits
+  int csplit;  // column splits; 0: staged (row tables through device memory)
   int grid;    // blocks (persistent)
 };
 
@@ -103,6 +103,13 @@ size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint
 // partial[item][M][M] = sum over the item's systems of w |E|^2 on the coarse grid
 hipError_t field_forward(int precision, const FieldConfig& c, const FieldArgs& a, void* partial,
                          hipStream_t st);
+// The staged forward (FieldConfig csplit 0): images [img0, img0 + n_img) of the
+// item map, their row tables (n_img * sys_per_plane of staged_table_bytes
+// each) in G; writes the same partials as field_forward.
+size_t staged_table_bytes(int precision, int M, int W);
+hipError_t field_forward_staged(int precision, const FieldConfig& c, const FieldArgs& a,
+                                void* partial, void* G, int img0, int n_img, int cus,
+                                hipStream_t st);
 // gX[item][Wb][Wb] = sum over the item's systems of 2 w conj(H) FFT(gM E) on the box
 hipError_t field_adjoint(int precision, const FieldConfig& c, const FieldArgs& a,
                          const void* gM, void* gX, hipStream_t st);

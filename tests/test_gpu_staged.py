@@ -1,0 +1,99 @@
+"""The staged coherent-system path (row tables through device memory, column
+transforms in a second launch) against the single-launch kernel and the oracle.
+
+Same arithmetic stage by stage, so FP64 agrees to <= 1e-12 and FP32 to <= 1e-5 of the
+peak; covers several passes of row tables, chunked work items and both block sizes.
+"""
+import numpy as np
+import pytest
+
+from oracle import litho_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _euv(**kw):
+    from lithobat_amd.config import OpticsConfig
+    return OpticsConfig(wavelength=13.5e-9, NA=0.33, source_type="annular", sigma_outer=0.8,
+                        sigma_inner=0.4, source_grid=11, normalisation="none", **kw)
+
+
+def _forward(plan, masks, shape):
+    from lithobat_amd import autotune
+    plan._tuned_for.add(masks.shape[0])
+    autotune.set_shape(plan, False, *shape)
+    assert plan.info["column_splits"] == shape[1]
+    return plan.forward(masks)
+
+
+def _single_launch(plan):
+    """The first single-launch kernel shape that fits this plan (column splits > 0)."""
+    from lithobat_amd import autotune
+    nt, cs = next((nt, cs) for nt, cs in autotune.shapes(plan, False) if cs > 0)
+    return nt, cs, 0
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_staged_matches_single_launch_and_oracle_m256(device, precision):
+    import torch
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=512, pixel_size=4e-9)
+    optics = _euv()
+    planes = [(-40e-9, 1.0), (25e-9, 1.0)]
+    plan = ImagingPlan(optics, grid, planes, precision=precision)
+    assert plan.info["m"] == 256
+    dt = torch.float64 if precision == 64 else torch.float32
+    masks_np = np.stack([manhattan(512, 4e-9, seed=70 + i) for i in range(12)])
+    masks = torch.tensor(masks_np, device=device, dtype=dt)
+    assert plan.info["column_splits"] == 0  # staged by default on this grid
+    ref, ref_pk = _forward(plan, masks, _single_launch(plan))
+    tol = 1e-12 if precision == 64 else 1e-5
+    # 24 images: several passes of row tables in FP64, one in FP32
+    for shape in [(256, 0, 0), (256, 0, 5), (512, 0, 0)]:
+        img, pk = _forward(plan, masks, shape)
+        assert _rel(img.cpu(), ref.cpu()) <= tol, shape
+        assert _rel(pk.cpu(), ref_pk.cpu()) <= tol, shape
+    if precision == 64:
+        o, _, _ = O.compute_aerial_image(masks_np[3], _euv(defocus=25e-9), grid, return_raw=True)
+        assert _rel(img[3, 1].cpu(), o) <= 1e-12
+
+
+def test_staged_m128_and_single_image(device):
+    """The staged shape on a 128-point coarse grid (radix 16 x 8, two stage-2 tasks per
+    thread) and a single mask (one pass, chunked items reduced by the split sum)."""
+    import torch
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=512, pixel_size=8e-9)
+    optics = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="annular",
+                          sigma_outer=0.9, sigma_inner=0.6, source_grid=15,
+                          normalisation="none")
+    plan = ImagingPlan(optics, grid, precision=64)
+    assert plan.info["m"] == 128
+    m_np = manhattan(512, 8e-9, seed=5)
+    m = torch.tensor(m_np[None], device=device)
+    ref, _ = _forward(plan, m, _single_launch(plan))
+    for chunk in (0, 1, 7):
+        img, _ = _forward(plan, m, (256, 0, chunk))
+        assert _rel(img.cpu(), ref.cpu()) <= 1e-12, chunk
+    o, _, _ = O.compute_aerial_image(m_np, optics, grid, return_raw=True)
+    assert _rel(img[0, 0].cpu(), o) <= 1e-12
+
+
+def test_staged_is_forward_only(device):
+    from lithobat_amd import autotune
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.plan import ImagingPlan
+    plan = ImagingPlan(_euv(), GridConfig(n_pixels=512, pixel_size=4e-9), precision=32)
+    assert (256, 0) in autotune.shapes(plan, False)
+    assert all(cs != 0 for _, cs in autotune.shapes(plan, True))
+    with pytest.raises(Exception, match="forward only"):
+        autotune.set_shape(plan, True, 256, 0, 0)
