@@ -86,6 +86,52 @@ struct FieldShape {
   static_assert(CPR <= MC && MC % CPR == 0, "rounds tile the split");
 };
 
+// Row-pass scratch layout per coarse grid: stage-1 task j's R1 outputs at
+// j * SP, rows LDR apart (SP >= R1, LDR >= T1 * SP), chosen so that a wave's
+// stage-1 writes and stage-2 reads spread over the LDS banks (unpadded, a row
+// with R1 = 16 puts its stage-1 tasks on two banks, and rows M + 1 apart
+// overlap the stage-2 reads of neighbouring rows).  Element x of a row, in
+// stage-1 output order x = j * R1 + r, sits at rpos(x).
+template <int M>
+struct RowLayout;
+template <>
+struct RowLayout<16> {
+  static constexpr int SP = 5, LDR = 20;
+};
+template <>
+struct RowLayout<32> {
+  static constexpr int SP = 9, LDR = 36;
+};
+template <>
+struct RowLayout<48> {
+  static constexpr int SP = 13, LDR = 60;
+};
+template <>
+struct RowLayout<64> {
+  static constexpr int SP = 9, LDR = 72;
+};
+template <>
+struct RowLayout<80> {
+  static constexpr int SP = 10, LDR = 81;
+};
+template <>
+struct RowLayout<96> {
+  static constexpr int SP = 13, LDR = 108;
+};
+template <>
+struct RowLayout<128> {
+  static constexpr int SP = 17, LDR = 136;
+};
+template <>
+struct RowLayout<256> {
+  static constexpr int SP = 17, LDR = 272;
+};
+
+template <int M, int R1>
+__device__ __forceinline__ int rpos(int x) {
+  return (x / R1) * RowLayout<M>::SP + x % R1;
+}
+
 __device__ __forceinline__ int wrap_idx(int i, int n) {
   if (n <= 0) return i;
   int r = i % n;
@@ -99,6 +145,7 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
                                          const cx<T>* __restrict__ Hj, int W, int2 bx,
                                          const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs, int tid) {
   using F = FieldShape<T, M, NT, CSPLIT>;
+  using RL = RowLayout<M>;
   const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
   for (int a0 = 0; a0 < W; a0 += F::CPR) {
     {
@@ -118,7 +165,7 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
         }
         dft<F::R1, +1>(v);
 #pragma unroll
-        for (int r = 0; r < F::R1; ++r) S[rl1 * F::LDC + j1 * F::R1 + r] = v[r];
+        for (int r = 0; r < F::R1; ++r) S[rl1 * RL::LDR + j1 * RL::SP + r] = v[r];
       }
     }
     __syncthreads();
@@ -131,7 +178,7 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
         cx<T> v[F::R2];
 #pragma unroll
         for (int r = 0; r < F::R2; ++r) {
-          v[r] = S[rl * F::LDC + j2 + r * F::R1];
+          v[r] = S[rl * RL::LDR + r * RL::SP + j2];
           if (r > 0) v[r] = v[r] * tw[r * j2];
         }
         dft<F::R2, +1>(v);
@@ -171,6 +218,14 @@ struct RowPre {
   static constexpr bool DB = ON && F::ROUNDS >= 2 && 2 * F::CPR <= MAXW + 16;
   static constexpr int CB_ROWS_FWD =
       DB ? (MAXW > 2 * F::CPR ? MAXW : 2 * F::CPR) : (ON && MAXW > F::CPR ? MAXW : F::CPR);
+  // Cb in complex elements: the column rounds' rows (LDC apart) or the row
+  // pass's rows (RowLayout), whichever needs more
+  static constexpr int ROW_ROWS_FWD = ON ? MAXW : F::CPR;
+  static constexpr int CB_FWD = CB_ROWS_FWD * F::LDC > ROW_ROWS_FWD * RowLayout<M>::LDR
+                                    ? CB_ROWS_FWD * F::LDC
+                                    : ROW_ROWS_FWD * RowLayout<M>::LDR;
+  static constexpr int CB_ADJ = F::LDC > RowLayout<M>::LDR ? F::CPR * F::LDC
+                                                           : F::CPR * RowLayout<M>::LDR;
   cx<T> x[NCH][NR], h[NCH][NR];
 
   __device__ __forceinline__ void fetch(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
@@ -206,6 +261,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
                                              int tid) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   using P = RowPre<T, M, NT, CSPLIT>;
+  using RL = RowLayout<M>;
   const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
   if constexpr (!ALL) {
 #pragma unroll
@@ -218,7 +274,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
         for (int r = 0; r < F::R1; ++r) v[r] = r < P::NR ? pre.x[c][r] * pre.h[c][r] : cx<T>{0, 0};
         dft<F::R1, +1>(v);
 #pragma unroll
-        for (int r = 0; r < F::R1; ++r) S[rl1 * F::LDC + j1 * F::R1 + r] = v[r];
+        for (int r = 0; r < F::R1; ++r) S[rl1 * RL::LDR + j1 * RL::SP + r] = v[r];
       }
       __syncthreads();
 #pragma unroll
@@ -230,7 +286,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
           cx<T> v[F::R2];
 #pragma unroll
           for (int r = 0; r < F::R2; ++r) {
-            v[r] = S[rl * F::LDC + j2 + r * F::R1];
+            v[r] = S[rl * RL::LDR + r * RL::SP + j2];
             if (r > 0) v[r] = v[r] * tw[r * j2];
           }
           dft<F::R2, +1>(v);
@@ -254,7 +310,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
       for (int r = 0; r < F::R1; ++r) v[r] = r < P::NR ? pre.x[c][r] * pre.h[c][r] : cx<T>{0, 0};
       dft<F::R1, +1>(v);
 #pragma unroll
-      for (int r = 0; r < F::R1; ++r) S[a * F::LDC + j1 * F::R1 + r] = v[r];
+      for (int r = 0; r < F::R1; ++r) S[a * RL::LDR + j1 * RL::SP + r] = v[r];
     }
   }
   __syncthreads();
@@ -267,7 +323,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
       cx<T> v[F::R2];
 #pragma unroll
       for (int r = 0; r < F::R2; ++r) {
-        v[r] = S[a * F::LDC + j2 + r * F::R1];
+        v[r] = S[a * RL::LDR + r * RL::SP + j2];
         if (r > 0) v[r] = v[r] * tw[r * j2];
       }
       dft<F::R2, +1>(v);
@@ -379,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
   cx<T>* Cb = tw + M;                              // CB_ROWS_FWD x LDC (row scratch too)
-  cx<T>* Tb = Cb + RowPre<T, M, NT, CSPLIT>::CB_ROWS_FWD * F::LDC;  // W x MC
+  cx<T>* Tb = Cb + RowPre<T, M, NT, CSPLIT>::CB_FWD;  // W x MC
   const int tid = threadIdx.x;
   load_roots(tw, twg, M, tid, NT);
   __syncthreads();
@@ -479,8 +535,8 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
   using F = FieldShape<T, M, NT, CSPLIT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
-  cx<T>* Cb = tw + M;                              // CPR x LDC
-  cx<T>* Tb = Cb + F::CPR * F::LDC;                // W x MC
+  cx<T>* Cb = tw + M;                              // CPR rows (column or row layout)
+  cx<T>* Tb = Cb + RowPre<T, M, NT, CSPLIT>::CB_ADJ;  // W x MC
   // The transposed column stage writes its box rows over the row table: round
   // rd's stage 1 has read Tb at exactly those (row, column) places two
   // barriers earlier, later rounds read other columns, and the transposed
@@ -595,7 +651,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
             const int xl = x - cs * F::MC;
             cx<T> val = cx<T>{0, 0};
             if (a < W && (CSPLIT == 1 || (xl >= 0 && xl < F::MC))) val = Tp[a * F::MC + xl];
-            Cb[rl * F::LDC + x] = val;
+            Cb[rl * RowLayout<M>::LDR + rpos<M, F::R1>(x)] = val;
           }
           __syncthreads();
 #pragma unroll
@@ -605,12 +661,12 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
             const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
             cx<T> v[F::R2];
 #pragma unroll
-            for (int r = 0; r < F::R2; ++r) v[r] = Cb[rl * F::LDC + j2 + r * F::R1];
+            for (int r = 0; r < F::R2; ++r) v[r] = Cb[rl * RowLayout<M>::LDR + r * RowLayout<M>::SP + j2];
             dft<F::R2, -1>(v);
 #pragma unroll
             for (int r = 0; r < F::R2; ++r) {
               if (r > 0) v[r] = mulc(v[r], tw[r * j2]);
-              Cb[rl * F::LDC + j2 + r * F::R1] = v[r];
+              Cb[rl * RowLayout<M>::LDR + r * RowLayout<M>::SP + j2] = v[r];
             }
           }
           __syncthreads();
@@ -618,7 +674,8 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
             const int a = a0 + rl1;
             cx<T> v[F::R1];
 #pragma unroll
-            for (int r = 0; r < F::R1; ++r) v[r] = Cb[rl1 * F::LDC + j1r * F::R1 + r];
+            for (int r = 0; r < F::R1; ++r)
+              v[r] = Cb[rl1 * RowLayout<M>::LDR + j1r * RowLayout<M>::SP + r];
             dft<F::R1, -1>(v);
             if (a < W) {
               const int xr = wrap_idx(bx.x + a, wrapN);
@@ -656,9 +713,10 @@ __global__ __launch_bounds__(NT) void k_rows_staged(MapSource<T, M> src, int img
                                                     cx<T>* __restrict__ G,
                                                     const cx<T>* __restrict__ twg) {
   using F = FieldShape<T, M, NT, 1>;
+  using L = RowLayout<M>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
-  cx<T>* S = tw + M;                               // CPR x LDC
+  cx<T>* S = tw + M;                               // CPR x LDR
   const int tid = threadIdx.x;
   load_roots(tw, twg, M, tid, NT);
   __syncthreads();
@@ -693,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_rows_staged(MapSource<T, M> src, int img
         }
         dft<F::R1, +1>(v);
 #pragma unroll
-        for (int r = 0; r < F::R1; ++r) S[rl1 * F::LDC + j1 * F::R1 + r] = v[r];
+        for (int r = 0; r < F::R1; ++r) S[rl1 * L::LDR + j1 * L::SP + r] = v[r];
       }
     }
     __syncthreads();
@@ -703,10 +761,11 @@ __global__ __launch_bounds__(NT) void k_rows_staged(MapSource<T, M> src, int img
       const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
       const int a = a0 + rl;
       if (t < F::CPR * F::T2 && a < W) {
+        // stage-2 input r is stage-1 output j2 of task r
         cx<T> v[F::R2];
 #pragma unroll
         for (int r = 0; r < F::R2; ++r) {
-          v[r] = S[rl * F::LDC + j2 + r * F::R1];
+          v[r] = S[rl * L::LDR + r * L::SP + j2];
           if (r > 0) v[r] = v[r] * tw[r * j2];
         }
         dft<F::R2, +1>(v);
@@ -848,16 +907,18 @@ bool field_config_supported(int precision, const FieldConfig& c) {
 }
 
 template <typename T, int M, int NT>
-static size_t staged_smem() {
+static size_t staged_smem() {  // the larger of the two kernels' scratch
   using F = FieldShape<T, M, NT, 1>;
-  return (M + (size_t)F::CPR * F::LDC) * sizeof(cx<T>);
+  const size_t rows = (size_t)F::CPR * RowLayout<M>::LDR, cols = (size_t)F::CPR * F::LDC;
+  return (M + (rows > cols ? rows : cols)) * sizeof(cx<T>);
 }
 
 template <typename T, int M, int NT, int CS>
 static size_t smem_for(int W, bool adjoint) {
   using F = FieldShape<T, M, NT, CS>;
-  const int cb_rows = adjoint ? F::CPR : RowPre<T, M, NT, CS>::CB_ROWS_FWD;
-  size_t n = M + (size_t)cb_rows * F::LDC + (size_t)W * F::MC;  // the adjoint's Tp aliases Tb
+  using P = RowPre<T, M, NT, CS>;
+  const size_t cb = adjoint ? P::CB_ADJ : P::CB_FWD;
+  size_t n = M + cb + (size_t)W * F::MC;  // the adjoint's Tp aliases Tb
   return n * sizeof(cx<T>);
 }
 
