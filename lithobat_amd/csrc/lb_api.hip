@@ -329,12 +329,16 @@ struct Layout {
 
 struct Work {
   int n_masks, n_img, n_chunks, chunk, n_items;
-  int pass_img;  // staged forward: images whose row tables are held at once
+  int pass_img;  // staged path: images whose row tables are held at once
+  int nrc;       // staged adjoint: band-gradient partials per item (row chunks)
   size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, queue, G;
   size_t total;
 };
 
-Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
+// use: the coherent-system kernel shape the work is laid out for (default:
+// the plan's own forward or adjoint shape).
+Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0,
+            const FieldConfig* use = nullptr) {
   Work w{};
   w.n_masks = n_masks;
   w.n_img = n_masks * p.n_planes;
@@ -356,9 +360,11 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
   w.Xm = L.take(C * n_masks * p.Wbm * p.Wbm);
   w.RTm = L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
-  if (!backward && p.fwd.csplit == 0) {
-    // row tables of the staged forward, a pass of images at a time, sized to
-    // stay in the last-level cache between its two launches
+  const FieldConfig& fc = use ? *use : (backward ? p.adj : p.fwd);
+  w.nrc = backward && fc.csplit == 0 ? staged_row_chunks(fc, p.W) : 1;
+  if (fc.csplit == 0) {
+    // row tables of the staged path, a pass of images at a time, sized to
+    // stay in the last-level cache between its launches
     const size_t per_img = staged_table_bytes(p.precision, p.M, p.W) * (size_t)std::max(p.spp, 1);
     w.pass_img = (int)std::max<size_t>(1, std::min<size_t>((size_t)w.n_img,
                                                            kStagedPassBytes / per_img));
@@ -370,7 +376,8 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
     size_t seg = R * (size_t)w.n_img * segment_splits(w.n_img, w.n_chunks, mm) * mm;
     if (backward)
       seg = std::max<size_t>(seg, R * (size_t)n_masks *
-                                      segment_splits(n_masks, p.n_planes * w.n_chunks, bb) * bb);
+                                      segment_splits(n_masks, p.n_planes * w.n_chunks * w.nrc,
+                                                     bb) * bb);
     w.seg = L.take(seg);
   }
   if (!p.direct) w.IM = L.take(R * (size_t)w.n_img * p.M * p.M);
@@ -386,7 +393,7 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0) {
       w.GY = L.take(C * (size_t)w.n_img * p.WbI * p.WbI);
       w.TTc = L.take(C * (size_t)w.n_img * band_ld(p.WbI) * p.M);
     }
-    w.gX = L.take(C * (size_t)w.n_items * p.Wbm * p.Wbm);
+    w.gX = L.take(C * (size_t)w.n_items * w.nrc * p.Wbm * p.Wbm);
     w.GX = L.take(C * (size_t)n_masks * p.Wbm * p.Wbm);
     w.TTm = L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   }
@@ -516,7 +523,7 @@ template <typename T>
 lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, hipStream_t st) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
   cx<T>* GX = (cx<T>*)(ws + w.GX);
-  LB_HIP(segment_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes * w.n_chunks,
+  LB_HIP(segment_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes * w.n_chunks * w.nrc,
                         2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, (T*)(ws + w.seg), st));
   LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, 1, invN2, 0, (cx<T>*)(ws + w.TTm),
                            (T*)dmask, (T*)nullptr, st));
@@ -534,7 +541,13 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
   FieldArgs a = field_args(p, ws + w.Xm, w);
   a.queue = (int*)(ws + w.queue);
   LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
-  LB_HIP(field_adjoint(p.precision, fc, a, gM, ws + w.gX, st));
+  if (fc.csplit == 0) {
+    for (int i0 = 0; i0 < w.n_img; i0 += w.pass_img)
+      LB_HIP(field_adjoint_staged(p.precision, fc, a, gM, ws + w.gX, ws + w.G, i0,
+                                  std::min(w.pass_img, w.n_img - i0), p.cus, st));
+  } else {
+    LB_HIP(field_adjoint(p.precision, fc, a, gM, ws + w.gX, st));
+  }
   return bwd_post<T>(p, n_masks, dmask, ws, w, st);
 }
 
@@ -960,8 +973,6 @@ lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads
   if (!plan) return fail(LB_ERR_ARG, "null plan");
   Plan& p = plan->p;
   FieldConfig fc{p.M, block_threads, column_splits, 0};
-  if (adjoint && column_splits == 0)
-    return fail(LB_ERR_UNSUPPORTED, "the staged coherent-system path (splits 0) is forward only");
   if (!field_config_supported(p.precision, fc))
     return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel of shape M=" +
                                         std::to_string(p.M) + " threads=" +
@@ -1124,7 +1135,7 @@ lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) 
   size_t off_f = 0, off_b = 0;
   for (lb_plan h : b->plans) {
     const Plan& p = h->p;
-    Work wf = layout(p, 1, false, b->chunk), wb = layout(p, 1, true, b->chunk);
+    Work wf = layout(p, 1, false, b->chunk, &b->fwd), wb = layout(p, 1, true, b->chunk, &b->adj);
     b->wf.push_back(wf);
     b->wb.push_back(wb);
     b->base_f.push_back(off_f);

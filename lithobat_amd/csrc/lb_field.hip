@@ -858,6 +858,190 @@ __global__ __launch_bounds__(NT) void k_cols_staged(MapSource<T, M> src, int img
   }
 }
 
+// Staged adjoint.  k_rows_staged recomputes the row tables; k_cols_adj_staged
+// runs, per (item, column group), each system's column stages, multiplies
+// the field by the coarse sensitivity gM and runs the transposed column
+// stages, writing the result over the system's row table (each block owns
+// its columns of every table it touches); k_rows_adj_staged runs the
+// transposed row pass of one chunk of CPR rows of an item over the item's
+// systems and accumulates 2 w G_C conj(H) into its own band-gradient partial
+// (item * nrc + chunk), so no two blocks share an accumulator even when the
+// systems' boxes differ.
+template <typename T, int M, int NT>
+__global__ __launch_bounds__(NT) void k_cols_adj_staged(MapSource<T, M> src, int img0, int n_img,
+                                                        cx<T>* __restrict__ G,
+                                                        const cx<T>* __restrict__ twg) {
+  using F = FieldShape<T, M, NT, 1>;
+  constexpr int NG = M / F::CPR;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
+  cx<T>* Cb = tw + M;                              // CPR x LDC
+  const int tid = threadIdx.x;
+  load_roots(tw, twg, M, tid, NT);
+  __syncthreads();
+  const int W = src.W, spp = src.map.sys_per_plane, n_planes = src.map.n_planes;
+  const int n_chunks = src.map.n_chunks;
+  const int c1 = tid % F::CPR, jc1 = tid / F::CPR;
+  const long long n_tasks = (long long)n_img * n_chunks * NG;
+  for (long long task = blockIdx.x; task < n_tasks; task += gridDim.x) {
+    const int g = (int)(task % NG);
+    const int item = img0 * n_chunks + (int)(task / NG);
+    const ItemView<T> iv = src.get(item);
+    const int img = item / n_chunks, plane = img % n_planes;
+    cx<T>* Gi = G + ((long long)(img - img0) * spp - (long long)plane * spp) * W * M;
+    const int x1 = g * F::CPR + c1;
+    T gm[F::Q2 * F::R2];
+#pragma unroll
+    for (int q = 0; q < F::Q2; ++q) {
+      const int t = tid + q * NT;
+      const bool ok = F::NT2 % NT == 0 || t < F::NT2;
+      const int c2 = t % F::CPR, j2 = t / F::CPR;
+#pragma unroll
+      for (int r = 0; r < F::R2; ++r)
+        gm[q * F::R2 + r] = ok ? iv.gM[(j2 + r * F::R1) * M + g * F::CPR + c2] : (T)0;
+    }
+    cx<T> nx[F::NRW];
+    auto fetch = [&](int j) {
+      const cx<T>* Gt = Gi + (long long)j * W * M;
+#pragma unroll
+      for (int r = 0; r < F::NRW; ++r) {
+        const int a = jc1 + r * F::T1;
+        nx[r] = a < W ? Gt[(long long)a * M + x1] : cx<T>{0, 0};
+      }
+    };
+    if (iv.s0 < iv.s1) fetch(iv.s0);
+    for (int j = iv.s0; j < iv.s1; ++j) {
+      cx<T>* Gt = Gi + (long long)j * W * M;
+      {
+        cx<T> v[F::R1];
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) v[r] = r < F::NRW ? nx[r] : cx<T>{0, 0};
+        dft<F::R1, +1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) Cb[c1 * F::LDC + jc1 * F::R1 + r] = v[r];
+      }
+      if (j + 1 < iv.s1) fetch(j + 1);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < F::Q2; ++q) {
+        const int t = tid + q * NT;
+        if (!(F::NT2 % NT == 0 || t < F::NT2)) continue;
+        const int c2 = t % F::CPR, j2 = t / F::CPR;
+        cx<T> v[F::R2];
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
+          if (r > 0) v[r] = v[r] * tw[r * j2];
+        }
+        dft<F::R2, +1>(v);
+        // h = gM * E, then the transposed stage 2: forward DFT, conj twiddle
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) v[r] = scale(v[r], gm[q * F::R2 + r]);
+        dft<F::R2, -1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          if (r > 0) v[r] = mulc(v[r], tw[r * j2]);
+          Cb[c2 * F::LDC + j2 + r * F::R1] = v[r];
+        }
+      }
+      __syncthreads();
+      {
+        // transposed stage 1, outputs pruned to the W box rows; a thread reads
+        // back exactly the places it writes in the next system's stage 1
+        cx<T> v[F::R1];
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) v[r] = Cb[c1 * F::LDC + jc1 * F::R1 + r];
+        dft<F::R1, -1>(v);
+#pragma unroll
+        for (int r = 0; r < F::NRW; ++r) {
+          const int a = jc1 + r * F::T1;
+          if (a < W) Gt[(long long)a * M + x1] = v[r];
+        }
+      }
+    }
+    __syncthreads();  // the next task's first stage writes Cb
+  }
+}
+
+template <typename T, int M, int NT>
+__global__ __launch_bounds__(NT) void k_rows_adj_staged(MapSource<T, M> src, int img0, int n_img,
+                                                        const cx<T>* __restrict__ G,
+                                                        cx<T>* __restrict__ gXp,
+                                                        const cx<T>* __restrict__ twg) {
+  using F = FieldShape<T, M, NT, 1>;
+  using RL = RowLayout<M>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
+  cx<T>* S = tw + M;                               // CPR x LDR
+  const int tid = threadIdx.x;
+  load_roots(tw, twg, M, tid, NT);
+  __syncthreads();
+  const int W = src.W, Wb = src.Wb, wrapN = src.wrapN, spp = src.map.sys_per_plane;
+  const int n_planes = src.map.n_planes, n_chunks = src.map.n_chunks;
+  const int nrc = (W + F::CPR - 1) / F::CPR;
+  const int rl1 = tid / F::T1, j1r = tid - rl1 * F::T1;
+  const long long n_tasks = (long long)n_img * n_chunks * nrc;
+  for (long long task = blockIdx.x; task < n_tasks; task += gridDim.x) {
+    const int rc = (int)(task % nrc);
+    const int item = img0 * n_chunks + (int)(task / nrc);
+    const ItemView<T> iv = src.get(item);
+    const int img = item / n_chunks, plane = img % n_planes;
+    const cx<T>* Gi = G + ((long long)(img - img0) * spp - (long long)plane * spp) * W * M;
+    cx<T>* gout = gXp + ((long long)item * nrc + rc) * Wb * Wb;
+    for (int e = tid; e < Wb * Wb; e += NT) gout[e] = cx<T>{0, 0};
+    __syncthreads();
+    const int a0 = rc * F::CPR;
+    for (int j = iv.s0; j < iv.s1; ++j) {
+      const cx<T>* Gt = Gi + (long long)j * W * M;
+      const int2 bx = src.box[j];
+      const T w2 = (T)2 * src.wsys[j];
+      const cx<T>* Hj = src.H + (long long)j * W * W;
+      for (int e = tid; e < F::CPR * M; e += NT) {
+        const int rl = e / M, x = e - rl * M;
+        const int a = a0 + rl;
+        S[rl * RL::LDR + rpos<M, F::R1>(x)] = a < W ? Gt[(long long)a * M + x] : cx<T>{0, 0};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < F::Q2; ++q) {
+        const int t = tid + q * NT;
+        if (!(t < F::CPR * F::T2)) continue;
+        const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+        cx<T> v[F::R2];
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) v[r] = S[rl * RL::LDR + r * RL::SP + j2];
+        dft<F::R2, -1>(v);
+#pragma unroll
+        for (int r = 0; r < F::R2; ++r) {
+          if (r > 0) v[r] = mulc(v[r], tw[r * j2]);
+          S[rl * RL::LDR + r * RL::SP + j2] = v[r];
+        }
+      }
+      __syncthreads();
+      {
+        const int a = a0 + rl1;
+        cx<T> v[F::R1];
+#pragma unroll
+        for (int r = 0; r < F::R1; ++r) v[r] = S[rl1 * RL::LDR + j1r * RL::SP + r];
+        dft<F::R1, -1>(v);
+        if (a < W) {
+          const int xr = wrap_idx(bx.x + a, wrapN);
+#pragma unroll
+          for (int r = 0; r < F::NRW; ++r) {
+            const int b = j1r + r * F::T1;
+            if (b < W) {
+              const cx<T> gc = scale(mulc(v[r], Hj[a * W + b]), w2);
+              cx<T>* dst = gout + (long long)xr * Wb + wrap_idx(bx.y + b, wrapN);
+              *dst = *dst + gc;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------- dispatch
 #define LB_FIELD_CONFIGS(X)                                                                \
   X(16, 64, 1) X(32, 128, 1) X(48, 64, 1) X(64, 256, 1) X(64, 128, 1) X(64, 256, 2)          \
@@ -924,7 +1108,7 @@ static size_t smem_for(int W, bool adjoint) {
 
 size_t field_smem_bytes(int precision, const FieldConfig& c, int W, bool adjoint) {
   if (c.csplit == 0) {
-    if (adjoint) return (size_t)-1;  // the staged path is forward only
+    (void)adjoint;  // the staged kernels share one LDS size
 #define LB_CASE(M_, NT_)                                                          \
   if (c.M == M_ && c.nt == NT_)                                                   \
     return precision == 64 ? staged_smem<double, M_, NT_>() : staged_smem<float, M_, NT_>();
@@ -1042,6 +1226,67 @@ static hipError_t launch_staged(const FieldArgs& a, void* partial, void* G, int 
   hipLaunchKernelGGL(kc, dim3((unsigned)std::max(1LL, std::min(col_tasks, max_blocks))), dim3(NT),
                      sm, st, src, img0, n_img, (const cx<T>*)G, twg);
   return hipGetLastError();
+}
+
+template <typename T, int M, int NT>
+static hipError_t launch_staged_adj(const FieldArgs& a, const void* gM, void* gXp, void* G,
+                                    int img0, int n_img, int cus, hipStream_t st) {
+  using F = FieldShape<T, M, NT, 1>;
+  const size_t sm = staged_smem<T, M, NT>();
+  const cx<T>* twg = nullptr;
+  hipError_t e = table_for<T>(M, &twg);
+  if (e != hipSuccess) return e;
+  const int occ = (int)std::min<size_t>(2048 / NT, (160 * 1024 - 64) / sm);
+  const long long max_blocks = (long long)cus * std::max(1, occ);
+  auto kr = k_rows_staged<T, M, NT>;
+  auto kc = k_cols_adj_staged<T, M, NT>;
+  auto ka = k_rows_adj_staged<T, M, NT>;
+  for (const void* k : {reinterpret_cast<const void*>(kr), reinterpret_cast<const void*>(kc),
+                        reinterpret_cast<const void*>(ka)}) {
+    e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  const MapSource<T, M> src = map_source<T, M>(a, nullptr, gM, nullptr);
+  const int nrc = (a.W + F::CPR - 1) / F::CPR;
+  const long long row_tasks = (long long)n_img * a.map.sys_per_plane * nrc;
+  auto grid = [&](long long tasks) {
+    return dim3((unsigned)std::max(1LL, std::min(tasks, max_blocks)));
+  };
+  hipLaunchKernelGGL(kr, grid(row_tasks), dim3(NT), sm, st, src, img0, n_img, (cx<T>*)G, twg);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long col_tasks = (long long)n_img * a.map.n_chunks * (M / F::CPR);
+  hipLaunchKernelGGL(kc, grid(col_tasks), dim3(NT), sm, st, src, img0, n_img, (cx<T>*)G, twg);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long adj_tasks = (long long)n_img * a.map.n_chunks * nrc;
+  hipLaunchKernelGGL(ka, grid(adj_tasks), dim3(NT), sm, st, src, img0, n_img, (const cx<T>*)G,
+                     (cx<T>*)gXp, twg);
+  return hipGetLastError();
+}
+
+int staged_row_chunks(const FieldConfig& c, int W) {
+#define LB_CASE(M_, NT_)                                        \
+  if (c.M == M_ && c.nt == NT_) {                               \
+    constexpr int cpr = FieldShape<float, M_, NT_, 1>::CPR;     \
+    return (W + cpr - 1) / cpr;                                 \
+  }
+  LB_STAGED_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return 1;
+}
+
+hipError_t field_adjoint_staged(int precision, const FieldConfig& c, const FieldArgs& a,
+                                const void* gM, void* gXp, void* G, int img0, int n_img, int cus,
+                                hipStream_t st) {
+#define LB_CASE(M_, NT_)                                                                       \
+  if (c.M == M_ && c.nt == NT_ && c.csplit == 0)                                               \
+    return precision == 64                                                                     \
+               ? launch_staged_adj<double, M_, NT_>(a, gM, gXp, G, img0, n_img, cus, st)      \
+               : launch_staged_adj<float, M_, NT_>(a, gM, gXp, G, img0, n_img, cus, st);
+  LB_STAGED_CONFIGS(LB_CASE)
+#undef LB_CASE
+  return hipErrorInvalidValue;
 }
 
 size_t staged_table_bytes(int precision, int M, int W) {

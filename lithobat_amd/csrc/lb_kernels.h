@@ -110,6 +110,13 @@ size_t staged_table_bytes(int precision, int M, int W);
 hipError_t field_forward_staged(int precision, const FieldConfig& c, const FieldArgs& a,
                                 void* partial, void* G, int img0, int n_img, int cus,
                                 hipStream_t st);
+// The staged adjoint: band-gradient partials gXp[item * nrc + chunk][Wb][Wb],
+// nrc = staged_row_chunks(c, W) row chunks per item (the transposed row pass
+// of each chunk accumulates into its own partial).
+int staged_row_chunks(const FieldConfig& c, int W);
+hipError_t field_adjoint_staged(int precision, const FieldConfig& c, const FieldArgs& a,
+                                const void* gM, void* gXp, void* G, int img0, int n_img, int cus,
+                                hipStream_t st);
 // gX[item][Wb][Wb] = sum over the item's systems of 2 w conj(H) FFT(gM E) on the box
 hipError_t field_adjoint(int precision, const FieldConfig& c, const FieldArgs& a,
                          const void* gM, void* gX, hipStream_t st);

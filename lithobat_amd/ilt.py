@@ -55,18 +55,20 @@ def mask_of(theta: torch.Tensor, beta: float, smooth_px: float = 0.0) -> torch.T
 
 
 def relaxed_resist_loss(raw: torch.Tensor, target: torch.Tensor, scale: float, level: float,
-                        steepness: float, sign: float = 1.0, sigma_px: float = 0.0
-                        ) -> torch.Tensor:
+                        steepness: float, sign: float = 1.0, sigma_px: float = 0.0,
+                        side: dict | None = None) -> torch.Tensor:
     """ilt.py:311-330 without penalties: mean (sigmoid(a s) - t)^2.
 
     a = steepness / |level| and s = sign * (G(raw * scale) - level), the
     reference's relaxed resist, G the PEB blur of width ``sigma_px`` (identity
     below 1e-3 px).  The mean runs over the last two (image) axes and is
-    summed over any leading axes.
+    summed over any leading axes.  ``side`` (a dict) receives the signed field
+    s and the per-image mismatch count #{(s > 0) != (t > 0.5)} of the same pass.
     """
     if sigma_px < 1e-3:
-        return RelaxedResist.apply(raw, target, scale, level, steepness, sign)
-    return RelaxedResist.apply(blur(raw * scale, sigma_px), target, 1.0, level, steepness, sign)
+        return RelaxedResist.apply(raw, target, scale, level, steepness, sign, side)
+    return RelaxedResist.apply(blur(raw * scale, sigma_px), target, 1.0, level, steepness, sign,
+                               side)
 
 
 def mask_penalties(mask: torch.Tensor, weight_tv: float = 0.0,

@@ -109,12 +109,19 @@ def mask_sigmoid_grad(grad_mask: torch.Tensor, mask: torch.Tensor, beta: float,
 
 class RelaxedResist(torch.autograd.Function):
     """Sum over images of the per-image relaxed-resist cost; the derivative is
-    computed in the same pass as the cost and scaled by the incoming gradient."""
+    computed in the same pass as the cost and scaled by the incoming gradient.
+
+    ``side``: a dict that, when given, receives the same pass's signed field
+    (``"signed"``) and per-image pattern mismatch (``"mismatch"``), detached."""
 
     @staticmethod
-    def forward(ctx, intensity, target, scale, level, steepness, sign):
+    def forward(ctx, intensity, target, scale, level, steepness, sign, side=None):
         r = resist_objective(intensity, target, scale, level, steepness, sign,
-                             grad=intensity.requires_grad)
+                             grad=intensity.requires_grad, signed=side is not None,
+                             mismatch=side is not None)
+        if side is not None:
+            side["signed"] = r["signed"]
+            side["mismatch"] = r["mismatch"]
         ctx.save_for_backward(r["d_intensity"] if r["d_intensity"] is not None
                               else torch.empty(0, device=intensity.device))
         return r["total"]
@@ -122,7 +129,7 @@ class RelaxedResist(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         (d,) = ctx.saved_tensors
-        return d * g, None, None, None, None, None
+        return d * g, None, None, None, None, None, None
 
 
 class MaskSigmoid(torch.autograd.Function):

@@ -249,16 +249,16 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
     t_dev = torch.as_tensor(tgt, dtype=dt, device=dev)
 
     def objective(m):
-        """ilt.py:311-360 on the device: cost (device scalar), d cost / d mask, signed field."""
+        """ilt.py:311-360 on the device: cost (device scalar), d cost / d mask, the signed
+        field and the pattern error (device int), the last two from the loss's own pass."""
         m = m.detach().requires_grad_(True)
         raw = abbe_intensity(m[None], plan)  # [1, 1, n, n]
+        side = {}
         cost = (relaxed_resist_loss(raw, t_dev[None, None], scale, level, steepness, sign,
-                                    sigma_px)
+                                    sigma_px, side=side)
                 + mask_penalties(m, weight_tv, weight_discrete))
         cost.backward()
-        with torch.no_grad():
-            signed = sign * (blur(raw[0, 0] * scale, sigma_px) - level)
-        return cost.detach(), m.grad, signed
+        return cost.detach(), m.grad, side["signed"][0, 0], side["mismatch"][0, 0]
 
     def grad_theta(grad_m, m, beta_k):
         """ilt.py:381-389: back through the sigmoid's slope, then the Gaussian."""
@@ -278,7 +278,7 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
     beta_k = float(beta)
     with torch.no_grad():
         mask = mask_of(theta, beta_k, smooth_px)
-    cost, grad_m, signed = objective(mask)
+    cost, grad_m, signed, perr = objective(mask)
     lit = int((signed > 0.0).sum())
     if lit == 0 or lit == signed.numel():
         peak = float((signed.abs() + level).max())
@@ -288,31 +288,33 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
             f"field reaches {peak:.4g} at dose {model.dose:g}. Size the dose first.")
     rows = []
 
-    def record(it, cost, grad_m, signed, mask):
-        perr = int(((signed > 0.0) != (t_dev > 0.5)).sum())
-        rows.append({"iteration": it, "cost": float(cost), "pattern_error": perr,
-                     "beta": beta_k, "grad_norm": float(torch.linalg.vector_norm(grad_m))})
+    def record(it, cost, grad_m, signed, perr, mask):
+        # the three history values reach the host in one copy
+        vals = torch.stack([cost.double(), perr.double(),
+                            torch.linalg.vector_norm(grad_m).double()]).cpu().numpy()
+        c, pe = float(vals[0]), int(vals[1])
+        rows.append({"iteration": it, "cost": c, "pattern_error": pe, "beta": beta_k,
+                     "grad_norm": float(vals[2])})
         if progress is not None:
-            progress(ILTFrame(it, mask.cpu().numpy(), signed.cpu().numpy(), float(cost), perr,
-                              beta_k))
-        return float(cost)
+            progress(ILTFrame(it, mask.cpu().numpy(), signed.cpu().numpy(), c, pe, beta_k))
+        return c
 
-    record(0, cost, grad_m, signed, mask)
+    record(0, cost, grad_m, signed, perr, mask)
     prev_cost, converged, taken = np.inf, False, 0
     for it in range(1, max_iter + 1):
         with torch.no_grad():
             grad_t = grad_theta(grad_m, mask, beta_k)
         mom1.mul_(b1).add_(grad_t, alpha=1.0 - b1)
         mom2.mul_(b2).addcmul_(grad_t, grad_t, value=1.0 - b2)
-        m_hat = mom1 / (1.0 - b1 ** it)
-        v_hat = mom2 / (1.0 - b2 ** it)
-        theta = theta - step * m_hat / (torch.sqrt(v_hat) + eps)
+        # theta -= step * m_hat / (sqrt(v_hat) + eps), m_hat, v_hat the bias-corrected moments
+        denom = torch.sqrt(mom2 / (1.0 - b2 ** it)).add_(eps)
+        theta = torch.addcdiv(theta, mom1 / (1.0 - b1 ** it), denom, value=-step)
         beta_k *= growth
         with torch.no_grad():
             mask = mask_of(theta, beta_k, smooth_px)
-        cost, grad_m, signed = objective(mask)
+        cost, grad_m, signed, perr = objective(mask)
         taken = it
-        c = record(it, cost, grad_m, signed, mask)
+        c = record(it, cost, grad_m, signed, perr, mask)
         if abs(prev_cost - c) < tol:
             converged = True
             break

@@ -1,8 +1,10 @@
 """The staged coherent-system path (row tables through device memory, column
-transforms in a second launch) against the single-launch kernel and the oracle.
+transforms in a second launch; the adjoint adds a transposed row pass per row chunk)
+against the single-launch kernels, the oracle and central differences.
 
 Same arithmetic stage by stage, so FP64 agrees to <= 1e-12 and FP32 to <= 1e-5 of the
-peak; covers several passes of row tables, chunked work items and both block sizes.
+peak (adjoint 2e-5); covers several passes of row tables, chunked work items, systems
+with their own boxes and both block sizes.
 """
 import numpy as np
 import pytest
@@ -88,12 +90,78 @@ def test_staged_m128_and_single_image(device):
     assert _rel(img[0, 0].cpu(), o) <= 1e-12
 
 
-def test_staged_is_forward_only(device):
+def _backward(plan, masks, g, shape):
+    from lithobat_amd import autotune
+    plan._tuned_for.add(masks.shape[0])
+    autotune.set_shape(plan, True, *shape)
+    return plan.backward(masks, g)
+
+
+def _single_launch_adj(plan):
+    from lithobat_amd import autotune
+    nt, cs = next((nt, cs) for nt, cs in autotune.shapes(plan, True) if cs > 0)
+    return nt, cs, 0
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_staged_adjoint_matches_single_launch_m256(device, precision):
+    """Abbe systems with their own boxes (the per-row-chunk band-gradient partials must not
+    race), several passes of row tables in FP64, chunked items, both block sizes."""
+    import torch
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=512, pixel_size=4e-9)
+    plan = ImagingPlan(_euv(), grid, [(-40e-9, 1.0), (25e-9, 1.0)], precision=precision)
+    dt = torch.float64 if precision == 64 else torch.float32
+    masks = torch.tensor(np.stack([manhattan(512, 4e-9, seed=90 + i) for i in range(12)]),
+                         device=device, dtype=dt)
+    rng = np.random.default_rng(3)
+    g = torch.tensor(rng.standard_normal((12, 2, 512, 512)), device=device, dtype=dt)
+    ref = _backward(plan, masks, g, _single_launch_adj(plan)).cpu()
+    tol = 1e-12 if precision == 64 else 2e-5
+    for shape in [(256, 0, 0), (256, 0, 5), (512, 0, 0)]:
+        got = _backward(plan, masks, g, shape).cpu()
+        assert _rel(got, ref) <= tol, shape
+
+
+def test_staged_adjoint_socs_and_central_differences(device):
+    """The SOCS adjoint at 512^2 / 8 nm (K = 12, one shared box) through the staged path,
+    against the single-launch adjoint and a central difference of sum(g * I) (the image is
+    quadratic in the mask, so the difference is exact)."""
+    import torch
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.hopkins import SOCSKernels
+    from lithobat_amd.masks import manhattan
+    optics = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="annular",
+                          sigma_outer=0.9, sigma_inner=0.6, source_grid=11,
+                          normalisation="none")
+    k = SOCSKernels.build(optics, GridConfig(n_pixels=512, pixel_size=8e-9), tol=0.0,
+                          max_kernels=12, precision=64)
+    plan = k.plan
+    assert plan.info["m"] == 256
+    m = torch.tensor(manhattan(512, 8e-9, seed=12)[None], device=device)
+    g = torch.tensor(np.random.default_rng(4).standard_normal((1, 1, 512, 512)), device=device)
+    ref = _backward(plan, m, g, _single_launch_adj(plan))[0].cpu().numpy()
+    got = _backward(plan, m, g, (256, 0, 0))[0].cpu().numpy()
+    assert _rel(got, ref) <= 1e-12
+
+    def obj(x):
+        return float((plan.forward(x, with_peak=False)[0] * g).sum())
+
+    for (i, j) in [(0, 0), (100, 311), (511, 7)]:
+        mp, mm = m.clone(), m.clone()
+        mp[0, i, j] += 0.5
+        mm[0, i, j] -= 0.5
+        assert got[i, j] == pytest.approx(obj(mp) - obj(mm), rel=1e-9,
+                                          abs=1e-12 * np.abs(got).max())
+
+
+def test_staged_shapes_listed(device):
     from lithobat_amd import autotune
     from lithobat_amd.config import GridConfig
     from lithobat_amd.plan import ImagingPlan
     plan = ImagingPlan(_euv(), GridConfig(n_pixels=512, pixel_size=4e-9), precision=32)
-    assert (256, 0) in autotune.shapes(plan, False)
-    assert all(cs != 0 for _, cs in autotune.shapes(plan, True))
-    with pytest.raises(Exception, match="forward only"):
-        autotune.set_shape(plan, True, 256, 0, 0)
+    for adjoint in (False, True):
+        shapes = autotune.shapes(plan, adjoint)
+        assert shapes[0] == (256, 0) and any(cs > 0 for _, cs in shapes)
