@@ -208,6 +208,26 @@ lb_status lb_resist_objective(const void* intensity, const void* target, int32_t
                               void* signed_field, int32_t* mismatch, int32_t precision,
                               void* workspace, size_t workspace_bytes, void* stream);
 
+/* The mask penalties of opc/ilt.py:339-355 (the TV and discreteness terms of
+ * _Objective.__call__) over n_img masks [ny][nx], periodic neighbours:
+ *   cost_b = weight_tv sum (gx^2 + gy^2) / npix + weight_discrete sum m (1 - m) / npix,
+ *   grad   = weight_tv (-2 / npix) lap(m) + weight_discrete (1 - 2 m) / npix,
+ * one pass plus a deterministic two-level reduction.  cost [n_img], total
+ * (0-d) and grad (like mask) are optional.  Workspace: the size
+ * lb_resist_objective_workspace_bytes(n_img, ny * nx) returns. */
+lb_status lb_mask_penalties(const void* mask, int32_t n_img, int32_t ny, int32_t nx,
+                            double weight_tv, double weight_discrete, void* cost, void* total,
+                            void* grad, int32_t precision, void* workspace, size_t ws_bytes,
+                            void* stream);
+
+/* One Adam step of opc/ilt.py:587-591 in place over n values:
+ *   mom1 = b1 mom1 + (1 - b1) g,  mom2 = b2 mom2 + (1 - b2) g^2,
+ *   theta -= step (mom1 / (1 - b1^it)) / (sqrt(mom2 / (1 - b2^it)) + eps),
+ * iteration = it >= 1 (the bias corrections are formed in float64). */
+lb_status lb_adam_step(void* theta, void* mom1, void* mom2, const void* grad, int64_t n,
+                       int32_t iteration, double step, double beta1, double beta2, double eps,
+                       int32_t precision, void* stream);
+
 /* The ILT mask parameterisation (opc/ilt.py:368-389, _mask_of / _grad_theta
  * without their Gaussian, which lb_gaussian_blur applies): mask =
  * 1 / (1 + exp(-clip(beta * x, -60, 60))) and grad_x = grad_mask * beta *

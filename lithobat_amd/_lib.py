@@ -52,7 +52,7 @@ EXPORTS = [
     "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
     "lb_backward_dense", "lb_gaussian_blur_workspace_bytes", "lb_gaussian_blur",
     "lb_resist_objective_workspace_bytes", "lb_resist_objective", "lb_mask_sigmoid",
-    "lb_mask_sigmoid_grad",
+    "lb_mask_sigmoid_grad", "lb_mask_penalties", "lb_adam_step",
 ]
 
 _lib = None
@@ -100,6 +100,10 @@ def load() -> C.CDLL:
                                  C.c_double, vp, vp, vp, vp, vp, i32, vp, sz, vp], C.c_int),
         "lb_mask_sigmoid": ([vp, C.c_int64, C.c_double, vp, i32, vp], C.c_int),
         "lb_mask_sigmoid_grad": ([vp, vp, C.c_int64, C.c_double, vp, i32, vp], C.c_int),
+        "lb_mask_penalties": ([vp, i32, i32, i32, C.c_double, C.c_double, vp, vp, vp, i32, vp, sz,
+                               vp], C.c_int),
+        "lb_adam_step": ([vp, vp, vp, vp, C.c_int64, i32, C.c_double, C.c_double, C.c_double,
+                          C.c_double, i32, vp], C.c_int),
         "lb_batch_create": ([C.POINTER(vp), i32, plan_p], C.c_int),
         "lb_batch_destroy": ([vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),

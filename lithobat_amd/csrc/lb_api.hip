@@ -1446,6 +1446,53 @@ lb_status lb_resist_objective(const void* intensity, const void* target, int32_t
   return LB_OK;
 }
 
+lb_status lb_mask_penalties(const void* mask, int32_t n_img, int32_t ny, int32_t nx,
+                            double weight_tv, double weight_discrete, void* cost, void* total,
+                            void* grad, int32_t precision, void* workspace, size_t ws_bytes,
+                            void* stream) {
+  g_err.clear();
+  if (!mask) return fail(LB_ERR_ARG, "null argument");
+  if (n_img < 1 || ny < 1 || nx < 1) return fail(LB_ERR_ARG, "n_img, ny and nx must be >= 1");
+  if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
+  const long long npix = (long long)ny * nx;
+  const size_t need = lb_resist_objective_workspace_bytes(n_img, npix);
+  if (!workspace || ws_bytes < need)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  const size_t k = (size_t)n_img * resist_blocks(npix);
+  double* pc = (double*)workspace;
+  int* pz = (int*)((char*)workspace + ((k * sizeof(double) + n_img * sizeof(double) + 255) / 256) *
+                                          256);
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 64)
+    LB_HIP(mask_penalties<double>((const double*)mask, n_img, ny, nx, weight_tv, weight_discrete,
+                                  (double*)cost, (double*)total, (double*)grad, pc, pz, st));
+  else
+    LB_HIP(mask_penalties<float>((const float*)mask, n_img, ny, nx, (float)weight_tv,
+                                 (float)weight_discrete, (float*)cost, (float*)total, (float*)grad,
+                                 pc, pz, st));
+  return LB_OK;
+}
+
+lb_status lb_adam_step(void* theta, void* mom1, void* mom2, const void* grad, int64_t n,
+                       int32_t iteration, double step, double beta1, double beta2, double eps,
+                       int32_t precision, void* stream) {
+  g_err.clear();
+  if (!theta || !mom1 || !mom2 || !grad) return fail(LB_ERR_ARG, "null argument");
+  if (n < 0) return fail(LB_ERR_ARG, "n must be >= 0");
+  if (iteration < 1) return fail(LB_ERR_ARG, "iteration counts from 1");
+  if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
+  const double c1 = 1.0 - std::pow(beta1, iteration), c2 = 1.0 - std::pow(beta2, iteration);
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 64)
+    LB_HIP(adam_step<double>((double*)theta, (double*)mom1, (double*)mom2, (const double*)grad, n,
+                             beta1, beta2, 1.0 - beta1, 1.0 - beta2, eps, step, c1, c2, st));
+  else
+    LB_HIP(adam_step<float>((float*)theta, (float*)mom1, (float*)mom2, (const float*)grad, n,
+                            (float)beta1, (float)beta2, (float)(1.0 - beta1), (float)(1.0 - beta2),
+                            (float)eps, (float)step, (float)c1, (float)c2, st));
+  return LB_OK;
+}
+
 lb_status lb_mask_sigmoid(const void* x, int64_t n, double beta, void* mask, int32_t precision,
                           void* stream) {
   g_err.clear();

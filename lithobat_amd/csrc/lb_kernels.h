@@ -176,6 +176,12 @@ template <typename T>
 hipError_t mask_sigmoid(const T* x, long long n, T beta, T* m, hipStream_t st);
 template <typename T>
 hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, T* gx, hipStream_t st);
+template <typename T>
+hipError_t mask_penalties(const T* m, int n_img, int ny, int nx, T wtv, T wd, T* cost, T* total,
+                          T* grad, double* part_cost, int* part_zero, hipStream_t st);
+template <typename T>
+hipError_t adam_step(T* theta, T* m1, T* m2, const T* g, long long n, T b1, T b2, T o1, T o2,
+                     T eps, T step, T c1, T c2, hipStream_t st);
 
 // ---- pupil tables (lb_pupil.hip) ------------------------------------------
 struct PupilSys {

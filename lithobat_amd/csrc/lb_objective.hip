@@ -203,11 +203,107 @@ hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, T* gx
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- penalties
+// Mask penalties of opc/ilt.py:339-355 over images [n_img][ny][nx] with
+// periodic neighbours: cost_b = wtv sum (gx^2 + gy^2) / npix + wd sum m (1 - m)
+// / npix, gx = m - m(x - 1), gy = m - m(y - 1), and its gradient
+// wtv (-2 / npix) lap + wd (1 - 2 m) / npix, lap the 5-point Laplacian; the
+// costs reduce in the same two deterministic levels as the resist objective.
+template <typename T>
+__global__ __launch_bounds__(kObjThreads) void k_penalty_partial(
+    const T* __restrict__ m, int ny, int nx, T wtv, T wd, T* __restrict__ grad,
+    double* __restrict__ part_cost, int* __restrict__ part_zero) {
+  const long long npix = (long long)ny * nx;
+  const long long base = (long long)blockIdx.y * npix;
+  const T inv = (T)1 / (T)npix;
+  double c = 0;
+  for (long long e = blockIdx.x * (long long)kObjThreads + threadIdx.x; e < npix;
+       e += (long long)gridDim.x * kObjThreads) {
+    const int y = (int)(e / nx), x = (int)(e - (long long)y * nx);
+    const T v = m[base + e];
+    T g = 0;
+    if (wtv > (T)0) {
+      const T l = m[base + (long long)y * nx + (x == 0 ? nx - 1 : x - 1)];
+      const T r = m[base + (long long)y * nx + (x == nx - 1 ? 0 : x + 1)];
+      const T u = m[base + (long long)(y == 0 ? ny - 1 : y - 1) * nx + x];
+      const T d = m[base + (long long)(y == ny - 1 ? 0 : y + 1) * nx + x];
+      const T gx = v - l, gy = v - u;
+      c += (double)wtv * (double)(gx * gx + gy * gy);
+      g += wtv * ((T)-2 * inv) * (u + d + l + r - (T)4 * v);
+    }
+    if (wd > (T)0) {
+      c += (double)wd * (double)(v * ((T)1 - v));
+      g += wd * ((T)1 - (T)2 * v) * inv;
+    }
+    if (grad) grad[base + e] = g;
+  }
+  __shared__ double sc[kObjThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sc[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double cs = 0;
+    for (int w = 0; w < kObjThreads / 64; ++w) cs += sc[w];
+    const long long k = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    part_cost[k] = cs;
+    part_zero[k] = 0;
+  }
+}
+
+template <typename T>
+hipError_t mask_penalties(const T* m, int n_img, int ny, int nx, T wtv, T wd, T* cost, T* total,
+                          T* grad, double* part_cost, int* part_zero, hipStream_t st) {
+  const long long npix = (long long)ny * nx;
+  const int nblk = resist_blocks(npix);
+  hipLaunchKernelGGL(k_penalty_partial<T>, dim3(nblk, n_img), dim3(kObjThreads), 0, st, m, ny,
+                     nx, wtv, wd, grad, part_cost, part_zero);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_resist_final<T>, dim3(1), dim3(64 * kFinalWaves), 0, st, part_cost,
+                     part_zero, n_img, nblk, npix, part_cost + (long long)n_img * nblk, cost, total,
+                     (int*)nullptr);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- Adam
+// opc/ilt.py:587-591 in place: m1 = b1 m1 + (1 - b1) g, m2 = b2 m2 + (1 - b2)
+// g^2, theta -= step (m1 / c1) / (sqrt(m2 / c2) + eps), c_k = 1 - b_k^it;
+// o_k = 1 - b_k and c_k are formed in float64 by the caller (1 - 0.999f in
+// float would be off by 1.3e-5).
+template <typename T>
+__global__ void k_adam(T* __restrict__ theta, T* __restrict__ m1, T* __restrict__ m2,
+                       const T* __restrict__ g, long long n, T b1, T b2, T o1, T o2, T eps,
+                       T step, T c1, T c2) {
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += st) {
+    const T gi = g[i];
+    const T a = b1 * m1[i] + o1 * gi;
+    const T b = b2 * m2[i] + o2 * (gi * gi);
+    m1[i] = a;
+    m2[i] = b;
+    theta[i] = theta[i] - step * (a / c1) / (sqrt(b / c2) + eps);
+  }
+}
+
+template <typename T>
+hipError_t adam_step(T* theta, T* m1, T* m2, const T* g, long long n, T b1, T b2, T o1, T o2,
+                     T eps, T step, T c1, T c2, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adam<T>, dim3(elementwise_blocks(n)), dim3(256), 0, st, theta, m1, m2, g, n,
+                     b1, b2, o1, o2, eps, step, c1, c2);
+  return hipGetLastError();
+}
+
 #define LB_OBJ_INST(T)                                                                          \
   template hipError_t resist_objective<T>(const T*, const T*, int, long long, T, T, T, T, T*, T*, \
                                           T*, T*, int*, double*, int*, hipStream_t);           \
   template hipError_t mask_sigmoid<T>(const T*, long long, T, T*, hipStream_t);                 \
-  template hipError_t mask_sigmoid_grad<T>(const T*, const T*, long long, T, T*, hipStream_t);
+  template hipError_t mask_sigmoid_grad<T>(const T*, const T*, long long, T, T*, hipStream_t);    \
+  template hipError_t mask_penalties<T>(const T*, int, int, int, T, T, T*, T*, T*, double*, int*, \
+                                        hipStream_t);                                           \
+  template hipError_t adam_step<T>(T*, T*, T*, const T*, long long, T, T, T, T, T, T, T, T,      \
+                                   hipStream_t);
 LB_OBJ_INST(float)
 LB_OBJ_INST(double)
 #undef LB_OBJ_INST

@@ -17,7 +17,7 @@ import torch
 
 from .blur import blur
 from .expose import _to_device
-from .objective import MaskSigmoid, RelaxedResist
+from .objective import MaskPenalties, MaskSigmoid, RelaxedResist
 from .plan import ImagingPlan
 
 __all__ = ["AbbeIntensity", "abbe_intensity", "socs_fields", "socs_gradient",
@@ -74,16 +74,11 @@ def relaxed_resist_loss(raw: torch.Tensor, target: torch.Tensor, scale: float, l
 def mask_penalties(mask: torch.Tensor, weight_tv: float = 0.0,
                    weight_discrete: float = 0.0) -> torch.Tensor:
     """ilt.py:339-355: weight_tv * mean |grad m|^2 (periodic differences) +
-    weight_discrete * mean m (1 - m), per image, summed over leading axes."""
-    npix = float(mask.shape[-1] * mask.shape[-2])
-    cost = mask.new_zeros(())
-    if weight_tv > 0.0:
-        gx = mask - torch.roll(mask, 1, -1)
-        gy = mask - torch.roll(mask, 1, -2)
-        cost = cost + weight_tv * (gx * gx + gy * gy).sum() / npix
-    if weight_discrete > 0.0:
-        cost = cost + weight_discrete * (mask * (1.0 - mask)).sum() / npix
-    return cost
+    weight_discrete * mean m (1 - m), per image, summed over leading axes; cost and
+    gradient from one fused pass."""
+    if weight_tv <= 0.0 and weight_discrete <= 0.0:
+        return mask.new_zeros(())
+    return MaskPenalties.apply(mask, max(weight_tv, 0.0), max(weight_discrete, 0.0))
 
 
 class FieldStack(np.ndarray):

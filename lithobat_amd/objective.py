@@ -13,6 +13,10 @@ count_nonzero((s > 0) != (t > 0.5)) (ilt.py:357), all from one pass over the
 images.  ``mask_sigmoid`` is the mask parameterisation of ilt.py:368-389
 without its Gaussian: m = 1 / (1 + exp(-clip(beta x, +-60))), d x = d m beta
 m (1 - m).  ``RelaxedResist`` and ``MaskSigmoid`` are their autograd forms.
+
+``mask_penalties`` is the TV + discreteness term of ilt.py:339-355 with its
+gradient from one pass (``MaskPenalties`` its autograd form), and
+``adam_step`` the solver's Adam update (ilt.py:587-591), in place.
 """
 from __future__ import annotations
 
@@ -21,7 +25,7 @@ import torch
 from . import _lib
 
 __all__ = ["resist_objective", "resist_objective_into", "mask_sigmoid", "mask_sigmoid_grad",
-           "RelaxedResist", "MaskSigmoid"]
+           "mask_penalties", "adam_step", "RelaxedResist", "MaskSigmoid", "MaskPenalties"]
 
 
 def _prec(t: torch.Tensor) -> int:
@@ -105,6 +109,60 @@ def mask_sigmoid_grad(grad_mask: torch.Tensor, mask: torch.Tensor, beta: float,
     _lib.check(_lib.load().lb_mask_sigmoid_grad(g.data_ptr(), mask.data_ptr(), mask.numel(),
                                                 float(beta), res.data_ptr(), _prec(mask), st))
     return res
+
+
+def mask_penalties(mask: torch.Tensor, weight_tv: float, weight_discrete: float,
+                   grad: bool = True) -> dict:
+    """Per-image penalty ``cost`` [...], their sum ``total`` and, on request, the gradient
+    ``grad`` (like ``mask``) over the last two axes, periodic neighbours."""
+    _on_device(mask, "mask")
+    if mask.dim() < 2:
+        raise ValueError("mask must have (..., ny, nx) image axes")
+    m = mask.contiguous()
+    ny, nx = m.shape[-2], m.shape[-1]
+    n_img = m.numel() // (ny * nx)
+    cost = torch.empty(tuple(m.shape[:-2]), dtype=m.dtype, device=m.device)
+    total = torch.empty((), dtype=m.dtype, device=m.device)
+    g = torch.empty_like(m) if grad else None
+    ws = torch.empty(max(workspace_bytes(n_img, ny * nx), 256), dtype=torch.uint8,
+                     device=m.device)
+    st = torch.cuda.current_stream(m.device).cuda_stream
+    _lib.check(_lib.load().lb_mask_penalties(
+        m.data_ptr(), n_img, ny, nx, float(weight_tv), float(weight_discrete), cost.data_ptr(),
+        total.data_ptr(), g.data_ptr() if g is not None else None, _prec(m), ws.data_ptr(),
+        ws.numel(), st))
+    return {"cost": cost, "total": total, "grad": g}
+
+
+def adam_step(theta: torch.Tensor, mom1: torch.Tensor, mom2: torch.Tensor, grad: torch.Tensor,
+              iteration: int, step: float, beta1: float = 0.9, beta2: float = 0.999,
+              eps: float = 1e-8) -> None:
+    """theta, mom1, mom2 updated in place (contiguous tensors of one dtype)."""
+    for t in (theta, mom1, mom2, grad):
+        _on_device(t, "Adam operand")
+        if not t.is_contiguous() or t.dtype != theta.dtype or t.shape != theta.shape:
+            raise ValueError("Adam operands must be contiguous tensors of one shape and dtype")
+    st = torch.cuda.current_stream(theta.device).cuda_stream
+    _lib.check(_lib.load().lb_adam_step(theta.data_ptr(), mom1.data_ptr(), mom2.data_ptr(),
+                                        grad.data_ptr(), theta.numel(), int(iteration),
+                                        float(step), float(beta1), float(beta2), float(eps),
+                                        _prec(theta), st))
+
+
+class MaskPenalties(torch.autograd.Function):
+    """Sum over images of the mask penalties; the gradient comes from the same pass."""
+
+    @staticmethod
+    def forward(ctx, mask, weight_tv, weight_discrete):
+        r = mask_penalties(mask, weight_tv, weight_discrete, grad=mask.requires_grad)
+        ctx.save_for_backward(r["grad"] if r["grad"] is not None
+                              else torch.empty(0, device=mask.device))
+        return r["total"]
+
+    @staticmethod
+    def backward(ctx, g):
+        (d,) = ctx.saved_tensors
+        return d * g, None, None
 
 
 class RelaxedResist(torch.autograd.Function):

@@ -21,6 +21,7 @@ from .blur import blur, gaussian_blur
 from .expose import _to_device, normalisation_scale
 from .hopkins import SOCSKernels
 from .ilt import abbe_intensity, mask_of, mask_penalties, relaxed_resist_loss
+from .objective import adam_step, mask_sigmoid_grad
 
 
 def apply_peb(pac_map, diffusion_sigma: float, pixel_size: float):
@@ -262,7 +263,7 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
 
     def grad_theta(grad_m, m, beta_k):
         """ilt.py:381-389: back through the sigmoid's slope, then the Gaussian."""
-        return blur(grad_m * (beta_k * m * (1.0 - m)), smooth_px)
+        return blur(mask_sigmoid_grad(grad_m, m, beta_k), smooth_px)
 
     p = float(np.clip(initial_grey, 0.5 + 1e-3, 1.0 - 1e-6))
     if initial is None:
@@ -304,11 +305,7 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
     for it in range(1, max_iter + 1):
         with torch.no_grad():
             grad_t = grad_theta(grad_m, mask, beta_k)
-        mom1.mul_(b1).add_(grad_t, alpha=1.0 - b1)
-        mom2.mul_(b2).addcmul_(grad_t, grad_t, value=1.0 - b2)
-        # theta -= step * m_hat / (sqrt(v_hat) + eps), m_hat, v_hat the bias-corrected moments
-        denom = torch.sqrt(mom2 / (1.0 - b2 ** it)).add_(eps)
-        theta = torch.addcdiv(theta, mom1 / (1.0 - b1 ** it), denom, value=-step)
+        adam_step(theta, mom1, mom2, grad_t.contiguous(), it, step, b1, b2, eps)
         beta_k *= growth
         with torch.no_grad():
             mask = mask_of(theta, beta_k, smooth_px)

@@ -858,6 +858,15 @@ def dose_for_target(target, optics, grid, resist, dose=1.0, tone="clear", mask=N
     return float(np.sqrt(lo * hi))
 
 
+def adam_step(theta, mom1, mom2, grad_t, it, step, b1=0.9, b2=0.999, eps=1e-8):
+    """ilt.py:587-591: one Adam update; returns (theta, mom1, mom2)."""
+    mom1 = b1 * mom1 + (1.0 - b1) * grad_t
+    mom2 = b2 * mom2 + (1.0 - b2) * grad_t ** 2
+    m_hat = mom1 / (1.0 - b1 ** it)
+    v_hat = mom2 / (1.0 - b2 ** it)
+    return theta - step * m_hat / (np.sqrt(v_hat) + eps), mom1, mom2
+
+
 def run_ilt(target, optics, grid, resist, dose=1.0, tone="clear", *, max_iter=120, step=0.4,
             tol=1e-7, beta=4.0, beta_final=12.0, initial_grey=0.9, steepness=12.0,
             weight_tv=0.0, weight_discrete=0.0, smooth=0.0, socs_tol=1e-8):
@@ -896,11 +905,7 @@ def run_ilt(target, optics, grid, resist, dose=1.0, tone="clear", *, max_iter=12
     prev_cost, converged, taken = np.inf, False, 0
     for it in range(1, max_iter + 1):
         grad_t = grad_theta(grad_m, mask, beta_k, smooth_px)
-        mom1 = b1 * mom1 + (1.0 - b1) * grad_t
-        mom2 = b2 * mom2 + (1.0 - b2) * grad_t ** 2
-        m_hat = mom1 / (1.0 - b1 ** it)
-        v_hat = mom2 / (1.0 - b2 ** it)
-        theta = theta - step * m_hat / (np.sqrt(v_hat) + eps)
+        theta, mom1, mom2 = adam_step(theta, mom1, mom2, grad_t, it, step, b1, b2, eps)
         beta_k *= growth
         mask = mask_of(theta, beta_k, smooth_px)
         cost, grad_m, signed, perr = objective(mask)

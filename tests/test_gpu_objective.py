@@ -156,3 +156,56 @@ def test_odd_sizes_unaligned_views_and_many_images(device):
         assert int(r["mismatch"][i]) == pe
         tot += cost
     assert float(r["total"]) == pytest.approx(tot, rel=1e-12)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_mask_penalties_match_oracle(device, precision):
+    """TV + discreteness cost and gradient (periodic neighbours, odd sizes, several images)
+    against the oracle's restatement; each term alone and both together."""
+    import torch
+    from lithobat_amd.objective import mask_penalties
+    dt = torch.float64 if precision == 64 else torch.float32
+    tol = 1e-12 if precision == 64 else 1e-5
+    rng = np.random.default_rng(6)
+    m = rng.random((3, 37, 52))
+    for wtv, wd in [(0.3, 0.0), (0.0, 0.2), (0.3, 0.2)]:
+        r = mask_penalties(torch.tensor(m, device=device, dtype=dt), wtv, wd)
+        tot = 0.0
+        for b in range(3):
+            c, g = O.mask_penalties(m[b], wtv, wd)
+            assert float(r["cost"][b]) == pytest.approx(c, rel=tol)
+            assert _rel(r["grad"][b].cpu(), g) <= tol
+            tot += c
+        assert float(r["total"]) == pytest.approx(tot, rel=tol)
+
+
+def test_mask_penalties_autograd(device):
+    import torch
+    from lithobat_amd.ilt import mask_penalties
+    rng = np.random.default_rng(7)
+    m = rng.random((64, 64))
+    x = torch.tensor(m, device=device, requires_grad=True)
+    loss = mask_penalties(x, 0.25, 0.1) * 3.0
+    loss.backward()
+    c, g = O.mask_penalties(m, 0.25, 0.1)
+    assert float(loss) == pytest.approx(3.0 * c, rel=1e-12)
+    assert _rel(x.grad.cpu(), 3.0 * g) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_adam_step_matches_oracle(device, precision):
+    """Three in-place steps from zero moments with a changing gradient, against the oracle."""
+    import torch
+    from lithobat_amd.objective import adam_step
+    dt = torch.float64 if precision == 64 else torch.float32
+    tol = 1e-12 if precision == 64 else 1e-5
+    rng = np.random.default_rng(9)
+    th = rng.standard_normal(1001)
+    t = torch.tensor(th, device=device, dtype=dt)
+    m1, m2 = torch.zeros_like(t), torch.zeros_like(t)
+    a, b = np.zeros(1001), np.zeros(1001)
+    for it in range(1, 4):
+        g = rng.standard_normal(1001)
+        adam_step(t, m1, m2, torch.tensor(g, device=device, dtype=dt), it, 0.4)
+        th, a, b = O.adam_step(th, a, b, g, it, 0.4)
+        assert _rel(t.cpu(), th) <= tol and _rel(m1.cpu(), a) <= tol and _rel(m2.cpu(), b) <= tol

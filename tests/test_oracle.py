@@ -592,3 +592,20 @@ def test_host_reflection_table_matches_oracle():
         m3d.mask_material("Si", 248e-9)
     with pytest.raises(ValueError, match="needs a multilayer"):
         m3d.MaskStack(multilayer=None)
+
+
+def test_adam_step_closed_forms():
+    """First step from zero moments: the bias-corrected moments are g and g^2, so theta
+    moves by -step g / (|g| + eps); a constant gradient keeps m_hat = g, v_hat = g^2 at
+    every later step too (the bias corrections cancel the geometric sums exactly)."""
+    rng = np.random.default_rng(2)
+    g = rng.standard_normal(50)
+    th0 = rng.standard_normal(50)
+    th, m1, m2 = O.adam_step(th0, np.zeros(50), np.zeros(50), g, 1, 0.4)
+    assert np.allclose(th, th0 - 0.4 * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-15)
+    assert np.allclose(m1, 0.1 * g, rtol=1e-15) and np.allclose(m2, 0.001 * g ** 2, rtol=1e-13)
+    th, a, b = th0, np.zeros(50), np.zeros(50)
+    for it in range(1, 8):
+        prev = th
+        th, a, b = O.adam_step(th, a, b, g, it, 0.05)
+        assert np.allclose(th, prev - 0.05 * g / (np.abs(g) + 1e-8), rtol=0, atol=1e-12)
