@@ -228,6 +228,24 @@ lb_status lb_adam_step(void* theta, void* mom1, void* mom2, const void* grad, in
                        int32_t iteration, double step, double beta1, double beta2, double eps,
                        int32_t precision, void* stream);
 
+/* Device-scalar forms for solver loops captured into a graph (no counterpart
+ * in the reference, whose loop runs on the host: opc/ilt.py:583-592).
+ * lb_ilt_schedule advances schedule = [beta, it, 1 - beta1^it, 1 - beta2^it,
+ * beta_next] (float64, device) by one iteration: it += 1, beta = beta_next,
+ * beta_next = beta * growth, and the two bias corrections.  The _dev entry
+ * points read beta (lb_mask_sigmoid_dev / _grad_dev) or the two corrections
+ * (lb_adam_step_dev: corrections[0..1]) from device memory at run time. */
+lb_status lb_ilt_schedule(double* schedule, double growth, double beta1, double beta2,
+                          void* stream);
+lb_status lb_adam_step_dev(void* theta, void* mom1, void* mom2, const void* grad, int64_t n,
+                           const double* corrections, double step, double beta1, double beta2,
+                           double eps, int32_t precision, void* stream);
+lb_status lb_mask_sigmoid_dev(const void* x, int64_t n, const double* beta, void* mask,
+                              int32_t precision, void* stream);
+lb_status lb_mask_sigmoid_grad_dev(const void* grad_mask, const void* mask, int64_t n,
+                                   const double* beta, void* grad_x, int32_t precision,
+                                   void* stream);
+
 /* The ILT mask parameterisation (opc/ilt.py:368-389, _mask_of / _grad_theta
  * without their Gaussian, which lb_gaussian_blur applies): mask =
  * 1 / (1 + exp(-clip(beta * x, -60, 60))) and grad_x = grad_mask * beta *

@@ -52,7 +52,8 @@ EXPORTS = [
     "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
     "lb_backward_dense", "lb_gaussian_blur_workspace_bytes", "lb_gaussian_blur",
     "lb_resist_objective_workspace_bytes", "lb_resist_objective", "lb_mask_sigmoid",
-    "lb_mask_sigmoid_grad", "lb_mask_penalties", "lb_adam_step",
+    "lb_mask_sigmoid_grad", "lb_mask_penalties", "lb_adam_step", "lb_adam_step_dev",
+    "lb_ilt_schedule", "lb_mask_sigmoid_dev", "lb_mask_sigmoid_grad_dev",
 ]
 
 _lib = None
@@ -104,6 +105,11 @@ def load() -> C.CDLL:
                                vp], C.c_int),
         "lb_adam_step": ([vp, vp, vp, vp, C.c_int64, i32, C.c_double, C.c_double, C.c_double,
                           C.c_double, i32, vp], C.c_int),
+        "lb_adam_step_dev": ([vp, vp, vp, vp, C.c_int64, vp, C.c_double, C.c_double, C.c_double,
+                              C.c_double, i32, vp], C.c_int),
+        "lb_ilt_schedule": ([vp, C.c_double, C.c_double, C.c_double, vp], C.c_int),
+        "lb_mask_sigmoid_dev": ([vp, C.c_int64, vp, vp, i32, vp], C.c_int),
+        "lb_mask_sigmoid_grad_dev": ([vp, vp, C.c_int64, vp, vp, i32, vp], C.c_int),
         "lb_batch_create": ([C.POINTER(vp), i32, plan_p], C.c_int),
         "lb_batch_destroy": ([vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),

@@ -1473,54 +1473,101 @@ lb_status lb_mask_penalties(const void* mask, int32_t n_img, int32_t ny, int32_t
   return LB_OK;
 }
 
-lb_status lb_adam_step(void* theta, void* mom1, void* mom2, const void* grad, int64_t n,
-                       int32_t iteration, double step, double beta1, double beta2, double eps,
-                       int32_t precision, void* stream) {
+static lb_status adam_any(void* theta, void* mom1, void* mom2, const void* grad, int64_t n,
+                          double c1, double c2, const double* pc, double step, double beta1,
+                          double beta2, double eps, int32_t precision, void* stream) {
   g_err.clear();
   if (!theta || !mom1 || !mom2 || !grad) return fail(LB_ERR_ARG, "null argument");
   if (n < 0) return fail(LB_ERR_ARG, "n must be >= 0");
-  if (iteration < 1) return fail(LB_ERR_ARG, "iteration counts from 1");
   if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
-  const double c1 = 1.0 - std::pow(beta1, iteration), c2 = 1.0 - std::pow(beta2, iteration);
   hipStream_t st = (hipStream_t)stream;
   if (precision == 64)
     LB_HIP(adam_step<double>((double*)theta, (double*)mom1, (double*)mom2, (const double*)grad, n,
-                             beta1, beta2, 1.0 - beta1, 1.0 - beta2, eps, step, c1, c2, st));
+                             beta1, beta2, 1.0 - beta1, 1.0 - beta2, eps, step, c1, c2, pc, st));
   else
     LB_HIP(adam_step<float>((float*)theta, (float*)mom1, (float*)mom2, (const float*)grad, n,
                             (float)beta1, (float)beta2, (float)(1.0 - beta1), (float)(1.0 - beta2),
-                            (float)eps, (float)step, (float)c1, (float)c2, st));
+                            (float)eps, (float)step, (float)c1, (float)c2, pc, st));
   return LB_OK;
 }
 
-lb_status lb_mask_sigmoid(const void* x, int64_t n, double beta, void* mask, int32_t precision,
+lb_status lb_adam_step(void* theta, void* mom1, void* mom2, const void* grad, int64_t n,
+                       int32_t iteration, double step, double beta1, double beta2, double eps,
+                       int32_t precision, void* stream) {
+  if (iteration < 1) return fail(LB_ERR_ARG, "iteration counts from 1");
+  const double c1 = 1.0 - std::pow(beta1, iteration), c2 = 1.0 - std::pow(beta2, iteration);
+  return adam_any(theta, mom1, mom2, grad, n, c1, c2, nullptr, step, beta1, beta2, eps, precision,
+                  stream);
+}
+
+lb_status lb_adam_step_dev(void* theta, void* mom1, void* mom2, const void* grad, int64_t n,
+                           const double* corrections, double step, double beta1, double beta2,
+                           double eps, int32_t precision, void* stream) {
+  if (!corrections) return fail(LB_ERR_ARG, "null corrections");
+  return adam_any(theta, mom1, mom2, grad, n, 0.0, 0.0, corrections, step, beta1, beta2, eps,
+                  precision, stream);
+}
+
+lb_status lb_ilt_schedule(double* schedule, double growth, double beta1, double beta2,
                           void* stream) {
+  g_err.clear();
+  if (!schedule) return fail(LB_ERR_ARG, "null schedule");
+  LB_HIP(ilt_schedule(schedule, growth, beta1, beta2, (hipStream_t)stream));
+  return LB_OK;
+}
+
+static lb_status mask_sigmoid_any(const void* x, int64_t n, double beta, const double* pbeta,
+                                  void* mask, int32_t precision, void* stream) {
   g_err.clear();
   if (!x || !mask) return fail(LB_ERR_ARG, "null argument");
   if (n < 0) return fail(LB_ERR_ARG, "n must be >= 0");
   if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
   hipStream_t st = (hipStream_t)stream;
   if (precision == 64)
-    LB_HIP(mask_sigmoid<double>((const double*)x, n, beta, (double*)mask, st));
+    LB_HIP(mask_sigmoid<double>((const double*)x, n, beta, pbeta, (double*)mask, st));
   else
-    LB_HIP(mask_sigmoid<float>((const float*)x, n, (float)beta, (float*)mask, st));
+    LB_HIP(mask_sigmoid<float>((const float*)x, n, (float)beta, pbeta, (float*)mask, st));
   return LB_OK;
 }
 
-lb_status lb_mask_sigmoid_grad(const void* grad_mask, const void* mask, int64_t n, double beta,
-                               void* grad_x, int32_t precision, void* stream) {
+lb_status lb_mask_sigmoid(const void* x, int64_t n, double beta, void* mask, int32_t precision,
+                          void* stream) {
+  return mask_sigmoid_any(x, n, beta, nullptr, mask, precision, stream);
+}
+
+lb_status lb_mask_sigmoid_dev(const void* x, int64_t n, const double* beta, void* mask,
+                              int32_t precision, void* stream) {
+  if (!beta) return fail(LB_ERR_ARG, "null beta");
+  return mask_sigmoid_any(x, n, 0.0, beta, mask, precision, stream);
+}
+
+static lb_status mask_sigmoid_grad_any(const void* grad_mask, const void* mask, int64_t n,
+                                       double beta, const double* pbeta, void* grad_x,
+                                       int32_t precision, void* stream) {
   g_err.clear();
   if (!grad_mask || !mask || !grad_x) return fail(LB_ERR_ARG, "null argument");
   if (n < 0) return fail(LB_ERR_ARG, "n must be >= 0");
   if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
   hipStream_t st = (hipStream_t)stream;
   if (precision == 64)
-    LB_HIP(mask_sigmoid_grad<double>((const double*)grad_mask, (const double*)mask, n, beta,
+    LB_HIP(mask_sigmoid_grad<double>((const double*)grad_mask, (const double*)mask, n, beta, pbeta,
                                      (double*)grad_x, st));
   else
     LB_HIP(mask_sigmoid_grad<float>((const float*)grad_mask, (const float*)mask, n, (float)beta,
-                                    (float*)grad_x, st));
+                                    pbeta, (float*)grad_x, st));
   return LB_OK;
+}
+
+lb_status lb_mask_sigmoid_grad(const void* grad_mask, const void* mask, int64_t n, double beta,
+                               void* grad_x, int32_t precision, void* stream) {
+  return mask_sigmoid_grad_any(grad_mask, mask, n, beta, nullptr, grad_x, precision, stream);
+}
+
+lb_status lb_mask_sigmoid_grad_dev(const void* grad_mask, const void* mask, int64_t n,
+                                   const double* beta, void* grad_x, int32_t precision,
+                                   void* stream) {
+  if (!beta) return fail(LB_ERR_ARG, "null beta");
+  return mask_sigmoid_grad_any(grad_mask, mask, n, 0.0, beta, grad_x, precision, stream);
 }
 
 lb_status lb_forward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,

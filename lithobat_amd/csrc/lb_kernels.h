@@ -173,15 +173,19 @@ hipError_t resist_objective(const T* f, const T* target, int n_img, long long np
                             T level, T a, T sign, T* cost, T* total, T* d_f, T* signed_f,
                             int* mismatch, double* part_cost, int* part_err, hipStream_t st);
 template <typename T>
-hipError_t mask_sigmoid(const T* x, long long n, T beta, T* m, hipStream_t st);
+hipError_t mask_sigmoid(const T* x, long long n, T beta, const double* pbeta, T* m,
+                        hipStream_t st);
 template <typename T>
-hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, T* gx, hipStream_t st);
+hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, const double* pbeta,
+                             T* gx, hipStream_t st);
 template <typename T>
 hipError_t mask_penalties(const T* m, int n_img, int ny, int nx, T wtv, T wd, T* cost, T* total,
                           T* grad, double* part_cost, int* part_zero, hipStream_t st);
 template <typename T>
 hipError_t adam_step(T* theta, T* m1, T* m2, const T* g, long long n, T b1, T b2, T o1, T o2,
-                     T eps, T step, T c1, T c2, hipStream_t st);
+                     T eps, T step, T c1, T c2, const double* pc, hipStream_t st);
+// s = [beta, it, 1 - b1^it, 1 - b2^it, beta_next] advanced by one iteration
+hipError_t ilt_schedule(double* s, double growth, double b1, double b2, hipStream_t st);
 
 // ---- pupil tables (lb_pupil.hip) ------------------------------------------
 struct PupilSys {

@@ -143,8 +143,12 @@ struct Vec16 {
   T v[K];
 };
 
+// pbeta (device, optional): beta read from device memory instead of the
+// argument, so a captured graph can follow a schedule that changes per call.
 template <typename T, bool VEC>
-__global__ void k_mask_sigmoid(const T* __restrict__ x, long long n, T beta, T* __restrict__ m) {
+__global__ void k_mask_sigmoid(const T* __restrict__ x, long long n, T beta_arg,
+                               const double* __restrict__ pbeta, T* __restrict__ m) {
+  const T beta = pbeta ? (T)*pbeta : beta_arg;
   using V = Vec16<T>;
   const long long nv = VEC ? n / V::K : 0;
   const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -160,7 +164,9 @@ __global__ void k_mask_sigmoid(const T* __restrict__ x, long long n, T beta, T* 
 
 template <typename T, bool VEC>
 __global__ void k_mask_sigmoid_grad(const T* __restrict__ gm, const T* __restrict__ m, long long n,
-                                    T beta, T* __restrict__ gx) {
+                                    T beta_arg, const double* __restrict__ pbeta,
+                                    T* __restrict__ gx) {
+  const T beta = pbeta ? (T)*pbeta : beta_arg;
   using V = Vec16<T>;
   const long long nv = VEC ? n / V::K : 0;
   const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -187,19 +193,21 @@ static unsigned elementwise_blocks(long long n) {
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 template <typename T>
-hipError_t mask_sigmoid(const T* x, long long n, T beta, T* m, hipStream_t st) {
+hipError_t mask_sigmoid(const T* x, long long n, T beta, const double* pbeta, T* m,
+                        hipStream_t st) {
   if (n <= 0) return hipSuccess;
   auto k = aligned16(x) && aligned16(m) ? k_mask_sigmoid<T, true> : k_mask_sigmoid<T, false>;
-  hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, x, n, beta, m);
+  hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, x, n, beta, pbeta, m);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, T* gx, hipStream_t st) {
+hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, const double* pbeta,
+                             T* gx, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   auto k = aligned16(gm) && aligned16(m) && aligned16(gx) ? k_mask_sigmoid_grad<T, true>
                                                           : k_mask_sigmoid_grad<T, false>;
-  hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, gm, m, n, beta, gx);
+  hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, gm, m, n, beta, pbeta, gx);
   return hipGetLastError();
 }
 
@@ -271,10 +279,12 @@ hipError_t mask_penalties(const T* m, int n_img, int ny, int nx, T wtv, T wd, T*
 // g^2, theta -= step (m1 / c1) / (sqrt(m2 / c2) + eps), c_k = 1 - b_k^it;
 // o_k = 1 - b_k and c_k are formed in float64 by the caller (1 - 0.999f in
 // float would be off by 1.3e-5).
+// pc (device, optional): c1 = pc[0], c2 = pc[1] instead of the arguments.
 template <typename T>
 __global__ void k_adam(T* __restrict__ theta, T* __restrict__ m1, T* __restrict__ m2,
                        const T* __restrict__ g, long long n, T b1, T b2, T o1, T o2, T eps,
-                       T step, T c1, T c2) {
+                       T step, T c1_arg, T c2_arg, const double* __restrict__ pc) {
+  const T c1 = pc ? (T)pc[0] : c1_arg, c2 = pc ? (T)pc[1] : c2_arg;
   const long long st = (long long)gridDim.x * blockDim.x;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += st) {
     const T gi = g[i];
@@ -288,22 +298,41 @@ __global__ void k_adam(T* __restrict__ theta, T* __restrict__ m1, T* __restrict_
 
 template <typename T>
 hipError_t adam_step(T* theta, T* m1, T* m2, const T* g, long long n, T b1, T b2, T o1, T o2,
-                     T eps, T step, T c1, T c2, hipStream_t st) {
+                     T eps, T step, T c1, T c2, const double* pc, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_adam<T>, dim3(elementwise_blocks(n)), dim3(256), 0, st, theta, m1, m2, g, n,
-                     b1, b2, o1, o2, eps, step, c1, c2);
+                     b1, b2, o1, o2, eps, step, c1, c2, pc);
+  return hipGetLastError();
+}
+
+// The solver's per-iteration scalars on the device (opc/ilt.py:583-592):
+// s = [beta, it, 1 - b1^it, 1 - b2^it, beta_next]; one call advances it by
+// one, makes beta_next the current beta and grows the next by `growth`.
+__global__ void k_ilt_schedule(double* s, double growth, double b1, double b2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double it = s[1] + 1.0;
+  s[1] = it;
+  s[0] = s[4];
+  s[4] = s[0] * growth;
+  s[2] = 1.0 - pow(b1, it);
+  s[3] = 1.0 - pow(b2, it);
+}
+
+hipError_t ilt_schedule(double* s, double growth, double b1, double b2, hipStream_t st) {
+  hipLaunchKernelGGL(k_ilt_schedule, dim3(1), dim3(64), 0, st, s, growth, b1, b2);
   return hipGetLastError();
 }
 
 #define LB_OBJ_INST(T)                                                                          \
   template hipError_t resist_objective<T>(const T*, const T*, int, long long, T, T, T, T, T*, T*, \
                                           T*, T*, int*, double*, int*, hipStream_t);           \
-  template hipError_t mask_sigmoid<T>(const T*, long long, T, T*, hipStream_t);                 \
-  template hipError_t mask_sigmoid_grad<T>(const T*, const T*, long long, T, T*, hipStream_t);    \
+  template hipError_t mask_sigmoid<T>(const T*, long long, T, const double*, T*, hipStream_t);  \
+  template hipError_t mask_sigmoid_grad<T>(const T*, const T*, long long, T, const double*, T*,   \
+                                           hipStream_t);                                        \
   template hipError_t mask_penalties<T>(const T*, int, int, int, T, T, T*, T*, T*, double*, int*, \
                                         hipStream_t);                                           \
   template hipError_t adam_step<T>(T*, T*, T*, const T*, long long, T, T, T, T, T, T, T, T,      \
-                                   hipStream_t);
+                                   const double*, hipStream_t);
 LB_OBJ_INST(float)
 LB_OBJ_INST(double)
 #undef LB_OBJ_INST

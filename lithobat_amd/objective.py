@@ -25,7 +25,8 @@ import torch
 from . import _lib
 
 __all__ = ["resist_objective", "resist_objective_into", "mask_sigmoid", "mask_sigmoid_grad",
-           "mask_penalties", "adam_step", "RelaxedResist", "MaskSigmoid", "MaskPenalties"]
+           "mask_penalties", "adam_step", "ilt_schedule", "RelaxedResist", "MaskSigmoid",
+           "MaskPenalties"]
 
 
 def _prec(t: torch.Tensor) -> int:
@@ -90,25 +91,54 @@ def resist_objective_into(f, t, scale, level, steepness, sign, cost, total, d, s
         _prec(f), ws.data_ptr(), ws.numel(), st))
 
 
-def mask_sigmoid(x: torch.Tensor, beta: float, out: torch.Tensor | None = None) -> torch.Tensor:
+def _device_scalar(t: torch.Tensor, n: int, name: str) -> int:
+    """Address of n float64 values on the device (a schedule entry)."""
+    if t.dtype != torch.float64 or t.device.type != "cuda" or t.numel() < n or not t.is_contiguous():
+        raise TypeError(f"{name} must be a contiguous float64 device tensor of {n} value(s)")
+    return t.data_ptr()
+
+
+def mask_sigmoid(x: torch.Tensor, beta, out: torch.Tensor | None = None) -> torch.Tensor:
+    """``beta``: a number, or a float64 device tensor read when the pass runs."""
     _on_device(x, "x")
     prec = _prec(x)
     xc = x.contiguous()
     m = torch.empty_like(xc) if out is None else out
     st = torch.cuda.current_stream(xc.device).cuda_stream
-    _lib.check(_lib.load().lb_mask_sigmoid(xc.data_ptr(), xc.numel(), float(beta), m.data_ptr(),
-                                           prec, st))
+    lib = _lib.load()
+    if isinstance(beta, torch.Tensor):
+        _lib.check(lib.lb_mask_sigmoid_dev(xc.data_ptr(), xc.numel(),
+                                           _device_scalar(beta, 1, "beta"), m.data_ptr(), prec, st))
+    else:
+        _lib.check(lib.lb_mask_sigmoid(xc.data_ptr(), xc.numel(), float(beta), m.data_ptr(), prec,
+                                       st))
     return m
 
 
-def mask_sigmoid_grad(grad_mask: torch.Tensor, mask: torch.Tensor, beta: float,
+def mask_sigmoid_grad(grad_mask: torch.Tensor, mask: torch.Tensor, beta,
                       out: torch.Tensor | None = None) -> torch.Tensor:
+    """``beta``: a number, or a float64 device tensor read when the pass runs."""
     g = grad_mask.to(mask.dtype).contiguous()
     res = torch.empty_like(mask) if out is None else out
     st = torch.cuda.current_stream(mask.device).cuda_stream
-    _lib.check(_lib.load().lb_mask_sigmoid_grad(g.data_ptr(), mask.data_ptr(), mask.numel(),
-                                                float(beta), res.data_ptr(), _prec(mask), st))
+    lib = _lib.load()
+    if isinstance(beta, torch.Tensor):
+        _lib.check(lib.lb_mask_sigmoid_grad_dev(g.data_ptr(), mask.data_ptr(), mask.numel(),
+                                                _device_scalar(beta, 1, "beta"), res.data_ptr(),
+                                                _prec(mask), st))
+    else:
+        _lib.check(lib.lb_mask_sigmoid_grad(g.data_ptr(), mask.data_ptr(), mask.numel(),
+                                            float(beta), res.data_ptr(), _prec(mask), st))
     return res
+
+
+def ilt_schedule(schedule: torch.Tensor, growth: float, beta1: float = 0.9,
+                 beta2: float = 0.999) -> None:
+    """Advance ``schedule`` = [beta, it, 1 - beta1^it, 1 - beta2^it, beta_next] (float64,
+    device) by one solver iteration, on the device."""
+    _lib.check(_lib.load().lb_ilt_schedule(_device_scalar(schedule, 5, "schedule"), float(growth),
+                                           float(beta1), float(beta2),
+                                           torch.cuda.current_stream(schedule.device).cuda_stream))
 
 
 def mask_penalties(mask: torch.Tensor, weight_tv: float, weight_discrete: float,
@@ -135,18 +165,26 @@ def mask_penalties(mask: torch.Tensor, weight_tv: float, weight_discrete: float,
 
 
 def adam_step(theta: torch.Tensor, mom1: torch.Tensor, mom2: torch.Tensor, grad: torch.Tensor,
-              iteration: int, step: float, beta1: float = 0.9, beta2: float = 0.999,
+              iteration, step: float, beta1: float = 0.9, beta2: float = 0.999,
               eps: float = 1e-8) -> None:
-    """theta, mom1, mom2 updated in place (contiguous tensors of one dtype)."""
+    """theta, mom1, mom2 updated in place (contiguous tensors of one dtype).  ``iteration``:
+    the step number (>= 1), or a float64 device tensor holding the two bias corrections
+    [1 - beta1^it, 1 - beta2^it], read when the pass runs."""
     for t in (theta, mom1, mom2, grad):
         _on_device(t, "Adam operand")
         if not t.is_contiguous() or t.dtype != theta.dtype or t.shape != theta.shape:
             raise ValueError("Adam operands must be contiguous tensors of one shape and dtype")
     st = torch.cuda.current_stream(theta.device).cuda_stream
-    _lib.check(_lib.load().lb_adam_step(theta.data_ptr(), mom1.data_ptr(), mom2.data_ptr(),
-                                        grad.data_ptr(), theta.numel(), int(iteration),
-                                        float(step), float(beta1), float(beta2), float(eps),
-                                        _prec(theta), st))
+    lib = _lib.load()
+    if isinstance(iteration, torch.Tensor):
+        _lib.check(lib.lb_adam_step_dev(theta.data_ptr(), mom1.data_ptr(), mom2.data_ptr(),
+                                        grad.data_ptr(), theta.numel(),
+                                        _device_scalar(iteration, 2, "corrections"), float(step),
+                                        float(beta1), float(beta2), float(eps), _prec(theta), st))
+    else:
+        _lib.check(lib.lb_adam_step(theta.data_ptr(), mom1.data_ptr(), mom2.data_ptr(),
+                                    grad.data_ptr(), theta.numel(), int(iteration), float(step),
+                                    float(beta1), float(beta2), float(eps), _prec(theta), st))
 
 
 class MaskPenalties(torch.autograd.Function):

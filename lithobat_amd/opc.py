@@ -21,7 +21,7 @@ from .blur import blur, gaussian_blur
 from .expose import _to_device, normalisation_scale
 from .hopkins import SOCSKernels
 from .ilt import abbe_intensity, mask_of, mask_penalties, relaxed_resist_loss
-from .objective import adam_step, mask_sigmoid_grad
+from .objective import adam_step, ilt_schedule, mask_sigmoid_grad
 
 
 def apply_peb(pac_map, diffusion_sigma: float, pixel_size: float):
@@ -218,8 +218,17 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
             tol: float = 1e-7, beta: float = 4.0, beta_final: float | None = 12.0,
             initial_grey: float = 0.9, steepness: float = 12.0, weight_tv: float = 0.0,
             weight_discrete: float = 0.0, smooth: float = 0.0,
-            progress: Callable[[ILTFrame], None] | None = None) -> ILTResult:
-    """ilt.py:397-621 on the device (same parameters, checks, schedule and history)."""
+            progress: Callable[[ILTFrame], None] | None = None,
+            graph: bool = True) -> ILTResult:
+    """ilt.py:397-621 on the device (same parameters, checks, schedule and history).
+
+    ``graph``: after the first iteration, one iteration (the schedule update, the step
+    back through the mask parameterisation, Adam, the new mask, the forward image,
+    objective and adjoint) is captured into a HIP graph and replayed; its per-iteration
+    scalars (beta, the Adam bias corrections) live on the device, so replays and eager
+    iterations run the same kernels on the same values.  Each iteration still reads
+    its cost, pattern error and gradient norm back for the history and the
+    convergence test, as the reference does."""
     kernels = model.kernels()
     if kernels is None:
         raise ValueError(
@@ -270,13 +279,15 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
         m0 = 0.5 + (np.clip(tgt, 0.0, 1.0) - 0.5) * (2.0 * p - 1.0)
     else:
         m0 = np.clip(np.asarray(initial, dtype=np.float64), 1.0 - p, p)
-    theta = torch.as_tensor(np.log(m0 / (1.0 - m0)) / beta, dtype=dt, device=dev)
+    theta = torch.as_tensor(np.log(m0 / (1.0 - m0)) / beta, dtype=dt, device=dev).contiguous()
     growth = (1.0 if beta_final is None or max_iter <= 1
               else (beta_final / beta) ** (1.0 / (max_iter - 1)))
     mom1 = torch.zeros_like(theta)
     mom2 = torch.zeros_like(theta)
     b1, b2, eps = 0.9, 0.999, 1e-8
     beta_k = float(beta)
+    # [beta, it, 1 - b1^it, 1 - b2^it, beta_next] on the device (ilt_schedule)
+    sched = torch.tensor([beta_k, 0.0, 0.0, 0.0, beta_k], dtype=torch.float64, device=dev)
     with torch.no_grad():
         mask = mask_of(theta, beta_k, smooth_px)
     cost, grad_m, signed, perr = objective(mask)
@@ -301,21 +312,43 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
         return c
 
     record(0, cost, grad_m, signed, perr, mask)
+    # the iteration's inputs and outputs at fixed addresses (a graph replays on them)
+    state = {"mask": mask.detach().clone(), "grad_m": grad_m.detach().clone()}
+
+    def iteration():
+        ilt_schedule(sched, growth, b1, b2)
+        with torch.no_grad():
+            grad_t = grad_theta(state["grad_m"], state["mask"], sched[0:1]).contiguous()
+            adam_step(theta, mom1, mom2, grad_t, sched[2:4], step, b1, b2, eps)
+            new_mask = mask_of(theta, sched[4:5], smooth_px)
+        out = objective(new_mask)
+        with torch.no_grad():
+            state["mask"].copy_(new_mask)
+            state["grad_m"].copy_(out[1])
+        return out
+
     prev_cost, converged, taken = np.inf, False, 0
+    g = None
     for it in range(1, max_iter + 1):
-        with torch.no_grad():
-            grad_t = grad_theta(grad_m, mask, beta_k)
-        adam_step(theta, mom1, mom2, grad_t.contiguous(), it, step, b1, b2, eps)
+        if g is None:
+            outs = iteration()
+        else:
+            g.replay()
         beta_k *= growth
-        with torch.no_grad():
-            mask = mask_of(theta, beta_k, smooth_px)
-        cost, grad_m, signed, perr = objective(mask)
+        cost, _, signed, perr = outs
         taken = it
-        c = record(it, cost, grad_m, signed, perr, mask)
+        c = record(it, cost, state["grad_m"], signed, perr, state["mask"])
         if abs(prev_cost - c) < tol:
             converged = True
             break
         prev_cost = c
+        if graph and g is None and it < max_iter:
+            # capture records the iteration without running it; replays run it
+            torch.cuda.current_stream().synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                outs = iteration()
+    mask = state["mask"]
     try:
         import pandas as pd
         history = pd.DataFrame(rows)

@@ -61,6 +61,29 @@ def test_run_ilt_trajectory_matches_oracle(device):
     assert res.printed.field.shape == (64, 64)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_run_ilt_graph_replays_equal_eager(device, precision):
+    """The graph-captured solve (iterations 2..n replayed) runs the same kernels on the same
+    device-held schedule as the eager solve: identical history, mask and theta; and the
+    convergence test stops both at the same iteration."""
+    from lithobat_amd.opc import PrintModel, run_ilt
+    optics, grid, resist = _process()
+    target = O.lines_and_spaces(64, 8e-9, 160e-9, 80e-9)
+    model = PrintModel(optics, resist, grid, precision=precision)
+    kw = dict(max_iter=8, smooth=8e-9, weight_tv=1e-3, weight_discrete=1e-3, steepness=12.0)
+    a = run_ilt(target, model, graph=True, **kw)
+    b = run_ilt(target, model, graph=False, **kw)
+    assert a.iterations == b.iterations == 8
+    for col in ("cost", "pattern_error", "beta", "grad_norm"):
+        assert list(a.history[col]) == list(b.history[col]), col
+    assert np.array_equal(a.mask, b.mask) and np.array_equal(a.theta, b.theta)
+    tol = float(a.history["cost"].diff().abs().iloc[3]) * 1.0001  # stops after iteration 3
+    c = run_ilt(target, model, graph=True, **dict(kw, tol=tol))
+    d = run_ilt(target, model, graph=False, **dict(kw, tol=tol))
+    assert c.converged and d.converged and c.iterations == d.iterations <= 4
+    assert np.array_equal(c.mask, d.mask)
+
+
 def test_run_ilt_refuses_what_the_reference_refuses(device):
     import dataclasses
     from lithobat_amd.opc import PrintModel, run_ilt
