@@ -55,6 +55,27 @@ class ImagingBatch:
         return len(self.plans)
 
     @property
+    def groups(self) -> list[list[int]]:
+        """Clips with the same band geometry, whose band passes the library runs as one
+        pass when their buffers are consecutive in memory (see ``grouped_empty``)."""
+        keys = {}
+        for c, p in enumerate(self.plans):
+            i = p.info
+            keys.setdefault((p.n, len(p.planes), i["band"], i["m"], i["n_systems"]), []).append(c)
+        return list(keys.values())
+
+    def grouped_empty(self, per_clip_shape, device) -> list[torch.Tensor]:
+        """Per-clip tensors (shape ``per_clip_shape(plan)``) carved consecutively out of one
+        allocation per clip group, so batched passes run on them in place."""
+        out = [None] * len(self.plans)
+        for g in self.groups:
+            shape = tuple(per_clip_shape(self.plans[g[0]]))
+            block = torch.empty((len(g),) + shape, dtype=self.real_dtype, device=device)
+            for k, c in enumerate(g):
+                out[c] = block[k]
+        return out
+
+    @property
     def real_dtype(self):
         return self.plans[0].real_dtype
 
@@ -93,15 +114,14 @@ class ImagingBatch:
 
     def forward(self, masks):
         masks = self._check(masks)
-        images = [torch.empty((len(p.planes), p.n, p.n), dtype=self.real_dtype,
-                              device=m.device) for p, m in zip(self.plans, masks)]
+        images = self.grouped_empty(lambda p: (len(p.planes), p.n, p.n), masks[0].device)
         self.forward_into(masks, images)
         return images
 
     def backward(self, masks, d_images):
         masks = self._check(masks)
         d_images = [g.to(self.real_dtype).contiguous() for g in d_images]
-        d_masks = [torch.empty_like(m) for m in masks]
+        d_masks = self.grouped_empty(lambda p: (p.n, p.n), masks[0].device)
         self.backward_into(masks, d_images, d_masks)
         return d_masks
 
@@ -130,41 +150,65 @@ class ILTStep:
     reference's relaxed resist (a = steepness / |level|, opc/ilt.py:311-337),
     and d loss / d theta_c by the explicit chain rule and the batch adjoint.
     All buffers are preallocated, so ``run`` can be captured into a graph.
+    Clips of one shape group (``ImagingBatch.groups``) share consecutive buffers:
+    their mask transforms, objective and band passes run once per group.
+    ``costs`` holds the loss of each group (of each clip where a group's clips
+    have different normalisations).
     """
 
     def __init__(self, batch: ImagingBatch, thetas, targets, beta=8.0, level=0.3,
                  steepness=50.0):
         self.batch = batch
         self.beta, self.level, self.steep = beta, level, steepness
-        self.theta = [t.to(batch.real_dtype).contiguous() for t in thetas]
-        self.target = [t.to(batch.real_dtype).contiguous() for t in targets]
+        dev = thetas[0].device
+        dt = batch.real_dtype
+        plans = batch.plans
+        self.theta = batch.grouped_empty(lambda p: (p.n, p.n), dev)
+        for th, t in zip(self.theta, thetas):
+            th.copy_(t.to(dt))
+        self.mask = batch.grouped_empty(lambda p: (p.n, p.n), dev)
+        self.raw = batch.grouped_empty(lambda p: (len(p.planes), p.n, p.n), dev)
+        self.d_raw = batch.grouped_empty(lambda p: (len(p.planes), p.n, p.n), dev)
+        self.d_mask = batch.grouped_empty(lambda p: (p.n, p.n), dev)
+        self.grad = batch.grouped_empty(lambda p: (p.n, p.n), dev)
         # the target of every plane of the clip, as the objective reads it
-        self.target_planes = [t.expand(len(p.planes), *t.shape[-2:]).contiguous()
-                              for p, t in zip(batch.plans, self.target)]
-        self.scale = [1.0 / float(p.clear[0]) if p.clear[0] > 0 else 1.0 for p in batch.plans]
-        self.mask = [torch.empty_like(t) for t in self.theta]
-        self.raw = [torch.empty((len(p.planes), p.n, p.n), dtype=batch.real_dtype,
-                                device=t.device) for p, t in zip(batch.plans, self.theta)]
-        self.d_raw = [torch.empty_like(r) for r in self.raw]
-        self.d_mask = [torch.empty_like(t) for t in self.theta]
-        self.grad = [torch.empty_like(t) for t in self.theta]
-        self.loss = torch.zeros((), dtype=batch.real_dtype, device=self.theta[0].device)
-        self.costs = torch.zeros(len(self.theta), dtype=batch.real_dtype,
-                                 device=self.theta[0].device)
-        self.obj_ws = [torch.empty(max(objective.workspace_bytes(len(r), r.shape[-1] ** 2), 256),
-                                   dtype=torch.uint8, device=r.device) for r in self.raw]
-        batch.workspace(False, self.theta[0].device)
-        batch.workspace(True, self.theta[0].device)
+        self.target_planes = batch.grouped_empty(lambda p: (len(p.planes), p.n, p.n), dev)
+        for tp, t in zip(self.target_planes, targets):
+            tp.copy_(t.to(dt).expand(tp.shape))
+        scale = [1.0 / float(p.clear[0]) if p.clear[0] > 0 else 1.0 for p in plans]
+
+        def whole(views, g):  # the group's consecutive block behind its clips' views
+            v0 = views[g[0]]
+            return v0.as_strided((len(g),) + tuple(v0.shape), (v0.numel(),) + v0.stride())
+
+        # units of work: a whole group where its clips share a normalisation, else clips
+        self.units = []
+        for g in batch.groups:
+            if all(scale[c] == scale[g[0]] for c in g):
+                self.units.append((whole(self.theta, g), whole(self.mask, g), whole(self.raw, g),
+                                   whole(self.target_planes, g), whole(self.d_raw, g),
+                                   whole(self.d_mask, g), whole(self.grad, g), scale[g[0]]))
+            else:
+                for c in g:
+                    self.units.append((self.theta[c], self.mask[c], self.raw[c],
+                                       self.target_planes[c], self.d_raw[c], self.d_mask[c],
+                                       self.grad[c], scale[c]))
+        self.loss = torch.zeros((), dtype=dt, device=dev)
+        self.costs = torch.zeros(len(self.units), dtype=dt, device=dev)
+        self.obj_ws = [torch.empty(max(objective.workspace_bytes(u[2].numel() // u[2].shape[-1] ** 2,
+                                                                 u[2].shape[-1] ** 2), 256),
+                                   dtype=torch.uint8, device=dev) for u in self.units]
+        batch.workspace(False, dev)
+        batch.workspace(True, dev)
 
     def run(self) -> None:
-        for th, m in zip(self.theta, self.mask):
+        for th, m, *_ in self.units:
             objective.mask_sigmoid(th, self.beta, out=m)
         self.batch.forward_into(self.mask, self.raw)
-        for c, (raw, t, s, d, ws) in enumerate(zip(self.raw, self.target_planes, self.scale,
-                                                    self.d_raw, self.obj_ws)):
+        for u, (_, _, raw, t, d, _, _, s) in enumerate(self.units):
             objective.resist_objective_into(raw, t, s, self.level, self.steep, 1.0, None,
-                                            self.costs[c], d, None, None, ws)
+                                            self.costs[u], d, None, None, self.obj_ws[u])
         torch.sum(self.costs, 0, out=self.loss)
         self.batch.backward_into(self.mask, self.d_raw, self.d_mask)
-        for g, dm, m in zip(self.grad, self.d_mask, self.mask):
+        for _, m, _, _, _, dm, g, _ in self.units:
             objective.mask_sigmoid_grad(dm, m, self.beta, out=g)

@@ -573,12 +573,24 @@ struct lb_plan_s {
 // Several plans (clips of different sizes / optics) imaged together: their
 // band passes run per clip, their coherent systems in ONE persistent launch
 // of the coherent-system kernel over a descriptor table of work items.
+// Clips with the same band geometry (size, mask band, coarse grid and its
+// band, planes, systems per plane) form a group: their band passes run as one
+// multi-mask pass over a group workspace laid out like a plan's for that many
+// masks.  A group's masks and images are used in place when the caller's
+// buffers are consecutive in memory (clip after clip), else through staging
+// copies in the group workspace.
+struct BatchGroup {
+  std::vector<int> members;  // clip indices, in order
+  Work wf, wb;
+  size_t base_f = 0, base_b = 0;
+  size_t stage_f = 0, stage_b = 0;  // staging offsets (masks, then images / image gradients)
+};
+
 struct lb_batch_s {
   std::vector<lb_plan> plans;  // not owned
   int precision = 32, M = 0, Wmax = 0, cus = 256, chunk = 1;
   FieldConfig fwd{}, adj{};
-  std::vector<Work> wf, wb;
-  std::vector<size_t> base_f, base_b;
+  std::vector<BatchGroup> groups;
   size_t ws_f = 0, ws_b = 0;
   ItemDesc* items_f = nullptr;
   ItemDesc* items_b = nullptr;
@@ -1131,42 +1143,69 @@ lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) 
     return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel shape fits the widest clip");
   }
   b->chunk = (int)std::max<long long>(1, (total_sys + 4LL * b->cus - 1) / (4LL * b->cus));
+  // group the clips by band geometry
+  auto same = [](const Plan& a, const Plan& c) {
+    return a.N == c.N && a.nu0m == c.nu0m && a.Wbm == c.Wbm && a.wrapm == c.wrapm && a.M == c.M &&
+           a.QI == c.QI && a.WbI == c.WbI && a.n_planes == c.n_planes && a.spp == c.spp;
+  };
+  for (int c = 0; c < n_plans; ++c) {
+    BatchGroup* g = nullptr;
+    for (BatchGroup& h : b->groups)
+      if (same(b->plans[h.members[0]]->p, b->plans[c]->p)) g = &h;
+    if (!g) {
+      b->groups.emplace_back();
+      g = &b->groups.back();
+    }
+    g->members.push_back(c);
+  }
   std::vector<ItemDesc> itf, itb;
   size_t off_f = 0, off_b = 0;
-  for (lb_plan h : b->plans) {
-    const Plan& p = h->p;
-    Work wf = layout(p, 1, false, b->chunk, &b->fwd), wb = layout(p, 1, true, b->chunk, &b->adj);
-    b->wf.push_back(wf);
-    b->wb.push_back(wb);
-    b->base_f.push_back(off_f);
-    b->base_b.push_back(off_b);
+  for (BatchGroup& g : b->groups) {
+    const Plan& p = b->plans[g.members[0]]->p;
+    const int cnt = (int)g.members.size();
+    g.wf = layout(p, cnt, false, b->chunk, &b->fwd);
+    g.wb = layout(p, cnt, true, b->chunk, &b->adj);
+    const Work& wf = g.wf;
+    const Work& wb = g.wb;
+    g.base_f = off_f;
+    g.base_b = off_b;
     const size_t R = p.rsz(), C = p.csz();
-    for (int img = 0; img < wf.n_img; ++img)
-      for (int k = 0; k < wf.n_chunks; ++k) {
-        const int plane = img % p.n_planes;
-        ItemDesc d{};
-        d.Wb = p.Wbm;
-        d.wrapN = p.wrapm;
-        d.H = p.H;
-        d.W = p.W;
-        d.box = p.box;
-        d.wsys = p.wsys;
-        d.s0 = plane * p.spp + k * wf.chunk;
-        d.s1 = std::min(d.s0 + wf.chunk, (plane + 1) * p.spp);
-        const long long item = (long long)img * wf.n_chunks + k;
-        d.X_off = (long long)(off_f + wf.Xm);
-        d.part_off = (long long)(off_f + wf.partial + item * p.M * p.M * R);
-        d.gM_off = -1;
-        d.gX_off = -1;
-        itf.push_back(d);
-        d.X_off = (long long)(off_b + wb.Xm);
-        d.part_off = -1;
-        d.gM_off = (long long)(off_b + wb.gM + (size_t)img * p.M * p.M * R);
-        d.gX_off = (long long)(off_b + wb.gX + item * p.Wbm * p.Wbm * C);
-        itb.push_back(d);
-      }
-    off_f += (wf.total + 255) / 256 * 256;
-    off_b += (wb.total + 255) / 256 * 256;
+    const size_t nn = (size_t)p.N * p.N;
+    g.stage_f = off_f + (wf.total + 255) / 256 * 256;
+    g.stage_b = off_b + (wb.total + 255) / 256 * 256;
+    for (int k = 0; k < cnt; ++k) {
+      const Plan& q = b->plans[g.members[k]]->p;
+      for (int pl = 0; pl < q.n_planes; ++pl)
+        for (int ch = 0; ch < wf.n_chunks; ++ch) {
+          const long long gi = (long long)k * q.n_planes + pl;  // image in the group
+          const long long item = gi * wf.n_chunks + ch;
+          ItemDesc d{};
+          d.Wb = q.Wbm;
+          d.wrapN = q.wrapm;
+          d.H = q.H;
+          d.W = q.W;
+          d.box = q.box;
+          d.wsys = q.wsys;
+          d.s0 = pl * q.spp + ch * wf.chunk;
+          d.s1 = std::min(d.s0 + wf.chunk, (pl + 1) * q.spp);
+          d.X_off = (long long)(off_f + wf.Xm + (size_t)k * q.Wbm * q.Wbm * C);
+          d.part_off = (long long)(off_f + wf.partial + item * q.M * q.M * R);
+          d.gM_off = -1;
+          d.gX_off = -1;
+          itf.push_back(d);
+          d.X_off = (long long)(off_b + wb.Xm + (size_t)k * q.Wbm * q.Wbm * C);
+          d.part_off = -1;
+          d.gM_off = (long long)(off_b + wb.gM + (size_t)gi * q.M * q.M * R);
+          d.gX_off = (long long)(off_b + wb.gX + item * q.Wbm * q.Wbm * C);
+          itb.push_back(d);
+        }
+    }
+    // staging: masks, then images (forward) or image gradients and mask gradients (backward)
+    const size_t masks_b = (R * cnt * nn + 255) / 256 * 256;
+    const size_t imgs_b = (R * cnt * p.n_planes * nn + 255) / 256 * 256;
+    const size_t peak_b = (R * cnt * p.n_planes + 255) / 256 * 256;
+    off_f = g.stage_f + masks_b + imgs_b + peak_b;
+    off_b = g.stage_b + masks_b + imgs_b + masks_b;
   }
   b->ws_f = off_f;
   b->ws_b = off_b;
@@ -1206,23 +1245,69 @@ static int batch_grid(const lb_batch_s& b, const FieldConfig& fc, int n_items, b
 
 }  // extern "C"
 
+// Whether the clips of a group sit consecutively in the caller's memory at
+// `bytes` per clip (then the group runs on them in place).
+static bool consecutive(const void* const* ptr, const std::vector<int>& members, size_t bytes) {
+  for (size_t k = 1; k < members.size(); ++k)
+    if ((const char*)ptr[members[k]] != (const char*)ptr[members[0]] + k * bytes) return false;
+  return true;
+}
+
+static hipError_t gather(char* dst, const void* const* src, const std::vector<int>& members,
+                         size_t bytes, hipStream_t st) {
+  for (size_t k = 0; k < members.size(); ++k) {
+    hipError_t e = hipMemcpyAsync(dst + k * bytes, src[members[k]], bytes,
+                                  hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+static hipError_t scatter(void* const* dst, const char* src, const std::vector<int>& members,
+                          size_t bytes, hipStream_t st) {
+  for (size_t k = 0; k < members.size(); ++k) {
+    hipError_t e = hipMemcpyAsync(dst[members[k]], src + k * bytes, bytes,
+                                  hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template <typename T>
 static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* const* images,
                                  void* const* peaks, char* ws, hipStream_t st) {
-  const int n = (int)b.plans.size();
-  for (int c = 0; c < n; ++c) {
-    lb_status rc = fwd_pre<T>(b.plans[c]->p, masks[c], 0, 1, ws + b.base_f[c], b.wf[c], st);
+  for (const BatchGroup& g : b.groups) {
+    const Plan& p = b.plans[g.members[0]]->p;
+    const size_t mb = sizeof(T) * (size_t)p.N * p.N;
+    const void* m0 = masks[g.members[0]];
+    if (!consecutive(masks, g.members, mb)) {
+      LB_HIP(gather(ws + g.stage_f, masks, g.members, mb, st));
+      m0 = ws + g.stage_f;
+    }
+    lb_status rc = fwd_pre<T>(b.plans[g.members[0]]->p, m0, 0, (int)g.members.size(),
+                              ws + g.base_f, g.wf, st);
     if (rc != LB_OK) return rc;
   }
   FieldConfig fc = b.fwd;
   fc.grid = batch_grid(b, fc, b.n_items_f, false);
-  int* q = (int*)(ws + b.base_f[0] + b.wf[0].queue);
+  int* q = (int*)(ws + b.groups[0].base_f + b.groups[0].wf.queue);
   LB_HIP(fill<float>((float*)q, 0.0f, 1, st));
   LB_HIP(field_multi(b.precision, fc, b.items_f, b.n_items_f, b.Wmax, ws, q, false, st));
-  for (int c = 0; c < n; ++c) {
-    lb_status rc = fwd_post<T>(b.plans[c]->p, images[c], peaks ? peaks[c] : nullptr,
-                               ws + b.base_f[c], b.wf[c], st);
+  for (const BatchGroup& g : b.groups) {
+    Plan& p = b.plans[g.members[0]]->p;
+    const int cnt = (int)g.members.size();
+    const size_t nn = (size_t)p.N * p.N;
+    const size_t ib = sizeof(T) * p.n_planes * nn, pb = sizeof(T) * p.n_planes;
+    char* stage_img = ws + g.stage_f + (sizeof(T) * cnt * nn + 255) / 256 * 256;
+    char* stage_pk = stage_img + (sizeof(T) * cnt * p.n_planes * nn + 255) / 256 * 256;
+    const bool img_in_place = consecutive(images, g.members, ib);
+    const bool pk_in_place = !peaks || consecutive(peaks, g.members, pb);
+    void* out = img_in_place ? images[g.members[0]] : (void*)stage_img;
+    void* pk = peaks ? (pk_in_place ? peaks[g.members[0]] : (void*)stage_pk) : nullptr;
+    lb_status rc = fwd_post<T>(p, out, pk, ws + g.base_f, g.wf, st);
     if (rc != LB_OK) return rc;
+    if (!img_in_place) LB_HIP(scatter(images, stage_img, g.members, ib, st));
+    if (peaks && !pk_in_place) LB_HIP(scatter(peaks, stage_pk, g.members, pb, st));
   }
   return LB_OK;
 }
@@ -1231,21 +1316,45 @@ template <typename T>
 static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
                                   const void* const* dimg, void* const* dmask, char* ws,
                                   hipStream_t st) {
-  const int n = (int)b.plans.size();
-  for (int c = 0; c < n; ++c) {
+  for (const BatchGroup& g : b.groups) {
+    const Plan& p = b.plans[g.members[0]]->p;
+    const int cnt = (int)g.members.size();
+    const size_t nn = (size_t)p.N * p.N;
+    const size_t mb = sizeof(T) * nn, ib = sizeof(T) * p.n_planes * nn;
+    char* stage_m = ws + g.stage_b;
+    char* stage_g = stage_m + (sizeof(T) * cnt * nn + 255) / 256 * 256;
+    const void* m0 = masks[g.members[0]];
+    if (!consecutive(masks, g.members, mb)) {
+      LB_HIP(gather(stage_m, masks, g.members, mb, st));
+      m0 = stage_m;
+    }
+    const void* g0 = dimg[g.members[0]];
+    if (!consecutive(dimg, g.members, ib)) {
+      LB_HIP(gather(stage_g, dimg, g.members, ib, st));
+      g0 = stage_g;
+    }
     const T* gM = nullptr;
-    lb_status rc =
-        bwd_pre<T>(b.plans[c]->p, masks[c], 1, dimg[c], ws + b.base_b[c], b.wb[c], &gM, st);
+    lb_status rc = bwd_pre<T>(b.plans[g.members[0]]->p, m0, cnt, g0, ws + g.base_b, g.wb, &gM,
+                              st);
     if (rc != LB_OK) return rc;
   }
   FieldConfig fc = b.adj;
   fc.grid = batch_grid(b, fc, b.n_items_b, true);
-  int* q = (int*)(ws + b.base_b[0] + b.wb[0].queue);
+  int* q = (int*)(ws + b.groups[0].base_b + b.groups[0].wb.queue);
   LB_HIP(fill<float>((float*)q, 0.0f, 1, st));
   LB_HIP(field_multi(b.precision, fc, b.items_b, b.n_items_b, b.Wmax, ws, q, true, st));
-  for (int c = 0; c < n; ++c) {
-    lb_status rc = bwd_post<T>(b.plans[c]->p, 1, dmask[c], ws + b.base_b[c], b.wb[c], st);
+  for (const BatchGroup& g : b.groups) {
+    Plan& p = b.plans[g.members[0]]->p;
+    const int cnt = (int)g.members.size();
+    const size_t nn = (size_t)p.N * p.N;
+    const size_t mb = sizeof(T) * nn;
+    char* stage_d = ws + g.stage_b + (sizeof(T) * cnt * nn + 255) / 256 * 256 +
+                    (sizeof(T) * cnt * p.n_planes * nn + 255) / 256 * 256;
+    const bool in_place = consecutive(dmask, g.members, mb);
+    void* out = in_place ? dmask[g.members[0]] : (void*)stage_d;
+    lb_status rc = bwd_post<T>(p, cnt, out, ws + g.base_b, g.wb, st);
     if (rc != LB_OK) return rc;
+    if (!in_place) LB_HIP(scatter(dmask, stage_d, g.members, mb, st));
   }
   return LB_OK;
 }
