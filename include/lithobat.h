@@ -255,6 +255,12 @@ lb_status lb_mask_sigmoid(const void* x, int64_t n, double beta, void* mask, int
 lb_status lb_mask_sigmoid_grad(const void* grad_mask, const void* mask, int64_t n, double beta,
                                void* grad_x, int32_t precision, void* stream);
 
+/* No counterpart in the reference: the coherent-system kernel shape of a
+ * batch's forward or adjoint launch (single-launch shapes only; the work
+ * items and workspace do not depend on it). */
+lb_status lb_batch_set_shape(lb_batch batch, int32_t adjoint, int32_t block_threads,
+                             int32_t column_splits);
+
 /* Batches of clips (no direct counterpart in the reference, which images one
  * mask per call of compute_aerial_image / runs one run_ilt per clip): plans of
  * different grid sizes and optics -- one mask each -- imaged together.  The

@@ -54,6 +54,11 @@ class ImagingBatch:
     def __len__(self):
         return len(self.plans)
 
+    def set_shape(self, adjoint: bool, threads: int, splits: int) -> None:
+        """Coherent-system kernel shape of the batch's forward or adjoint launch."""
+        _lib.check(self._lib.lb_batch_set_shape(self._h, 1 if adjoint else 0, int(threads),
+                                                int(splits)))
+
     @property
     def groups(self) -> list[list[int]]:
         """Clips with the same band geometry, whose band passes the library runs as one

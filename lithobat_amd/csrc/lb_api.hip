@@ -1226,6 +1226,22 @@ lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) 
   return LB_OK;
 }
 
+lb_status lb_batch_set_shape(lb_batch batch, int32_t adjoint, int32_t block_threads,
+                             int32_t column_splits) {
+  g_err.clear();
+  if (!batch) return fail(LB_ERR_ARG, "null batch");
+  FieldConfig fc{batch->M, block_threads, column_splits, 0};
+  if (column_splits < 1 || !field_config_supported(batch->precision, fc))
+    return fail(LB_ERR_UNSUPPORTED, "no single-launch coherent-system kernel of shape M=" +
+                                        std::to_string(batch->M) + " threads=" +
+                                        std::to_string(block_threads) + " splits=" +
+                                        std::to_string(column_splits));
+  if (field_smem_bytes(batch->precision, fc, batch->Wmax, adjoint != 0) > kLdsLimit)
+    return fail(LB_ERR_UNSUPPORTED, "shape does not fit in LDS for the widest clip");
+  (adjoint ? batch->adj : batch->fwd) = fc;
+  return LB_OK;
+}
+
 lb_status lb_batch_destroy(lb_batch batch) {
   delete batch;
   return LB_OK;

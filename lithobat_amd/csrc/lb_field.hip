@@ -1046,7 +1046,7 @@ __global__ __launch_bounds__(NT) void k_rows_adj_staged(MapSource<T, M> src, int
 #define LB_FIELD_CONFIGS(X)                                                                \
   X(16, 64, 1) X(32, 128, 1) X(48, 64, 1) X(64, 256, 1) X(64, 128, 1) X(64, 256, 2)          \
       X(80, 320, 1) X(80, 320, 2) X(80, 160, 1)                                              \
-      X(96, 256, 1) X(96, 128, 1) X(96, 256, 3) X(96, 384, 1) X(128, 256, 1) X(128, 512, 1) X(128, 256, 2) \
+      X(96, 256, 1) X(96, 128, 1) X(96, 256, 3) X(96, 384, 1) X(128, 512, 1) X(128, 256, 1) X(128, 256, 2) \
       X(128, 256, 4) X(256, 256, 2) X(256, 256, 4) X(256, 256, 8)
 
 // staged shapes (column splits 0): coarse grid, threads per block
