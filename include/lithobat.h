@@ -228,6 +228,18 @@ lb_status lb_adam_step(void* theta, void* mom1, void* mom2, const void* grad, in
                        int32_t iteration, double step, double beta1, double beta2, double eps,
                        int32_t precision, void* stream);
 
+/* The voxel interpolation of develop/resist3d.py:172-185 (exposure_volume):
+ * for n_masks stacks of n_planes raw images [npix] and nz voxels (bottom-up),
+ * voxel z mixes planes plane_index[z] and plane_index[z] + 1 (each scaled by
+ * its image normalisation plane_scale[mask][plane]) with the weight
+ * weight[z] of the second: a + w (c - a).  plane_index (int32), weight and
+ * plane_scale are device arrays; volume is [n_masks][nz][npix].  The caller
+ * guarantees plane_index[z] + 1 < n_planes. */
+lb_status lb_volume_interp(const void* raw, int32_t n_masks, int32_t n_planes, int64_t npix,
+                           int32_t nz, const int32_t* plane_index, const void* weight,
+                           const void* plane_scale, void* volume, int32_t precision,
+                           void* stream);
+
 /* Device-scalar forms for solver loops captured into a graph (no counterpart
  * in the reference, whose loop runs on the host: opc/ilt.py:583-592).
  * lb_ilt_schedule advances schedule = [beta, it, 1 - beta1^it, 1 - beta2^it,

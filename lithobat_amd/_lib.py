@@ -54,6 +54,7 @@ EXPORTS = [
     "lb_resist_objective_workspace_bytes", "lb_resist_objective", "lb_mask_sigmoid",
     "lb_mask_sigmoid_grad", "lb_mask_penalties", "lb_adam_step", "lb_adam_step_dev",
     "lb_ilt_schedule", "lb_mask_sigmoid_dev", "lb_mask_sigmoid_grad_dev", "lb_batch_set_shape",
+    "lb_volume_interp",
 ]
 
 _lib = None
@@ -113,6 +114,7 @@ def load() -> C.CDLL:
         "lb_batch_create": ([C.POINTER(vp), i32, plan_p], C.c_int),
         "lb_batch_destroy": ([vp], C.c_int),
         "lb_batch_set_shape": ([vp, i32, i32, i32], C.c_int),
+        "lb_volume_interp": ([vp, i32, i32, C.c_int64, i32, vp, vp, vp, vp, i32, vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),
         "lb_batch_forward": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, sz, vp],
                              C.c_int),

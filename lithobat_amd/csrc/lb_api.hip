@@ -1633,6 +1633,28 @@ lb_status lb_adam_step_dev(void* theta, void* mom1, void* mom2, const void* grad
                   precision, stream);
 }
 
+lb_status lb_volume_interp(const void* raw, int32_t n_masks, int32_t n_planes, int64_t npix,
+                           int32_t nz, const int32_t* plane_index, const void* weight,
+                           const void* plane_scale, void* volume, int32_t precision,
+                           void* stream) {
+  g_err.clear();
+  if (!raw || !plane_index || !weight || !plane_scale || !volume)
+    return fail(LB_ERR_ARG, "null argument");
+  if (n_masks < 1 || n_planes < 2 || npix < 1 || nz < 1)
+    return fail(LB_ERR_ARG, "need n_masks >= 1, n_planes >= 2, npix >= 1, nz >= 1");
+  if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 64)
+    LB_HIP(volume_lerp<double>((const double*)raw, n_masks, n_planes, npix, nz, plane_index,
+                               (const double*)weight, (const double*)plane_scale, (double*)volume,
+                               st));
+  else
+    LB_HIP(volume_lerp<float>((const float*)raw, n_masks, n_planes, npix, nz, plane_index,
+                              (const float*)weight, (const float*)plane_scale, (float*)volume,
+                              st));
+  return LB_OK;
+}
+
 lb_status lb_ilt_schedule(double* schedule, double growth, double beta1, double beta2,
                           void* stream) {
   g_err.clear();

@@ -184,6 +184,9 @@ hipError_t mask_penalties(const T* m, int n_img, int ny, int nx, T wtv, T wd, T*
 template <typename T>
 hipError_t adam_step(T* theta, T* m1, T* m2, const T* g, long long n, T b1, T b2, T o1, T o2,
                      T eps, T step, T c1, T c2, const double* pc, hipStream_t st);
+template <typename T>
+hipError_t volume_lerp(const T* raw, int n_masks, int n_planes, long long npix, int nz,
+                       const int* jz, const T* tz, const T* scale, T* vol, hipStream_t st);
 // s = [beta, it, 1 - b1^it, 1 - b2^it, beta_next] advanced by one iteration
 hipError_t ilt_schedule(double* s, double growth, double b1, double b2, hipStream_t st);
 

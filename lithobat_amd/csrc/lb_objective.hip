@@ -323,6 +323,41 @@ hipError_t ilt_schedule(double* s, double growth, double b1, double b2, hipStrea
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- volume
+// develop/resist3d.py:172-185: voxel z of mask b takes the two optical planes
+// j_z, j_z + 1 around its depth, each normalised by its own scale (the image
+// normalisation of that plane), and mixes them linearly:
+//   a = s_j raw_j,  c = s_{j+1} raw_{j+1},  vol = a + t_z (c - a).
+// One read of the two planes and one write per voxel.
+template <typename T>
+__global__ void k_volume_lerp(const T* __restrict__ raw, int n_planes, long long npix, int nz,
+                              const int* __restrict__ jz, const T* __restrict__ tz,
+                              const T* __restrict__ scale, T* __restrict__ vol) {
+  const int z = blockIdx.y, b = blockIdx.z;
+  const int j = jz[z];
+  const T t = tz[z];
+  const T s0 = scale[(long long)b * n_planes + j], s1 = scale[(long long)b * n_planes + j + 1];
+  const T* r0 = raw + ((long long)b * n_planes + j) * npix;
+  const T* r1 = r0 + npix;
+  T* o = vol + ((long long)b * nz + z) * npix;
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < npix; e += st) {
+    const T a = s0 * r0[e], c = s1 * r1[e];
+    o[e] = a + t * (c - a);
+  }
+}
+
+template <typename T>
+hipError_t volume_lerp(const T* raw, int n_masks, int n_planes, long long npix, int nz,
+                       const int* jz, const T* tz, const T* scale, T* vol, hipStream_t st) {
+  if (n_masks <= 0 || nz <= 0 || npix <= 0) return hipSuccess;
+  long long bx = (npix + 4LL * 256 - 1) / (4LL * 256);
+  if (bx > 1024) bx = 1024;
+  hipLaunchKernelGGL(k_volume_lerp<T>, dim3((unsigned)bx, nz, n_masks), dim3(256), 0, st, raw,
+                     n_planes, npix, nz, jz, tz, scale, vol);
+  return hipGetLastError();
+}
+
 #define LB_OBJ_INST(T)                                                                          \
   template hipError_t resist_objective<T>(const T*, const T*, int, long long, T, T, T, T, T*, T*, \
                                           T*, T*, int*, double*, int*, hipStream_t);           \
@@ -332,7 +367,9 @@ hipError_t ilt_schedule(double* s, double growth, double b1, double b2, hipStrea
   template hipError_t mask_penalties<T>(const T*, int, int, int, T, T, T*, T*, T*, double*, int*, \
                                         hipStream_t);                                           \
   template hipError_t adam_step<T>(T*, T*, T*, const T*, long long, T, T, T, T, T, T, T, T,      \
-                                   const double*, hipStream_t);
+                                   const double*, hipStream_t);                                  \
+  template hipError_t volume_lerp<T>(const T*, int, int, long long, int, const int*, const T*,   \
+                                     const T*, T*, hipStream_t);
 LB_OBJ_INST(float)
 LB_OBJ_INST(double)
 #undef LB_OBJ_INST

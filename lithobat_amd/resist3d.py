@@ -3,14 +3,15 @@
 ``exposure_volume`` images ``grid.n_z_slices`` optical planes through the film in
 one batched launch (the planes share the mask spectrum and every source point's
 transfer geometry; only the defocus phase and the image index differ) and
-interpolates them linearly onto the voxel grid on the device.
+interpolates them linearly onto the voxel grid on the device, each plane's image
+normalisation folded into the same pass (lb_volume_interp).
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from .expose import compute_aerial_planes
+from . import _lib
 
 
 def focus_reference_depth(resist) -> float:
@@ -61,17 +62,38 @@ def exposure_volume(mask, optics, grid, resist, dose: float = 1.0, *,
     NumPy masks return float64 NumPy arrays computed in FP64 on the device; a torch
     mask on the device returns device tensors in its precision.
     """
-    from .expose import _to_device
+    from .expose import _raw
     planes, depths, nz = volume_planes(optics, grid, resist)
-    m, was_np, precision = _to_device(mask, precision)
-    st = compute_aerial_planes(m, optics, grid, planes, dose=dose, precision=precision)
-    j, t = _interp_weights(depths, nz, grid.dz)
-    jj = torch.as_tensor(j, device=st.device)
-    tt = torch.as_tensor(t, dtype=st.dtype, device=st.device).view(-1, 1, 1)
-    lo = st.index_select(0, jj)
-    hi = st.index_select(0, jj + 1)
-    vol = lo + tt * (hi - lo)
+    imgs, peak, plan, was_np, single = _raw(mask, optics, grid, planes, precision)
+    B, P, n = imgs.shape[0], imgs.shape[1], imgs.shape[-1]
+    dt, dev = imgs.dtype, imgs.device
+    # each plane's image normalisation (aerial_image.py:217-241), as compute_aerial_planes
+    mode = optics.normalisation.lower()
+    if mode not in ("peak", "clear", "none"):
+        raise ValueError(f"normalisation must be 'peak', 'clear' or 'none', got "
+                         f"'{optics.normalisation}'")
+    # voxel weights and fixed plane scales, kept on the plan (no host copies per call)
+    cache = plan.__dict__.setdefault("_volume_tables", {})
+    key = (nz, float(grid.dz), tuple(float(d) for d in depths), mode, float(dose), str(dev))
+    if key not in cache:
+        j, t = _interp_weights(depths, nz, grid.dz)
+        fixed = None
+        if mode != "peak":
+            fixed = torch.tensor([dose / c if (mode == "clear" and c > 0) else dose
+                                  for c in plan.clear], dtype=dt, device=dev)
+        cache[key] = (torch.as_tensor(j, dtype=torch.int32, device=dev),
+                      torch.as_tensor(t, dtype=dt, device=dev), fixed)
+    jj, tt, fixed = cache[key]
+    if mode == "peak":
+        scale = torch.where(peak > 0, dose / peak, torch.full_like(peak, dose)).contiguous()
+    else:
+        scale = fixed.expand(B, P).contiguous()
+    vol = torch.empty((B, nz, n, n), dtype=dt, device=dev)
+    _lib.check(_lib.load().lb_volume_interp(
+        imgs.data_ptr(), B, P, n * n, nz, jj.data_ptr(), tt.data_ptr(), scale.data_ptr(),
+        vol.data_ptr(), 64 if dt == torch.float64 else 32,
+        torch.cuda.current_stream(dev).cuda_stream))
     z = (np.arange(nz) + 0.5) * grid.dz
     if was_np:
-        return vol.double().cpu().numpy(), np.asarray(z, dtype=np.float64)
-    return vol, torch.as_tensor(z, dtype=st.dtype, device=st.device)
+        return vol[0].double().cpu().numpy(), np.asarray(z, dtype=np.float64)
+    return (vol[0] if single else vol), torch.as_tensor(z, dtype=dt, device=dev)
