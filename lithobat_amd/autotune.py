@@ -68,22 +68,39 @@ def shapes(plan, adjoint: bool) -> list[tuple[int, int]]:
     return [(buf[2 * k], buf[2 * k + 1]) for k in range(min(n.value, 32))]
 
 
-def set_shape(plan, adjoint: bool, nt: int, cs: int, chunk: int) -> None:
+def set_shape(plan, adjoint: bool, nt: int, cs: int, chunk: int, pin: bool = True) -> None:
+    """Select a kernel shape; ``pin`` keeps the plan's automatic per-batch-size selection
+    from replacing it (``unpin`` hands the choice back)."""
+    _set(plan, adjoint, nt, cs, chunk)
+    if pin:
+        plan._pinned = True
+
+
+def unpin(plan) -> None:
+    plan._pinned = False
+    plan._shape_for = None
+
+
+def _set(plan, adjoint: bool, nt: int, cs: int, chunk: int) -> None:
     _lib.check(_lib.load().lb_plan_set_shape(plan._h, 1 if adjoint else 0, nt, cs, chunk))
 
 
 def apply(plan, n_masks: int) -> bool:
-    """Select the tuned shapes for this batch size if the table has them."""
+    """Select the tuned shapes for this batch size, or the plan's defaults (own kernel
+    shape, automatic work items) where the table has none."""
     table = load_table()
     hit = False
     for adjoint in (False, True):
         e = table.get(key_of(plan, n_masks, adjoint))
+        done = False
         if e:
             try:
-                set_shape(plan, adjoint, e["threads"], e["splits"], e["chunk"])
-                hit = True
+                _set(plan, adjoint, e["threads"], e["splits"], e["chunk"])
+                hit = done = True
             except (ValueError, NotImplementedError):
                 pass
+        if not done:
+            _set(plan, adjoint, 0, 0, 0)
     return hit
 
 
@@ -108,7 +125,8 @@ def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
     (the saved table keeps the winners only).
     """
     n_masks = masks.shape[0] if masks.dim() == 3 else 1
-    plan._tuned_for.add(n_masks)  # keep the table from overriding the candidates
+    was_pinned = plan._pinned
+    plan._pinned = True  # keep the table from overriding the candidates
     P = len(plan.planes)
     results = {}
     spp = plan.info["n_systems"]
@@ -138,6 +156,9 @@ def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
         with _lock:
             table.update(results)
         save_table(table)
+    # the winners stay selected for this batch size; another size selects its own
+    plan._pinned = was_pinned
+    plan._shape_for = n_masks
     if candidates:
         for k, v in timings.items():
             results[k] = dict(results[k], candidates=v)

@@ -112,20 +112,19 @@ bool field_size(int m) {
 // Coherent-system kernel shape for (M, W): the first of field_configs() (the
 // staged shapes on M = 256, else no column split, then the smallest split)
 // whose LDS fits; the adjoint takes single-launch shapes only.
-bool pick_field_configs(Plan& p) {
-  bool got_f = false, got_a = false;
+bool default_field_config(const Plan& p, bool adjoint, FieldConfig* out) {
   for (const FieldConfig& c : field_configs()) {
     if (c.M != p.M) continue;
-    if (!got_f && field_smem_bytes(p.precision, c, p.W, false) <= kLdsLimit) {
-      p.fwd = c;
-      got_f = true;
-    }
-    if (!got_a && field_smem_bytes(p.precision, c, p.W, true) <= kLdsLimit) {
-      p.adj = c;
-      got_a = true;
+    if (field_smem_bytes(p.precision, c, p.W, adjoint) <= kLdsLimit) {
+      *out = c;
+      return true;
     }
   }
-  return got_f && got_a;
+  return false;
+}
+
+bool pick_field_configs(Plan& p) {
+  return default_field_config(p, false, &p.fwd) && default_field_config(p, true, &p.adj);
 }
 
 lb_status set_geometry_common(Plan& p, const lb_imaging_desc* d) {
@@ -984,6 +983,14 @@ lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads
                             int32_t column_splits, int32_t chunk) {
   if (!plan) return fail(LB_ERR_ARG, "null plan");
   Plan& p = plan->p;
+  if (block_threads == 0) {  // back to the plan's own choice and automatic work items
+    FieldConfig d{};
+    if (!default_field_config(p, adjoint != 0, &d))
+      return fail(LB_ERR_UNSUPPORTED, "no default coherent-system kernel shape");
+    (adjoint ? p.adj : p.fwd) = d;
+    (adjoint ? p.chunk_adj : p.chunk_fwd) = chunk > 0 ? chunk : 0;
+    return LB_OK;
+  }
   FieldConfig fc{p.M, block_threads, column_splits, 0};
   if (!field_config_supported(p.precision, fc))
     return fail(LB_ERR_UNSUPPORTED, "no coherent-system kernel of shape M=" +

@@ -132,7 +132,10 @@ class ImagingPlan:
         self._lib = lib
         self._ws = None
         self._lock = threading.Lock()
-        self._tuned_for = set()
+        # batch size whose tuned kernel shapes are selected; _pinned: shapes set by
+        # hand (autotune.set_shape), which the automatic selection leaves alone
+        self._shape_for = None
+        self._pinned = False
         clear = np.zeros(len(self.planes))
         _lib.check(lib.lb_plan_clear(self._h, _dptr(clear, C.c_double)))
         self.clear = clear
@@ -180,11 +183,13 @@ class ImagingPlan:
         return {k: getattr(info, k) for k, _ in _lib.PlanInfo._fields_}
 
     def _use_tuning(self, n_masks: int) -> None:
-        """Select the autotuned kernel shapes recorded for this batch size, once."""
-        if n_masks not in self._tuned_for:
-            from . import autotune
-            autotune.apply(self, n_masks)
-            self._tuned_for.add(n_masks)
+        """Select the autotuned kernel shapes recorded for this batch size (the plan's
+        defaults where the table has none), whenever the batch size changes."""
+        if self._pinned or self._shape_for == n_masks:
+            return
+        from . import autotune
+        autotune.apply(self, n_masks)
+        self._shape_for = n_masks
 
     def workspace_bytes(self, n_masks: int, backward: bool = False) -> int:
         return int(self._lib.lb_workspace_bytes(self._h, int(n_masks), 1 if backward else 0))

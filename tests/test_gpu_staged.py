@@ -27,7 +27,6 @@ def _euv(**kw):
 
 def _forward(plan, masks, shape):
     from lithobat_amd import autotune
-    plan._tuned_for.add(masks.shape[0])
     autotune.set_shape(plan, False, *shape)
     assert plan.info["column_splits"] == shape[1]
     return plan.forward(masks)
@@ -92,7 +91,6 @@ def test_staged_m128_and_single_image(device):
 
 def _backward(plan, masks, g, shape):
     from lithobat_amd import autotune
-    plan._tuned_for.add(masks.shape[0])
     autotune.set_shape(plan, True, *shape)
     return plan.backward(masks, g)
 
