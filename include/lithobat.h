@@ -155,6 +155,24 @@ lb_status lb_forward(lb_plan plan, const void* masks, int32_t mask_complex, int3
 lb_status lb_backward(lb_plan plan, const void* masks, int32_t n_masks, const void* d_images,
                       void* d_masks, void* workspace, size_t workspace_bytes, void* stream);
 
+/* The same forward and adjoint from the masks' band spectra (the
+ * ThinMaskSpectra of expose/aerial_image.py's provider, restricted to the
+ * band every system's box lies in): lb_mask_spectrum computes them
+ * (spectrum: [n_masks][Wb][Wb] complex, lb_mask_spectrum_bytes), and
+ * lb_forward_spectrum / lb_backward_spectrum image from them and run the
+ * adjoint from them, so an autograd step analyses each mask once (the
+ * reference recomputes the fields too, opc/ilt.py:235-266).  Workspaces as
+ * lb_forward / lb_backward (lb_mask_spectrum: the forward's). */
+size_t lb_mask_spectrum_bytes(lb_plan plan, int32_t n_masks);
+lb_status lb_mask_spectrum(lb_plan plan, const void* masks, int32_t mask_complex,
+                           int32_t n_masks, void* spectrum, void* workspace,
+                           size_t workspace_bytes, void* stream);
+lb_status lb_forward_spectrum(lb_plan plan, const void* spectrum, int32_t n_masks, void* images,
+                              void* peak, void* workspace, size_t workspace_bytes, void* stream);
+lb_status lb_backward_spectrum(lb_plan plan, const void* spectrum, int32_t n_masks,
+                               const void* d_images, void* d_masks, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
 /* Coherent fields of each system on the full grid, complex, [n_masks][n_sys][N][N]
  * for plane 0 (SOCS: one per kernel, the `fields` of socs_fields). */
 lb_status lb_fields(lb_plan plan, const void* masks, int32_t mask_complex, int32_t n_masks,

@@ -469,14 +469,18 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
   return LB_OK;
 }
 
+// spectrum: the masks' band spectra from lb_mask_spectrum (then masks is
+// unused), or null to compute them here.
 template <typename T>
 lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* images, void* peak,
-                    char* ws, const Work& w, hipStream_t st) {
-  lb_status rc = fwd_pre<T>(p, masks, cplx, n_masks, ws, w, st);
-  if (rc != LB_OK) return rc;
+                    char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr) {
+  if (!spectrum) {
+    lb_status rc = fwd_pre<T>(p, masks, cplx, n_masks, ws, w, st);
+    if (rc != LB_OK) return rc;
+  }
   FieldConfig fc = p.fwd;
   fc.grid = grid_for(p, fc, w.n_items, false);
-  FieldArgs a = field_args(p, ws + w.Xm, w);
+  FieldArgs a = field_args(p, spectrum ? spectrum : ws + w.Xm, w);
   a.queue = (int*)(ws + w.queue);
   LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
   int e0 = -1, e1 = -1;
@@ -499,10 +503,11 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
 // gM (the exact adjoint of the lift from M x M to N x N).
 template <typename T>
 lb_status bwd_pre(Plan& p, const void* masks, int n_masks, const void* dimg, char* ws,
-                  const Work& w, const T** gM, hipStream_t st) {
+                  const Work& w, const T** gM, hipStream_t st, bool have_spectrum = false) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
-  LB_HIP(band_analysis<T>(p.N, masks, 0, (long long)p.N * p.N, 0, 1, n_masks, p.nu0m, p.Wbm,
-                          invN2, (cx<T>*)(ws + w.RTm), (cx<T>*)(ws + w.Xm), st));
+  if (!have_spectrum)
+    LB_HIP(band_analysis<T>(p.N, masks, 0, (long long)p.N * p.N, 0, 1, n_masks, p.nu0m, p.Wbm,
+                            invN2, (cx<T>*)(ws + w.RTm), (cx<T>*)(ws + w.Xm), st));
   *gM = (const T*)dimg;
   if (!p.direct) {
     const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
@@ -531,13 +536,13 @@ lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, h
 
 template <typename T>
 lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, void* dmask,
-                     char* ws, const Work& w, hipStream_t st) {
+                     char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr) {
   const T* gM = nullptr;
-  lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st);
+  lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st, spectrum != nullptr);
   if (rc != LB_OK) return rc;
   FieldConfig fc = p.adj;
   fc.grid = grid_for(p, fc, w.n_items, true);
-  FieldArgs a = field_args(p, ws + w.Xm, w);
+  FieldArgs a = field_args(p, spectrum ? spectrum : ws + w.Xm, w);
   a.queue = (int*)(ws + w.queue);
   LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
   if (fc.csplit == 0) {
@@ -1085,6 +1090,74 @@ lb_status lb_backward(lb_plan plan, const void* masks, int32_t n_masks, const vo
   return p.precision == 64
              ? backward_t<double>(p, masks, n_masks, dimg, dmask, (char*)workspace, w, st)
              : backward_t<float>(p, masks, n_masks, dimg, dmask, (char*)workspace, w, st);
+}
+
+size_t lb_mask_spectrum_bytes(lb_plan plan, int32_t n_masks) {
+  if (!plan || n_masks < 1) return 0;
+  const Plan& p = plan->p;
+  return p.csz() * (size_t)n_masks * p.Wbm * p.Wbm;
+}
+
+lb_status lb_mask_spectrum(lb_plan plan, const void* masks, int32_t cplx, int32_t n_masks,
+                           void* spectrum, void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !masks || !spectrum) return fail(LB_ERR_ARG, "null plan, masks or spectrum");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  Work w = layout(p, n_masks, false);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  if (p.precision == 64) {
+    const double invN2 = 1.0 / ((double)p.N * (double)p.N);
+    LB_HIP(band_analysis<double>(p.N, masks, cplx, (long long)p.N * p.N * (cplx ? 2 : 1), 0, 1,
+                                 n_masks, p.nu0m, p.Wbm, invN2, (cx<double>*)(ws + w.RTm),
+                                 (cx<double>*)spectrum, st));
+  } else {
+    const float invN2 = (float)(1.0 / ((double)p.N * (double)p.N));
+    LB_HIP(band_analysis<float>(p.N, masks, cplx, (long long)p.N * p.N * (cplx ? 2 : 1), 0, 1,
+                                n_masks, p.nu0m, p.Wbm, invN2, (cx<float>*)(ws + w.RTm),
+                                (cx<float>*)spectrum, st));
+  }
+  return LB_OK;
+}
+
+lb_status lb_forward_spectrum(lb_plan plan, const void* spectrum, int32_t n_masks, void* images,
+                              void* peak, void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !spectrum || !images) return fail(LB_ERR_ARG, "null plan, spectrum or images");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  Work w = layout(p, n_masks, false);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? forward_t<double>(p, nullptr, 0, n_masks, images, peak, (char*)workspace, w, st,
+                                 spectrum)
+             : forward_t<float>(p, nullptr, 0, n_masks, images, peak, (char*)workspace, w, st,
+                                spectrum);
+}
+
+lb_status lb_backward_spectrum(lb_plan plan, const void* spectrum, int32_t n_masks,
+                               const void* dimg, void* dmask, void* workspace, size_t ws_bytes,
+                               void* stream) {
+  g_err.clear();
+  if (!plan || !spectrum || !dimg || !dmask) return fail(LB_ERR_ARG, "null argument");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  Work w = layout(p, n_masks, true);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? backward_t<double>(p, nullptr, n_masks, dimg, dmask, (char*)workspace, w, st,
+                                  spectrum)
+             : backward_t<float>(p, nullptr, n_masks, dimg, dmask, (char*)workspace, w, st,
+                                 spectrum);
 }
 
 lb_status lb_socs_kernels(lb_plan plan, void* kernels, void* stream) {

@@ -25,22 +25,26 @@ __all__ = ["AbbeIntensity", "abbe_intensity", "socs_fields", "socs_gradient",
 
 
 class AbbeIntensity(torch.autograd.Function):
-    """raw[B, P, N, N] = plan(mask[B, N, N]); backward by the recomputing adjoint."""
+    """raw[B, P, N, N] = plan(mask[B, N, N]); backward by the recomputing adjoint.
+
+    The masks' band spectra are computed once and kept for the backward (a few
+    hundred KB per mask), so the step analyses each mask once."""
 
     @staticmethod
     def forward(ctx, mask: torch.Tensor, plan: ImagingPlan):
-        raw, _ = plan.forward(mask, with_peak=False)
+        spec = plan.mask_spectrum(mask)
+        raw, _ = plan.forward_spectrum(spec, with_peak=False)
         ctx.plan = plan
-        ctx.save_for_backward(mask)
+        ctx.squeeze = mask.dim() == 2
+        ctx.dtype = mask.dtype
+        ctx.save_for_backward(spec)
         return raw
 
     @staticmethod
     def backward(ctx, grad_raw: torch.Tensor):
-        (mask,) = ctx.saved_tensors
-        squeeze = mask.dim() == 2
-        g = ctx.plan.backward(mask, grad_raw.contiguous())
-        g = g.to(mask.dtype)
-        return (g[0] if squeeze else g), None
+        (spec,) = ctx.saved_tensors
+        g = ctx.plan.backward_spectrum(spec, grad_raw.contiguous()).to(ctx.dtype)
+        return (g[0] if ctx.squeeze else g), None
 
 
 def abbe_intensity(mask: torch.Tensor, plan: ImagingPlan) -> torch.Tensor:

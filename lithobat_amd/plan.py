@@ -136,6 +136,7 @@ class ImagingPlan:
         # hand (autotune.set_shape), which the automatic selection leaves alone
         self._shape_for = None
         self._pinned = False
+        self._band = None  # mask band side (set on first use; fixed for the plan)
         clear = np.zeros(len(self.planes))
         _lib.check(lib.lb_plan_clear(self._h, _dptr(clear, C.c_double)))
         self.clear = clear
@@ -223,6 +224,66 @@ class ImagingPlan:
                                             peak.data_ptr() if peak is not None else None,
                                             ws.data_ptr(), ws.numel(), st))
         return out, peak
+
+    def mask_spectrum(self, masks: torch.Tensor) -> torch.Tensor:
+        """Band spectra [B, Wb, Wb] (complex) of masks [B, N, N] or [N, N], the input of
+        ``forward_spectrum`` / ``backward_spectrum``."""
+        m, cplx = self._check_masks(masks)
+        B = m.shape[0]
+        wb = self._band_side()
+        spec = torch.empty((B, wb, wb), dtype=self.complex_dtype, device=m.device)
+        with self._lock:
+            ws = self._workspace(self.workspace_bytes(B, False), m.device)
+            st = torch.cuda.current_stream(m.device).cuda_stream
+            _lib.check(self._lib.lb_mask_spectrum(self._h, m.data_ptr(), cplx, B,
+                                                  spec.data_ptr(), ws.data_ptr(), ws.numel(),
+                                                  st))
+        return spec
+
+    def _band_side(self) -> int:
+        if self._band is None:
+            self._band = int(self.info["band"])
+            assert self._band ** 2 * (16 if self.precision == 64 else 8) == int(
+                self._lib.lb_mask_spectrum_bytes(self._h, 1))
+        return self._band
+
+    def _check_spectrum(self, spec: torch.Tensor) -> int:
+        wb = self._band_side()
+        if (spec.dtype != self.complex_dtype or spec.dim() != 3 or spec.shape[1:] != (wb, wb)
+                or not spec.is_contiguous() or spec.device.type != "cuda"):
+            raise ValueError(f"spectrum must be a contiguous {self.complex_dtype} device tensor "
+                             f"[B, {wb}, {wb}] from mask_spectrum")
+        return spec.shape[0]
+
+    def forward_spectrum(self, spec: torch.Tensor, with_peak: bool = True):
+        """``forward`` from band spectra (``mask_spectrum``): images [B, P, N, N], peaks."""
+        B = self._check_spectrum(spec)
+        P = len(self.planes)
+        out = torch.empty((B, P, self.n, self.n), dtype=self.real_dtype, device=spec.device)
+        peak = torch.empty((B, P), dtype=self.real_dtype, device=spec.device) if with_peak else None
+        with self._lock:
+            self._use_tuning(B)
+            ws = self._workspace(self.workspace_bytes(B, False), spec.device)
+            st = torch.cuda.current_stream(spec.device).cuda_stream
+            _lib.check(self._lib.lb_forward_spectrum(
+                self._h, spec.data_ptr(), B, out.data_ptr(),
+                peak.data_ptr() if peak is not None else None, ws.data_ptr(), ws.numel(), st))
+        return out, peak
+
+    def backward_spectrum(self, spec: torch.Tensor, d_images: torch.Tensor) -> torch.Tensor:
+        """``backward`` from the band spectra of real masks (``mask_spectrum``)."""
+        B = self._check_spectrum(spec)
+        P = len(self.planes)
+        g = d_images.to(self.real_dtype).reshape(B, P, self.n, self.n).contiguous()
+        out = torch.empty((B, self.n, self.n), dtype=self.real_dtype, device=spec.device)
+        with self._lock:
+            self._use_tuning(B)
+            ws = self._workspace(self.workspace_bytes(B, True), spec.device)
+            st = torch.cuda.current_stream(spec.device).cuda_stream
+            _lib.check(self._lib.lb_backward_spectrum(self._h, spec.data_ptr(), B, g.data_ptr(),
+                                                      out.data_ptr(), ws.data_ptr(), ws.numel(),
+                                                      st))
+        return out
 
     def backward(self, masks: torch.Tensor, d_images: torch.Tensor) -> torch.Tensor:
         """d/dmask of sum(d_images * I) for real masks [B, N, N]; d_images [B, P, N, N]."""
