@@ -291,6 +291,14 @@ lb_status lb_mask_sigmoid_grad(const void* grad_mask, const void* mask, int64_t 
 lb_status lb_batch_set_shape(lb_batch batch, int32_t adjoint, int32_t block_threads,
                              int32_t column_splits);
 
+/* lb_batch_backward after an lb_batch_forward of the same masks: the masks'
+ * band spectra are taken from that call's workspace (forward_workspace,
+ * untouched since) instead of recomputed, so the masks are not passed. */
+lb_status lb_batch_backward_reuse(lb_batch batch, const void* const* d_images,
+                                  void* const* d_masks, void* workspace,
+                                  size_t workspace_bytes, const void* forward_workspace,
+                                  void* stream);
+
 /* Batches of clips (no direct counterpart in the reference, which images one
  * mask per call of compute_aerial_image / runs one run_ilt per clip): plans of
  * different grid sizes and optics -- one mask each -- imaged together.  The

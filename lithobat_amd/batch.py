@@ -111,9 +111,18 @@ class ImagingBatch:
         _lib.check(self._lib.lb_batch_forward(self._h, _ptrs(masks), _ptrs(images), None,
                                               ws.data_ptr(), ws.numel(), st))
 
-    def backward_into(self, masks, d_images, d_masks, stream=None) -> None:
+    def backward_into(self, masks, d_images, d_masks, stream=None,
+                      after_forward: bool = False) -> None:
+        """``after_forward``: the last ``forward_into`` imaged these same masks (and its
+        workspace is untouched since), so their band spectra are reused."""
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        ws = self.workspace(True, masks[0].device)
+        ws = self.workspace(True, d_images[0].device)
+        if after_forward:
+            fws = self.workspace(False, d_images[0].device)
+            _lib.check(self._lib.lb_batch_backward_reuse(self._h, _ptrs(d_images),
+                                                         _ptrs(d_masks), ws.data_ptr(),
+                                                         ws.numel(), fws.data_ptr(), st))
+            return
         _lib.check(self._lib.lb_batch_backward(self._h, _ptrs(masks), _ptrs(d_images),
                                                _ptrs(d_masks), ws.data_ptr(), ws.numel(), st))
 
@@ -214,6 +223,6 @@ class ILTStep:
             objective.resist_objective_into(raw, t, s, self.level, self.steep, 1.0, None,
                                             self.costs[u], d, None, None, self.obj_ws[u])
         torch.sum(self.costs, 0, out=self.loss)
-        self.batch.backward_into(self.mask, self.d_raw, self.d_mask)
+        self.batch.backward_into(self.mask, self.d_raw, self.d_mask, after_forward=True)
         for _, m, _, _, _, dm, g, _ in self.units:
             objective.mask_sigmoid_grad(dm, m, self.beta, out=g)

@@ -1408,10 +1408,12 @@ static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* 
   return LB_OK;
 }
 
+// fws (nullable): the workspace of the lb_batch_forward call that imaged these
+// masks; their band spectra are copied from it instead of recomputed.
 template <typename T>
 static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
                                   const void* const* dimg, void* const* dmask, char* ws,
-                                  hipStream_t st) {
+                                  hipStream_t st, const char* fws = nullptr) {
   for (const BatchGroup& g : b.groups) {
     const Plan& p = b.plans[g.members[0]]->p;
     const int cnt = (int)g.members.size();
@@ -1420,7 +1422,10 @@ static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
     char* stage_m = ws + g.stage_b;
     char* stage_g = stage_m + (sizeof(T) * cnt * nn + 255) / 256 * 256;
     const void* m0 = masks[g.members[0]];
-    if (!consecutive(masks, g.members, mb)) {
+    if (fws) {
+      LB_HIP(hipMemcpyAsync(ws + g.base_b + g.wb.Xm, fws + g.base_f + g.wf.Xm,
+                            p.csz() * (size_t)cnt * p.Wbm * p.Wbm, hipMemcpyDeviceToDevice, st));
+    } else if (!consecutive(masks, g.members, mb)) {
       LB_HIP(gather(stage_m, masks, g.members, mb, st));
       m0 = stage_m;
     }
@@ -1431,7 +1436,7 @@ static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
     }
     const T* gM = nullptr;
     lb_status rc = bwd_pre<T>(b.plans[g.members[0]]->p, m0, cnt, g0, ws + g.base_b, g.wb, &gM,
-                              st);
+                              st, fws != nullptr);
     if (rc != LB_OK) return rc;
   }
   FieldConfig fc = b.adj;
@@ -1469,6 +1474,24 @@ lb_status lb_batch_forward(lb_batch batch, const void* const* masks, void* const
   return batch->precision == 64
              ? batch_forward_t<double>(*batch, masks, images, peaks, (char*)workspace, st)
              : batch_forward_t<float>(*batch, masks, images, peaks, (char*)workspace, st);
+}
+
+lb_status lb_batch_backward_reuse(lb_batch batch, const void* const* d_images,
+                                  void* const* d_masks, void* workspace, size_t ws_bytes,
+                                  const void* forward_workspace, void* stream) {
+  g_err.clear();
+  if (!batch || !d_images || !d_masks || !forward_workspace) return fail(LB_ERR_ARG, "null argument");
+  for (size_t c = 0; c < batch->plans.size(); ++c)
+    if (!d_images[c] || !d_masks[c]) return fail(LB_ERR_ARG, "null pointer");
+  if (!workspace || ws_bytes < batch->ws_b)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(batch->ws_b) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<const void*> none(batch->plans.size(), forward_workspace);  // masks unused
+  return batch->precision == 64
+             ? batch_backward_t<double>(*batch, none.data(), d_images, d_masks, (char*)workspace,
+                                        st, (const char*)forward_workspace)
+             : batch_backward_t<float>(*batch, none.data(), d_images, d_masks, (char*)workspace,
+                                       st, (const char*)forward_workspace);
 }
 
 lb_status lb_batch_backward(lb_batch batch, const void* const* masks,

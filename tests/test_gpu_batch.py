@@ -182,3 +182,25 @@ def test_grouped_clips_in_place_and_staged(device, precision):
         for (ri, rg), im, gr in zip(refs, outs, douts):
             assert _rel(im[0].cpu().numpy(), ri) <= tol_img
             assert _rel(gr.cpu().numpy(), rg) <= tol_grad
+
+
+def test_batch_backward_reuses_forward_spectra(device):
+    """The adjoint after a forward of the same masks takes their band spectra from the
+    forward's workspace: same gradients as the adjoint that analyses the masks."""
+    import torch
+    from lithobat_amd.batch import ImagingBatch
+    clips = _clips()
+    batch = ImagingBatch(clips, precision=64)
+    rng = np.random.default_rng(31)
+    masks = batch.grouped_empty(lambda p: (p.n, p.n), device)
+    for m in masks:
+        m.copy_(torch.tensor(rng.random(tuple(m.shape)), device=device))
+    sens = [torch.tensor(rng.standard_normal((1,) + tuple(m.shape)), device=device)
+            for m in masks]
+    ref = batch.backward(masks, sens)
+    imgs = batch.grouped_empty(lambda p: (len(p.planes), p.n, p.n), device)
+    batch.forward_into(masks, imgs)
+    got = batch.grouped_empty(lambda p: (p.n, p.n), device)
+    batch.backward_into(masks, sens, got, after_forward=True)
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
