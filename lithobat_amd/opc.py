@@ -345,9 +345,13 @@ def run_ilt(target, model: PrintModel, *, initial=None, max_iter: int = 120, ste
         if graph and g is None and it < max_iter:
             # capture records the iteration without running it; replays run it
             torch.cuda.current_stream().synchronize()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                outs = iteration()
+            try:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    outs = iteration()
+            except RuntimeError:  # a stack that cannot capture this step: stay eager
+                g, graph = None, False
+                torch.cuda.synchronize()
     mask = state["mask"]
     try:
         import pandas as pd
