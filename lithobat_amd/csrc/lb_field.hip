@@ -643,25 +643,23 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
           }
           __syncthreads();
         }
-        // transposed row pass over Tp, in blocks of CPR rows through Cb
+        // transposed row pass over Tp, in blocks of CPR rows: the transposed
+        // stage 2 reads its rows straight from Tp (columns outside this split
+        // are zero) and leaves its outputs in Cb in the row-pass layout
         for (int a0 = 0; a0 < W; a0 += F::CPR) {
-          for (int e = tid; e < F::CPR * M; e += NT) {
-            const int rl = e / M, x = e - rl * M;
-            const int a = a0 + rl;
-            const int xl = x - cs * F::MC;
-            cx<T> val = cx<T>{0, 0};
-            if (a < W && (CSPLIT == 1 || (xl >= 0 && xl < F::MC))) val = Tp[a * F::MC + xl];
-            Cb[rl * RowLayout<M>::LDR + rpos<M, F::R1>(x)] = val;
-          }
-          __syncthreads();
 #pragma unroll
           for (int q = 0; q < F::Q2; ++q) {
             const int t = tid + q * NT;
             if (!(F::NT2 % NT == 0 || t < F::NT2)) continue;
             const int rl = t / F::T2, j2 = t - (t / F::T2) * F::T2;
+            const int a = a0 + rl;
             cx<T> v[F::R2];
 #pragma unroll
-            for (int r = 0; r < F::R2; ++r) v[r] = Cb[rl * RowLayout<M>::LDR + r * RowLayout<M>::SP + j2];
+            for (int r = 0; r < F::R2; ++r) {
+              const int xl = j2 + r * F::R1 - cs * F::MC;
+              v[r] = (a < W && (CSPLIT == 1 || (xl >= 0 && xl < F::MC))) ? Tp[a * F::MC + xl]
+                                                                          : cx<T>{0, 0};
+            }
             dft<F::R2, -1>(v);
 #pragma unroll
             for (int r = 0; r < F::R2; ++r) {
