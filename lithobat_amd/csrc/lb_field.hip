@@ -630,7 +630,10 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
           }
           __syncthreads();
           {
-            // transposed stage 1: outputs pruned to the W box rows
+            // transposed stage 1: outputs pruned to the W box rows.  A thread
+            // reads back exactly the slots its own stage 1 wrote, and the next
+            // round's stage 1 writes those same slots, so only the last round
+            // needs a barrier (the transposed row pass reads the whole table).
             cx<T> v[F::R1];
 #pragma unroll
             for (int r = 0; r < F::R1; ++r) v[r] = Cb[c1 * F::LDC + jc1 * F::R1 + r];
@@ -641,7 +644,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
               if (r < F::NRW && a < W) Tp[a * F::MC + xl] = v[r];
             }
           }
-          __syncthreads();
+          if (rd + 1 == F::ROUNDS) __syncthreads();
         }
         // transposed row pass over Tp, in blocks of CPR rows: the transposed
         // stage 2 reads its rows straight from Tp (columns outside this split
