@@ -97,35 +97,57 @@ struct RowLayout;
 template <>
 struct RowLayout<16> {
   static constexpr int SP = 5, LDR = 20;
+  static constexpr bool SWZ = false;
 };
 template <>
 struct RowLayout<32> {
   static constexpr int SP = 9, LDR = 36;
+  static constexpr bool SWZ = false;
 };
 template <>
 struct RowLayout<48> {
   static constexpr int SP = 13, LDR = 60;
+  static constexpr bool SWZ = false;
 };
 template <>
 struct RowLayout<64> {
   static constexpr int SP = 9, LDR = 72;
+  static constexpr bool SWZ = false;
 };
 template <>
 struct RowLayout<80> {
   static constexpr int SP = 10, LDR = 81;
+  static constexpr bool SWZ = false;
 };
 template <>
 struct RowLayout<96> {
   static constexpr int SP = 13, LDR = 108;
+  static constexpr bool SWZ = true;
 };
 template <>
 struct RowLayout<128> {
   static constexpr int SP = 17, LDR = 136;
+  static constexpr bool SWZ = false;
 };
 template <>
 struct RowLayout<256> {
   static constexpr int SP = 17, LDR = 272;
+  static constexpr bool SWZ = false;
 };
+
+// Row taken by thread group g = tid / T1 in the stage-1 mapping (T1 threads
+// per row).  The two rows that share a 16-lane half of a 64-bit LDS access
+// write {a LDR + j SP} for j < T1; with SWZ those rows are two apart rather
+// than adjacent (bits 0 and 1 of g swapped), which at M = 96 (LDR = 108,
+// SP = 13, T1 = 8) puts their writes on disjoint banks where adjacent rows
+// overlap in half of them.
+template <int M>
+__device__ __forceinline__ int stage1_row(int g) {
+  if constexpr (RowLayout<M>::SWZ)
+    return (g & ~3) | ((g & 1) << 1) | ((g >> 1) & 1);
+  else
+    return g;
+}
 
 template <int M, int R1>
 __device__ __forceinline__ int rpos(int x) {
@@ -146,7 +168,7 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
                                          const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs, int tid) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   using RL = RowLayout<M>;
-  const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
+  const int rl1 = stage1_row<M>(tid / F::T1), j1 = tid - (tid / F::T1) * F::T1;
   for (int a0 = 0; a0 < W; a0 += F::CPR) {
     {
       const int a = a0 + rl1;
@@ -226,6 +248,14 @@ struct RowPre {
                                     : ROW_ROWS_FWD * RowLayout<M>::LDR;
   static constexpr int CB_ADJ = F::LDC > RowLayout<M>::LDR ? F::CPR * F::LDC
                                                            : F::CPR * RowLayout<M>::LDR;
+  // TWR: the forward row pass's stage-2 twiddles come from an (R2 - 1) x R1
+  // table by (r, j2) rather than from the roots table at r * j2: the R1 tasks
+  // of a row read R1 distinct entries there, while r * j2 puts several of
+  // them on one bank (up to 8 at M = 128).  On at M = 96 and 128, where the
+  // table costs no resident block.
+  static constexpr bool TWR = ON && (M == 96 || M == 128);
+  static_assert(!RowLayout<M>::SWZ || F::CPR % 4 == 0, "the row swizzle permutes rows in fours");
+  static constexpr int TWR_N = TWR ? (F::R2 - 1) * F::R1 : 0;
   cx<T> x[NCH][NR], h[NCH][NR];
 
   __device__ __forceinline__ void fetch(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
@@ -258,11 +288,11 @@ struct RowPre {
 template <bool ALL, typename T, int M, int NT, int CSPLIT>
 __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre, int W,
                                              const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs,
-                                             int tid) {
+                                             int tid, const cx<T>* twr = nullptr) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   using P = RowPre<T, M, NT, CSPLIT>;
   using RL = RowLayout<M>;
-  const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
+  const int rl1 = stage1_row<M>(tid / F::T1), j1 = tid - (tid / F::T1) * F::T1;
   if constexpr (!ALL) {
 #pragma unroll
     for (int c = 0; c < P::NCH; ++c) {
@@ -324,7 +354,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
 #pragma unroll
       for (int r = 0; r < F::R2; ++r) {
         v[r] = S[a * RL::LDR + r * RL::SP + j2];
-        if (r > 0) v[r] = v[r] * tw[r * j2];
+        if (r > 0) v[r] = v[r] * (P::TWR ? twr[(r - 1) * F::R1 + j2] : tw[r * j2]);
       }
       dft<F::R2, +1>(v);
 #pragma unroll
@@ -432,12 +462,15 @@ __device__ __forceinline__ int next_item(int* queue, int item, int* s_next) {
 template <typename T, int M, int NT, int CSPLIT, typename Src>
 __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restrict__ twg) {
   using F = FieldShape<T, M, NT, CSPLIT>;
+  using Pre = RowPre<T, M, NT, CSPLIT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
-  cx<T>* Cb = tw + M;                              // CB_ROWS_FWD x LDC (row scratch too)
-  cx<T>* Tb = Cb + RowPre<T, M, NT, CSPLIT>::CB_FWD;  // W x MC
+  cx<T>* twr = tw + M;                             // TWR_N: row stage-2 twiddles by (r, j2)
+  cx<T>* Cb = twr + Pre::TWR_N;                    // CB_ROWS_FWD x LDC (row scratch too)
+  cx<T>* Tb = Cb + Pre::CB_FWD;                    // W x MC
   const int tid = threadIdx.x;
   load_roots(tw, twg, M, tid, NT);
+  for (int i = tid; i < Pre::TWR_N; i += NT) twr[i] = twg[(i / F::R1 + 1) * (i % F::R1)];
   __syncthreads();
   const int c1 = tid % F::CPR, jc1 = tid / F::CPR;  // column stage-1 mapping
 
@@ -450,9 +483,8 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
     const cx<T>* H = iv.H;
     const int2* box = iv.box;
     const T* wsys = iv.wsys;
-    using Pre = RowPre<T, M, NT, CSPLIT>;
     Pre pre;
-    const int rl1 = tid / F::T1, j1 = tid - (tid / F::T1) * F::T1;
+    const int rl1 = stage1_row<M>(tid / F::T1), j1 = tid - (tid / F::T1) * F::T1;
     if (Pre::ON && s0 < s1)
       pre.fetch(Xm, Wb, wrapN, H + (long long)s0 * W * W, W, box[s0], rl1, j1);
     for (int cs = 0; cs < CSPLIT; ++cs) {
@@ -462,7 +494,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
       for (int j = s0; j < s1; ++j) {
         const T w = wsys[j];
         if constexpr (Pre::ON) {
-          row_pass_pre<true>(pre, W, tw, Cb, Tb, cs, tid);
+          row_pass_pre<true>(pre, W, tw, Cb, Tb, cs, tid, twr);
           // the next system's inputs load while this one runs its columns
           const int jn = j + 1 < s1 ? j + 1 : (cs + 1 < CSPLIT ? s0 : -1);
           if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1);
@@ -547,7 +579,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
   load_roots(tw, twg, M, tid, NT);
   __syncthreads();
   const int c1 = tid % F::CPR, jc1 = tid / F::CPR;
-  const int rl1 = tid / F::T1, j1r = tid - (tid / F::T1) * F::T1;
+  const int rl1 = stage1_row<M>(tid / F::T1), j1r = tid - (tid / F::T1) * F::T1;
 
   const int n_items = src.count();
   __shared__ int s_next;
@@ -1102,7 +1134,7 @@ template <typename T, int M, int NT, int CS>
 static size_t smem_for(int W, bool adjoint) {
   using F = FieldShape<T, M, NT, CS>;
   using P = RowPre<T, M, NT, CS>;
-  const size_t cb = adjoint ? P::CB_ADJ : P::CB_FWD;
+  const size_t cb = adjoint ? P::CB_ADJ : P::CB_FWD + P::TWR_N;
   size_t n = M + cb + (size_t)W * F::MC;  // the adjoint's Tp aliases Tb
   return n * sizeof(cx<T>);
 }
