@@ -84,7 +84,31 @@ struct FieldShape {
   static_assert(T2 == R1, "two-stage plan");
   static_assert(CPR * T1 == NT, "stage 1 maps one task per thread");
   static_assert(CPR <= MC && MC % CPR == 0, "rounds tile the split");
+  // TWR: the row passes' stage-2 twiddles e^{2 pi i r j2 / M} come from an
+  // (R2 - 1) x R1 table by (r, j2) rather than from the roots table at r j2:
+  // the R1 tasks of a row read R1 distinct entries there, while r j2 puts
+  // several of them on one bank (up to 8 at M = 128).  On at M = 96 and 128,
+  // whose kernels hold two resident blocks at most (registers), which the
+  // table's LDS does not reduce.
+  static constexpr bool TWR = M == 96 || M == 128;
+  static constexpr int TWR_N = TWR ? (R2 - 1) * R1 : 0;
 };
+
+// The by-(r, j2) table of a shape with TWR, filled from the global roots.
+template <typename F, typename T>
+__device__ __forceinline__ void load_row_tw(cx<T>* twr, const cx<T>* __restrict__ twg, int tid,
+                                            int nt) {
+  for (int i = tid; i < F::TWR_N; i += nt) twr[i] = twg[(i / F::R1 + 1) * (i % F::R1)];
+}
+
+// Row-pass stage-2 twiddle for r >= 1, j2 < R1.
+template <typename F, typename T>
+__device__ __forceinline__ cx<T> row_tw(const cx<T>* tw, const cx<T>* twr, int r, int j2) {
+  if constexpr (F::TWR)
+    return twr[(r - 1) * F::R1 + j2];
+  else
+    return tw[r * j2];
+}
 
 // Row-pass scratch layout per coarse grid: stage-1 task j's R1 outputs at
 // j * SP, rows LDR apart (SP >= R1, LDR >= T1 * SP), chosen so that a wave's
@@ -165,7 +189,8 @@ __device__ __forceinline__ int wrap_idx(int i, int n) {
 template <typename T, int M, int NT, int CSPLIT>
 __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
                                          const cx<T>* __restrict__ Hj, int W, int2 bx,
-                                         const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs, int tid) {
+                                         const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs, int tid,
+                                         const cx<T>* twr) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   using RL = RowLayout<M>;
   const int rl1 = stage1_row<M>(tid / F::T1), j1 = tid - (tid / F::T1) * F::T1;
@@ -201,7 +226,7 @@ __device__ __forceinline__ void row_pass(const cx<T>* __restrict__ Xm, int Wb, i
 #pragma unroll
         for (int r = 0; r < F::R2; ++r) {
           v[r] = S[rl * RL::LDR + r * RL::SP + j2];
-          if (r > 0) v[r] = v[r] * tw[r * j2];
+          if (r > 0) v[r] = v[r] * row_tw<F>(tw, twr, r, j2);
         }
         dft<F::R2, +1>(v);
 #pragma unroll
@@ -248,14 +273,7 @@ struct RowPre {
                                     : ROW_ROWS_FWD * RowLayout<M>::LDR;
   static constexpr int CB_ADJ = F::LDC > RowLayout<M>::LDR ? F::CPR * F::LDC
                                                            : F::CPR * RowLayout<M>::LDR;
-  // TWR: the forward row pass's stage-2 twiddles come from an (R2 - 1) x R1
-  // table by (r, j2) rather than from the roots table at r * j2: the R1 tasks
-  // of a row read R1 distinct entries there, while r * j2 puts several of
-  // them on one bank (up to 8 at M = 128).  On at M = 96 and 128, where the
-  // table costs no resident block.
-  static constexpr bool TWR = ON && (M == 96 || M == 128);
   static_assert(!RowLayout<M>::SWZ || F::CPR % 4 == 0, "the row swizzle permutes rows in fours");
-  static constexpr int TWR_N = TWR ? (F::R2 - 1) * F::R1 : 0;
   cx<T> x[NCH][NR], h[NCH][NR];
 
   __device__ __forceinline__ void fetch(const cx<T>* __restrict__ Xm, int Wb, int wrapN,
@@ -288,7 +306,7 @@ struct RowPre {
 template <bool ALL, typename T, int M, int NT, int CSPLIT>
 __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre, int W,
                                              const cx<T>* tw, cx<T>* S, cx<T>* Tb, int cs,
-                                             int tid, const cx<T>* twr = nullptr) {
+                                             int tid, const cx<T>* twr) {
   using F = FieldShape<T, M, NT, CSPLIT>;
   using P = RowPre<T, M, NT, CSPLIT>;
   using RL = RowLayout<M>;
@@ -317,7 +335,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
 #pragma unroll
           for (int r = 0; r < F::R2; ++r) {
             v[r] = S[rl * RL::LDR + r * RL::SP + j2];
-            if (r > 0) v[r] = v[r] * tw[r * j2];
+            if (r > 0) v[r] = v[r] * row_tw<F>(tw, twr, r, j2);
           }
           dft<F::R2, +1>(v);
 #pragma unroll
@@ -354,7 +372,7 @@ __device__ __forceinline__ void row_pass_pre(const RowPre<T, M, NT, CSPLIT>& pre
 #pragma unroll
       for (int r = 0; r < F::R2; ++r) {
         v[r] = S[a * RL::LDR + r * RL::SP + j2];
-        if (r > 0) v[r] = v[r] * (P::TWR ? twr[(r - 1) * F::R1 + j2] : tw[r * j2]);
+        if (r > 0) v[r] = v[r] * row_tw<F>(tw, twr, r, j2);
       }
       dft<F::R2, +1>(v);
 #pragma unroll
@@ -466,11 +484,11 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
   cx<T>* twr = tw + M;                             // TWR_N: row stage-2 twiddles by (r, j2)
-  cx<T>* Cb = twr + Pre::TWR_N;                    // CB_ROWS_FWD x LDC (row scratch too)
+  cx<T>* Cb = twr + F::TWR_N;                      // CB_ROWS_FWD x LDC (row scratch too)
   cx<T>* Tb = Cb + Pre::CB_FWD;                    // W x MC
   const int tid = threadIdx.x;
   load_roots(tw, twg, M, tid, NT);
-  for (int i = tid; i < Pre::TWR_N; i += NT) twr[i] = twg[(i / F::R1 + 1) * (i % F::R1)];
+  load_row_tw<F>(twr, twg, tid, NT);
   __syncthreads();
   const int c1 = tid % F::CPR, jc1 = tid / F::CPR;  // column stage-1 mapping
 
@@ -500,7 +518,7 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
           if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1);
         } else {
           row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, H + (long long)j * W * W, W, box[j], tw, Cb,
-                                     Tb, cs, tid);
+                                     Tb, cs, tid, twr);
         }
         // Column rounds.  With Pre::DB round rd uses half (rd & 1) of Cb: a
         // round's stage 1 may overwrite the other half while slower threads
@@ -567,7 +585,8 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
   using F = FieldShape<T, M, NT, CSPLIT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // M
-  cx<T>* Cb = tw + M;                              // CPR rows (column or row layout)
+  cx<T>* twr = tw + M;                             // TWR_N: row stage-2 twiddles by (r, j2)
+  cx<T>* Cb = twr + F::TWR_N;                      // CPR rows (column or row layout)
   cx<T>* Tb = Cb + RowPre<T, M, NT, CSPLIT>::CB_ADJ;  // W x MC
   // The transposed column stage writes its box rows over the row table: round
   // rd's stage 1 has read Tb at exactly those (row, column) places two
@@ -577,6 +596,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
   (void)Wmax;  // the table extent is the launch's LDS size
   const int tid = threadIdx.x;
   load_roots(tw, twg, M, tid, NT);
+  load_row_tw<F>(twr, twg, tid, NT);
   __syncthreads();
   const int c1 = tid % F::CPR, jc1 = tid / F::CPR;
   const int rl1 = stage1_row<M>(tid / F::T1), j1r = tid - (tid / F::T1) * F::T1;
@@ -617,11 +637,11 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
         const T w2 = (T)2 * wsys[j];
         const cx<T>* Hj = H + (long long)j * W * W;
         if constexpr (Pre::ON_ADJ) {
-          row_pass_pre<false>(pre, W, tw, Cb, Tb, cs, tid);
+          row_pass_pre<false>(pre, W, tw, Cb, Tb, cs, tid, twr);
           const int jn = j + 1 < s1 ? j + 1 : (cs + 1 < CSPLIT ? s0 : -1);
           if (jn >= 0) pre.fetch(Xm, Wb, wrapN, H + (long long)jn * W * W, W, box[jn], rl1, j1r);
         } else {
-          row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, Hj, W, bx, tw, Cb, Tb, cs, tid);
+          row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, Hj, W, bx, tw, Cb, Tb, cs, tid, twr);
         }
 #pragma unroll
         for (int rd = 0; rd < F::ROUNDS; ++rd) {
@@ -698,7 +718,7 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
             dft<F::R2, -1>(v);
 #pragma unroll
             for (int r = 0; r < F::R2; ++r) {
-              if (r > 0) v[r] = mulc(v[r], tw[r * j2]);
+              if (r > 0) v[r] = mulc(v[r], row_tw<F>(tw, twr, r, j2));
               Cb[rl * RowLayout<M>::LDR + r * RowLayout<M>::SP + j2] = v[r];
             }
           }
@@ -1134,7 +1154,7 @@ template <typename T, int M, int NT, int CS>
 static size_t smem_for(int W, bool adjoint) {
   using F = FieldShape<T, M, NT, CS>;
   using P = RowPre<T, M, NT, CS>;
-  const size_t cb = adjoint ? P::CB_ADJ : P::CB_FWD + P::TWR_N;
+  const size_t cb = (adjoint ? P::CB_ADJ : P::CB_FWD) + F::TWR_N;
   size_t n = M + cb + (size_t)W * F::MC;  // the adjoint's Tp aliases Tb
   return n * sizeof(cx<T>);
 }
