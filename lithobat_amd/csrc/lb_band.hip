@@ -810,9 +810,14 @@ static hipError_t set_smem(K kern, size_t bytes) {
 
 // Persistent row kernels: units per wavefront, and the wavefront count
 // (about 8 resident per SIMD over 256 CUs) below which fewer units per
-// wavefront are used.
+// wavefront are used.  The synthesis rows take more units per wavefront:
+// each block loads the roots table and the register twiddles once, and a
+// longer loop keeps the next unit's band values in flight (a 320-image batch
+// at 1024^2: 349 us with 2 units per wavefront, 276 us with 8).
 constexpr int kUnitsPerWave = 4;
 constexpr int kRowWavesTarget = 8192;
+constexpr int kSynthUnitsPerWave = 8;
+constexpr int kSynthWavesTarget = 1024;
 
 static unsigned blocks_for(int lines, int per_block) {
   return (unsigned)((lines + per_block - 1) / per_block);
@@ -889,8 +894,9 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
       // the batch is too small to fill the device
       const int units = ((cout ? N : N / 2) + tpw<N>() - 1) / tpw<N>();
       const int nw = row_waves<T, N>();
-      int upw = (int)std::min<long long>(kUnitsPerWave, std::max<long long>(
-                                             1, (long long)units * nimg / (nw * kRowWavesTarget)));
+      int upw = (int)std::min<long long>(
+          kSynthUnitsPerWave,
+          std::max<long long>(1, (long long)units * nimg / (nw * kSynthWavesTarget)));
       const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
       auto k2 = cout ? k_synth_rows_sp<T, N, true> : k_synth_rows_sp<T, N, false>;
       if ((e = set_smem(k2, sm)) != hipSuccess) return e;
