@@ -196,3 +196,27 @@ def test_plain_fp64_lines_and_spaces_488_points(device):
     assert img.dtype == np.float64 and img.shape == (256, 256)
     assert _rel(img, ref) <= 1e-12, _rel(img, ref)
 
+
+
+_SIDE_REF = {}
+
+
+@pytest.mark.parametrize("n", [48, 80, 96, 192, 384, 768, 1536, 2048])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_every_grid_family_matches_oracle(device, n, precision):
+    """Grid sides 3 * 2^k and 80 (radix-3 and radix-5 band plans, the 1536-point one in four
+    stages) and the largest power of two, 4 nm pixels, NA 0.85 annular on a 7-point source
+    grid: the band transforms' block shapes change with the side and the precision."""
+    from lithobat_amd import compute_aerial_image
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.masks import manhattan
+    grid = GridConfig(n_pixels=n, pixel_size=4e-9)
+    optics = OpticsConfig(NA=0.85, source_type="annular", sigma_outer=0.8, sigma_inner=0.5,
+                          source_grid=7, defocus=30e-9, normalisation="none")
+    if n not in _SIDE_REF:
+        mask = manhattan(n, 4e-9, seed=n, n_rect=(max(2, n // 64), max(4, n // 32)))
+        _SIDE_REF[n] = (mask, O.compute_aerial_image(mask, optics, grid, return_raw=True)[0])
+    mask, ref = _SIDE_REF[n]
+    raw = compute_aerial_image(mask, optics, grid, return_raw=True, precision=precision)[0]
+    tol = 1e-12 if precision == 64 else 1e-5
+    assert _rel(raw, ref) <= tol, (n, precision, _rel(raw, ref))
