@@ -87,9 +87,12 @@ struct FieldShape {
   // TWR: the row passes' stage-2 twiddles e^{2 pi i r j2 / M} come from an
   // (R2 - 1) x R1 table by (r, j2) rather than from the roots table at r j2:
   // the R1 tasks of a row read R1 distinct entries there, while r j2 puts
-  // several of them on one bank (up to 8 at M = 128).  On at M = 96 and 128,
-  // whose kernels hold two resident blocks at most (registers), which the
-  // table's LDS does not reduce.
+  // several of them on one bank (up to 8 at M = 128).  On at M = 96 and 128.
+  // The table (672 / 896 B in FP32) leaves the resident block count of the
+  // 96 x 256-thread and 128 x 512-thread kernels unchanged at every box
+  // width; a few other shapes lose a block at some widths (128 x 256 threads
+  // x 4 splits at W = 41-43, 96 x 128 threads at W = 34, several FP64 ones),
+  // which the autotuner's timings include.
   static constexpr bool TWR = M == 96 || M == 128;
   static constexpr int TWR_N = TWR ? (R2 - 1) * R1 : 0;
 };
