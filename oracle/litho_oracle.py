@@ -25,7 +25,10 @@ Pinning (what checks each function independently of this file):
 * ``socs_build`` / ``socs_raw``: full rank == Abbe (1e-12), eigen-reconstruction
   of the TCC, golden eigenvalues from the imported reference.
 * ``socs_gradient``, ``abbe_gradient``, ``objective``: central finite
-  differences.
+  differences, and the reference's own outputs (tests/golden/ilt32.npz) for
+  ``objective``, ``socs_gradient``, ``mask_of`` / ``grad_theta``,
+  ``dose_for_target``, ``run_ilt`` (history, mask, theta, print) and
+  ``exposure_volume`` (every focus reference).
 """
 from __future__ import annotations
 
@@ -736,6 +739,19 @@ def mask_penalties(mask, weight_tv=0.0, weight_discrete=0.0):
     return cost, grad
 
 
+def objective(k: "SOCS", mask, target, scale, level, steepness, sign=1.0, sigma_px=0.0,
+              weight_tv=0.0, weight_discrete=0.0):
+    """ilt.py:292-360 (_Objective.__call__): the relaxed-resist cost of the SOCS image of
+    ``mask`` with its gradient through socs_gradient, plus the TV / discreteness penalties.
+    Returns (cost, grad, signed, pattern_error)."""
+    fields, raw = socs_fields(k, mask)
+    cost, d_raw, signed, perr = relaxed_objective(raw, target, scale, level, steepness, sign,
+                                                  sigma_px)
+    grad = socs_gradient(k, fields, d_raw)
+    c2, g2 = mask_penalties(mask, weight_tv, weight_discrete)
+    return cost + c2, grad + g2, signed, perr
+
+
 def mask_of(theta, beta, smooth_px=0.0):
     """ilt.py:368-378."""
     return sigmoid_clipped(beta * blur(theta, smooth_px))
@@ -824,6 +840,16 @@ def develop_field(aerial, resist, grid):
     return apply_peb(aerial, resist.diffusion_sigma, grid.pixel_size), float(resist.threshold)
 
 
+def print_mask(mask, optics, grid, resist, dose=1.0, socs_tol=1e-8):
+    """ilt.py:624-648 through SOCS kernels: the aerial image at ``dose`` (normalised as the
+    optics say) and its threshold-model field.  Returns (aerial, field, level)."""
+    k = socs_build(optics, grid, tol=socs_tol)
+    raw, peak, clear = socs_raw(k, mask)
+    aerial = raw * normalisation_scale(optics.normalisation, peak, clear, dose)
+    fld, level = develop_field(aerial, resist, grid)
+    return aerial, fld, level
+
+
 def dose_for_target(target, optics, grid, resist, dose=1.0, tone="clear", mask=None,
                     socs_tol=1e-8):
     """ilt.py:651-721: the dose at which ``mask`` (default: the target) prints as many
@@ -881,13 +907,9 @@ def run_ilt(target, optics, grid, resist, dose=1.0, tone="clear", *, max_iter=12
     sigma_px = float(resist.diffusion_sigma / grid.pixel_size)
     sign = 1.0 if tone == "clear" else -1.0
 
-    def objective(mask):
-        fields, raw = socs_fields(k, mask)
-        cost, d_raw, signed, perr = relaxed_objective(raw, tgt, scale, level, steepness, sign,
-                                                      sigma_px)
-        grad = socs_gradient(k, fields, d_raw)
-        c2, g2 = mask_penalties(mask, weight_tv, weight_discrete)
-        return cost + c2, grad + g2, signed, perr
+    def obj(mask):
+        return objective(k, mask, tgt, scale, level, steepness, sign, sigma_px, weight_tv,
+                         weight_discrete)
 
     p = float(np.clip(initial_grey, 0.5 + 1e-3, 1.0 - 1e-6))
     m0 = 0.5 + (np.clip(tgt, 0.0, 1.0) - 0.5) * (2.0 * p - 1.0)
@@ -899,7 +921,7 @@ def run_ilt(target, optics, grid, resist, dose=1.0, tone="clear", *, max_iter=12
     b1, b2, eps = 0.9, 0.999, 1e-8
     beta_k = float(beta)
     mask = mask_of(theta, beta_k, smooth_px)
-    cost, grad_m, signed, perr = objective(mask)
+    cost, grad_m, signed, perr = obj(mask)
     rows = [{"iteration": 0, "cost": cost, "pattern_error": perr, "beta": beta_k,
              "grad_norm": float(np.linalg.norm(grad_m))}]
     prev_cost, converged, taken = np.inf, False, 0
@@ -908,7 +930,7 @@ def run_ilt(target, optics, grid, resist, dose=1.0, tone="clear", *, max_iter=12
         theta, mom1, mom2 = adam_step(theta, mom1, mom2, grad_t, it, step, b1, b2, eps)
         beta_k *= growth
         mask = mask_of(theta, beta_k, smooth_px)
-        cost, grad_m, signed, perr = objective(mask)
+        cost, grad_m, signed, perr = obj(mask)
         taken = it
         rows.append({"iteration": it, "cost": cost, "pattern_error": perr, "beta": beta_k,
                      "grad_norm": float(np.linalg.norm(grad_m))})

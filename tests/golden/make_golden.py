@@ -221,8 +221,157 @@ def main_multilayer():
     print("multilayer golden vectors written to", HERE)
 
 
+def main_ilt():
+    """The ILT objective family, written to ilt32.npz by ``make_golden.py ilt``:
+    ``_Objective.__call__`` (opc/ilt.py:292-360) for both signs, with and without the
+    PEB blur and the penalties; ``_mask_of`` / ``_grad_theta`` (:368-389) with and
+    without smoothing; two short ``run_ilt`` solves (:397-621, one per tone);
+    ``dose_for_target`` (:651-721) for both tones and an explicit mask; and
+    ``exposure_volume`` (develop/resist3d.py:113-185) for every focus reference."""
+    sys.path.insert(0, REF_SRC)
+    from litho_sim.core.config import GridConfig, OpticsConfig, ResistConfig
+    from litho_sim.develop.resist3d import exposure_volume
+    from litho_sim.expose import SOCSKernels
+    from litho_sim.expose.aerial_image import normalisation_scale
+    from litho_sim.opc import ilt as I
+    from litho_sim.opc.model import PrintModel
+
+    meta = {"numpy": np.__version__, "scipy": scipy.__version__}
+    keys = ["wavelength", "NA", "n_immersion", "defocus", "sigma_outer", "sigma_inner",
+            "source_type", "source_grid", "source_kwargs", "zernike_coeffs", "exact_defocus",
+            "n_image", "imaging_model", "polarisation", "obliquity", "normalisation"]
+
+    def optics_dict(o):
+        d = {k: getattr(o, k) for k in keys}
+        d["zernike_coeffs"] = {str(k): v for k, v in d["zernike_coeffs"].items()}
+        return d
+
+    n, px = 32, 8e-9
+    grid = GridConfig(n_pixels=n, pixel_size=px)
+    optics = OpticsConfig(NA=1.35, n_immersion=1.44, source_type="quadrupole",
+                          sigma_outer=0.85, sigma_inner=0.55, source_grid=21,
+                          normalisation="clear")
+    arrs = {"n": n, "pixel_size": px, "optics": np.array(json.dumps(optics_dict(optics)))}
+
+    # --- the objective on a sigmoid(8 theta) mask ------------------------------
+    k = SOCSKernels.build(optics, grid, tol=1e-8)
+    rng = np.random.default_rng(3000)
+    theta = rng.standard_normal((n, n))
+    mask = I._mask_of(theta, 8.0, 0.0)
+    target = manhattan(n, px, seed=3001, n_rect=(3, 6))
+    dose = 1.0
+    scale = normalisation_scale("clear", 1.0, k.clear, dose)
+    arrs.update(obj_theta=theta, obj_mask=mask, obj_target=target, obj_scale=scale,
+                obj_socs_tol=1e-8, obj_clear=k.clear, obj_weights=k.weights)
+    cases = {
+        "plain": dict(sign=1.0, sigma_px=0.0, steepness=50.0, level=0.3, wtv=0.0, wd=0.0),
+        "dark_blur": dict(sign=-1.0, sigma_px=1.5, steepness=50.0, level=0.3, wtv=0.0,
+                          wd=0.0),
+        "blur_penalties": dict(sign=1.0, sigma_px=1.5, steepness=12.0, level=0.25,
+                               wtv=0.05, wd=0.02),
+    }
+    for name, c in cases.items():
+        obj = I._Objective(kernels=k, target=target, scale=scale, level=c["level"],
+                           sigma_px=c["sigma_px"], sign=c["sign"], steepness=c["steepness"],
+                           weight_tv=c["wtv"], weight_discrete=c["wd"])
+        cost, grad, signed, perr = obj(mask)
+        arrs[f"obj_{name}__params"] = np.array(json.dumps(c))
+        arrs[f"obj_{name}__cost"] = cost
+        arrs[f"obj_{name}__grad"] = grad
+        arrs[f"obj_{name}__signed"] = signed
+        arrs[f"obj_{name}__pattern_error"] = perr
+    # socs_gradient on the fields socs_fields returns, for a plain sensitivity
+    fields, inten = I.socs_fields(k, mask)
+    sens = rng.standard_normal((n, n))
+    arrs["sg_sensitivity"] = sens
+    arrs["sg_intensity"] = inten
+    arrs["sg_grad"] = I.socs_gradient(k, fields, sens)
+
+    # --- the parameterisation ---------------------------------------------------
+    gm = rng.standard_normal((n, n))
+    arrs["par_grad_mask"] = gm
+    for name, beta, smooth_px in (("b8_s0", 8.0, 0.0), ("b4_s1.3", 4.0, 1.3),
+                                  ("b12_s2.5", 12.0, 2.5)):
+        m = I._mask_of(theta, beta, smooth_px)
+        arrs[f"par_{name}__beta"] = beta
+        arrs[f"par_{name}__smooth_px"] = smooth_px
+        arrs[f"par_{name}__mask"] = m
+        arrs[f"par_{name}__grad_theta"] = I._grad_theta(gm, m, beta, smooth_px)
+
+    # --- dose_for_target and two short solves -------------------------------------
+    resist = ResistConfig(threshold=0.3, diffusion_sigma=8e-9)
+    ilt_target = manhattan(n, px, seed=3003, n_rect=(2, 4), wmin=48e-9, wmax=120e-9)
+    other = manhattan(n, px, seed=3004, n_rect=(2, 4), wmin=48e-9, wmax=120e-9)
+    arrs["ilt_target"] = ilt_target
+    arrs["dose_other_mask"] = other
+    arrs["resist_threshold"] = resist.threshold
+    arrs["resist_diffusion_sigma"] = resist.diffusion_sigma
+    for tone in ("clear", "dark"):
+        model = PrintModel(optics=optics, resist=resist, grid=grid, tone=tone)
+        arrs[f"dose_{tone}"] = I.dose_for_target(ilt_target, model)
+        arrs[f"dose_{tone}__other"] = I.dose_for_target(ilt_target, model, mask=other)
+    solves = {
+        "clear": dict(tone="clear", max_iter=5, smooth=8e-9, weight_tv=1e-3,
+                      weight_discrete=1e-3, beta=4.0, beta_final=12.0, steepness=12.0),
+        "dark": dict(tone="dark", max_iter=5, smooth=0.0, weight_tv=0.0,
+                     weight_discrete=0.0, beta=4.0, beta_final=None, steepness=4.0),
+    }
+    for name, s in solves.items():
+        # the dark-tone solve prints the complement (its features are the dark side)
+        tgt = ilt_target if s["tone"] == "clear" else 1.0 - ilt_target
+        model = PrintModel(optics=optics, resist=resist, grid=grid, tone=s["tone"])
+        model.dose = I.dose_for_target(tgt, model)
+        kw = {key: v for key, v in s.items() if key != "tone"}
+        res = I.run_ilt(tgt, model, **kw)
+        h = res.history
+        arrs[f"run_{name}__target"] = tgt
+        arrs[f"run_{name}__params"] = np.array(json.dumps(s))
+        arrs[f"run_{name}__dose"] = model.dose
+        for col in ("cost", "pattern_error", "beta", "grad_norm"):
+            arrs[f"run_{name}__{col}"] = h[col].to_numpy()
+        arrs[f"run_{name}__mask"] = res.mask
+        arrs[f"run_{name}__theta"] = res.theta
+        arrs[f"run_{name}__iterations"] = res.iterations
+        arrs[f"run_{name}__converged"] = res.converged
+        arrs[f"run_{name}__printed_aerial"] = res.printed.aerial
+        arrs[f"run_{name}__printed_field"] = res.printed.field
+
+    # --- exposure_volume for every focus reference ----------------------------------
+    vgrid = GridConfig(n_pixels=16, pixel_size=px, dz=4e-9, n_z_slices=6)
+    voptics = {
+        "none": OpticsConfig(NA=0.93, source_type="annular", sigma_outer=0.8,
+                             sigma_inner=0.5, source_grid=7, defocus=20e-9,
+                             normalisation="none"),
+        "clear": OpticsConfig(NA=1.35, n_immersion=1.44, source_type="quadrupole",
+                              sigma_outer=0.85, sigma_inner=0.55, source_grid=11,
+                              defocus=-30e-9, normalisation="clear"),
+    }
+    vmask = manhattan(16, px, seed=3005, n_rect=(2, 3), wmin=16e-9, wmax=48e-9)
+    arrs["vol_mask"] = vmask
+    arrs["vol_dz"] = vgrid.dz
+    arrs["vol_n_z_slices"] = vgrid.n_z_slices
+    vcases = []
+    for oname, o in voptics.items():
+        arrs[f"vol_optics_{oname}"] = np.array(json.dumps(optics_dict(o)))
+        for ref in ("top", "mid", "bottom"):
+            r = ResistConfig(thickness=90e-9, n_resist=1.7, focus_reference=ref)
+            vol, z = exposure_volume(vmask, o, vgrid, r, dose=1.3)
+            key = f"vol_{oname}_{ref}"
+            arrs[f"{key}__intensity"] = vol
+            arrs[f"{key}__z"] = z
+            vcases.append(key)
+    arrs["vol_thickness"] = 90e-9
+    arrs["vol_n_resist"] = 1.7
+    arrs["vol_dose"] = 1.3
+    np.savez_compressed(os.path.join(HERE, "ilt32.npz"), meta=json.dumps(meta),
+                        obj_cases=np.array(list(cases)), vol_cases=np.array(vcases), **arrs)
+    print("ILT golden vectors written to", HERE)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["multilayer"]:
         main_multilayer()
+    elif sys.argv[1:] == ["ilt"]:
+        main_ilt()
     else:
         main()
