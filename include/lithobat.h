@@ -150,6 +150,21 @@ lb_status lb_forward(lb_plan plan, const void* masks, int32_t mask_complex, int3
                      void* images, void* peak, void* workspace, size_t workspace_bytes,
                      void* stream);
 
+/* No counterpart in the reference: lb_forward for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a fixed device array of the library,
+ * LB_STATIC_SCRATCH_BYTES on each GPU it runs on (held until the process
+ * ends), instead of a caller buffer; the call allocates nothing.  Same kernels
+ * and outputs as lb_forward.  When lb_forward's workspace for the call would
+ * not fit, the work items take more coherent systems each (fewer coarse
+ * partial images; the images agree with lb_forward's to rounding), and staged
+ * plans take smaller passes; a call that does not fit even then returns
+ * LB_ERR_WORKSPACE.  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (other streams or threads) race. */
+#define LB_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+lb_status lb_forward_static(lb_plan plan, const void* masks, int32_t mask_complex,
+                            int32_t n_masks, void* images, void* peak, void* stream);
+
 /* d_masks[mask][N][N] = d/d mask of sum_{plane,x} d_images[mask][plane][x] * I(x)
  * for real masks (fields recomputed, not stored). */
 lb_status lb_backward(lb_plan plan, const void* masks, int32_t n_masks, const void* d_images,

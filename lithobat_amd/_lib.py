@@ -43,10 +43,14 @@ class PlanInfo(C.Structure):
         "block_threads", "column_splits", "chunk", "grid_blocks")]
 
 
+# LB_STATIC_SCRATCH_BYTES of the header: lb_forward_static's fixed device workspace
+STATIC_SCRATCH_BYTES = 64 << 20
+
 EXPORTS = [
     "lb_version", "lb_last_error", "lb_abbe_plan_create", "lb_socs_plan_create",
     "lb_plan_destroy", "lb_plan_info_get", "lb_plan_clear", "lb_socs_weights",
-    "lb_socs_kernels", "lb_workspace_bytes", "lb_forward", "lb_backward", "lb_fields",
+    "lb_socs_kernels", "lb_workspace_bytes", "lb_forward", "lb_forward_static", "lb_backward",
+    "lb_fields",
     "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read", "lb_plan_shapes",
     "lb_plan_set_shape", "lb_batch_create", "lb_batch_destroy", "lb_batch_workspace_bytes",
     "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
@@ -86,6 +90,7 @@ def load() -> C.CDLL:
         "lb_socs_kernels": ([vp, vp, vp], C.c_int),
         "lb_workspace_bytes": ([vp, i32, i32], sz),
         "lb_forward": ([vp, vp, i32, i32, vp, vp, vp, sz, vp], C.c_int),
+        "lb_forward_static": ([vp, vp, i32, i32, vp, vp, vp], C.c_int),
         "lb_backward": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
         "lb_fields": ([vp, vp, i32, i32, vp, vp, sz, vp], C.c_int),
         "lb_fields_workspace_bytes": ([vp, i32], sz),

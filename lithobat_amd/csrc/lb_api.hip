@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -45,6 +46,36 @@ const int kFieldSizes[] = {16, 32, 48, 64, 80, 96, 128, 256};
 // runs slower per image than the 96-point one on the same boxes, so it is
 // used only when requested.
 const int kAutoFieldSizes[] = {16, 32, 48, 64, 96, 128, 256};
+
+// The fixed device workspace of lb_forward_static, for callers that cannot
+// allocate.  Each GPU the library runs on holds its own copy for the life of
+// the process.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol): set_geometry_common, the plan
+// set-up, looks up each device's address of it once with that device current,
+// and the entry point passes that address.
+static __device__ __align__(256) unsigned char scratch[LB_STATIC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch` (looked up once per device).
+lb_status static_scratch(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(LB_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(LB_ERR_UNSUPPORTED, "device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(LB_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return LB_OK;
+}
 
 // Noll j -> (n, m), expose/pupil.py:56-78.
 const int kNoll[22][2] = {{0, 0},  {0, 0},  {1, 1},  {1, -1}, {2, 0},  {2, -2}, {2, 2},  {3, -1},
@@ -190,7 +221,8 @@ lb_status set_geometry_common(Plan& p, const lb_imaging_desc* d) {
   int dev = 0;
   LB_HIP(hipGetDevice(&dev));
   LB_HIP(hipDeviceGetAttribute(&p.cus, hipDeviceAttributeMultiprocessorCount, dev));
-  return LB_OK;
+  void* sp = nullptr;
+  return static_scratch(&sp);  // this device's lb_forward_static workspace, once per device
 }
 
 lb_status check_desc(const lb_imaging_desc* d) {
@@ -337,7 +369,7 @@ struct Work {
 // use: the coherent-system kernel shape the work is laid out for (default:
 // the plan's own forward or adjoint shape).
 Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0,
-            const FieldConfig* use = nullptr) {
+            const FieldConfig* use = nullptr, size_t staged_pass = kStagedPassBytes) {
   Work w{};
   w.n_masks = n_masks;
   w.n_img = n_masks * p.n_planes;
@@ -366,7 +398,7 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0,
     // stay in the last-level cache between its launches
     const size_t per_img = staged_table_bytes(p.precision, p.M, p.W) * (size_t)std::max(p.spp, 1);
     w.pass_img = (int)std::max<size_t>(1, std::min<size_t>((size_t)w.n_img,
-                                                           kStagedPassBytes / per_img));
+                                                           staged_pass / per_img));
     w.G = L.take(per_img * (size_t)w.pass_img);
   }
   {
@@ -473,7 +505,8 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
 // unused), or null to compute them here.
 template <typename T>
 lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* images, void* peak,
-                    char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr) {
+                    char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr,
+                    bool may_profile = true) {
   if (!spectrum) {
     lb_status rc = fwd_pre<T>(p, masks, cplx, n_masks, ws, w, st);
     if (rc != LB_OK) return rc;
@@ -484,7 +517,8 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
   a.queue = (int*)(ws + w.queue);
   LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
   int e0 = -1, e1 = -1;
-  if (p.profile) {
+  const bool prof = p.profile && may_profile;
+  if (prof) {
     LB_HIP(profile_events(p, &e0, &e1));
     LB_HIP(hipEventRecord(p.ev_pool[e0], st));
   }
@@ -495,7 +529,7 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
   } else {
     LB_HIP(field_forward(p.precision, fc, a, ws + w.partial, st));
   }
-  if (p.profile) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
+  if (prof) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
   return fwd_post<T>(p, images, peak, ws, w, st);
 }
 
@@ -1074,6 +1108,40 @@ lb_status lb_forward(lb_plan plan, const void* masks, int32_t cplx, int32_t n_ma
   return p.precision == 64
              ? forward_t<double>(p, masks, cplx, n_masks, images, peak, (char*)workspace, w, st)
              : forward_t<float>(p, masks, cplx, n_masks, images, peak, (char*)workspace, w, st);
+}
+
+lb_status lb_forward_static(lb_plan plan, const void* masks, int32_t cplx, int32_t n_masks,
+                            void* images, void* peak, void* stream) {
+  g_err.clear();
+  if (!plan || !masks || !images) return fail(LB_ERR_ARG, "null plan, masks or images");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  void* ws = nullptr;
+  lb_status rc = static_scratch(&ws);
+  if (rc != LB_OK) return rc;
+  const size_t cap = LB_STATIC_SCRATCH_BYTES;
+  // lb_forward's own layout when it fits; else more systems per work item
+  // (fewer coarse partials), then smaller staged passes
+  Work w = layout(p, n_masks, false);
+  for (int chunk = w.chunk; w.total > cap && chunk < p.spp;) {
+    chunk = std::min(p.spp, chunk * 2);
+    w = layout(p, n_masks, false, chunk);
+  }
+  for (size_t pass = kStagedPassBytes / 2; w.total > cap && p.fwd.csplit == 0 && pass >= (1u << 20);
+       pass /= 2)
+    w = layout(p, n_masks, false, p.spp, nullptr, pass);
+  if (w.total > cap)
+    return fail(LB_ERR_WORKSPACE, "lb_forward_static: " + std::to_string(n_masks) +
+                                      " masks need " + std::to_string(w.total) +
+                                      " bytes of workspace, beyond the static scratch of " +
+                                      std::to_string(cap) + " bytes; use lb_forward");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? forward_t<double>(p, masks, cplx, n_masks, images, peak, (char*)ws, w, st, nullptr,
+                                 false)
+             : forward_t<float>(p, masks, cplx, n_masks, images, peak, (char*)ws, w, st, nullptr,
+                                false);
 }
 
 lb_status lb_backward(lb_plan plan, const void* masks, int32_t n_masks, const void* dimg,
