@@ -188,11 +188,30 @@ lb_status lb_backward_spectrum(lb_plan plan, const void* spectrum, int32_t n_mas
                                const void* d_images, void* d_masks, void* workspace,
                                size_t workspace_bytes, void* stream);
 
+/* lb_backward_spectrum for complex masks: d_masks[mask][N][N] (interleaved complex) =
+ * dJ/dRe m + i dJ/dIm m of J = sum d_images * I, the gradient torch's autograd uses for a
+ * complex input; its real part is the reference's projection (opc/ilt.py:235-266 takes
+ * 2 Re of the adjoint). */
+lb_status lb_backward_spectrum_complex(lb_plan plan, const void* spectrum, int32_t n_masks,
+                                       const void* d_images, void* d_masks, void* workspace,
+                                       size_t workspace_bytes, void* stream);
+
 /* Coherent fields of each system on the full grid, complex, [n_masks][n_sys][N][N]
  * for plane 0 (SOCS: one per kernel, the `fields` of socs_fields). */
 lb_status lb_fields(lb_plan plan, const void* masks, int32_t mask_complex, int32_t n_masks,
                     void* fields, void* workspace, size_t workspace_bytes, void* stream);
 size_t lb_fields_workspace_bytes(lb_plan plan, int32_t n_masks);
+
+/* socs_gradient(kernels, fields, sensitivity) of opc/ilt.py:235-266 on the fields it is
+ * given (SOCS plans): d_masks[mask][N][N] = 2 Re sum_k lambda_k ifft2(conj(phi_k)
+ * fft2(sensitivity[mask] * fields[mask][k])) for fields [n_masks][K][N][N] (complex) and
+ * sensitivity [n_masks][N][N] (real), in the plan's precision; complex_out != 0 writes the
+ * complex adjoint 2 sum_k ... before its real part (interleaved complex d_masks).  The
+ * fields need not come from lb_fields.  Workspace: lb_fields_backward_workspace_bytes. */
+size_t lb_fields_backward_workspace_bytes(lb_plan plan);
+lb_status lb_fields_backward(lb_plan plan, const void* fields, int32_t n_masks,
+                             const void* sensitivity, void* d_masks, int32_t complex_out,
+                             void* workspace, size_t workspace_bytes, void* stream);
 
 /* The reference's own dataflow on the device, for comparison: the raw image
  * of one real mask on one plane as one dense N x N inverse transform per

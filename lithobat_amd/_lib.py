@@ -50,7 +50,8 @@ EXPORTS = [
     "lb_version", "lb_last_error", "lb_abbe_plan_create", "lb_socs_plan_create",
     "lb_plan_destroy", "lb_plan_info_get", "lb_plan_clear", "lb_socs_weights",
     "lb_socs_kernels", "lb_workspace_bytes", "lb_forward", "lb_forward_static", "lb_backward",
-    "lb_fields",
+    "lb_fields", "lb_fields_backward", "lb_fields_backward_workspace_bytes",
+    "lb_backward_spectrum_complex",
     "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read", "lb_plan_shapes",
     "lb_plan_set_shape", "lb_batch_create", "lb_batch_destroy", "lb_batch_workspace_bytes",
     "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
@@ -94,6 +95,9 @@ def load() -> C.CDLL:
         "lb_backward": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
         "lb_fields": ([vp, vp, i32, i32, vp, vp, sz, vp], C.c_int),
         "lb_fields_workspace_bytes": ([vp, i32], sz),
+        "lb_fields_backward_workspace_bytes": ([vp], sz),
+        "lb_fields_backward": ([vp, vp, i32, vp, vp, i32, vp, sz, vp], C.c_int),
+        "lb_backward_spectrum_complex": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
         "lb_profile_enable": ([vp, i32], C.c_int),
         "lb_profile_read": ([vp, C.POINTER(i32), C.POINTER(C.c_double)], C.c_int),
         "lb_plan_shapes": ([vp, i32, C.POINTER(i32), i32, C.POINTER(i32)], C.c_int),

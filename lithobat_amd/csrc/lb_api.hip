@@ -557,20 +557,23 @@ lb_status bwd_pre(Plan& p, const void* masks, int n_masks, const void* dimg, cha
 
 // Backward, phase 3: sum the band-gradient partials per mask and synthesise
 // the real mask gradient.
+// cout: the complex gradient d/dRe + i d/dIm (complex masks) instead of its real part.
 template <typename T>
-lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, hipStream_t st) {
+lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, hipStream_t st,
+                   bool cout = false) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
   cx<T>* GX = (cx<T>*)(ws + w.GX);
   LB_HIP(segment_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes * w.n_chunks * w.nrc,
                         2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, (T*)(ws + w.seg), st));
-  LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, 1, invN2, 0, (cx<T>*)(ws + w.TTm),
-                           (T*)dmask, (T*)nullptr, st));
+  LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, cout ? 0 : 1, invN2, cout ? 1 : 0,
+                           (cx<T>*)(ws + w.TTm), (T*)dmask, (T*)nullptr, st));
   return LB_OK;
 }
 
 template <typename T>
 lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, void* dmask,
-                     char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr) {
+                     char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr,
+                     bool cout = false) {
   const T* gM = nullptr;
   lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st, spectrum != nullptr);
   if (rc != LB_OK) return rc;
@@ -586,7 +589,7 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
   } else {
     LB_HIP(field_adjoint(p.precision, fc, a, gM, ws + w.gX, st));
   }
-  return bwd_post<T>(p, n_masks, dmask, ws, w, st);
+  return bwd_post<T>(p, n_masks, dmask, ws, w, st, cout);
 }
 
 // mask band covering every box: symmetric [-Q, Q], or the whole grid
@@ -1228,6 +1231,25 @@ lb_status lb_backward_spectrum(lb_plan plan, const void* spectrum, int32_t n_mas
                                  spectrum);
 }
 
+lb_status lb_backward_spectrum_complex(lb_plan plan, const void* spectrum, int32_t n_masks,
+                                       const void* dimg, void* dmask, void* workspace,
+                                       size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !spectrum || !dimg || !dmask) return fail(LB_ERR_ARG, "null argument");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  Work w = layout(p, n_masks, true);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? backward_t<double>(p, nullptr, n_masks, dimg, dmask, (char*)workspace, w, st,
+                                  spectrum, true)
+             : backward_t<float>(p, nullptr, n_masks, dimg, dmask, (char*)workspace, w, st,
+                                 spectrum, true);
+}
+
 lb_status lb_socs_kernels(lb_plan plan, void* kernels, void* stream) {
   g_err.clear();
   if (!plan || !kernels) return fail(LB_ERR_ARG, "null argument");
@@ -1245,6 +1267,8 @@ lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) 
   g_err.clear();
   if (!plans || n_plans < 1 || !out) return fail(LB_ERR_ARG, "need >= 1 plan and an output");
   *out = nullptr;
+  for (int32_t i = 0; i < n_plans; ++i)
+    if (!plans[i]) return fail(LB_ERR_ARG, "null plan in batch");
   lb_batch b = new lb_batch_s();
   b->plans.assign(plans, plans + n_plans);
   const Plan& p0 = plans[0]->p;
@@ -1254,10 +1278,6 @@ lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) 
   long long total_sys = 0;
   for (lb_plan h : b->plans) {
     const Plan& p = h->p;
-    if (!h) {
-      delete b;
-      return fail(LB_ERR_ARG, "null plan in batch");
-    }
     if (p.precision != b->precision || p.M != b->M) {
       delete b;
       return fail(LB_ERR_ARG, "batched plans must share precision and coarse grid side "
@@ -1951,6 +1971,88 @@ lb_status lb_fields(lb_plan plan, const void* masks, int32_t cplx, int32_t n_mas
   hipStream_t st = (hipStream_t)stream;
   return p.precision == 64 ? fields_t<double>(p, masks, cplx, n_masks, fields, (char*)workspace, st)
                            : fields_t<float>(p, masks, cplx, n_masks, fields, (char*)workspace, st);
+}
+
+}  // extern "C"
+
+// systems whose products g E_k are transformed together by lb_fields_backward
+constexpr int kFieldsAdjChunk = 16;
+
+namespace {
+struct FieldsAdjWork {
+  size_t Y, RT, X, A, TT, total;
+};
+FieldsAdjWork fields_adj_layout(const Plan& p) {
+  FieldsAdjWork f{};
+  const size_t C = p.csz(), nn = (size_t)p.N * p.N;
+  const int ch = std::max(1, std::min(kFieldsAdjChunk, p.spp));
+  Layout L;
+  f.Y = L.take(C * (size_t)ch * nn);
+  f.RT = L.take(C * (size_t)ch * band_ld(p.Wbm) * p.N);
+  f.X = L.take(C * (size_t)ch * p.Wbm * p.Wbm);
+  f.A = L.take(C * (size_t)p.Wbm * p.Wbm);
+  f.TT = L.take(C * (size_t)band_ld(p.Wbm) * p.N);
+  f.total = L.off;
+  return f;
+}
+}  // namespace
+
+// socs_gradient (opc/ilt.py:235-266) on the fields given: per mask, for chunks of
+// systems, Y_k = g E_k, X_k = fft2(Y_k) on the mask band, A += w_k conj(H_k) X_k on each
+// box; then d mask = (2 / N^2) Re sum A e^{+...} (or the complex sum when cout).
+template <typename T>
+static lb_status fields_bwd_t(Plan& p, const void* fields, int n_masks, const void* sens,
+                              void* dmask, bool cout, char* ws, hipStream_t st) {
+  const FieldsAdjWork f = fields_adj_layout(p);
+  const long long nn = (long long)p.N * p.N;
+  cx<T>* Y = (cx<T>*)(ws + f.Y);
+  cx<T>* RT = (cx<T>*)(ws + f.RT);
+  cx<T>* X = (cx<T>*)(ws + f.X);
+  cx<T>* A = (cx<T>*)(ws + f.A);
+  cx<T>* TT = (cx<T>*)(ws + f.TT);
+  const T s2 = (T)(2.0 / ((double)p.N * (double)p.N));
+  for (int m = 0; m < n_masks; ++m) {
+    const cx<T>* E = (const cx<T>*)fields + (long long)m * p.spp * nn;
+    const T* g = (const T*)sens + (long long)m * nn;
+    for (int j0 = 0; j0 < p.spp; j0 += kFieldsAdjChunk) {
+      const int nk = std::min(kFieldsAdjChunk, p.spp - j0);
+      LB_HIP(mul_real_into<T>(E + (long long)j0 * nn, nk, nn, g, Y, st));
+      LB_HIP(band_analysis<T>(p.N, Y, 1, 2 * nn, 0, 1, nk, p.nu0m, p.Wbm, (T)1, RT, X, st));
+      LB_HIP(fields_adj_band<T>(X, p.Wbm, p.wrapm, (const cx<T>*)p.H, p.W, p.box,
+                                (const T*)p.wsys, j0, nk, j0 > 0 ? 1 : 0, A, st));
+    }
+    T* out = (T*)dmask + (long long)m * nn * (cout ? 2 : 1);
+    LB_HIP(band_synthesis<T>(p.N, A, 1, p.nu0m, p.Wbm, cout ? 0 : 1, s2, cout ? 1 : 0, TT, out,
+                             (T*)nullptr, st));
+  }
+  return LB_OK;
+}
+
+extern "C" {
+
+size_t lb_fields_backward_workspace_bytes(lb_plan plan) {
+  if (!plan) return 0;
+  return fields_adj_layout(plan->p).total;
+}
+
+lb_status lb_fields_backward(lb_plan plan, const void* fields, int32_t n_masks,
+                             const void* sensitivity, void* d_masks, int32_t complex_out,
+                             void* workspace, size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !fields || !sensitivity || !d_masks) return fail(LB_ERR_ARG, "null argument");
+  Plan& p = plan->p;
+  if (!p.socs) return fail(LB_ERR_UNSUPPORTED, "fields are exported for SOCS plans only");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no kernels");
+  const size_t need = lb_fields_backward_workspace_bytes(plan);
+  if (!workspace || ws_bytes < need)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(need) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? fields_bwd_t<double>(p, fields, n_masks, sensitivity, d_masks, complex_out != 0,
+                                    (char*)workspace, st)
+             : fields_bwd_t<float>(p, fields, n_masks, sensitivity, d_masks, complex_out != 0,
+                                   (char*)workspace, st);
 }
 
 }  // extern "C"

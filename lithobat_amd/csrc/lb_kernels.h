@@ -150,6 +150,15 @@ hipError_t accum_norm2(const cx<T>* E, int nsys, long long len, const T* w, int 
 // E_k *= g (real) for a batch of fields; A += w conj(H) G over each box (atomic)
 template <typename T>
 hipError_t mul_real(cx<T>* E, int nsys, long long len, const T* g, hipStream_t st);
+// Y_k = g E_k (copy); A (+)= sum_k w conj(H) X_k over each system's box on a band
+// (deterministic order): the adjoint from given fields.
+template <typename T>
+hipError_t mul_real_into(const cx<T>* E, int nsys, long long len, const T* g, cx<T>* Y,
+                         hipStream_t st);
+template <typename T>
+hipError_t fields_adj_band(const cx<T>* X, int Wb, int wrapN, const cx<T>* H, int W,
+                           const int2* box, const T* w, int j0, int nsys, int accumulate,
+                           cx<T>* A, hipStream_t st);
 template <typename T>
 hipError_t dense_adjoint_accum(const cx<T>* G, int N, const cx<T>* H, int W, const int2* box,
                                const T* w, int j0, int nsys, int nu0m, cx<T>* A, hipStream_t st);

@@ -321,6 +321,73 @@ hipError_t dense_adjoint_accum(const cx<T>* G, int N, const cx<T>* H, int W, con
   return hipGetLastError();
 }
 
+// Y_k = g E_k for a chunk of fields (the fields-consuming adjoint's first step).
+template <typename T>
+__global__ void k_mul_real_into(const cx<T>* __restrict__ E, int nsys, long long len,
+                                const T* __restrict__ g, cx<T>* __restrict__ Y) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+       e += (long long)gridDim.x * blockDim.x) {
+    const T gv = g[e];
+    for (int k = 0; k < nsys; ++k) Y[(long long)k * len + e] = scale(E[(long long)k * len + e], gv);
+  }
+}
+
+// A[u][v] = (A[u][v] if accumulate) + sum_{k < nsys} w_j conj(H_j[a][c]) X_k[u][v] with
+// j = j0 + k and (a, c) the band bin's place in system j's box (modulo wrapN when the
+// band spans the whole grid).  One thread per band bin, systems in order: the sums do not
+// depend on the launch (no atomics).
+template <typename T>
+__global__ void k_fields_adj_band(const cx<T>* __restrict__ X, int Wb, int wrapN,
+                                  const cx<T>* __restrict__ H, int W, const int2* __restrict__ box,
+                                  const T* __restrict__ w, int j0, int nsys, int accumulate,
+                                  cx<T>* __restrict__ A) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= Wb * Wb) return;
+  const int u = e / Wb, v = e - u * Wb;
+  cx<T> acc = accumulate ? A[e] : cx<T>{0, 0};
+  for (int k = 0; k < nsys; ++k) {
+    const int j = j0 + k;
+    const int2 b = box[j];
+    int a = u - b.x, c = v - b.y;
+    if (wrapN > 0) {
+      a = ((a % wrapN) + wrapN) % wrapN;
+      c = ((c % wrapN) + wrapN) % wrapN;
+    }
+    if (a < 0 || a >= W || c < 0 || c >= W) continue;
+    const cx<T> t = mulc(X[(long long)k * Wb * Wb + e], H[(long long)j * W * W + a * W + c]);
+    acc = acc + scale(t, w[j]);
+  }
+  A[e] = acc;
+}
+
+template <typename T>
+hipError_t mul_real_into(const cx<T>* E, int nsys, long long len, const T* g, cx<T>* Y,
+                         hipStream_t st) {
+  long long b = (len + 255) / 256;
+  if (b > 8192) b = 8192;
+  hipLaunchKernelGGL(k_mul_real_into<T>, dim3((unsigned)b), dim3(256), 0, st, E, nsys, len, g, Y);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fields_adj_band(const cx<T>* X, int Wb, int wrapN, const cx<T>* H, int W,
+                           const int2* box, const T* w, int j0, int nsys, int accumulate,
+                           cx<T>* A, hipStream_t st) {
+  hipLaunchKernelGGL(k_fields_adj_band<T>, dim3((Wb * Wb + 255) / 256), dim3(256), 0, st, X, Wb,
+                     wrapN, H, W, box, w, j0, nsys, accumulate, A);
+  return hipGetLastError();
+}
+
+template hipError_t mul_real_into<float>(const cx<float>*, int, long long, const float*,
+                                         cx<float>*, hipStream_t);
+template hipError_t mul_real_into<double>(const cx<double>*, int, long long, const double*,
+                                          cx<double>*, hipStream_t);
+template hipError_t fields_adj_band<float>(const cx<float>*, int, int, const cx<float>*, int,
+                                           const int2*, const float*, int, int, int, cx<float>*,
+                                           hipStream_t);
+template hipError_t fields_adj_band<double>(const cx<double>*, int, int, const cx<double>*, int,
+                                            const int2*, const double*, int, int, int,
+                                            cx<double>*, hipStream_t);
 template hipError_t mul_real<float>(cx<float>*, int, long long, const float*, hipStream_t);
 template hipError_t mul_real<double>(cx<double>*, int, long long, const double*, hipStream_t);
 template hipError_t dense_adjoint_accum<float>(const cx<float>*, int, const cx<float>*, int,

@@ -6,9 +6,10 @@ per-system coherent field from the mask instead of storing them, and returns
 d/dmask of <dI, I> for real masks.  It works for Abbe plans (scalar or
 vector, several planes) and SOCS plans alike.
 
-``socs_fields`` / ``socs_gradient`` keep the reference's two-call contract
-(fields are returned, then handed back with the sensitivity); the fields
-array remembers its mask, and the gradient is the recomputing adjoint.
+``socs_fields`` / ``socs_gradient`` keep the reference's two-call contract:
+the fields are returned, then handed back with the sensitivity, and the
+gradient is computed from the fields it is given (``lb_fields_backward``); with
+the mask passed as well it recomputes them instead (the autograd path).
 """
 from __future__ import annotations
 
@@ -28,7 +29,9 @@ class AbbeIntensity(torch.autograd.Function):
     """raw[B, P, N, N] = plan(mask[B, N, N]); backward by the recomputing adjoint.
 
     The masks' band spectra are computed once and kept for the backward (a few
-    hundred KB per mask), so the step analyses each mask once."""
+    hundred KB per mask), so the step analyses each mask once.  A complex mask
+    gets torch's complex gradient dJ/dRe m + i dJ/dIm m (its real part is the
+    reference's real projection, opc/ilt.py:235-266)."""
 
     @staticmethod
     def forward(ctx, mask: torch.Tensor, plan: ImagingPlan):
@@ -37,13 +40,15 @@ class AbbeIntensity(torch.autograd.Function):
         ctx.plan = plan
         ctx.squeeze = mask.dim() == 2
         ctx.dtype = mask.dtype
+        ctx.complex = mask.is_complex()
         ctx.save_for_backward(spec)
         return raw
 
     @staticmethod
     def backward(ctx, grad_raw: torch.Tensor):
         (spec,) = ctx.saved_tensors
-        g = ctx.plan.backward_spectrum(spec, grad_raw.contiguous()).to(ctx.dtype)
+        g = ctx.plan.backward_spectrum(spec, grad_raw.contiguous(),
+                                       complex_out=ctx.complex).to(ctx.dtype)
         return (g[0] if ctx.squeeze else g), None
 
 
@@ -85,18 +90,6 @@ def mask_penalties(mask: torch.Tensor, weight_tv: float = 0.0,
     return MaskPenalties.apply(mask, max(weight_tv, 0.0), max(weight_discrete, 0.0))
 
 
-class FieldStack(np.ndarray):
-    """The (K, n, n) fields of ``socs_fields``, remembering the mask they came from."""
-
-    def __new__(cls, arr, mask):
-        obj = np.asarray(arr).view(cls)
-        obj.mask = mask
-        return obj
-
-    def __array_finalize__(self, obj):
-        self.mask = getattr(obj, "mask", None)
-
-
 def socs_fields(kernels, mask):
     """ilt.py:208-232: (fields (K, n, n) complex, raw intensity (n, n))."""
     n = kernels.grid.n_pixels
@@ -106,23 +99,33 @@ def socs_fields(kernels, mask):
     fields = kernels.plan.fields(m)[0]
     raw, _ = kernels.plan.forward(m, with_peak=False)
     if was_np:
-        return FieldStack(fields.cpu().numpy(), np.array(mask, copy=True)), raw[0, 0].double().cpu().numpy()
+        return fields.cpu().numpy(), raw[0, 0].double().cpu().numpy()
     return fields, raw[0, 0]
 
 
 def socs_gradient(kernels, fields, sensitivity, mask=None):
-    """ilt.py:235-266: d/dm of sum(sensitivity * I) for the raw SOCS intensity.
+    """ilt.py:235-266: d/dm of sum(sensitivity * I) for the raw SOCS intensity, real.
 
-    The fields are recomputed from their mask (carried by the array
-    ``socs_fields`` returned, or passed as ``mask``).
-    """
-    if mask is None:
-        mask = getattr(fields, "mask", None)
-    if mask is None:
-        raise ValueError("socs_gradient recomputes the fields: pass the fields returned by "
-                         "socs_fields, or the mask they came from")
-    m, was_np, _ = _to_device(mask, kernels.plan.precision)
-    g = torch.as_tensor(np.asarray(sensitivity) if was_np else sensitivity,
-                        device=m.device, dtype=kernels.plan.real_dtype)
-    out = kernels.plan.backward(m.reshape(1, *m.shape[-2:]), g.reshape(1, 1, *g.shape[-2:]))[0]
+    Computed from the ``fields`` given (any (K, n, n) complex array, NumPy or a device
+    tensor), as the reference does; when ``mask`` is passed the fields are recomputed
+    from it instead (``fields`` is then not read)."""
+    n = kernels.grid.n_pixels
+    plan = kernels.plan
+    if mask is not None:
+        m, was_np, _ = _to_device(mask, plan.precision)
+        g = torch.as_tensor(np.asarray(sensitivity) if was_np else sensitivity,
+                            device=m.device, dtype=plan.real_dtype)
+        out = plan.backward(m.reshape(1, n, n), g.reshape(1, 1, n, n))[0]
+        return out.double().cpu().numpy() if was_np else out
+    was_np = not isinstance(fields, torch.Tensor)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    f = torch.as_tensor(np.asarray(fields), device=dev) if was_np else fields
+    if not f.is_complex():
+        raise TypeError("fields must be complex (the coherent fields of socs_fields)")
+    g = torch.as_tensor(np.asarray(sensitivity, dtype=np.float64) if was_np else sensitivity,
+                        device=f.device)
+    if tuple(g.shape) != (n, n):
+        raise ValueError(f"sensitivity must be {(n, n)} for these kernels, got "
+                         f"{tuple(g.shape)}")
+    out = plan.fields_backward(f, g)[0]
     return out.double().cpu().numpy() if was_np else out

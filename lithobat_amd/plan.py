@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import json
+import logging
 import threading
 from collections import OrderedDict
 from dataclasses import asdict, is_dataclass
@@ -20,6 +21,7 @@ from . import _lib
 from .source import SourcePoints, source_points
 
 _DTYPES = {32: (torch.float32, torch.complex64), 64: (torch.float64, torch.complex128)}
+logger = logging.getLogger(__name__)
 
 
 def _image_index(optics) -> float:
@@ -61,6 +63,9 @@ class ImagingPlan:
         self.n = int(grid.n_pixels)
         self.src = src if src is not None else source_points(optics)
         zern = {int(k): float(v) for k, v in (optics.zernike_coeffs or {}).items()}
+        for j in zern:
+            if not 1 <= j <= 21:  # expose/pupil.py:128-130: the term contributes zero
+                logger.warning("Zernike Noll index %d not implemented; returning 0.", j)
         pol = optics.polarisation.lower()
         if pol not in _lib.POLARISATION_CODES:
             raise ValueError(f"polarisation must be one of {list(_lib.POLARISATION_CODES)}, "
@@ -131,6 +136,7 @@ class ImagingPlan:
         self._h = handle
         self._lib = lib
         self._ws = None
+        self._retired = []  # earlier workspaces, which graphs may still address
         self._lock = threading.Lock()
         # batch size whose tuned kernel shapes are selected; _pinned: shapes set by
         # hand (autotune.set_shape), which the automatic selection leaves alone
@@ -172,7 +178,12 @@ class ImagingPlan:
         return _DTYPES[self.precision][1]
 
     def _workspace(self, nbytes: int, device) -> torch.Tensor:
+        """The plan's device workspace, grown when a call needs more.  A replaced
+        workspace is kept alive with the plan, not released: a graph captured on it
+        still writes there when it replays, after later calls moved to a larger one."""
         if self._ws is None or self._ws.numel() < nbytes or self._ws.device != device:
+            if self._ws is not None:
+                self._retired.append(self._ws)
             self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
         return self._ws
 
@@ -270,19 +281,49 @@ class ImagingPlan:
                 peak.data_ptr() if peak is not None else None, ws.data_ptr(), ws.numel(), st))
         return out, peak
 
-    def backward_spectrum(self, spec: torch.Tensor, d_images: torch.Tensor) -> torch.Tensor:
-        """``backward`` from the band spectra of real masks (``mask_spectrum``)."""
+    def backward_spectrum(self, spec: torch.Tensor, d_images: torch.Tensor,
+                          complex_out: bool = False) -> torch.Tensor:
+        """``backward`` from the band spectra (``mask_spectrum``): d/dmask of
+        sum(d_images * I), real [B, N, N]; ``complex_out`` gives dJ/dRe m + i dJ/dIm m
+        (complex [B, N, N]), the gradient of a complex mask."""
         B = self._check_spectrum(spec)
         P = len(self.planes)
         g = d_images.to(self.real_dtype).reshape(B, P, self.n, self.n).contiguous()
-        out = torch.empty((B, self.n, self.n), dtype=self.real_dtype, device=spec.device)
+        dt = self.complex_dtype if complex_out else self.real_dtype
+        out = torch.empty((B, self.n, self.n), dtype=dt, device=spec.device)
+        fn = (self._lib.lb_backward_spectrum_complex if complex_out
+              else self._lib.lb_backward_spectrum)
         with self._lock:
             self._use_tuning(B)
             ws = self._workspace(self.workspace_bytes(B, True), spec.device)
             st = torch.cuda.current_stream(spec.device).cuda_stream
-            _lib.check(self._lib.lb_backward_spectrum(self._h, spec.data_ptr(), B, g.data_ptr(),
-                                                      out.data_ptr(), ws.data_ptr(), ws.numel(),
-                                                      st))
+            _lib.check(fn(self._h, spec.data_ptr(), B, g.data_ptr(), out.data_ptr(),
+                          ws.data_ptr(), ws.numel(), st))
+        return out
+
+    def fields_backward(self, fields: torch.Tensor, sensitivity: torch.Tensor,
+                        complex_out: bool = False) -> torch.Tensor:
+        """SOCS plans: socs_gradient on the fields given, [K, N, N] or [B, K, N, N] complex,
+        with the sensitivity [N, N] or [B, N, N]: real [B, N, N] (or the complex adjoint)."""
+        K = self.info["n_systems"]
+        f = fields.to(self.complex_dtype)
+        if f.dim() == 3:
+            f = f.unsqueeze(0)
+        if f.dim() != 4 or tuple(f.shape[1:]) != (K, self.n, self.n):
+            raise ValueError(f"fields must be ({K}, {self.n}, {self.n}) for these kernels, got "
+                             f"{tuple(fields.shape)}")
+        B = f.shape[0]
+        g = sensitivity.to(device=f.device, dtype=self.real_dtype).reshape(B, self.n, self.n)
+        f, g = f.contiguous(), g.contiguous()
+        dt = self.complex_dtype if complex_out else self.real_dtype
+        out = torch.empty((B, self.n, self.n), dtype=dt, device=f.device)
+        with self._lock:
+            ws = self._workspace(int(self._lib.lb_fields_backward_workspace_bytes(self._h)),
+                                 f.device)
+            st = torch.cuda.current_stream(f.device).cuda_stream
+            _lib.check(self._lib.lb_fields_backward(self._h, f.data_ptr(), B, g.data_ptr(),
+                                                    out.data_ptr(), 1 if complex_out else 0,
+                                                    ws.data_ptr(), ws.numel(), st))
         return out
 
     def backward(self, masks: torch.Tensor, d_images: torch.Tensor) -> torch.Tensor:
