@@ -22,10 +22,14 @@ Also reported on the same line:
 * the depth-resolved image through a resist film and the full ILT solve
   (run_ilt) on the device;
 * the FP32 image error against the float64 oracle on a sample image;
+* the plain FP64 image of the CPU-runnable configuration (256^2 lines and
+  spaces, NA 0.85 annular, 488 source points) on the device;
 * roofline of the coherent-system kernel (HIP events on its stream; HBM
-  traffic from the committed counter record when it matches the shape);
+  traffic from the committed counter record when it matches the shape), and
+  of the ILT adjoint kernel of the 16-mask step;
 * the oracle (the reference algorithm in float64 NumPy/SciPy) timed on the
-  host cores on a bounded sample, as the CPU baseline.
+  host cores on a bounded sample, as the CPU baseline (the headline batch),
+  and for the plain FP64, SOCS, ILT and clip configurations.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--legs batched,plain,...]
 """
@@ -50,6 +54,9 @@ N_MASKS = 64
 MASK_SEEDS = range(1000, 1000 + N_MASKS)
 DEFOCUS = [-100e-9, -50e-9, 0.0, 50e-9, 100e-9]
 PEAK_FP32_TFLOPS = 157.3
+# the FP32 vector peak counts packed (two-wide) FP32 instructions; the kernels here are
+# built without them, so one FP32 FMA per lane per cycle is their ceiling
+VALU_UNPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
@@ -64,6 +71,14 @@ def optics_cfg4():
     return OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="quadrupole",
                         sigma_outer=0.85, sigma_inner=0.55, source_grid=47,
                         normalisation="clear")
+
+
+def optics_cfg1():
+    """The CPU-runnable plain configuration: NA 0.85 annular 0.5-0.8 on a 41-point source
+    grid (488 points, 0.05 sigma spacing), zero defocus, peak normalisation."""
+    from lithobat_amd.config import OpticsConfig
+    return OpticsConfig(wavelength=193e-9, NA=0.85, source_type="annular", sigma_outer=0.8,
+                        sigma_inner=0.5, source_grid=41, defocus=0.0)
 
 
 def clips_cfg5():
@@ -94,17 +109,25 @@ def field_flops(M: int, W: int) -> float:
 
 
 COUNTERS = os.path.join(ROOT, "profiles", "k_field_fwd_abbe_1024_64x5_counters.md")
+COUNTERS_ADJ = os.path.join(ROOT, "profiles", "k_field_adj_ilt_1024_16_counters.md")
 
 
-def recorded_traffic(plan):
+def kernel_shape(plan, n_masks, adjoint=False):
+    """The coherent-system kernel shape a call of ``n_masks`` masks runs."""
+    i = plan.info
+    pre = "adj_" if adjoint else ""
+    chunk = plan.work_items(n_masks, adjoint)[0]
+    return (f"M={i['m']} W={i['box']} threads={i[pre + 'block_threads']} "
+            f"splits={i[pre + 'column_splits']} chunk={chunk}")
+
+
+def recorded_traffic(plan, path=COUNTERS, n_masks=N_MASKS, adjoint=False):
     """HBM bytes per launch of the coherent-system kernel, as the committed counter summary
     (FETCH_SIZE and WRITE_SIZE passes, scaled by a 1 GiB copy) recorded them, when that
     record is for the kernel shape this run uses; else None."""
-    i = plan.info
-    shape = (f"M={i['m']} W={i['box']} threads={i['block_threads']} "
-             f"splits={i['column_splits']} chunk={i['chunk']}")
+    shape = kernel_shape(plan, n_masks, adjoint)
     try:
-        with open(COUNTERS) as f:
+        with open(path) as f:
             text = f.read()
     except OSError:
         return None, shape
@@ -176,6 +199,31 @@ def ilt_leg(args, ilt, dev, sync):
         loss.backward()
 
     batched = timed(batch_step, max(2, args.ilt_steps // 2), 1, sync) / ILT_BATCH
+    # the adjoint coherent-system kernel of the batched step, from HIP events on its stream
+    ilt.profile(True)
+    for _ in range(max(3, args.ilt_steps // 2)):
+        batch_step()
+    sync()
+    n_adj, adj_ms = ilt.profile_read(adjoint=True)
+    n_fwd, fwd_ms = ilt.profile_read()
+    ilt.profile(False)
+    info = ilt.info
+    M, W = info["m"], info["box"]
+    flops = 2.0 * ILT_BATCH * info["n_systems"] * field_flops(M, W)
+    adj_s = adj_ms / 1e3 / max(n_adj, 1)
+    achieved = flops / adj_s / 1e12
+    traffic, shape = recorded_traffic(ilt, COUNTERS_ADJ, ILT_BATCH, adjoint=True)
+    roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "valu_unpacked_ceiling": VALU_UNPACKED_TFLOPS,
+            "frac_of_unpacked": achieved / VALU_UNPACKED_TFLOPS, "traffic": traffic,
+            "traffic_source": (os.path.relpath(COUNTERS_ADJ, ROOT) if traffic is not None
+                               else None),
+            "kernel_shape": shape,
+            "kernel": "k_field_adj (ILT adjoint of the 16-mask step: fields recomputed, "
+                      "transposed radix stages, FP32 VALU)",
+            "kernel_ms": adj_s * 1e3, "flops_per_launch": flops,
+            "forward_kernel_ms": fwd_ms / max(n_fwd, 1)}
     th = theta.detach()[0]
 
     def dense_step():
@@ -200,7 +248,8 @@ def ilt_leg(args, ilt, dev, sync):
     # replayed: the single-mask step is bound by host launches when run eagerly
     g_one = graph_step_seconds(ilt_step, lambda: theta.grad, args, sync)
     g_batch = graph_step_seconds(batch_step, lambda: thetas.grad, args, sync)
-    return one, batched, dense, blurred, g_one, None if g_batch is None else g_batch / ILT_BATCH
+    return (one, batched, dense, blurred, g_one,
+            None if g_batch is None else g_batch / ILT_BATCH, roof)
 
 
 def graph_step_seconds(fn, result, args, sync):
@@ -319,7 +368,7 @@ def socs_leg(args, dev, sync):
     ref = abbe.forward(masks[:1])[0][0, 0]
     img = plan.forward(masks[:1])[0][0, 0]
     err = float((img - ref).abs().max() / ref.abs().max())
-    return {"images_per_s": 256 / sec, "build_s": build_s, "kernels": len(k24),
+    return k24, {"images_per_s": 256 / sec, "build_s": build_s, "kernels": len(k24),
             "n_source": k24.n_source, "n_band": k24.n_band,
             "energy_kept": float(k24.energy_kept), "rel_err_vs_abbe": err,
             "config": "256 masks 512^2 @ 8 nm, NA 1.35 annular 0.6-0.9 (142 points), "
@@ -402,6 +451,143 @@ def run_ilt_leg(args, sync):
     return (time.perf_counter() - t0) / res.iterations, res.iterations
 
 
+def cfg1_leg(args, dev, sync, with_cpu):
+    """The plain FP64 configuration on the device: one 256^2 image per call (5 nm pixels,
+    vertical lines / spaces pitch 200 nm CD 100 nm, 488 source points), images/s with the
+    mask resident; its error against the float64 oracle; and the oracle's own image timed
+    in full on the host (median of 5)."""
+    import torch
+
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.masks import lines_and_spaces
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=256, pixel_size=5e-9)
+    optics = optics_cfg1()
+    mask = lines_and_spaces(256, 5e-9, 200e-9, 100e-9)
+    plan = ImagingPlan(optics, grid, precision=64)
+    m = torch.tensor(mask, device=dev, dtype=torch.float64)
+    sec = timed(lambda: plan.forward(m), max(10, args.steps), 3, sync)
+    raw = plan.forward(m)[0][0, 0].cpu().numpy()
+    out = {"images_per_s": 1.0 / sec, "source_points": len(plan.src),
+           "coarse_grid": plan.info["m"], "pupil_box": plan.info["box"],
+           "config": "1 image 256^2 @ 5 nm, lines/spaces pitch 200 nm CD 100 nm, NA 0.85 "
+                     "annular 0.5-0.8 (488 points), FP64 (complex128), one image per call"}
+    if with_cpu:
+        from oracle import litho_oracle as O
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ref = O.compute_aerial_image(mask, optics, grid, return_raw=True)[0]
+            times.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(times))
+        out["fp64_rel_err_vs_oracle"] = float(np.abs(raw - ref).max() / np.abs(ref).max())
+        out["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "images/s", "cores": _cores(),
+                               "kind": "port",
+                               "sample": f"the whole image, median of 5 ({cpu_s:.3f} s)"}
+        out["vs_cpu"] = cpu_s / sec
+    return out
+
+
+def _cores():
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_socs_leg(k24, sync):
+    """CPU baseline of the SOCS configuration: the reference's convolution (fft2 of the
+    mask, K inverse transforms through the kernels, sum of lambda |E|^2) on one 512^2 mask,
+    three times, with the device's K = 24 kernels; and the reference's TCC build and
+    eigendecomposition (the full n_band x n_band TCC, eigh) timed on the same optics at half
+    the field (n_band / 4) and scaled by n_band^3 to the full band."""
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.masks import manhattan
+    from oracle import litho_oracle as O
+    optics, grid = socs_cfg3()
+    kern = k24.plan.kernels().cpu().numpy()
+    k = O.SOCS(kern, np.asarray(k24.weights, dtype=np.float64), float(k24.clear),
+               float(k24.energy_kept), int(k24.n_band), int(k24.n_source), grid.n_pixels)
+    mask = manhattan(512, 8e-9, seed=2000)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.socs_raw(k, mask)
+        times.append(time.perf_counter() - t0)
+    conv_s = float(np.median(times))
+    half = GridConfig(n_pixels=256, pixel_size=8e-9)
+    t0 = time.perf_counter()
+    kh = O.socs_build(optics, half, tol=0.0, max_kernels=24)
+    tcc_half = time.perf_counter() - t0
+    scale = (k24.n_band / kh.n_band) ** 3
+    return {"value": 1.0 / conv_s, "unit": "images/s", "cores": _cores(), "kind": "port",
+            "sample": f"1 mask (seed 2000) x 3, median {conv_s:.3f} s, K = 24 kernels",
+            "tcc_build_s_sample": tcc_half, "tcc_n_band_sample": int(kh.n_band),
+            "tcc_build_s_extrapolated": tcc_half * scale,
+            "tcc_sample": f"TCC + eigh at 256^2 @ 8 nm (n_band {kh.n_band}) in "
+                          f"{tcc_half:.2f} s, scaled by (n_band ratio)^3 = {scale:.1f} to "
+                          f"n_band {k24.n_band}"}
+
+
+def _oracle_ilt_step(mask, target, optics, grid, every):
+    """Seconds of one oracle ILT step (forward image, relaxed resist, Abbe adjoint,
+    sigmoid chain) over every ``every``-th source point, and the full source size."""
+    from oracle import litho_oracle as O
+    full = O.source_points(optics)
+    sub = O.SourcePts(full.weight[::every], full.fx[::every], full.fy[::every],
+                      full.phi[::every], full.grid)
+    planes = [(optics.defocus, optics.n_immersion if optics.n_image is None
+               else optics.n_image)]
+    t0 = time.perf_counter()
+    raw, _, clear = O.abbe_sum(mask, optics, grid, planes, src=sub)
+    _, d_raw, _, _ = O.relaxed_objective(raw[0], target, 1.0 / max(clear[0], 1e-30), 0.3, 50.0)
+    O.grad_theta(O.abbe_gradient(mask, optics, grid, d_raw, src=sub), mask, 8.0)
+    return time.perf_counter() - t0, len(full), len(sub)
+
+
+def cpu_ilt_leg():
+    """CPU baseline of the ILT step at 1024^2, NA 1.35 (148 points): the oracle's forward
+    image, relaxed-resist cost and derivative, and Abbe adjoint (opc/ilt.py:311-337 and
+    235-266 restated for the Abbe systems) for one mask over every 4th source point, three
+    times (median), scaled by the source count (the step is one transform pair per source
+    point, plus the mask's own transform)."""
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.masks import manhattan
+    from oracle import litho_oracle as O
+    optics = optics_cfg4()
+    grid = GridConfig(n_pixels=N, pixel_size=PIXEL)
+    mask = O.mask_of(np.random.default_rng(3000).standard_normal((N, N)), 8.0)
+    target = manhattan(N, PIXEL, seed=3001)
+    runs = [_oracle_ilt_step(mask, target, optics, grid, 4) for _ in range(3)]
+    t, s_full, s_sub = sorted(runs)[1]
+    step_s = t * s_full / s_sub
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": _cores(), "kind": "port",
+            "sample": f"1 step (theta seed 3000) over {s_sub} of the {s_full} source points, "
+                      f"x 3, median {t:.2f} s, scaled by {s_full}/{s_sub}: {step_s:.1f} s"}
+
+
+def cpu_clips_leg():
+    """CPU baseline of the 16-clip vector ILT step: one clip of each size (512, 768, 1024)
+    through the oracle's vector forward, relaxed resist and Abbe adjoint over every 8th
+    source point, scaled by the source count; the step time is the sum over the 16 clips
+    of the time of a clip of the same size."""
+    from lithobat_amd.masks import manhattan
+    from oracle import litho_oracle as O
+    clips = clips_cfg5()
+    per = {}
+    for c, (o, g) in enumerate(clips):
+        n = int(g.n_pixels)
+        if n in per:
+            continue
+        mask = O.mask_of(np.random.default_rng(5000 + c).standard_normal((n, n)), 8.0)
+        t, s_full, s_sub = _oracle_ilt_step(mask, manhattan(n, PIXEL, seed=6000 + c), o, g, 8)
+        per[n] = t * s_full / s_sub
+    counts = {n: sum(1 for _, g in clips if int(g.n_pixels) == n) for n in per}
+    step_s = sum(per[n] * counts[n] for n in per)
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": _cores(), "kind": "port",
+            "sample": f"one clip of each size over {s_sub} of the {s_full} source points, "
+                      "scaled by the source count (" +
+                      ", ".join(f"{n}^2: {per[n]:.2f} s x {counts[n]}" for n in sorted(per)) +
+                      f"), summed over the 16 clips: {step_s:.1f} s per step"}
+
+
 def dense_leg(args, plan, masks, sync):
     """Images/s of the reference's dataflow on the device: one dense N x N inverse
     transform per source point (8 per launch), host synchronisation per image."""
@@ -428,8 +614,10 @@ def main():
     ap.add_argument("--ilt-steps", type=int, default=10)
     ap.add_argument("--dense-images", type=int, default=3,
                     help="images timed for the dense per-source-point form")
-    ap.add_argument("--legs", default="batched,plain,ilt,clips,dense,socs,volume,solve,euv,cpu",
-                    help="comma list of measurements to run (batched always runs)")
+    ap.add_argument("--legs",
+                    default="batched,plain,ilt,clips,dense,socs,volume,solve,euv,cfg1,cpu",
+                    help="comma list of measurements to run (batched always runs; cpu = the "
+                         "CPU baselines of every configuration)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the legs, copy a MIB-sized buffer once (a plain copy kernel of "
                          "known size, to scale hardware byte counters)")
@@ -558,9 +746,10 @@ def main():
     ilt_s = ilt_batched_s = ilt_dense_s = ilt_blur_s = vol_s = None
     ilt_graph_s = ilt_graph_batched_s = None
     clip_ilt_s = clip_plain_s = dense_images_per_s = None
+    ilt_roof = None
     if "ilt" in legs:
         (ilt_s, ilt_batched_s, ilt_dense_s, ilt_blur_s, ilt_graph_s,
-         ilt_graph_batched_s) = ilt_leg(args, ilt, dev, sync)
+         ilt_graph_batched_s, ilt_roof) = ilt_leg(args, ilt, dev, sync)
     if "volume" in legs:
         vol_s = volume_leg(args, masks, sync)
     solve = run_ilt_leg(args, sync) if "solve" in legs else None
@@ -568,7 +757,10 @@ def main():
         clip_ilt_s, clip_plain_s = clips_leg(args, dev, sync)
     if "dense" in legs:
         dense_images_per_s = dense_leg(args, plain_plans[0], masks, sync)
-    socs = socs_leg(args, dev, sync) if "socs" in legs else None
+    socs = k24 = None
+    if "socs" in legs:
+        k24, socs = socs_leg(args, dev, sync)
+    cfg1 = cfg1_leg(args, dev, sync, "cpu" in legs) if "cfg1" in legs else None
     euv = euv_leg(args, dev, sync) if "euv" in legs else None
     if args.copy_calibration > 0:
         nb = args.copy_calibration << 20
@@ -591,6 +783,13 @@ def main():
                "sample": "1 image (mask seed 1000, defocus -100 nm) of the 64 x 5 batch: the "
                          "oracle's float64 Abbe sum with scipy.fft workers=-1 and 64 MiB "
                          f"transform batches; {cpu_s:.2f} s on {cores} host threads"}
+        if socs is not None:
+            socs["cpu_baseline"] = cpu_socs_leg(k24, sync)
+            socs["vs_cpu"] = socs["images_per_s"] / socs["cpu_baseline"]["value"]
+        cpu_ilt = cpu_ilt_leg() if "ilt" in legs else None
+        cpu_clips = cpu_clips_leg() if "clips" in legs else None
+    else:
+        cpu_ilt = cpu_clips = None
 
     line = {
         "metric": "aerial images/s and ILT grad steps/s at 1024² (batched vs plain, ≤1e-5 rel err)",
@@ -625,6 +824,9 @@ def main():
                            "defocus +-50 nm, forward + adjoint, one persistent coherent-system "
                            "launch per pass, graph-captured step; plain = one clip per call, "
                            "eager, sync per clip",
+        "clip_ilt_cpu_baseline": cpu_clips,
+        "clip_ilt_vs_cpu": ratio(None if cpu_clips is None else 1.0 / cpu_clips["value"],
+                                 clip_ilt_s),
         "ilt_steps_per_s": ratio(1.0, ilt_s),
         "ilt_batched_steps_per_s": ratio(1.0, ilt_batched_s),
         "ilt_blur_steps_per_s": ratio(1.0, ilt_blur_s),
@@ -648,11 +850,18 @@ def main():
                       "blur = PEB sigma 2 px and mask smoothing 1 px on; graph = the "
                       "same autograd step captured once into a graph and replayed, 1 mask "
                       f"and {ILT_BATCH} masks per step",
+        "ilt_cpu_baseline": cpu_ilt,
+        "ilt_batched_vs_cpu": ratio(None if cpu_ilt is None else 1.0 / cpu_ilt["value"],
+                                    ilt_batched_s),
+        "ilt_roofline": ilt_roof,
         "socs": socs,
+        "plain_fp64": cfg1,
         "euv_multilayer": euv,
         "fp32_rel_err_vs_fp64": err,
-        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+                     "valu_unpacked_ceiling": VALU_UNPACKED_TFLOPS,
+                     "frac_of_unpacked": achieved_tflops / VALU_UNPACKED_TFLOPS,
                      "traffic": traffic,
                      "traffic_source": ("profiles/k_field_fwd_abbe_1024_64x5_counters.md"
                                         if traffic is not None else None),

@@ -108,6 +108,10 @@ typedef struct {
   int32_t column_splits;
   int32_t chunk;           /* systems per work item */
   int32_t grid_blocks;     /* persistent blocks of the coherent-system kernel */
+  int32_t adj_block_threads; /* the adjoint kernel's shape */
+  int32_t adj_column_splits;
+  int32_t adj_chunk;       /* systems per adjoint work item for one mask */
+  int32_t socs_sweeps;     /* SOCS plans: Jacobi sweeps of the decomposition (else 0) */
 } lb_plan_info;
 
 /* Library version (major * 10000 + minor * 100 + patch). */
@@ -360,13 +364,18 @@ lb_status lb_plan_shapes(lb_plan plan, int32_t adjoint, int32_t* shapes_pairs,
                          int32_t max_shapes, int32_t* n_shapes);
 lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads,
                             int32_t column_splits, int32_t chunk);
+/* The work items a call of n_masks masks runs: systems per item and item count. */
+lb_status lb_plan_work_items(lb_plan plan, int32_t n_masks, int32_t adjoint, int32_t* chunk,
+                             int32_t* n_items);
 
-/* Timing aid (no counterpart in the reference): when enabled, lb_forward
- * records a HIP event pair on its stream around each launch of the
- * coherent-system kernel; lb_profile_read waits for them and returns the
+/* Timing aid (no counterpart in the reference): when enabled, the forward and
+ * adjoint entry points record a HIP event pair on their stream around each
+ * launch (or staged pass) of the coherent-system kernel; lb_profile_read
+ * (forward launches) and lb_profile_read_adjoint wait for them and return the
  * launch count and summed kernel time.  Enabling resets the record. */
 lb_status lb_profile_enable(lb_plan plan, int32_t on);
 lb_status lb_profile_read(lb_plan plan, int32_t* launches, double* total_ms);
+lb_status lb_profile_read_adjoint(lb_plan plan, int32_t* launches, double* total_ms);
 
 #ifdef __cplusplus
 }

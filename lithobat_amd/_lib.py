@@ -40,7 +40,8 @@ class ImagingDesc(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [(k, C.c_int32) for k in (
         "n", "m", "box", "band", "n_planes", "n_systems", "precision", "n_kernels",
-        "block_threads", "column_splits", "chunk", "grid_blocks")]
+        "block_threads", "column_splits", "chunk", "grid_blocks", "adj_block_threads",
+        "adj_column_splits", "adj_chunk", "socs_sweeps")]
 
 
 # LB_STATIC_SCRATCH_BYTES of the header: lb_forward_static's fixed device workspace
@@ -52,7 +53,8 @@ EXPORTS = [
     "lb_socs_kernels", "lb_workspace_bytes", "lb_forward", "lb_forward_static", "lb_backward",
     "lb_fields", "lb_fields_backward", "lb_fields_backward_workspace_bytes",
     "lb_backward_spectrum_complex",
-    "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read", "lb_plan_shapes",
+    "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read",
+    "lb_profile_read_adjoint", "lb_plan_shapes", "lb_plan_work_items",
     "lb_plan_set_shape", "lb_batch_create", "lb_batch_destroy", "lb_batch_workspace_bytes",
     "lb_batch_forward", "lb_batch_backward", "lb_dense_workspace_bytes", "lb_forward_dense",
     "lb_backward_dense", "lb_gaussian_blur_workspace_bytes", "lb_gaussian_blur",
@@ -100,8 +102,10 @@ def load() -> C.CDLL:
         "lb_backward_spectrum_complex": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
         "lb_profile_enable": ([vp, i32], C.c_int),
         "lb_profile_read": ([vp, C.POINTER(i32), C.POINTER(C.c_double)], C.c_int),
+        "lb_profile_read_adjoint": ([vp, C.POINTER(i32), C.POINTER(C.c_double)], C.c_int),
         "lb_plan_shapes": ([vp, i32, C.POINTER(i32), i32, C.POINTER(i32)], C.c_int),
         "lb_plan_set_shape": ([vp, i32, i32, i32, i32], C.c_int),
+        "lb_plan_work_items": ([vp, i32, i32, C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "lb_dense_workspace_bytes": ([vp, i32], sz),
         "lb_forward_dense": ([vp, vp, i32, i32, vp, vp, sz, vp], C.c_int),
         "lb_backward_dense": ([vp, vp, i32, i32, vp, vp, vp, sz, vp], C.c_int),

@@ -115,12 +115,14 @@ struct Plan {
   std::vector<double> weights;
   double energy = 1.0;
   int n_band = 0;
+  int jacobi_sweeps = 0;
   void* phi128 = nullptr;  // [K][W][W] complex128
   double* refl = nullptr;  // reflective mask: [3][n_refl] angle, |r|, arg r
   // optional event timing of the coherent-system kernel (lb_profile_*)
   int profile = 0;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, int>> ev_pairs;  // (start, stop) indices per launch
+  std::vector<std::pair<int, int>> ev_pairs;  // (start, stop) indices per forward launch
+  std::vector<std::pair<int, int>> ev_pairs_adj;  // the same for the adjoint launches
 
   ~Plan() {
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -452,7 +454,7 @@ int grid_for(const Plan& p, const FieldConfig& fc, int n_items, bool adjoint) {
   return std::max(1, std::min(n_items, p.cus * occ));
 }
 
-hipError_t profile_events(Plan& p, int* e0, int* e1) {
+hipError_t profile_events(Plan& p, int* e0, int* e1, bool adjoint = false) {
   for (int k = 0; k < 2; ++k) {
     hipEvent_t e;
     hipError_t r = hipEventCreate(&e);
@@ -461,7 +463,7 @@ hipError_t profile_events(Plan& p, int* e0, int* e1) {
   }
   *e0 = (int)p.ev_pool.size() - 2;
   *e1 = (int)p.ev_pool.size() - 1;
-  p.ev_pairs.emplace_back(*e0, *e1);
+  (adjoint ? p.ev_pairs_adj : p.ev_pairs).emplace_back(*e0, *e1);
   return hipSuccess;
 }
 
@@ -582,6 +584,11 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
   FieldArgs a = field_args(p, spectrum ? spectrum : ws + w.Xm, w);
   a.queue = (int*)(ws + w.queue);
   LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
+  int e0 = -1, e1 = -1;
+  if (p.profile) {
+    LB_HIP(profile_events(p, &e0, &e1, true));
+    LB_HIP(hipEventRecord(p.ev_pool[e0], st));
+  }
   if (fc.csplit == 0) {
     for (int i0 = 0; i0 < w.n_img; i0 += w.pass_img)
       LB_HIP(field_adjoint_staged(p.precision, fc, a, gM, ws + w.gX, ws + w.G, i0,
@@ -589,6 +596,7 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
   } else {
     LB_HIP(field_adjoint(p.precision, fc, a, gM, ws + w.gX, st));
   }
+  if (p.profile) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
   return bwd_post<T>(p, n_masks, dmask, ws, w, st, cout);
 }
 
@@ -866,6 +874,7 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
   cx<double>* V = nullptr;
   double* ev = nullptr;
   int* sweeps = nullptr;
+  void* jlog = nullptr;
   PupilSys* sys_dev = nullptr;
   double* dc = nullptr;
   int* idx_dev = nullptr;
@@ -876,6 +885,7 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
     (void)hipFree(V);
     (void)hipFree(ev);
     (void)hipFree(sweeps);
+    (void)hipFree(jlog);
     (void)hipFree(sys_dev);
     (void)hipFree(dc);
     (void)hipFree(idx_dev);
@@ -897,6 +907,8 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
   LB_HIP_C(hipMalloc(&V, sizeof(cx<double>) * Sa * Sa));
   LB_HIP_C(hipMalloc(&ev, sizeof(double) * Sa));
   LB_HIP_C(hipMalloc(&sweeps, sizeof(int)));
+  constexpr int kSweeps = 40;
+  LB_HIP_C(hipMalloc(&jlog, jacobi_log_bytes(Sa, kSweeps)));
   LB_HIP_C(hipMalloc(&sys_dev, sizeof(PupilSys) * Sa));
   LB_HIP_C(hipMalloc(&dc, sizeof(double) * Sa));
   std::vector<double> evals(S);
@@ -905,9 +917,10 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
     p.pp.W = Wt;
     LB_HIP_C(pupil_tables<double>(sys_dev, S, p.pp, A, dc, 0));
     LB_HIP_C(gram_f64(A, S, F, G, 0));
-    LB_HIP_C(jacobi_eigh_f64(G, V, S, 40, ev, sweeps, 0));
+    LB_HIP_C(jacobi_eigh_f64(G, V, S, kSweeps, jlog, ev, sweeps, 0));
     LB_HIP_C(hipDeviceSynchronize());
     LB_HIP_C(hipMemcpy(evals.data(), ev, sizeof(double) * S, hipMemcpyDeviceToHost));
+    LB_HIP_C(hipMemcpy(&p.jacobi_sweeps, sweeps, sizeof(int), hipMemcpyDeviceToHost));
   }
   // order, truncation: hopkins.py:172-182
   std::vector<int> order(S);
@@ -999,6 +1012,10 @@ lb_status lb_plan_info_get(lb_plan plan, lb_plan_info* info) {
   info->column_splits = p.fwd.csplit;
   info->chunk = w.chunk;
   info->grid_blocks = grid_for(p, p.fwd, w.n_items, false);
+  info->adj_block_threads = p.adj.nt;
+  info->adj_column_splits = p.adj.csplit;
+  info->adj_chunk = layout(p, 1, true).chunk;
+  info->socs_sweeps = p.socs ? p.jacobi_sweeps : 0;
   return LB_OK;
 }
 
@@ -1049,6 +1066,15 @@ lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads
   return LB_OK;
 }
 
+lb_status lb_plan_work_items(lb_plan plan, int32_t n_masks, int32_t adjoint, int32_t* chunk,
+                             int32_t* n_items) {
+  if (!plan || !chunk || !n_items || n_masks < 1) return fail(LB_ERR_ARG, "bad argument");
+  const Work w = layout(plan->p, n_masks, adjoint != 0);
+  *chunk = w.chunk;
+  *n_items = w.n_items;
+  return LB_OK;
+}
+
 lb_status lb_plan_shapes(lb_plan plan, int32_t adjoint, int32_t* shapes, int32_t max_shapes,
                          int32_t* n_shapes) {
   if (!plan || !n_shapes) return fail(LB_ERR_ARG, "null argument");
@@ -1073,23 +1099,37 @@ lb_status lb_profile_enable(lb_plan plan, int32_t on) {
   for (hipEvent_t e : p.ev_pool) (void)hipEventDestroy(e);
   p.ev_pool.clear();
   p.ev_pairs.clear();
+  p.ev_pairs_adj.clear();
   p.profile = on ? 1 : 0;
   return LB_OK;
 }
 
-lb_status lb_profile_read(lb_plan plan, int32_t* launches, double* total_ms) {
-  if (!plan || !launches || !total_ms) return fail(LB_ERR_ARG, "null argument");
-  Plan& p = plan->p;
+}  // extern "C"
+
+static lb_status profile_sum(Plan& p, const std::vector<std::pair<int, int>>& pairs,
+                             int32_t* launches, double* total_ms) {
   double t = 0;
-  for (auto& pr : p.ev_pairs) {
+  for (auto& pr : pairs) {
     LB_HIP(hipEventSynchronize(p.ev_pool[pr.second]));
     float ms = 0;
     LB_HIP(hipEventElapsedTime(&ms, p.ev_pool[pr.first], p.ev_pool[pr.second]));
     t += ms;
   }
-  *launches = (int32_t)p.ev_pairs.size();
+  *launches = (int32_t)pairs.size();
   *total_ms = t;
   return LB_OK;
+}
+
+extern "C" {
+
+lb_status lb_profile_read(lb_plan plan, int32_t* launches, double* total_ms) {
+  if (!plan || !launches || !total_ms) return fail(LB_ERR_ARG, "null argument");
+  return profile_sum(plan->p, plan->p.ev_pairs, launches, total_ms);
+}
+
+lb_status lb_profile_read_adjoint(lb_plan plan, int32_t* launches, double* total_ms) {
+  if (!plan || !launches || !total_ms) return fail(LB_ERR_ARG, "null argument");
+  return profile_sum(plan->p, plan->p.ev_pairs_adj, launches, total_ms);
 }
 
 size_t lb_workspace_bytes(lb_plan plan, int32_t n_masks, int32_t backward) {

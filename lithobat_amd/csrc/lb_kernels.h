@@ -241,9 +241,11 @@ hipError_t pupil_tables(const PupilSys* sys_dev, int n_sys, const PupilParams& p
 // ---- SOCS decomposition (lb_socs.hip), FP64 ------------------------------
 // G[s][t] = sum_f conj(A[s][f]) A[t][f]
 hipError_t gram_f64(const cx<double>* A, int S, long long F, cx<double>* G, hipStream_t st);
-// Hermitian Jacobi: G overwritten (diagonal = eigenvalues), V = eigenvectors (columns)
-hipError_t jacobi_eigh_f64(cx<double>* G, cx<double>* V, int S, int max_sweeps, double* evals,
-                           int* sweeps_out, hipStream_t st);
+// Hermitian Jacobi: G overwritten (diagonal = eigenvalues), V = eigenvectors (columns);
+// log: jacobi_log_bytes(S, max_sweeps) of scratch for the rotations
+size_t jacobi_log_bytes(int S, int max_sweeps);
+hipError_t jacobi_eigh_f64(cx<double>* G, cx<double>* V, int S, int max_sweeps, void* log,
+                           double* evals, int* sweeps_out, hipStream_t st);
 // phi[k][f] = sum_s A[s][f] V[s][idx[k]] * inv_sqrt[k]
 hipError_t socs_phi_f64(const cx<double>* A, const cx<double>* V, int S, long long F,
                         const int* idx, const double* inv_sqrt, int K, cx<double>* phi,

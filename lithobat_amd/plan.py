@@ -203,6 +203,14 @@ class ImagingPlan:
         autotune.apply(self, n_masks)
         self._shape_for = n_masks
 
+    def work_items(self, n_masks: int, adjoint: bool = False) -> tuple[int, int]:
+        """(systems per work item, work items) of a call of ``n_masks`` masks with the
+        shapes currently selected (lb_plan_work_items)."""
+        chunk, items = C.c_int32(), C.c_int32()
+        _lib.check(self._lib.lb_plan_work_items(self._h, int(n_masks), 1 if adjoint else 0,
+                                                C.byref(chunk), C.byref(items)))
+        return int(chunk.value), int(items.value)
+
     def workspace_bytes(self, n_masks: int, backward: bool = False) -> int:
         return int(self._lib.lb_workspace_bytes(self._h, int(n_masks), 1 if backward else 0))
 
@@ -395,11 +403,13 @@ class ImagingPlan:
         """Record HIP events around every coherent-system kernel launch (resets)."""
         _lib.check(self._lib.lb_profile_enable(self._h, 1 if on else 0))
 
-    def profile_read(self) -> tuple[int, float]:
-        """(launches, summed kernel milliseconds) since ``profile(True)``."""
+    def profile_read(self, adjoint: bool = False) -> tuple[int, float]:
+        """(launches, summed kernel milliseconds) of the forward (or adjoint)
+        coherent-system launches since ``profile(True)``."""
         n = C.c_int32()
         t = C.c_double()
-        _lib.check(self._lib.lb_profile_read(self._h, C.byref(n), C.byref(t)))
+        fn = self._lib.lb_profile_read_adjoint if adjoint else self._lib.lb_profile_read
+        _lib.check(fn(self._h, C.byref(n), C.byref(t)))
         return int(n.value), float(t.value)
 
     def kernels(self, device=None) -> torch.Tensor:

@@ -628,13 +628,14 @@ def socs_gradient(k: SOCS, fields, sensitivity):
     return 2.0 * np.real(np.tensordot(k.weights, adj, axes=1))
 
 
-def abbe_gradient(mask, optics, grid, sensitivity, planes=None):
+def abbe_gradient(mask, optics, grid, sensitivity, planes=None, src=None):
     """d/dm of sum_x g(x) I_raw(x) for the scalar/vector Abbe sum.
 
     The same adjoint as ilt.py:262-266 with the source points in place of the
     eigen-kernels: 2 Re sum_j w_j ifft2(conj(H_j) fft2(g E_j)), summed over
     planes when ``sensitivity`` is (P, n, n).  Pinned by central finite
-    differences, and at full rank by ``socs_gradient``.
+    differences, and at full rank by ``socs_gradient``.  ``src``: the source points
+    (default: the optics' own).
     """
     n = grid.n_pixels
     if planes is None:
@@ -644,7 +645,8 @@ def abbe_gradient(mask, optics, grid, sensitivity, planes=None):
     vector = optics.imaging_model.lower() == "vector"
     exact = bool(optics.exact_defocus) or vector
     zern = optics.zernike_coeffs or None
-    src = source_points(optics)
+    if src is None:
+        src = source_points(optics)
     Mf = np.fft.fft2(np.asarray(mask, dtype=np.complex128))
     FX, FY = make_freq_grid(n, grid.pixel_size)
     acc = np.zeros((n, n), dtype=np.complex128)
