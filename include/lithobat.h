@@ -129,6 +129,14 @@ lb_status lb_abbe_plan_create(const lb_imaging_desc* desc, lb_plan* out);
 lb_status lb_socs_plan_create(const lb_imaging_desc* desc, double tol, int32_t max_kernels,
                               lb_plan* out);
 
+/* _build_fft_pupil (expose/aerial_image.py:169-209): the scalar pupil P(f - f_s) =
+ * exp(i W) on the in-pupil bins of the whole N x N grid, zeros elsewhere, in FFT order
+ * (DC at [0][0]), complex128, for source offset (fsx, fsy) [1/m] and the defocus / image
+ * index of `plane`; W is the defocus OPD plus the Zernike terms, as the plans use.  Any
+ * N >= 1.  Allocates and synchronises (a set-up call, not for graph capture). */
+lb_status lb_pupil_dense(const lb_imaging_desc* desc, double fsx, double fsy, int32_t plane,
+                         void* out_c128, void* stream);
+
 lb_status lb_plan_destroy(lb_plan plan);
 lb_status lb_plan_info_get(lb_plan plan, lb_plan_info* info);
 

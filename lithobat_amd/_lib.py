@@ -52,7 +52,7 @@ EXPORTS = [
     "lb_plan_destroy", "lb_plan_info_get", "lb_plan_clear", "lb_socs_weights",
     "lb_socs_kernels", "lb_workspace_bytes", "lb_forward", "lb_forward_static", "lb_backward",
     "lb_fields", "lb_fields_backward", "lb_fields_backward_workspace_bytes",
-    "lb_backward_spectrum_complex",
+    "lb_backward_spectrum_complex", "lb_pupil_dense",
     "lb_fields_workspace_bytes", "lb_profile_enable", "lb_profile_read",
     "lb_profile_read_adjoint", "lb_plan_shapes", "lb_plan_work_items",
     "lb_plan_set_shape", "lb_batch_create", "lb_batch_destroy", "lb_batch_workspace_bytes",
@@ -86,6 +86,8 @@ def load() -> C.CDLL:
         "lb_abbe_plan_create": ([C.POINTER(ImagingDesc), plan_p], C.c_int),
         "lb_socs_plan_create": ([C.POINTER(ImagingDesc), C.c_double, i32, plan_p], C.c_int),
         "lb_plan_destroy": ([vp], C.c_int),
+        "lb_pupil_dense": ([C.POINTER(ImagingDesc), C.c_double, C.c_double, i32, vp, vp],
+                           C.c_int),
         "lb_plan_info_get": ([vp, C.POINTER(PlanInfo)], C.c_int),
         "lb_plan_clear": ([vp, C.POINTER(C.c_double)], C.c_int),
         "lb_socs_weights": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double),

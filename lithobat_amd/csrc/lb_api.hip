@@ -991,6 +991,45 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
   return LB_OK;
 }
 
+lb_status lb_pupil_dense(const lb_imaging_desc* d, double fsx, double fsy, int32_t plane,
+                         void* out, void* stream) {
+  g_err.clear();
+  if (!d || !out) return fail(LB_ERR_ARG, "null descriptor or output");
+  if (d->n < 1) return fail(LB_ERR_ARG, "grid side must be >= 1");
+  if (!(d->pixel_size > 0) || !(d->wavelength > 0) || !(d->na > 0))
+    return fail(LB_ERR_ARG, "pixel_size, wavelength and NA must be > 0");
+  if (d->n_planes < 1 || !d->defocus || !d->n_image || plane < 0 || plane >= d->n_planes)
+    return fail(LB_ERR_ARG, "plane out of range");
+  if (d->n_zernike > 0 && (!d->zernike_j || !d->zernike_c))
+    return fail(LB_ERR_ARG, "null Zernike arrays");
+  lb_imaging_desc thin = *d;
+  thin.n_refl = 0;  // the pupil alone: no mask-side reflection
+  Plan p;
+  lb_status rc = set_geometry_common(p, &thin);
+  if (rc != LB_OK) return rc;
+  PupilSys s{};
+  s.fsx = fsx;
+  s.fsy = fsy;
+  s.weight = 1.0;
+  s.defocus = d->defocus[plane];
+  s.n_image = d->n_image[plane];
+  s.comp = -1;
+  s.box_y = s.box_x = 0;  // the whole grid from bin 0: FFT order
+  p.pp.W = p.N;
+  PupilSys* sys_dev = nullptr;
+  double* dc = nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMalloc(&sys_dev, sizeof(PupilSys));
+  if (e == hipSuccess) e = hipMalloc(&dc, sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(sys_dev, &s, sizeof(PupilSys), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = pupil_tables<double>(sys_dev, 1, p.pp, (cx<double>*)out, dc, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(sys_dev);
+  (void)hipFree(dc);
+  if (e != hipSuccess) return fail(LB_ERR_HIP, std::string("dense pupil: ") + hipGetErrorString(e));
+  return LB_OK;
+}
+
 lb_status lb_plan_destroy(lb_plan plan) {
   delete plan;
   return LB_OK;

@@ -12,12 +12,12 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .plan import ImagingPlan, _image_index, get_plan
+from .plan import ImagingPlan, _image_index, get_plan, pupil_dense
 from .source import SourcePoints, source_points  # noqa: F401  (re-exported)
 
 __all__ = ["compute_aerial_image", "compute_aerial_planes", "compute_aerial_batch",
            "normalisation_scale", "source_points", "SourcePoints", "SOURCE_TILT_SIGN",
-           "FFT_BATCH_BYTES"]
+           "FFT_BATCH_BYTES", "build_fft_pupil"]
 
 #: aerial_image.py:73 -- the pupil is evaluated at f - f_s (a source point at
 #: +sigma tilts the mask by exp(-i 2 pi f_s . x)); the device tables follow it.
@@ -25,6 +25,17 @@ SOURCE_TILT_SIGN = -1.0
 #: aerial_image.py:78 -- kept for API compatibility; the device path batches by
 #: work items instead of by bytes.
 FFT_BATCH_BYTES = 64 * 2 ** 20
+
+
+def build_fft_pupil(optics, grid, fs_x: float = 0.0, fs_y: float = 0.0, defocus=None,
+                    n_image=None) -> np.ndarray:
+    """aerial_image.py:169-209 (_build_fft_pupil): the pupil P(f - f_s) = exp(i W) on the
+    in-pupil bins of the N x N grid, zeros elsewhere, in np.fft.fft2 ordering (DC at
+    [0, 0]), complex128.  The reference takes the geometry arrays of _pupil_geometry; here
+    the source offset (fs_x, fs_y) [1/m] is given and the geometry, defocus OPD (the
+    optics' defocus and image index unless passed) and Zernike phase are evaluated on
+    the device in FP64."""
+    return pupil_dense(optics, grid, fs_x, fs_y, defocus, n_image)
 
 
 def normalisation_scale(mode: str, peak: float, clear: float, dose: float = 1.0) -> float:

@@ -423,6 +423,45 @@ class ImagingPlan:
         return out
 
 
+def pupil_dense(optics, grid, fs_x: float = 0.0, fs_y: float = 0.0, defocus=None,
+                n_image=None) -> np.ndarray:
+    """The scalar pupil P(f - f_s) on the whole grid in FFT order (complex128), computed on
+    the device (lb_pupil_dense) with the optics' defocus (or ``defocus``), image index,
+    Zernike terms and defocus model."""
+    lib = _lib.load()
+    if not torch.cuda.is_available():
+        raise RuntimeError("the imaging library needs a HIP device (torch.cuda is unavailable)")
+    keep = []
+
+    def arr(x, dt=np.float64):
+        a = np.ascontiguousarray(np.asarray(x, dtype=dt))
+        keep.append(a)
+        return a
+
+    zern = {int(k): float(v) for k, v in (optics.zernike_coeffs or {}).items()}
+    for j in zern:
+        if not 1 <= j <= 21:  # expose/pupil.py:128-130: the term contributes zero
+            logger.warning("Zernike Noll index %d not implemented; returning 0.", j)
+    d = _lib.ImagingDesc()
+    d.n = int(grid.n_pixels)
+    d.pixel_size = float(grid.pixel_size)
+    d.wavelength = float(optics.wavelength)
+    d.na = float(optics.NA)
+    d.exact_defocus = 1 if optics.exact_defocus else 0
+    d.n_planes = 1
+    d.defocus = _dptr(arr([optics.defocus if defocus is None else defocus]), C.c_double)
+    d.n_image = _dptr(arr([_image_index(optics) if n_image is None else n_image]), C.c_double)
+    d.n_zernike = len(zern)
+    d.zernike_j = _dptr(arr(list(zern.keys()) or [0], np.int32), C.c_int32)
+    d.zernike_c = _dptr(arr(list(zern.values()) or [0.0]), C.c_double)
+    d.precision = 64
+    n = d.n
+    out = torch.empty((n, n), dtype=torch.complex128, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.check(lib.lb_pupil_dense(C.byref(d), float(fs_x), float(fs_y), 0, out.data_ptr(), st))
+    return out.cpu().numpy()
+
+
 # ----------------------------------------------------------------------------
 # Plan cache keyed on the optical set-up (the role of PrintModel.kernels'
 # cache in the reference, opc/model.py:201-221).

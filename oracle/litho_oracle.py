@@ -308,6 +308,17 @@ def pupil_phase(rho, phi, wavelength, NA, defocus, n_image, zernike, exact):
     return W
 
 
+def build_fft_pupil(rho, phi, inside, wavelength, NA, defocus, n_image, zernike, exact=False):
+    """aerial_image.py:169-209: exp(i W) on the in-pupil bins, zeros elsewhere, in the
+    layout of the geometry arrays (FFT order when they come from make_freq_grid)."""
+    idx = np.nonzero(inside)
+    W = pupil_phase(rho[idx], None if phi is None else phi[idx], wavelength, NA, defocus,
+                    n_image, zernike, exact)
+    pupil = np.zeros(rho.shape, dtype=np.complex128)
+    pupil[idx] = np.exp(1j * W)
+    return pupil
+
+
 def normalisation_scale(mode, peak, clear, dose=1.0):
     """aerial_image.py:217-241."""
     m = mode.lower()

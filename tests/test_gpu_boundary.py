@@ -239,3 +239,51 @@ def test_unknown_noll_indices_contribute_zero_with_a_warning(device, caplog):
     assert np.array_equal(odd, base)
     msgs = [r.getMessage() for r in caplog.records]
     assert sum("not implemented" in s for s in msgs) >= 3, msgs
+
+
+@pytest.mark.parametrize("n,px,shift,defocus,zern,exact", [
+    (64, 32e-9, (3, -2), 0.0, {4: 0.05}, True),
+    (100, 8e-9, (0, 0), 60e-9, {7: 0.03, 12: -0.02}, False),
+    (37, 20e-9, (-1, 2), -40e-9, {}, True),
+])
+def test_dense_pupil_matches_the_oracle(device, n, px, shift, defocus, zern, exact):
+    """_build_fft_pupil (aerial_image.py:169-209) on the device against the oracle's
+    restatement, FFT order, for on- and off-axis source points, any grid side."""
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.expose import build_fft_pupil
+    optics = OpticsConfig(NA=0.93, defocus=defocus, zernike_coeffs=zern, exact_defocus=exact)
+    grid = GridConfig(n_pixels=n, pixel_size=px)
+    fsx, fsy = shift[0] / (n * px), shift[1] / (n * px)
+    got = build_fft_pupil(optics, grid, fsx, fsy)
+    FX, FY = O.make_freq_grid(n, px)
+    rho, phi, inside = O.pupil_geometry(FX, FY, optics.wavelength, optics.NA, fsx, fsy,
+                                        need_phi=True)
+    want = O.build_fft_pupil(rho, phi, inside, optics.wavelength, optics.NA, defocus,
+                             optics.n_immersion, zern, exact)
+    assert got.shape == (n, n) and got.dtype == np.complex128
+    assert np.array_equal(got != 0, want != 0)
+    assert np.abs(got - want).max() <= 1e-12
+
+
+def test_dense_pupil_tilts_the_mask_the_other_way(device):
+    """tests/test_aerial_image.py:136-202 of the reference with the device pupil: a source
+    point at +f_s images the mask as the on-axis pupil does the mask tilted by
+    exp(-i 2 pi f_s x)."""
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.expose import build_fft_pupil
+    n, dx = 64, 32e-9
+    mask = np.random.default_rng(1).random((n, n))
+    sx, sy = 3, -2
+    optics = OpticsConfig(NA=0.93, source_grid=3, normalisation="none",
+                          zernike_coeffs={4: 0.05}, exact_defocus=True)
+    grid = GridConfig(n_pixels=n, pixel_size=dx)
+    eng = np.abs(np.fft.ifft2(np.fft.fft2(mask) * build_fft_pupil(
+        optics, grid, sx / (n * dx), sy / (n * dx)))) ** 2
+    on_axis = build_fft_pupil(optics, grid)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n))
+
+    def imaged(sign):
+        t = np.exp(sign * 2j * np.pi * (sx * X + sy * Y) / n)
+        return np.abs(np.fft.ifft2(np.fft.fft2(mask * t) * on_axis)) ** 2
+    assert np.abs(eng - imaged(-1)).max() < 1e-12 * eng.max()
+    assert np.abs(eng - imaged(+1)).max() > 1e-3 * eng.max()

@@ -156,7 +156,10 @@ def test_source_point_tilts_the_mask_the_other_way():
     def pupil(fx0, fy0):
         rho, phi, inside = O.pupil_geometry(FX, FY, lam, NA, fx0, fy0, need_phi=True)
         W = O.pupil_phase(rho, phi, lam, NA, 0.0, 1.0, {4: 0.05}, True)
-        return np.where(inside, np.exp(1j * W), 0)
+        P = O.build_fft_pupil(rho, phi, inside, lam, NA, 0.0, 1.0, {4: 0.05}, True)
+        assert np.array_equal(P, np.where(inside, np.exp(1j * W), 0))
+        assert P[0, 0] != 0 and np.count_nonzero(P) == np.count_nonzero(inside)
+        return P
 
     eng = np.abs(np.fft.ifft2(np.fft.fft2(mask) * pupil(fsx, fsy))) ** 2
     X, Y = np.meshgrid(np.arange(n), np.arange(n))
