@@ -18,8 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblithobat_amd.so")
-SOURCES = ["lb_api.hip", "lb_band.hip", "lb_blur.hip", "lb_field.hip", "lb_objective.hip",
-           "lb_pupil.hip", "lb_socs.hip", "lb_util.hip"]
+SOURCES = ["lb_api.hip", "lb_band.hip", "lb_band_any.hip", "lb_blur.hip", "lb_field.hip",
+           "lb_objective.hip", "lb_pupil.hip", "lb_socs.hip", "lb_util.hip"]
 ARCH = os.environ.get("LITHOBAT_ARCH", "gfx950")
 
 

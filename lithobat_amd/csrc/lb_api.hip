@@ -229,11 +229,10 @@ lb_status set_geometry_common(Plan& p, const lb_imaging_desc* d) {
 
 lb_status check_desc(const lb_imaging_desc* d) {
   if (!d) return fail(LB_ERR_ARG, "null descriptor");
-  if (d->n < 2) return fail(LB_ERR_ARG, "grid side must be >= 2");
-  if (!band_size_supported(d->n))
-    return fail(LB_ERR_UNSUPPORTED,
-                "grid side " + std::to_string(d->n) +
-                    " is not a supported FFT size (2^k for 8..2048, 3*2^k for 48..1536, or 80)");
+  if (d->n < 1) return fail(LB_ERR_ARG, "grid side must be >= 1");
+  if (!band_any_supported(d->n))
+    return fail(LB_ERR_UNSUPPORTED, "grid side " + std::to_string(d->n) + " exceeds " +
+                                        std::to_string(kBandAnyMax));
   if (!(d->pixel_size > 0) || !(d->wavelength > 0) || !(d->na > 0))
     return fail(LB_ERR_ARG, "pixel_size, wavelength and NA must be > 0");
   if (d->n_planes < 1 || !d->defocus || !d->n_image) return fail(LB_ERR_ARG, "need >= 1 plane");
@@ -260,7 +259,10 @@ lb_status choose_grid(Plan& p, int Wbox, int forced_m) {
         M = s;
         break;
       }
-    if (M == 0 || M >= p.N) M = p.N;
+    // a coarse grid no smaller than the image grid images on the image grid itself where
+    // that is a coherent-system size; otherwise it stays coarser than needed and the band
+    // synthesis folds its frequencies onto the image grid
+    if (M == 0 || (M >= p.N && field_size(p.N))) M = p.N;
   }
   if (!field_size(M))
     return fail(LB_ERR_UNSUPPORTED, "coarse grid side " + std::to_string(M) +

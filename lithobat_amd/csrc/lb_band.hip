@@ -948,6 +948,9 @@ template <typename T>
 hipError_t band_analysis(int N, const void* in, int cplx, long long img_stride,
                          long long slice_stride, int nsum, int nimg, int nu0, int Wb, T s,
                          cx<T>* RT, cx<T>* X, hipStream_t st) {
+  if (!band_size_supported(N) || Wb > N)
+    return band_analysis_any<T>(N, in, cplx, img_stride, slice_stride, nsum, nimg, nu0, Wb, s,
+                                RT, X, st);
   switch (N) {
 #define LB_CASE(n)                                                                             \
   case n:                                                                                      \
@@ -963,6 +966,8 @@ hipError_t band_analysis(int N, const void* in, int cplx, long long img_stride,
 template <typename T>
 hipError_t band_synthesis(int N, const cx<T>* Y, int nimg, int nu0, int Wb, int herm, T s,
                           int cout, cx<T>* TT, T* out, T* peak, hipStream_t st) {
+  if (!band_size_supported(N) || Wb > N)
+    return band_synthesis_any<T>(N, Y, nimg, nu0, Wb, herm, s, cout, TT, out, peak, st);
   switch (N) {
 #define LB_CASE(n) \
   case n:          \

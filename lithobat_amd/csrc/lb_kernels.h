@@ -25,7 +25,19 @@ inline hipError_t table_for(int L, const cx<T>** tw) {
 }
 
 // ---- band transforms (lb_band.hip) ---------------------------------------
+// band_size_supported: the sides with radix plans; band_analysis / band_synthesis
+// take the direct-sum path of lb_band_any.hip for any other side up to
+// kBandAnyMax and for bands wider than the grid.
 bool band_size_supported(int n);
+constexpr int kBandAnyMax = 4096;
+bool band_any_supported(int n);
+template <typename T>
+hipError_t band_analysis_any(int N, const void* in, int cplx, long long img_stride,
+                             long long slice_stride, int nsum, int nimg, int nu0, int Wb, T s,
+                             cx<T>* RT, cx<T>* X, hipStream_t st);
+template <typename T>
+hipError_t band_synthesis_any(int N, const cx<T>* Y, int nimg, int nu0, int Wb, int herm, T s,
+                              int cout, cx<T>* TT, T* out, T* peak, hipStream_t st);
 
 // Row stride of the band x N intermediates (RT / TT, [img][y][v]): the band
 // width rounded up to 16 complex values, so rows start on 128-byte lines
