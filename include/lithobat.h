@@ -366,8 +366,12 @@ lb_status lb_batch_backward(lb_batch batch, const void* const* masks,
  * list the coherent-system kernel shapes (threads, column splits) that fit
  * this plan, and select one for the forward (adjoint = 0) or adjoint
  * (adjoint = 1) kernel; chunk >= 0 also sets the systems per work item
- * (0 = automatic).  block_threads = 0 restores the plan's own default shape
- * for that kernel (and automatic work items unless chunk > 0). */
+ * (0 = automatic), and chunk = LB_CHUNK_SPREAD splits the call's systems,
+ * in image order, into one equal run per resident block of the kernel (at
+ * most a few image segments per block; shapes with column splits only).
+ * block_threads = 0 restores the plan's own default shape for that kernel
+ * (and automatic work items unless chunk > 0 or LB_CHUNK_SPREAD). */
+#define LB_CHUNK_SPREAD (-2)
 lb_status lb_plan_shapes(lb_plan plan, int32_t adjoint, int32_t* shapes_pairs,
                          int32_t max_shapes, int32_t* n_shapes);
 lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads,

@@ -23,6 +23,7 @@ from . import _lib
 TABLE_PATH = os.environ.get(
     "LITHOBAT_TUNING", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
 _lock = threading.Lock()
+SPREAD = -2  # chunk value: the call's systems in equal runs, one per resident block
 _table: dict | None = None
 
 
@@ -130,7 +131,18 @@ def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
     P = len(plan.planes)
     results = {}
     spp = plan.info["n_systems"]
-    chunk_cands = sorted({c for c in chunks if c <= max(spp, 1)})
+    chunk_cands = set(chunks)
+    # work-item sizes that fill the CUs in k rounds of one block each: a block
+    # of the large shapes holds a whole CU, and a round with a few idle CUs or
+    # a short last round costs as much as a full one
+    cus = torch.cuda.get_device_properties(masks.device).multi_processor_count
+    n_img = n_masks * P
+    for k in (1, 2, 3, 4):
+        per_img = (k * cus) // max(n_img, 1)
+        if per_img >= 1:
+            chunk_cands.add(-(-spp // per_img))
+    chunk_cands = sorted({c for c in chunk_cands if c <= max(spp, 1)})
+    chunk_cands.append(SPREAD)  # one run of systems per resident block
     timings = {}
     for adj in ([False, True] if adjoint else [False]):
         best = None
@@ -139,6 +151,8 @@ def tune(plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
                             dtype=plan.real_dtype) if adj else None
         for nt, cs in shapes(plan, adj):
             for ch in chunk_cands:
+                if ch == SPREAD and cs == 0:
+                    continue  # the staged kernels lay out their own work
                 set_shape(plan, adj, nt, cs, ch)
                 if adj:
                     ms = _time(lambda: plan.backward(masks, d_img), reps)

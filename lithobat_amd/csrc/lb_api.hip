@@ -364,11 +364,23 @@ struct Layout {
 
 struct Work {
   int n_masks, n_img, n_chunks, chunk, n_items;
+  // spread layout (chunk LB_CHUNK_SPREAD, fused kernels): one run of systems
+  // per resident block (runs of them), up to n_chunks image segments each
+  int spread, runs;
   int pass_img;  // staged path: images whose row tables are held at once
   int nrc;       // staged adjoint: band-gradient partials per item (row chunks)
   size_t Xm, RTm, partial, IM, seg, RTc, Y, TT, gM, RTg, GY, TTc, gX, GX, TTm, queue, G;
   size_t total;
 };
+
+// Blocks of a fused coherent-system kernel resident on the whole GPU at once
+// (LDS and threads per CU).
+int resident_blocks(const Plan& p, const FieldConfig& fc, bool adjoint) {
+  const size_t sm = field_smem_bytes(p.precision, fc, p.W, adjoint);
+  int occ = (int)std::max<size_t>(1, kLdsLimit / std::max<size_t>(sm, 1));
+  occ = std::min(occ, std::max(1, 2048 / fc.nt));
+  return p.cus * occ;
+}
 
 // use: the coherent-system kernel shape the work is laid out for (default:
 // the plan's own forward or adjoint shape).
@@ -378,24 +390,34 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0,
   w.n_masks = n_masks;
   w.n_img = n_masks * p.n_planes;
   int chunk = chunk_forced > 0 ? chunk_forced : (backward ? p.chunk_adj : p.chunk_fwd);
-  if (chunk <= 0) {
+  const FieldConfig& fc = use ? *use : (backward ? p.adj : p.fwd);
+  if (chunk == LB_CHUNK_SPREAD && fc.csplit > 0 && p.spp > 0) {
+    w.spread = 1;
+    w.runs = resident_blocks(p, fc, backward);
+    const long long total = (long long)w.n_img * p.spp;
+    const long long run = (total + w.runs - 1) / w.runs;
+    w.n_chunks = (int)((run + p.spp - 1) / p.spp) + 1;  // image segments per run, at most
+    w.chunk = LB_CHUNK_SPREAD;
+    w.n_items = w.n_chunks * w.runs;
+  } else if (chunk <= 0) {
     const long long total = (long long)w.n_img * p.spp;
     const long long target = 4LL * p.cus;
     int n_chunks = (int)std::max<long long>(1, std::min<long long>(p.spp, (target + w.n_img - 1) / std::max(w.n_img, 1)));
     (void)total;
     chunk = (p.spp + n_chunks - 1) / n_chunks;
   }
-  chunk = std::max(1, std::min(chunk, std::max(p.spp, 1)));
-  w.chunk = chunk;
-  w.n_chunks = std::max(1, (p.spp + chunk - 1) / chunk);
-  w.n_items = w.n_img * w.n_chunks;
+  if (!w.spread) {
+    chunk = std::max(1, std::min(chunk, std::max(p.spp, 1)));
+    w.chunk = chunk;
+    w.n_chunks = std::max(1, (p.spp + chunk - 1) / chunk);
+    w.n_items = w.n_img * w.n_chunks;
+  }
   const size_t C = p.csz(), R = p.rsz();
   Layout L;
   w.queue = L.take(256);  // work-item counter of the coherent-system launch
   w.Xm = L.take(C * n_masks * p.Wbm * p.Wbm);
   w.RTm = L.take(C * (size_t)n_masks * band_ld(p.Wbm) * p.N);
   w.partial = L.take(R * (size_t)w.n_items * p.M * p.M);
-  const FieldConfig& fc = use ? *use : (backward ? p.adj : p.fwd);
   w.nrc = backward && fc.csplit == 0 ? staged_row_chunks(fc, p.W) : 1;
   if (fc.csplit == 0) {
     // row tables of the staged path, a pass of images at a time, sized to
@@ -405,7 +427,7 @@ Work layout(const Plan& p, int n_masks, bool backward, int chunk_forced = 0,
                                                            staged_pass / per_img));
     w.G = L.take(per_img * (size_t)w.pass_img);
   }
-  {
+  if (!w.spread) {
     // split-sum scratch: coarse partials (forward) or band gradients (backward)
     const long long mm = (long long)p.M * p.M, bb = 2LL * p.Wbm * p.Wbm;
     size_t seg = R * (size_t)w.n_img * segment_splits(w.n_img, w.n_chunks, mm) * mm;
@@ -445,15 +467,13 @@ FieldArgs field_args(const Plan& p, const void* Xm, const Work& w) {
   a.W = p.W;
   a.box = p.box;
   a.wsys = p.wsys;
-  a.map = ItemMap{w.n_chunks, p.n_planes, p.spp, w.chunk, w.n_items};
+  a.map = ItemMap{w.n_chunks, p.n_planes, p.spp, w.chunk, w.n_items, w.spread, w.runs,
+                  (long long)w.n_img * p.spp};
   return a;
 }
 
 int grid_for(const Plan& p, const FieldConfig& fc, int n_items, bool adjoint) {
-  const size_t sm = field_smem_bytes(p.precision, fc, p.W, adjoint);
-  int occ = (int)std::max<size_t>(1, kLdsLimit / std::max<size_t>(sm, 1));
-  occ = std::min(occ, std::max(1, 2048 / fc.nt));
-  return std::max(1, std::min(n_items, p.cus * occ));
+  return std::max(1, std::min(n_items, resident_blocks(p, fc, adjoint)));
 }
 
 hipError_t profile_events(Plan& p, int* e0, int* e1, bool adjoint = false) {
@@ -488,15 +508,24 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
   // computed only when the caller asks for it
   T* pk = (T*)peak;
   if (pk) LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
+  const long long total = (long long)w.n_img * p.spp;
   if (p.direct) {
-    LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.N * p.N, (T*)images, pk,
-                          (T*)(ws + w.seg), st));
+    if (w.spread)
+      LB_HIP(spread_sum<T>(partial, w.n_img, 1, p.spp, total, w.runs, w.n_chunks,
+                           (long long)p.N * p.N, (T*)images, pk, st));
+    else
+      LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.N * p.N, (T*)images, pk,
+                            (T*)(ws + w.seg), st));
   } else {
     const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
     cx<T>* Y = (cx<T>*)(ws + w.Y);
     T* IM = (T*)(ws + w.IM);
-    LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.M * p.M, IM, (T*)nullptr,
-                          (T*)(ws + w.seg), st));
+    if (w.spread)
+      LB_HIP(spread_sum<T>(partial, w.n_img, 1, p.spp, total, w.runs, w.n_chunks,
+                           (long long)p.M * p.M, IM, (T*)nullptr, st));
+    else
+      LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.M * p.M, IM, (T*)nullptr,
+                            (T*)(ws + w.seg), st));
     LB_HIP(band_analysis<T>(p.M, IM, 0, (long long)p.M * p.M, 0, 1, w.n_img, -p.QI, p.WbI, invM2,
                             (cx<T>*)(ws + w.RTc), Y, st));
     LB_HIP(band_synthesis<T>(p.N, Y, w.n_img, -p.QI, p.WbI, 1, (T)1, 0, (cx<T>*)(ws + w.TT),
@@ -516,10 +545,11 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
     if (rc != LB_OK) return rc;
   }
   FieldConfig fc = p.fwd;
-  fc.grid = grid_for(p, fc, w.n_items, false);
+  fc.grid = w.spread ? w.runs : grid_for(p, fc, w.n_items, false);
   FieldArgs a = field_args(p, spectrum ? spectrum : ws + w.Xm, w);
-  a.queue = (int*)(ws + w.queue);
-  LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
+  // spread runs are static (block b takes items b, b + G, ...): no queue
+  a.queue = w.spread ? nullptr : (int*)(ws + w.queue);
+  if (a.queue) LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
   int e0 = -1, e1 = -1;
   const bool prof = p.profile && may_profile;
   if (prof) {
@@ -567,8 +597,13 @@ lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, h
                    bool cout = false) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
   cx<T>* GX = (cx<T>*)(ws + w.GX);
-  LB_HIP(segment_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes * w.n_chunks * w.nrc,
-                        2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, (T*)(ws + w.seg), st));
+  if (w.spread)
+    LB_HIP(spread_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes, p.spp,
+                         (long long)w.n_img * p.spp, w.runs, w.n_chunks, 2LL * p.Wbm * p.Wbm,
+                         (T*)GX, (T*)nullptr, st));
+  else
+    LB_HIP(segment_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes * w.n_chunks * w.nrc,
+                          2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, (T*)(ws + w.seg), st));
   LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, cout ? 0 : 1, invN2, cout ? 1 : 0,
                            (cx<T>*)(ws + w.TTm), (T*)dmask, (T*)nullptr, st));
   return LB_OK;
@@ -582,10 +617,10 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
   lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st, spectrum != nullptr);
   if (rc != LB_OK) return rc;
   FieldConfig fc = p.adj;
-  fc.grid = grid_for(p, fc, w.n_items, true);
+  fc.grid = w.spread ? w.runs : grid_for(p, fc, w.n_items, true);
   FieldArgs a = field_args(p, spectrum ? spectrum : ws + w.Xm, w);
-  a.queue = (int*)(ws + w.queue);
-  LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
+  a.queue = w.spread ? nullptr : (int*)(ws + w.queue);
+  if (a.queue) LB_HIP(fill<float>((float*)a.queue, 0.0f, 1, st));
   int e0 = -1, e1 = -1;
   if (p.profile) {
     LB_HIP(profile_events(p, &e0, &e1, true));
@@ -1052,7 +1087,7 @@ lb_status lb_plan_info_get(lb_plan plan, lb_plan_info* info) {
   info->block_threads = p.fwd.nt;
   info->column_splits = p.fwd.csplit;
   info->chunk = w.chunk;
-  info->grid_blocks = grid_for(p, p.fwd, w.n_items, false);
+  info->grid_blocks = w.spread ? w.runs : grid_for(p, p.fwd, w.n_items, false);
   info->adj_block_threads = p.adj.nt;
   info->adj_column_splits = p.adj.csplit;
   info->adj_chunk = layout(p, 1, true).chunk;
@@ -1088,7 +1123,7 @@ lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads
     if (!default_field_config(p, adjoint != 0, &d))
       return fail(LB_ERR_UNSUPPORTED, "no default coherent-system kernel shape");
     (adjoint ? p.adj : p.fwd) = d;
-    (adjoint ? p.chunk_adj : p.chunk_fwd) = chunk > 0 ? chunk : 0;
+    (adjoint ? p.chunk_adj : p.chunk_fwd) = chunk > 0 || chunk == LB_CHUNK_SPREAD ? chunk : 0;
     return LB_OK;
   }
   FieldConfig fc{p.M, block_threads, column_splits, 0};
@@ -1103,7 +1138,7 @@ lb_status lb_plan_set_shape(lb_plan plan, int32_t adjoint, int32_t block_threads
     p.adj = fc;
   else
     p.fwd = fc;
-  if (chunk >= 0) (adjoint ? p.chunk_adj : p.chunk_fwd) = chunk;
+  if (chunk >= 0 || chunk == LB_CHUNK_SPREAD) (adjoint ? p.chunk_adj : p.chunk_fwd) = chunk;
   return LB_OK;
 }
 
@@ -1208,7 +1243,7 @@ lb_status lb_forward_static(lb_plan plan, const void* masks, int32_t cplx, int32
   // lb_forward's own layout when it fits; else more systems per work item
   // (fewer coarse partials), then smaller staged passes
   Work w = layout(p, n_masks, false);
-  for (int chunk = w.chunk; w.total > cap && chunk < p.spp;) {
+  for (int chunk = std::max(w.chunk, 1); w.total > cap && chunk < p.spp;) {
     chunk = std::min(p.spp, chunk * 2);
     w = layout(p, n_masks, false, chunk);
   }

@@ -421,7 +421,29 @@ struct MapSource {
   int* queue;
   __device__ int count() const { return map.n_items; }
   __device__ ItemView<T> get(int item) const {
-    const int img = item / map.n_chunks, chunk = item - img * map.n_chunks;
+    int img, s0, s1;
+    const int spp = map.sys_per_plane;
+    if (map.spread) {
+      // block b's run [lo, hi) of the image-ordered systems, its k-th image
+      const int b = item % map.blocks, k = item / map.blocks;
+      const long long lo = (long long)b * map.total / map.blocks;
+      const long long hi = (long long)(b + 1) * map.total / map.blocks;
+      const long long im = lo / spp + k;
+      const long long a0 = max(lo, im * spp), a1 = min(hi, (im + 1) * spp);
+      if (a0 < a1 && im * spp < map.total) {
+        img = (int)im;
+        s0 = (int)(a0 - im * spp);
+        s1 = (int)(a1 - im * spp);
+      } else {
+        img = 0;  // empty segment: a valid image, no systems
+        s0 = s1 = 0;
+      }
+    } else {
+      img = item / map.n_chunks;
+      const int chunk = item - img * map.n_chunks;
+      s0 = chunk * map.chunk_sz;
+      s1 = min(s0 + map.chunk_sz, spp);
+    }
     const int plane = img % map.n_planes, mask = img / map.n_planes;
     ItemView<T> v;
     v.X = X + (long long)mask * Wb * Wb;
@@ -431,8 +453,8 @@ struct MapSource {
     v.W = W;
     v.box = box;
     v.wsys = wsys;
-    v.s0 = plane * map.sys_per_plane + chunk * map.chunk_sz;
-    v.s1 = min(v.s0 + map.chunk_sz, (plane + 1) * map.sys_per_plane);
+    v.s0 = plane * spp + s0;
+    v.s1 = plane * spp + s1;
     v.part = partial ? partial + (long long)item * M * M : nullptr;
     v.gM = gM ? gM + (long long)img * M * M : nullptr;
     v.gX = gX ? gX + (long long)item * Wb * Wb : nullptr;
@@ -705,6 +727,24 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
         // stage 2 reads its rows straight from Tp (columns outside this split
         // are zero) and leaves its outputs in Cb in the row-pass layout
         for (int a0 = 0; a0 < W; a0 += F::CPR) {
+          // This thread's band-gradient entries and transfer values for the
+          // stage 1 below.  In FP32 they load before the stage-2 work so their
+          // latency overlaps it (the previous system's stores to these entries
+          // were made visible by the barriers since); FP64 has no registers to
+          // spare and loads them after its transform.
+          const int a1 = a0 + rl1;
+          const int xr1 = wrap_idx(bx.x + a1, wrapN);
+          constexpr bool kEarly = sizeof(T) == 4;
+          cx<T> old[F::NRW], hv[F::NRW];
+          if constexpr (kEarly) {
+#pragma unroll
+            for (int r = 0; r < F::NRW; ++r) {
+              const int b = j1r + r * F::T1;
+              const bool in = a1 < W && b < W;
+              old[r] = in ? gout[(long long)xr1 * Wb + wrap_idx(bx.y + b, wrapN)] : cx<T>{0, 0};
+              hv[r] = in ? Hj[a1 * W + b] : cx<T>{0, 0};
+            }
+          }
 #pragma unroll
           for (int q = 0; q < F::Q2; ++q) {
             const int t = tid + q * NT;
@@ -727,21 +767,21 @@ __global__ __launch_bounds__(NT) void k_field_adj(Src src, int Wmax,
           }
           __syncthreads();
           {
-            const int a = a0 + rl1;
             cx<T> v[F::R1];
 #pragma unroll
             for (int r = 0; r < F::R1; ++r)
               v[r] = Cb[rl1 * RowLayout<M>::LDR + j1r * RowLayout<M>::SP + r];
             dft<F::R1, -1>(v);
-            if (a < W) {
-              const int xr = wrap_idx(bx.x + a, wrapN);
+            if (a1 < W) {
 #pragma unroll
-              for (int r = 0; r < F::R1; ++r) {
+              for (int r = 0; r < F::NRW; ++r) {
                 const int b = j1r + r * F::T1;
-                if (r < F::NRW && b < W) {
-                  const cx<T> gc = scale(mulc(v[r], Hj[a * W + b]), w2);
-                  cx<T>* dst = gout + (long long)xr * Wb + wrap_idx(bx.y + b, wrapN);
-                  *dst = *dst + gc;
+                if (b < W) {
+                  cx<T>* dst = gout + (long long)xr1 * Wb + wrap_idx(bx.y + b, wrapN);
+                  if constexpr (kEarly)
+                    *dst = old[r] + scale(mulc(v[r], hv[r]), w2);
+                  else
+                    *dst = *dst + scale(mulc(v[r], Hj[a1 * W + b]), w2);
                 }
               }
             }

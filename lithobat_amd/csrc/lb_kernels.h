@@ -69,6 +69,12 @@ struct ItemMap {
   int sys_per_plane;
   int chunk_sz;
   int n_items;
+  // spread: the n_img * sys_per_plane systems of the call, in image order,
+  // split into `blocks` equal runs, one per resident block; item k * blocks
+  // + b is the k-th image segment of block b's run (empty past its end)
+  int spread;
+  int blocks;
+  long long total;
 };
 
 struct FieldArgs {
@@ -140,6 +146,12 @@ hipError_t fill(T* p, T v, long long n, hipStream_t st);
 // scratch (nullable) holds groups * kSegmentSplits * len values for the split path.
 constexpr int kSegmentSplits = 16;
 int segment_splits(int groups, int cnt, long long len);  // scratch slices the split path uses
+// Sums the spread-layout partials (ItemMap::spread) of each group of `ipg`
+// consecutive images: out[g] = sum over the segments of images g*ipg ..
+// (g+1)*ipg - 1, in block order (deterministic).
+template <typename T>
+hipError_t spread_sum(const T* in, int groups, int ipg, int spp, long long total, int blocks,
+                      int segs, long long len, T* out, T* peak, hipStream_t st);
 template <typename T>
 hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, T* peak,
                        T* scratch, hipStream_t st);

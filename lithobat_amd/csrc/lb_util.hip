@@ -77,6 +77,59 @@ __global__ void k_segment_sum(const T* __restrict__ in, int cnt, long long len, 
   }
 }
 
+// Block whose run [b T / B, (b+1) T / B) holds system q.
+__device__ __forceinline__ int run_of(long long q, long long total, int blocks) {
+  return (int)(((q + 1) * blocks + total - 1) / total) - 1;
+}
+
+template <typename T>
+__global__ void k_spread_sum(const T* __restrict__ in, int ipg, int spp, long long total,
+                             int blocks, int segs, long long len, T* __restrict__ out,
+                             T* __restrict__ peak) {
+  const int g = blockIdx.y;
+  const long long q0 = (long long)g * ipg * spp, q1 = min(total, q0 + (long long)ipg * spp);
+  const int bf = run_of(q0, total, blocks), bl = run_of(q1 - 1, total, blocks);
+  T lmax = 0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
+       e += (long long)gridDim.x * blockDim.x) {
+    T s = 0;
+    for (int b = bf; b <= bl; ++b) {
+      const long long lo = (long long)b * total / blocks, hi = (long long)(b + 1) * total / blocks;
+      if (lo >= hi) continue;  // an empty run
+      const long long im0 = lo / spp;
+      for (int k = 0; k < segs; ++k) {
+        const long long im = im0 + k;
+        if (im * spp >= hi) break;
+        if (im / ipg == g) s += in[((long long)k * blocks + b) * len + e];
+      }
+    }
+    out[(long long)g * len + e] = s;
+    lmax = tmax(lmax, s);
+  }
+  if (peak) {
+    __shared__ T red[16];
+    for (int o = 32; o > 0; o >>= 1) lmax = tmax(lmax, (T)__shfl_xor(lmax, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T m = red[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = tmax(m, red[w]);
+      amax(peak + g, m);
+    }
+  }
+}
+
+template <typename T>
+hipError_t spread_sum(const T* in, int groups, int ipg, int spp, long long total, int blocks,
+                      int segs, long long len, T* out, T* peak, hipStream_t st) {
+  if (groups <= 0 || len <= 0) return hipSuccess;
+  long long bx = (len + 255) / 256;
+  if (bx > 4096) bx = 4096;
+  hipLaunchKernelGGL(k_spread_sum<T>, dim3((unsigned)bx, groups), dim3(256), 0, st, in, ipg, spp,
+                     total, blocks, segs, len, out, peak);
+  return hipGetLastError();
+}
+
 int segment_splits(int groups, int cnt, long long len) {
   long long bx = (len + 255) / 256;
   if (bx > 4096) bx = 4096;
@@ -116,6 +169,10 @@ template hipError_t fill<float>(float*, float, long long, hipStream_t);
 template hipError_t fill<double>(double*, double, long long, hipStream_t);
 template hipError_t segment_sum<float>(const float*, int, int, long long, float*, float*, float*,
                                        hipStream_t);
+template hipError_t spread_sum<float>(const float*, int, int, int, long long, int, int, long long,
+                                      float*, float*, hipStream_t);
+template hipError_t spread_sum<double>(const double*, int, int, int, long long, int, int,
+                                       long long, double*, double*, hipStream_t);
 template hipError_t segment_sum<double>(const double*, int, int, long long, double*, double*,
                                         double*, hipStream_t);
 
