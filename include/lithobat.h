@@ -111,7 +111,7 @@ typedef struct {
   int32_t adj_block_threads; /* the adjoint kernel's shape */
   int32_t adj_column_splits;
   int32_t adj_chunk;       /* systems per adjoint work item for one mask */
-  int32_t socs_sweeps;     /* SOCS plans: Jacobi sweeps of the decomposition (else 0) */
+  int32_t socs_sweeps;     /* SOCS plans: QL sweeps of the eigensolve (else 0) */
 } lb_plan_info;
 
 /* Library version (major * 10000 + minor * 100 + patch). */

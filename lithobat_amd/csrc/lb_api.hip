@@ -115,7 +115,7 @@ struct Plan {
   std::vector<double> weights;
   double energy = 1.0;
   int n_band = 0;
-  int jacobi_sweeps = 0;
+  int eig_sweeps = 0;  // QL sweeps of the SOCS eigensolve
   void* phi128 = nullptr;  // [K][W][W] complex128
   double* refl = nullptr;  // reflective mask: [3][n_refl] angle, |r|, arg r
   // optional event timing of the coherent-system kernel (lb_profile_*)
@@ -910,8 +910,8 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
   cx<double>* G = nullptr;
   cx<double>* V = nullptr;
   double* ev = nullptr;
-  int* sweeps = nullptr;
-  void* jlog = nullptr;
+  int* counts = nullptr;
+  void* ework = nullptr;
   PupilSys* sys_dev = nullptr;
   double* dc = nullptr;
   int* idx_dev = nullptr;
@@ -921,8 +921,8 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
     (void)hipFree(G);
     (void)hipFree(V);
     (void)hipFree(ev);
-    (void)hipFree(sweeps);
-    (void)hipFree(jlog);
+    (void)hipFree(counts);
+    (void)hipFree(ework);
     (void)hipFree(sys_dev);
     (void)hipFree(dc);
     (void)hipFree(idx_dev);
@@ -943,9 +943,8 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
   LB_HIP_C(hipMalloc(&G, sizeof(cx<double>) * Sa * Sa));
   LB_HIP_C(hipMalloc(&V, sizeof(cx<double>) * Sa * Sa));
   LB_HIP_C(hipMalloc(&ev, sizeof(double) * Sa));
-  LB_HIP_C(hipMalloc(&sweeps, sizeof(int)));
-  constexpr int kSweeps = 40;
-  LB_HIP_C(hipMalloc(&jlog, jacobi_log_bytes(Sa, kSweeps)));
+  LB_HIP_C(hipMalloc(&counts, 4 * sizeof(int)));
+  LB_HIP_C(hipMalloc(&ework, std::max(eigh_work_bytes(Sa), gram_work_bytes(Sa, F))));
   LB_HIP_C(hipMalloc(&sys_dev, sizeof(PupilSys) * Sa));
   LB_HIP_C(hipMalloc(&dc, sizeof(double) * Sa));
   std::vector<double> evals(S);
@@ -953,11 +952,19 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
     LB_HIP_C(hipMemcpy(sys_dev, sys.data(), sizeof(PupilSys) * S, hipMemcpyHostToDevice));
     p.pp.W = Wt;
     LB_HIP_C(pupil_tables<double>(sys_dev, S, p.pp, A, dc, 0));
-    LB_HIP_C(gram_f64(A, S, F, G, 0));
-    LB_HIP_C(jacobi_eigh_f64(G, V, S, kSweeps, jlog, ev, sweeps, 0));
+    LB_HIP_C(gram_f64(A, S, F, G, ework, 0));
+    LB_HIP_C(hermitian_eigh_f64(G, V, S, ework, ev, counts, 0));
     LB_HIP_C(hipDeviceSynchronize());
     LB_HIP_C(hipMemcpy(evals.data(), ev, sizeof(double) * S, hipMemcpyDeviceToHost));
-    LB_HIP_C(hipMemcpy(&p.jacobi_sweeps, sweeps, sizeof(int), hipMemcpyDeviceToHost));
+    int cnt[4] = {0, 0, 0, 0};
+    LB_HIP_C(hipMemcpy(cnt, counts, 3 * sizeof(int), hipMemcpyDeviceToHost));
+    if (cnt[2] != 0) {
+      cleanup();
+      delete h;
+      return fail(LB_ERR_HIP, cnt[2] == 1 ? "SOCS eigensolve: the QL iteration did not converge"
+                                          : "SOCS eigensolve: rotation log full");
+    }
+    p.eig_sweeps = cnt[0];
   }
   // order, truncation: hopkins.py:172-182
   std::vector<int> order(S);
@@ -1091,7 +1098,7 @@ lb_status lb_plan_info_get(lb_plan plan, lb_plan_info* info) {
   info->adj_block_threads = p.adj.nt;
   info->adj_column_splits = p.adj.csplit;
   info->adj_chunk = layout(p, 1, true).chunk;
-  info->socs_sweeps = p.socs ? p.jacobi_sweeps : 0;
+  info->socs_sweeps = p.socs ? p.eig_sweeps : 0;
   return LB_OK;
 }
 

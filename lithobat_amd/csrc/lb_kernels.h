@@ -263,13 +263,17 @@ hipError_t pupil_tables(const PupilSys* sys_dev, int n_sys, const PupilParams& p
                         double* dc, hipStream_t st);
 
 // ---- SOCS decomposition (lb_socs.hip), FP64 ------------------------------
-// G[s][t] = sum_f conj(A[s][f]) A[t][f]
-hipError_t gram_f64(const cx<double>* A, int S, long long F, cx<double>* G, hipStream_t st);
-// Hermitian Jacobi: G overwritten (diagonal = eigenvalues), V = eigenvectors (columns);
-// log: jacobi_log_bytes(S, max_sweeps) of scratch for the rotations
-size_t jacobi_log_bytes(int S, int max_sweeps);
-hipError_t jacobi_eigh_f64(cx<double>* G, cx<double>* V, int S, int max_sweeps, void* log,
-                           double* evals, int* sweeps_out, hipStream_t st);
+// G[s][t] = sum_f conj(A[s][f]) A[t][f]; work: gram_work_bytes(S, F) of scratch
+size_t gram_work_bytes(int S, long long F);
+hipError_t gram_f64(const cx<double>* A, int S, long long F, cx<double>* G, void* work,
+                    hipStream_t st);
+// Hermitian eigensolve (Householder + implicit QL): G destroyed, evals (S,
+// unordered), V = eigenvectors (columns); work: eigh_work_bytes(S) of scratch;
+// counts (4 ints) = {QL sweeps, rotations, status (0 ok, 1 no convergence,
+// 2 rotation log full)}
+size_t eigh_work_bytes(int S);
+hipError_t hermitian_eigh_f64(cx<double>* G, cx<double>* V, int S, void* work, double* evals,
+                              int* counts, hipStream_t st);
 // phi[k][f] = sum_s A[s][f] V[s][idx[k]] * inv_sqrt[k]
 hipError_t socs_phi_f64(const cx<double>* A, const cx<double>* V, int S, long long F,
                         const int* idx, const double* inv_sqrt, int K, cx<double>* phi,

@@ -5,9 +5,9 @@
 // TCC = B B^H with B = A^T (n_band x S), so its non-zero eigenpairs are those
 // of the S x S Gram matrix G = B^H B (G[s][t] = sum_f conj(A[s][f]) A[t][f]):
 // G u = lambda u  <=>  TCC (B u) = lambda (B u), and phi = B u / sqrt(lambda)
-// is unit-norm.  The Gram matrix is built here, diagonalised by a parallel
-// cyclic two-sided Jacobi (Brent-Luk round-robin pairing, complex Hermitian
-// rotations), and the kept eigenvectors are mapped back to kernels.
+// is unit-norm.  The Gram matrix is built here, diagonalised by a
+// Householder tridiagonalisation followed by the implicit QL iteration, and
+// the kept eigenvectors are mapped back to kernels.
 #include <algorithm>
 
 #include "lb_kernels.h"
@@ -18,19 +18,23 @@ using c64d = cx<double>;
 
 constexpr int kGT = 16;
 
+// One 16 x 16 tile of G over one slice [z fs, z fs + fs) of the band (blockIdx.z):
+// partial Gram matrices, summed in slice order by k_gram_sum (deterministic).
 __global__ __launch_bounds__(kGT* kGT) void k_gram(const c64d* __restrict__ A, int S, long long F,
-                                                   c64d* __restrict__ G) {
+                                                   long long fs, c64d* __restrict__ Gp) {
   __shared__ c64d As[kGT][kGT + 1];
   __shared__ c64d Bs[kGT][kGT + 1];
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int s = blockIdx.y * kGT + ty;  // row of G
   const int t = blockIdx.x * kGT + tx;  // col of G
+  const long long fa = (long long)blockIdx.z * fs;
+  const long long fb = fa + fs < F ? fa + fs : F;
   double re = 0, im = 0;
-  for (long long f0 = 0; f0 < F; f0 += kGT) {
+  for (long long f0 = fa; f0 < fb; f0 += kGT) {
     const int sr = blockIdx.y * kGT + ty, tr = blockIdx.x * kGT + ty;
     const long long f = f0 + tx;
-    As[ty][tx] = (sr < S && f < F) ? A[(long long)sr * F + f] : c64d{0, 0};
-    Bs[ty][tx] = (tr < S && f < F) ? A[(long long)tr * F + f] : c64d{0, 0};
+    As[ty][tx] = (sr < S && f < fb) ? A[(long long)sr * F + f] : c64d{0, 0};
+    Bs[ty][tx] = (tr < S && f < fb) ? A[(long long)tr * F + f] : c64d{0, 0};
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kGT; ++k) {
@@ -41,527 +45,536 @@ __global__ __launch_bounds__(kGT* kGT) void k_gram(const c64d* __restrict__ A, i
     }
     __syncthreads();
   }
-  if (s < S && t < S) G[(long long)s * S + t] = c64d{re, im};
+  if (s < S && t < S) Gp[(long long)blockIdx.z * S * S + (long long)s * S + t] = c64d{re, im};
 }
 
-hipError_t gram_f64(const c64d* A, int S, long long F, c64d* G, hipStream_t st) {
-  dim3 g((S + kGT - 1) / kGT, (S + kGT - 1) / kGT);
-  hipLaunchKernelGGL(k_gram, g, dim3(kGT, kGT), 0, st, A, S, F, G);
+__global__ void k_gram_sum(const c64d* __restrict__ Gp, int ks, long long ss,
+                           c64d* __restrict__ G) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ss;
+       e += (long long)gridDim.x * blockDim.x) {
+    c64d acc = Gp[e];
+    for (int z = 1; z < ks; ++z) acc = acc + Gp[(long long)z * ss + e];
+    G[e] = acc;
+  }
+}
+
+// Slices of the band: enough tiles x slices for about 1024 blocks, at least
+// 64 band entries per slice.
+static int gram_slices(int S, long long F) {
+  const long long nb = (S + kGT - 1) / kGT;
+  long long ks = (1024 + nb * nb - 1) / (nb * nb);
+  const long long cap = (F + 63) / 64;
+  if (ks > cap) ks = cap;
+  return (int)(ks < 1 ? 1 : ks);
+}
+
+size_t gram_work_bytes(int S, long long F) {
+  return sizeof(c64d) * (size_t)gram_slices(S, F) * (size_t)S * (size_t)S;
+}
+
+hipError_t gram_f64(const c64d* A, int S, long long F, c64d* G, void* work, hipStream_t st) {
+  if (S <= 0) return hipSuccess;
+  const int ks = gram_slices(S, F);
+  long long fs = (F + ks - 1) / ks;
+  fs = (fs + kGT - 1) / kGT * kGT;
+  c64d* Gp = static_cast<c64d*>(work);
+  dim3 g((S + kGT - 1) / kGT, (S + kGT - 1) / kGT, ks);
+  hipLaunchKernelGGL(k_gram, g, dim3(kGT, kGT), 0, st, A, S, F, fs, Gp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long ss = (long long)S * S;
+  const long long b = std::min<long long>((ss + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_gram_sum, dim3((unsigned)b), dim3(256), 0, st, Gp, ks, ss, G);
   return hipGetLastError();
 }
 
-constexpr int kJT = 512;
-constexpr int kMaxPairs = 1024;  // players (source points) per decomposition: S <= 1023
-constexpr int kJU = 5;  // 2 x 2 blocks per thread and pass (all loads issued first)
+// ---------------------------------------------------------------- eigensolve
+// G = Q T Q^H (Householder tridiagonalisation, T real symmetric), T = Z L Z^T
+// (implicit QL with Wilkinson shifts), so the eigenvectors are the columns of
+// V = Q Z.  The QL runs on one thread (its rotations form one dependent chain)
+// and logs every rotation; the rows of Z, which the rotations treat
+// independently, replay the log in parallel afterwards, and the reflectors are
+// applied to the columns of Z last.  Householder, "Unitary triangularization
+// of a nonsymmetric matrix" (J. ACM 5, 1958) in the Hermitian reduction form
+// of LAPACK's zhetd2 (lower); the QL iteration of Bowdler, Martin, Reinsch and
+// Wilkinson, "The QR and QL algorithms for symmetric matrices" (Numer. Math.
+// 11, 1968).
 
-// One rotation of the log: U = [[c, -s e], [s conj(e), c]] on columns (p, q).
-struct JacobiRot {
-  double c, s;
-  c64d e;
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+constexpr int kHT = 1024;  // threads of the tridiagonalisation block
+
+// One block.  A (n x n, full Hermitian storage, row-major) is reduced in
+// place: step i forms the reflector H(i) = I - tau[i] v v^H (v[0] = 1 at
+// index i + 1, stored in row i of R) with H(i)^H A H(i) zero in column i below
+// the subdiagonal, whose entry becomes e[i] (real).  Row i of A holds the
+// conjugate of column i, so that column is read from the row (contiguous).
+// Every wavefront forms the reflector itself, its lanes holding v at
+// c = lane + 64 q (NV slots, n <= 64 NV), so a step needs two barriers: after
+// p = tau A22 v, and after the rank-2 update.  From step i_lds on, the
+// trailing matrix A[i_lds:, i_lds:] is held in LDS.
+template <int NV>
+__global__ __launch_bounds__(kHT) void k_hetrd(c64d* __restrict__ A, int n, int i_lds,
+                                               double* __restrict__ d, double* __restrict__ e,
+                                               c64d* __restrict__ tau, c64d* __restrict__ R) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  c64d* p = reinterpret_cast<c64d*>(smem_raw);  // n: tau A22 v
+  c64d* L = p + n;                              // (n - i_lds)^2 from step i_lds on
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = kHT / 64;
+  const int nt = n - i_lds;
+  for (int i = 0; i + 1 < n; ++i) {
+    if (i == i_lds) {
+      for (int q = tid; q < nt * nt; q += kHT) {
+        const int a = q / nt, b = q - a * nt;
+        L[q] = A[(long long)(i_lds + a) * n + i_lds + b];
+      }
+      __syncthreads();
+    }
+    const bool in_lds = i >= i_lds;
+    // Ai[j] = A[i][j] (j >= i); A22[r * ld + c] = A[i+1+r][i+1+c]
+    const c64d* Ai = in_lds ? L + (long long)(i - i_lds) * nt - i_lds : A + (long long)i * n;
+    c64d* A22 = in_lds ? L + (long long)(i + 1 - i_lds) * nt + (i + 1 - i_lds)
+                       : A + (long long)(i + 1) * n + (i + 1);
+    const int ld = in_lds ? nt : n;
+    const int m = n - i - 1;  // order of A22
+    c64d x[NV];  // column i below the subdiagonal, at c = lane + 64 q (c >= 1)
+    double part = 0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int c = lane + 64 * q;
+      x[q] = (c >= 1 && c < m) ? conj(Ai[i + 1 + c]) : c64d{0, 0};
+      part += norm2(x[q]);
+    }
+    const double xn2 = wave_sum(part);
+    const c64d alpha = conj(Ai[i + 1]);
+    c64d t = c64d{0, 0}, scal = c64d{0, 0};
+    double beta = alpha.re;
+    if (xn2 != 0.0 || alpha.im != 0.0) {
+      const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+      beta = alpha.re >= 0.0 ? -nrm : nrm;
+      t = c64d{(beta - alpha.re) / beta, -alpha.im / beta};
+      const c64d den = c64d{alpha.re - beta, alpha.im};  // scal = 1 / (alpha - beta)
+      const double dd = den.re * den.re + den.im * den.im;
+      scal = c64d{den.re / dd, -den.im / dd};
+    }
+    c64d v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int c = lane + 64 * q;
+      v[q] = c == 0 ? c64d{1, 0} : x[q] * scal;
+      if (wave == 0 && c < m) R[(long long)i * n + i + 1 + c] = v[q];
+    }
+    if (tid == 0) {
+      d[i] = Ai[i].re;
+      e[i] = beta;
+      tau[i] = t;
+    }
+    // every wave formed the same t from the same values in the same order
+    if (t.re == 0.0 && t.im == 0.0) continue;
+    for (int r = wave; r < m; r += NW) {
+      const c64d* row = A22 + (long long)r * ld;
+      double re = 0, im = 0;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const int c = lane + 64 * q;
+        if (c < m) {
+          const c64d a = row[c];
+          re += a.re * v[q].re - a.im * v[q].im;
+          im += a.re * v[q].im + a.im * v[q].re;
+        }
+      }
+      re = wave_sum(re);
+      im = wave_sum(im);
+      if (lane == 0) p[r] = t * c64d{re, im};
+    }
+    __syncthreads();
+    // a2 = -tau/2 (p^H v), formed by every wave; w = p + a2 v
+    double re = 0, im = 0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int c = lane + 64 * q;
+      if (c < m) {
+        const c64d a = p[c];
+        re += a.re * v[q].re + a.im * v[q].im;
+        im += a.re * v[q].im - a.im * v[q].re;
+      }
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    const c64d a2 = scale(t * c64d{re, im}, -0.5);
+    c64d w[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int c = lane + 64 * q;
+      w[q] = c < m ? p[c] + a2 * v[q] : c64d{0, 0};
+    }
+    // A22 <- A22 - v w^H - w v^H (the diagonal kept real)
+    for (int r = wave; r < m; r += NW) {
+      // v_r, w_r from lane r % 64, slot r / 64 (uniform over the wave)
+      const int qr = r >> 6, lr = r & 63;
+      c64d vr = c64d{0, 0}, wr = c64d{0, 0};
+#pragma unroll
+      for (int q = 0; q < NV; ++q)
+        if (q == qr) {
+          vr = v[q];
+          wr = w[q];
+        }
+      vr = c64d{__shfl(vr.re, lr, 64), __shfl(vr.im, lr, 64)};
+      wr = c64d{__shfl(wr.re, lr, 64), __shfl(wr.im, lr, 64)};
+      c64d* row = A22 + (long long)r * ld;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        const int c = lane + 64 * q;
+        if (c < m) {
+          c64d val = row[c] - (mulc(vr, w[q]) + mulc(wr, v[q]));
+          if (c == r) val.im = 0.0;
+          row[c] = val;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && n > 0)
+    d[n - 1] = (i_lds <= n - 2 ? L[(long long)(nt - 1) * nt + (nt - 1)]
+                               : A[(long long)(n - 1) * n + (n - 1)]).re;
+}
+
+// One QL sweep of the log: rotations at i = m - 1, m - 2, ..., m - n_rot,
+// their (c, s) at rots[off ...].
+struct QLSweep {
+  int l, m, n_rot, off;
 };
 
-// Round-robin pairing (Brent-Luk): pair i of round r over n players (n even).
-__device__ __forceinline__ void jacobi_pair(int r, int i, int n, int* p, int* q) {
-  // r, i < n - 1: each sum lies in [0, 2 (n - 1)), so one conditional subtraction
-  // is the remainder
-  const int m = n - 1;
-  int a = r + i;
-  if (a >= m) a -= m;
-  int b = r - i + m;
-  if (b >= m) b -= m;
-  if (i == 0) b = m;
-  *p = a < b ? a : b;
-  *q = a < b ? b : a;
-}
-
-// Task w of the upper triangle of the half x half block grid, row-major: (a, b), a <= b.
-__device__ __forceinline__ void jacobi_task(long long w, int half, int* pa, int* pb) {
-  int a = (int)((2.0 * half + 1 - sqrt((2.0 * half + 1) * (2.0 * half + 1) - 8.0 * w)) / 2);
-  long long base = (long long)a * half - (long long)a * (a - 1) / 2;
-  while (base > w) {
-    --a;
-    base = (long long)a * half - (long long)a * (a - 1) / 2;
-  }
-  while (base + (half - a) <= w) {
-    base += half - a;
-    ++a;
-  }
-  *pa = a;
-  *pb = a + (int)(w - base);
-}
-
-// Whether a pair rotates: its coupling must stand above the rounding noise of
-// the pair (1e-15 of sqrt(G_pp G_qq)) and of the matrix (1e-17 of its largest
-// diagonal entry); rotations below those change nothing a double can hold.
-__device__ __forceinline__ bool jacobi_rotates(double ab, double a, double d, double floor_ab) {
-  return ab > 1e-300 && ab > floor_ab && ab > 1e-15 * sqrt(fabs(a * d));
-}
-
-// The sweep-start test over per-thread partial sums (block of kJT threads):
-// done when the off-diagonal Frobenius mass is below 1e-30 of the total or
-// the previous sweep applied no rotation.  Returns the noise floor of the
-// coming sweep.
-template <int NT>
-__device__ double jacobi_test(double off, double tot, double dmax, int applied,
-                              double (*red)[NT / 64], int* done) {
-  const int tid = threadIdx.x;
-  for (int o = 32; o > 0; o >>= 1) {
-    off += __shfl_xor(off, o, 64);
-    tot += __shfl_xor(tot, o, 64);
-    dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
-  }
-  if ((tid & 63) == 0) {
-    red[0][tid >> 6] = off;
-    red[1][tid >> 6] = tot;
-    red[2][tid >> 6] = dmax;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double so = 0, st = 0, sm = 0;
-    for (int w = 0; w < NT / 64; ++w) {
-      so += red[0][w];
-      st += red[1][w];
-      sm = fmax(sm, red[2][w]);
-    }
-    *done = (so <= 1e-30 * st || applied == 0) ? 1 : 0;
-    red[2][0] = sm;
-  }
-  __syncthreads();
-  const double floor_ab = 1e-17 * red[2][0];
-  __syncthreads();  // red is rewritten by the next test
-  return floor_ab;
-}
-
-// The cyclic Jacobi sweeps on G (one block; G in device memory, the LDS holds
-// a round's rotations and the next round's pivots).  Per round: the pair
-// threads form their rotation from the pivots (G_pp, G_qq, G_pq) the previous
-// round left in LDS and log it; then every 2 x 2 block (a, b), a <= b, of G is
-// rotated on its rows by pair a and on its columns by pair b and written with
-// its Hermitian mirror (b, a), and the entries that are the next round's
-// pivots are also put in LDS.  The eigenvectors are not updated here: the
-// logged rotations are applied to V afterwards (k_jacobi_vectors), row by
-// row in parallel.
-__global__ __launch_bounds__(kJT) void k_jacobi(c64d* __restrict__ G, int S, int max_sweeps,
-                                                JacobiRot* __restrict__ log,
-                                                double* __restrict__ evals,
-                                                int* __restrict__ sweeps_out) {
-  __shared__ JacobiRot rot[kMaxPairs / 2];
-  __shared__ int rp[kMaxPairs / 2], rq[kMaxPairs / 2];
-  __shared__ double piv_a[kMaxPairs / 2], piv_d[kMaxPairs / 2];
-  __shared__ c64d piv_b[kMaxPairs / 2];
-  __shared__ short slot_of[kMaxPairs];  // next round: pair index * 2 + (0: p, 1: q)
-  __shared__ double red[3][kJT / 64];
-  __shared__ int done;
-  const int tid = threadIdx.x;
-  const int n = S + (S & 1);  // even number of players; index S is a dummy when S is odd
-  const int half = n / 2;
-  const long long n_blk = (long long)half * (half + 1) / 2;  // blocks (a, b) with a <= b
-  int sweep = 0, applied = 1;
-  for (; sweep < max_sweeps; ++sweep) {
-    // convergence test: off-diagonal vs total Frobenius mass, or a sweep that
-    // rotated nothing; the largest diagonal entry sets the noise floor
-    double off = 0, tot = 0, dmax = 0;
-    for (long long e = tid; e < (long long)S * S; e += kJT) {
-      const int r = (int)(e / S), c = (int)(e % S);
-      const double m = G[e].re * G[e].re + G[e].im * G[e].im;
-      tot += m;
-      if (r != c) off += m;
-      else dmax = fmax(dmax, fabs(G[e].re));
-    }
-    const double floor_ab = jacobi_test<kJT>(off, tot, dmax, applied, red, &done);
-    if (done) break;
-    applied = 0;
-    // round 0's pivots from G
-    for (int i = tid; i < half; i += kJT) {
-      int p, q;
-      jacobi_pair(0, i, n, &p, &q);
-      piv_a[i] = G[(long long)p * S + p].re;
-      piv_d[i] = q < S ? G[(long long)q * S + q].re : 0.0;
-      piv_b[i] = q < S ? G[(long long)p * S + q] : c64d{0, 0};
-    }
-    __syncthreads();
-    for (int round = 0; round < n - 1; ++round) {
-      for (int i = tid; i < half; i += kJT) {
-        int p, q;
-        jacobi_pair(round, i, n, &p, &q);
-        double c = 1, s = 0;
-        c64d ea = c64d{1, 0};
-        if (q < S) {
-          const double a = piv_a[i], d = piv_d[i];
-          const c64d b = piv_b[i];
-          const double ab = sqrt(b.re * b.re + b.im * b.im);
-          if (jacobi_rotates(ab, a, d, floor_ab)) {
-            const double tau = (d - a) / (2.0 * ab);
-            const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-            ea = c64d{b.re / ab, b.im / ab};
-          }
-        }
-        rp[i] = p;
-        rq[i] = q;
-        rot[i] = JacobiRot{c, s, ea};
-        log[((long long)sweep * (n - 1) + round) * half + i] = JacobiRot{c, s, ea};
-        // where each player sits in the next round (round n - 1 is round 0 again)
-        int pn, qn;
-        jacobi_pair((round + 1) % (n - 1), i, n, &pn, &qn);
-        slot_of[pn] = (short)(2 * i);
-        slot_of[qn] = (short)(2 * i + 1);
-      }
-      applied += __syncthreads_count(tid < half && rot[tid < half ? tid : 0].s != 0.0);
-      for (long long w0 = tid; w0 < n_blk; w0 += (long long)kJU * kJT) {
-        c64d g[kJU][4];
-        int ra[kJU], rb[kJU], ca[kJU], cb[kJU], pa[kJU], pb[kJU];
+// 1 / sqrt(x) for x > 0: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt_refined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
 #pragma unroll
-        for (int u = 0; u < kJU; ++u) {
-          const long long w = w0 + (long long)u * kJT;
-          pa[u] = -1;
-          if (w < n_blk) {
-            int a, b;
-            jacobi_task(w, half, &a, &b);
-            pa[u] = a;
-            pb[u] = b;
-            ra[u] = rp[a];
-            rb[u] = rq[a] < S ? rq[a] : -1;
-            ca[u] = rp[b];
-            cb[u] = rq[b] < S ? rq[b] : -1;
-            g[u][0] = G[(long long)ra[u] * S + ca[u]];
-            g[u][1] = cb[u] >= 0 ? G[(long long)ra[u] * S + cb[u]] : c64d{0, 0};
-            g[u][2] = rb[u] >= 0 ? G[(long long)rb[u] * S + ca[u]] : c64d{0, 0};
-            g[u][3] = (rb[u] >= 0 && cb[u] >= 0) ? G[(long long)rb[u] * S + cb[u]] : c64d{0, 0};
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < kJU; ++u) {
-          if (pa[u] < 0) continue;
-          const int a = pa[u], b = pb[u];
-          if (rb[u] >= 0 && rot[a].s != 0.0) {  // rows: G <- U_a^H G
-            const double c = rot[a].c, s_ = rot[a].s;
-            const c64d ea = rot[a].e;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              const c64d gp = g[u][j], gq = g[u][2 + j];
-              g[u][j] = scale(gp, c) - scale(ea * gq, s_);
-              g[u][2 + j] = scale(mulc(gp, ea), s_) + scale(gq, c);
-            }
-          }
-          if (cb[u] >= 0 && rot[b].s != 0.0) {  // columns: G <- G U_b
-            const double c = rot[b].c, s_ = rot[b].s;
-            const c64d eb = rot[b].e;
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-              const c64d gp = g[u][2 * i], gq = g[u][2 * i + 1];
-              g[u][2 * i] = scale(gp, c) - scale(mulc(gq, eb), s_);
-              g[u][2 * i + 1] = scale(eb * gp, s_) + scale(gq, c);
-            }
-          }
-          const int rows[2] = {ra[u], rb[u]}, cols[2] = {ca[u], cb[u]};
-#pragma unroll
-          for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              const int r = rows[i], c = cols[j];
-              if (r < 0 || c < 0) continue;
-              const c64d v = g[u][2 * i + j];
-              G[(long long)r * S + c] = v;
-              if (a != b) G[(long long)c * S + r] = conj(v);
-              // the next round's pivots: entries (r, c) of one next-round pair
-              const int sr = slot_of[r], sc = slot_of[c];
-              if ((sr >> 1) == (sc >> 1)) {
-                const int k = sr >> 1;
-                if (r == c) {
-                  if (sr & 1) piv_d[k] = v.re; else piv_a[k] = v.re;
-                } else if (!(sr & 1)) {  // r is the pair's p, c its q
-                  piv_b[k] = v;
-                } else if (a != b) {     // the mirror (c, r) is (p, q)
-                  piv_b[k] = conj(v);
-                }
-              }
-            }
-        }
-      }
-      __syncthreads();
-    }
+  for (int k = 0; k < 2; ++k) {
+    const double h = x * y;
+    y = fma(0.5 * y, fma(-h, y, 1.0), y);
   }
-  for (int i = tid; i < S; i += kJT) evals[i] = G[(long long)i * S + i].re;
-  if (tid == 0) *sweeps_out = sweep;
+  return y;
 }
 
-// The same sweeps with G held in LDS: the strict upper triangle packed row by
-// row (complex), the real diagonal beside it, and a round's rotations as
-// (c, s e) with their players -- 16 S (S - 1) / 2 + 8 S + 28 n / 2 bytes, which
-// fits a CU's 160 KB for S <= 142.  Every unordered entry {r, c} belongs to
-// exactly one 2 x 2 block (a, b), a <= b, so a round reads and writes each slot
-// from one thread and needs no second copy; the pivots are read straight from
-// the LDS matrix, and a block whose two pairs both stay unrotated is skipped.
-
-__device__ __forceinline__ int tri_idx(int r, int c, int S) {  // r < c < S < 2^12
-  return __mul24(r, S) - (__mul24(r, r + 1) >> 1) + (c - r - 1);
-}
-
-// the rotation cosines' slots, which the sweep test's partial sums also use
-__host__ __device__ inline int jacobi_rc_slots(int half) {
-  return half > 3 * 16 ? half : 3 * 16;  // 3 x (1024 / 64) partials
-}
-
-size_t jacobi_lds_bytes(int S) {
-  const int n = S + (S & 1), half = n / 2;
-  return sizeof(c64d) * ((size_t)S * (S - 1) / 2 + (size_t)half) +
-         sizeof(double) * ((size_t)S + (size_t)jacobi_rc_slots(half)) +
-         2 * sizeof(short) * (size_t)half;
-}
-
-constexpr int kJTL = 1024;  // threads of the LDS form (jacobi_rc_slots holds 3 x kJTL / 64)
-constexpr int kJUL = 3;     // its blocks per thread: kJUL * kJTL >= 71 * 72 / 2
-__global__ __launch_bounds__(kJTL) void k_jacobi_lds(const c64d* __restrict__ G, int S,
-                                                    int max_sweeps, JacobiRot* __restrict__ log,
-                                                    double* __restrict__ evals,
-                                                    int* __restrict__ sweeps_out) {
+// Implicit QL on the real symmetric tridiagonal (d, e), e[i] coupling i and
+// i + 1, on one thread.  d becomes the eigenvalues; every rotation is logged.
+// counts = {sweeps, rotations, status}: status 1 when an eigenvalue takes more
+// than 60 iterations, 2 when the log is full (both reported as errors).  The
+// rotations form one dependent chain, so each is kept short: the next
+// rotation's d and e are loaded ahead and 1 / r comes from a refined
+// reciprocal square root instead of a square root and a division.
+__global__ void k_tql(double* __restrict__ d_io, const double* __restrict__ e_in, int n,
+                      QLSweep* __restrict__ sweeps, double2* __restrict__ rots, int max_sweeps,
+                      int max_rots, int* __restrict__ counts) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int tid = threadIdx.x;
-  const int n = S + (S & 1), half = n / 2;
-  const int n_off = S * (S - 1) / 2;
-  c64d* Go = reinterpret_cast<c64d*>(smem_raw);  // packed strict upper triangle
-  c64d* rse = Go + n_off;                          // this round's rotations: s e
-  double* Gd = reinterpret_cast<double*>(rse + half);  // diagonal
-  double* rc = Gd + S;                                 // this round's rotations: c
-  short* rp = reinterpret_cast<short*>(rc + jacobi_rc_slots(half));  // this round's players
-  short* rq = rp + half;                               //   (-1: the dummy)
-  __shared__ int done;
-  for (int e = tid; e < S * S; e += kJTL) {
-    const int r = e / S, c = e - (e / S) * S;
-    if (r == c) Gd[r] = G[e].re;
-    else if (r < c) Go[tri_idx(r, c, S)] = G[e];
+  double* d = reinterpret_cast<double*>(smem_raw);  // n
+  double* e = d + n;                                // n + 3 (e[n-1 ...] = 0)
+  for (int i = threadIdx.x; i < n + 3; i += blockDim.x) {
+    if (i < n) d[i] = d_io[i];
+    e[i] = i + 1 < n ? e_in[i] : 0.0;
   }
   __syncthreads();
-  const long long n_blk = (long long)half * (half + 1) / 2;
-  // this thread's blocks, the same in every round (-1: none): the half diagonal
-  // blocks first, so that they share a few wavefronts and the rest take the
-  // off-diagonal path only
-  int ta[kJUL], tb[kJUL];
+  if (threadIdx.x != 0) return;
+  // A coupling deflates once it is below the rounding level of the whole
+  // matrix (eps times its largest row sum).  A test relative to the two
+  // neighbouring diagonal entries never passes in a block of rounding-level
+  // entries, which a rank-deficient Gram matrix has (more source points than
+  // band frequencies): its near-zero eigenvalues would never converge.
+  const double eps = 2.220446049250313e-16;
+  double tnorm = 0.0;
+  for (int i = 0; i < n; ++i)
+    tnorm = fmax(tnorm, fabs(d[i]) + fabs(e[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0));
+  const double tol = eps * tnorm;
+  int n_sw = 0, n_rot = 0, status = 0;
+  for (int l = 0; l < n && status == 0; ++l) {
+    int iter = 0;
+    for (;;) {
+      // first m >= l with a negligible coupling (e[n-1] = 0 ends the search);
+      // four couplings per step, loaded together
+      int m = l;
+      for (;;) {
+        const double e0 = e[m], e1 = e[m + 1], e2 = e[m + 2], e3 = e[m + 3];
+        if (fabs(e0) <= tol) break;
+        if (fabs(e1) <= tol) { m += 1; break; }
+        if (fabs(e2) <= tol) { m += 2; break; }
+        if (fabs(e3) <= tol) { m += 3; break; }
+        m += 4;
+      }
+      if (m > n - 1) m = n - 1;
+      if (m == l) break;
+      if (++iter > 60) {
+        status = 1;
+        break;
+      }
+      if (n_sw >= max_sweeps || n_rot + (m - l) > max_rots) {
+        status = 2;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = sqrt(g * g + 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+      double s = 1.0, c = 1.0, p = 0.0;
+      const int off = n_rot;
+      bool under = false;
+      double di1 = d[m];
+      // rotations i = m-1 down to l in chunks of up to 8: the chunk's d and e
+      // are loaded together and its outputs stored together, so the chain
+      // waits on LDS once per chunk
+      int i = m - 1;
+      while (i >= l && !under) {
+        const int nk = i - l + 1 < 8 ? i - l + 1 : 8;
+        double dv[8], ev[8], dout[8], eout[8];
 #pragma unroll
-  for (int u = 0; u < kJUL; ++u) {
-    const long long w = tid + (long long)u * kJTL;
-    ta[u] = tb[u] = -1;
-    if (w < half) {
-      ta[u] = tb[u] = (int)w;
-    } else if (w < n_blk) {  // strict upper triangle, row-major
-      int a, b;
-      jacobi_task(w - half, half - 1, &a, &b);
-      ta[u] = a;
-      tb[u] = b + 1;
+        for (int k = 0; k < 8; ++k) {
+          const int j = i - k >= 0 ? i - k : 0;
+          dv[k] = d[j];
+          ev[k] = e[j];
+        }
+        int done = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (k < nk && !under) {
+            const double f = s * ev[k], b = c * ev[k];
+            const double rr = fma(f, f, g * g);
+            if (rr == 0.0) {  // the coupling underflowed: split here and start again
+              eout[k] = 0.0;
+              dout[k] = di1 - p;
+              under = true;
+            } else {
+              const double inv = rsqrt_refined(rr);
+              eout[k] = rr * inv;
+              s = f * inv;
+              c = g * inv;
+              g = di1 - p;
+              r = fma(dv[k] - g, s, 2.0 * c * b);
+              p = s * r;
+              dout[k] = g + p;
+              g = fma(c, r, -b);
+              rots[n_rot + k] = make_double2(c, s);
+              di1 = dv[k];
+              ++done;
+            }
+          }
+        }
+        // outputs of rotation k: e[i-k+1], d[i-k+1] (the underflowed one too)
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < done + (under ? 1 : 0)) {
+            e[i - k + 1] = eout[k];
+            d[i - k + 1] = dout[k];
+          }
+        n_rot += done;
+        i -= done;
+      }
+      if (under) e[m] = 0.0;
+      sweeps[n_sw++] = QLSweep{l, m, n_rot - off, off};
+      if (under) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
     }
   }
-  int sweep = 0, applied = 1;
-  for (; sweep < max_sweeps; ++sweep) {
-    double off = 0, tot = 0, dmax = 0;
-    for (int e = tid; e < n_off; e += kJTL) off += 2.0 * norm2(Go[e]);
-    for (int e = tid; e < S; e += kJTL) {
-      tot += Gd[e] * Gd[e];
-      dmax = fmax(dmax, fabs(Gd[e]));
-    }
-    tot += off;
-    // the test's partial sums borrow the rotation cosines' slots (written after it)
-    const double floor_ab = jacobi_test<kJTL>(off, tot, dmax, applied,
-                                              reinterpret_cast<double (*)[kJTL / 64]>(rc),
-                                              &done);
-    if (done) break;
-    applied = 0;
-    for (int round = 0; round < n - 1; ++round) {
-      for (int i = tid; i < half; i += kJTL) {
-        int p, q;
-        jacobi_pair(round, i, n, &p, &q);
-        double c = 1, s = 0;
-        c64d ea = c64d{1, 0};
-        if (q < S) {
-          const double a = Gd[p], d = Gd[q];
-          const c64d b = Go[tri_idx(p, q, S)];
-          const double ab = sqrt(b.re * b.re + b.im * b.im);
-          if (jacobi_rotates(ab, a, d, floor_ab)) {
-            const double tau = (d - a) / (2.0 * ab);
-            const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-            ea = c64d{b.re / ab, b.im / ab};
-          }
-        }
-        rc[i] = c;
-        rse[i] = scale(ea, s);
-        rp[i] = (short)p;
-        rq[i] = (short)(q < S ? q : -1);
-        log[((long long)sweep * (n - 1) + round) * half + i] = JacobiRot{c, s, ea};
-      }
-      applied += __syncthreads_count(tid < half && (rse[tid < half ? tid : 0].re != 0.0 ||
-                                                    rse[tid < half ? tid : 0].im != 0.0));
-#pragma unroll
-      for (int u = 0; u < kJUL; ++u) {
-        const int a = ta[u], b = tb[u];
-        if (a < 0) continue;
-        const c64d sa = rse[a], sb = rse[b];
-        const bool rot_a = sa.re != 0.0 || sa.im != 0.0;
-        const bool rot_b = sb.re != 0.0 || sb.im != 0.0;
-        if (!rot_a && !rot_b) continue;  // both pairs left as they are: the block is too
-        const double ca_ = rc[a], cb_ = rc[b];
-        const int ra = rp[a], rb = rq[a];
-        if (a == b) {
-          // the pair's own 2 x 2 block [[G_pp, G_pq], [conj G_pq, G_qq]]: U^H B U
-          if (rb < 0) continue;  // paired with the dummy: nothing to rotate
-          const int spq = tri_idx(ra, rb, S);
-          c64d g0 = c64d{Gd[ra], 0.0}, g1 = Go[spq], g2 = conj(g1), g3 = c64d{Gd[rb], 0.0};
-          {
-            const c64d n0 = scale(g0, ca_) - sa * g2, n2 = mulc(g0, sa) + scale(g2, ca_);
-            const c64d n1 = scale(g1, ca_) - sa * g3, n3 = mulc(g1, sa) + scale(g3, ca_);
-            g0 = n0; g1 = n1; g2 = n2; g3 = n3;
-          }
-          const c64d m0 = scale(g0, ca_) - mulc(g1, sa), m1 = sa * g0 + scale(g1, ca_);
-          const c64d m3 = sa * g2 + scale(g3, ca_);
-          Gd[ra] = m0.re;
-          Gd[rb] = m3.re;
-          Go[spq] = m1;
-          continue;
-        }
-        // off-diagonal block: rows {p_a, q_a} x columns {p_b, q_b}, all players distinct
-        const int ca = rp[b], cb = rq[b];
-        const int rows[2] = {ra, rb}, cols[2] = {ca, cb};
-        int sl[4];
-        bool cj[4];
-        c64d g[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int r = rows[k >> 1], c = cols[k & 1];
-          cj[k] = r > c;
-          sl[k] = (r < 0 || c < 0) ? -1 : (r < c ? tri_idx(r, c, S) : tri_idx(c, r, S));
-          const c64d v = sl[k] >= 0 ? Go[sl[k]] : c64d{0, 0};
-          g[k] = cj[k] ? conj(v) : v;
-        }
-        if (rb >= 0 && rot_a) {  // rows: G <- U_a^H G
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            const c64d gp = g[j], gq = g[2 + j];
-            g[j] = scale(gp, ca_) - sa * gq;
-            g[2 + j] = mulc(gp, sa) + scale(gq, ca_);
-          }
-        }
-        if (cb >= 0 && rot_b) {  // columns: G <- G U_b
-#pragma unroll
-          for (int i = 0; i < 2; ++i) {
-            const c64d gp = g[2 * i], gq = g[2 * i + 1];
-            g[2 * i] = scale(gp, cb_) - mulc(gq, sb);
-            g[2 * i + 1] = sb * gp + scale(gq, cb_);
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (sl[k] >= 0) Go[sl[k]] = cj[k] ? conj(g[k]) : g[k];
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = tid; i < S; i += kJTL) evals[i] = Gd[i];
-  if (tid == 0) *sweeps_out = sweep;
+  for (int i = 0; i < n; ++i) d_io[i] = d[i];
+  counts[0] = n_sw;
+  counts[1] = n_rot;
+  counts[2] = status;
 }
 
-// V = product of the logged rotations (V starts as the identity): every row
-// of V is independent, so one wavefront per row holds it in LDS and applies
-// the rounds in order.  The block stages the rotations of kJVR rounds at a
-// time in LDS (one cooperative load, double-buffered), which its waves share.
-constexpr int kJVW = 8;  // rows (wavefronts) per block, at most
-__global__ __launch_bounds__(64 * kJVW) void k_jacobi_vectors(c64d* __restrict__ V, int S,
-                                                              const JacobiRot* __restrict__ log,
-                                                              const int* __restrict__ sweeps,
-                                                              int nrc) {
+// Z = product of the logged rotations (Z starts as the identity), rows held
+// in LDS, two rows per lane when a block's 128 rows fit (two independent
+// chains per lane: the rotations are latency-bound).  A sweep's rotation at i
+// turns (z[i], z[i+1]); going down a sweep, the updated z[i] is the next
+// rotation's partner, so it stays in a register and each rotation reads one
+// element and writes one.
+// The log is read with vector loads (an offset the compiler cannot prove
+// uniform, zero at run time): scalar loads share the LDS wait counter and
+// return out of order, so every wait for a row element would also wait for
+// them.
+template <int RPL>
+__global__ __launch_bounds__(64) void k_ql_rows(int n, const QLSweep* __restrict__ sweeps,
+                                                const double2* __restrict__ rots,
+                                                const int* __restrict__ counts,
+                                                double* __restrict__ Z, int zero) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int wv = (int)(blockDim.x >> 6);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = S + (S & 1), half = n / 2;
-  JacobiRot* stage = reinterpret_cast<JacobiRot*>(smem_raw);  // 2 x nrc x half
-  c64d* rows = reinterpret_cast<c64d*>(stage + 2LL * nrc * half);
-  const int k = blockIdx.x * wv + wave;
-  c64d* row = rows + (long long)wave * (S + 1);
-  const bool live = k < S;
-  if (live)
-    for (int c = lane; c < S; c += 64) row[c] = c64d{c == k ? 1.0 : 0.0, 0.0};
-  const long long rounds = (long long)(*sweeps) * (n - 1);
-  const long long chunks = (rounds + nrc - 1) / nrc;
-  auto load = [&](long long ch, int buf) {
-    const long long r0 = ch * nrc;
-    const int nr = (int)min<long long>(nrc, rounds - r0);
-    JacobiRot* dst = stage + (long long)buf * nrc * half;
-    for (int e = threadIdx.x; e < nr * half; e += (int)blockDim.x) dst[e] = log[r0 * half + e];
+  const int ld = n + 1;
+  double* z[RPL];
+  int row[RPL];
+#pragma unroll
+  for (int u = 0; u < RPL; ++u) {
+    z[u] = reinterpret_cast<double*>(smem_raw) + (long long)(threadIdx.x + 64 * u) * ld;
+    row[u] = blockIdx.x * 64 * RPL + threadIdx.x + 64 * u;  // >= n: a spare slot
+#pragma unroll 1
+    for (int c = 0; c < n; ++c) z[u][c] = c == row[u] ? 1.0 : 0.0;
+  }
+  const int n_sw = counts[0];
+  for (int w = 0; w < n_sw; ++w) {
+    const QLSweep h = sweeps[w];
+    const double2* rr = rots + h.off + zero * (int)threadIdx.x;
+    double carry[RPL];
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) carry[u] = z[u][h.m];
+    int i = h.m - 1;
+    int k = 0;
+    for (; k + 4 <= h.n_rot; k += 4, i -= 4) {
+      const double2 r0 = rr[k], r1 = rr[k + 1], r2 = rr[k + 2], r3 = rr[k + 3];
+      double a[RPL][4];
+#pragma unroll
+      for (int u = 0; u < RPL; ++u)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) a[u][t] = z[u][i - t];
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) {
+        double cy = carry[u];
+        z[u][i + 1] = r0.y * a[u][0] + r0.x * cy;
+        cy = r0.x * a[u][0] - r0.y * cy;
+        z[u][i] = r1.y * a[u][1] + r1.x * cy;
+        cy = r1.x * a[u][1] - r1.y * cy;
+        z[u][i - 1] = r2.y * a[u][2] + r2.x * cy;
+        cy = r2.x * a[u][2] - r2.y * cy;
+        z[u][i - 2] = r3.y * a[u][3] + r3.x * cy;
+        carry[u] = r3.x * a[u][3] - r3.y * cy;
+      }
+    }
+    for (; k < h.n_rot; ++k, --i) {
+      const double2 rk = rr[k];
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) {
+        const double a = z[u][i];
+        z[u][i + 1] = rk.y * a + rk.x * carry[u];
+        carry[u] = rk.x * a - rk.y * carry[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) z[u][h.m - h.n_rot] = carry[u];
+  }
+#pragma unroll
+  for (int u = 0; u < RPL; ++u)
+    if (row[u] < n)
+      for (int c = 0; c < n; ++c) Z[(long long)row[u] * n + c] = z[u][c];
+}
+
+// V = H(0) H(1) ... H(n-2) Z: one wavefront per column of Z (held in LDS),
+// the reflectors applied from the last to the first.
+constexpr int kQBW = 4;  // columns (wavefronts) per block
+__global__ __launch_bounds__(64 * kQBW) void k_ql_back(int n, const c64d* __restrict__ R,
+                                                       const c64d* __restrict__ tau,
+                                                       const double* __restrict__ Z,
+                                                       c64d* __restrict__ V) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  c64d* z = reinterpret_cast<c64d*>(smem_raw) + (long long)wave * n;
+  const int j = blockIdx.x * kQBW + wave;
+  if (j >= n) return;
+  for (int r = lane; r < n; r += 64) z[r] = c64d{Z[(long long)r * n + j], 0.0};
+  for (int i = n - 2; i >= 0; --i) {
+    const c64d t = tau[i];
+    if (t.re == 0.0 && t.im == 0.0) continue;
+    const c64d* vi = R + (long long)i * n;
+    double re = 0, im = 0;
+    for (int r = i + 1 + lane; r < n; r += 64) {
+      const c64d a = vi[r], b = z[r];  // conj(a) b
+      re += a.re * b.re + a.im * b.im;
+      im += a.re * b.im - a.im * b.re;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    const c64d coef = t * c64d{re, im};
+    for (int r = i + 1 + lane; r < n; r += 64) z[r] = z[r] - coef * vi[r];
+  }
+  for (int r = lane; r < n; r += 64) V[(long long)r * n + j] = z[r];
+}
+
+// Log capacity: the QL takes about n^2 rotations in n + n/2 sweeps (1.4e4 to
+// 2e4 rotations at n = 142); four times that is a generous bound.
+static size_t eigh_max_sweeps(size_t n) { return 8 * n + 64; }
+static size_t eigh_max_rots(size_t n) { return 4 * n * n + 1024; }
+
+size_t eigh_work_bytes(int S) {
+  const size_t n = (size_t)std::max(S, 1);
+  const size_t max_sw = eigh_max_sweeps(n), max_rot = eigh_max_rots(n);
+  return sizeof(c64d) * (n * n + n) + sizeof(double) * (n * n + 2 * n) +
+         sizeof(QLSweep) * max_sw + sizeof(double2) * max_rot + 6 * 256;  // 6 aligned pieces
+}
+
+constexpr size_t kEighLdsMax = 160 * 1024 - 1024;
+
+hipError_t hermitian_eigh_f64(c64d* G, c64d* V, int S, void* work, double* evals, int* counts,
+                              hipStream_t st) {
+  if (S <= 0) return hipSuccess;
+  const int n = S;
+  const size_t max_sw = eigh_max_sweeps(n), max_rot = eigh_max_rots(n);
+  if (max_rot > (size_t)INT32_MAX) return hipErrorInvalidValue;
+  char* w = static_cast<char*>(work);
+  auto take = [&](size_t bytes) {
+    char* q = w;
+    w += (bytes + 255) & ~(size_t)255;
+    return q;
   };
-  if (chunks > 0) load(0, 0);
-  __syncthreads();
-  for (long long ch = 0; ch < chunks; ++ch) {
-    if (ch + 1 < chunks) load(ch + 1, (int)((ch + 1) & 1));  // in flight during this chunk
-    const long long r0 = ch * nrc;
-    const int nr = (int)min<long long>(nrc, rounds - r0);
-    const JacobiRot* src = stage + (long long)(ch & 1) * nrc * half;
-    if (live) {
-      for (int t = 0; t < nr; ++t) {
-        const int r = (int)((r0 + t) % (n - 1));
-        for (int i = lane; i < half; i += 64) {
-          int p, q;
-          jacobi_pair(r, i, n, &p, &q);
-          const JacobiRot u = src[t * half + i];
-          if (q >= S || u.s == 0.0) continue;
-          const c64d vp = row[p], vq = row[q];
-          row[p] = scale(vp, u.c) - scale(mulc(vq, u.e), u.s);
-          row[q] = scale(u.e * vp, u.s) + scale(vq, u.c);
-        }
-      }
+  c64d* R = reinterpret_cast<c64d*>(take(sizeof(c64d) * n * n));
+  c64d* tau = reinterpret_cast<c64d*>(take(sizeof(c64d) * n));
+  double* e = reinterpret_cast<double*>(take(sizeof(double) * n));
+  double* Z = reinterpret_cast<double*>(take(sizeof(double) * n * n));
+  QLSweep* sw = reinterpret_cast<QLSweep*>(take(sizeof(QLSweep) * max_sw));
+  double2* rots = reinterpret_cast<double2*>(take(sizeof(double2) * max_rot));
+  hipError_t err;
+  // the trailing matrix moves into LDS at the first step where it fits there
+  int i_lds = n;
+  for (int i = 0; i + 1 < n; ++i)
+    if (sizeof(c64d) * ((size_t)n + (size_t)(n - i) * (n - i)) <= kEighLdsMax) {
+      i_lds = i;
+      break;
     }
-    // every wave is done with this buffer (reloaded next chunk) and the next one has landed
-    __syncthreads();
-  }
-  if (live)
-    for (int c = lane; c < S; c += 64) V[(long long)k * S + c] = row[c];
-}
-
-size_t jacobi_log_bytes(int S, int max_sweeps) {
-  const int n = S + (S & 1);
-  return sizeof(JacobiRot) * (size_t)max_sweeps * (size_t)(n - 1) * (size_t)(n / 2);
-}
-
-constexpr size_t kJacobiLdsMax = 160 * 1024 - 256;  // dynamic LDS beside the static slots
-
-hipError_t jacobi_eigh_f64(c64d* G, c64d* V, int S, int max_sweeps, void* log, double* evals,
-                           int* sweeps_out, hipStream_t st) {
-  if (S > kMaxPairs - 1) return hipErrorInvalidValue;
-  JacobiRot* lg = (JacobiRot*)log;
-  hipError_t e;
-  const size_t lds = jacobi_lds_bytes(S);
-  const long long half = (S + (S & 1)) / 2;
-  bool in_lds = S >= 2 && lds <= kJacobiLdsMax && half * (half + 1) / 2 <= (long long)kJUL * kJTL;
-  if (in_lds) {
-    // a device that grants less LDS per block takes the device-memory form
-    in_lds = hipFuncSetAttribute(reinterpret_cast<const void*>(k_jacobi_lds),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds) == hipSuccess;
-    (void)hipGetLastError();
-  }
-  if (in_lds) {
-    hipLaunchKernelGGL(k_jacobi_lds, dim3(1), dim3(kJTL), lds, st, G, S, max_sweeps, lg, evals,
-                       sweeps_out);
-  } else {
-    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(kJT), 0, st, G, S, max_sweeps, lg, evals,
-                       sweeps_out);
-  }
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  // eigenvector pass: up to kJVW rows per block and 16 staged rounds, fewer where the
-  // rows and the two staging buffers would not fit in LDS
-  const int n = S + (S & 1);
-  const size_t per_round = sizeof(JacobiRot) * 2 * (size_t)(n / 2);
-  int wv = kJVW, nrc = 16;
-  while (wv > 1 && sizeof(c64d) * wv * (size_t)(S + 1) + per_round > kJacobiLdsMax) wv /= 2;
-  const size_t rows_b = sizeof(c64d) * wv * (size_t)(S + 1);
-  if (rows_b + per_round > kJacobiLdsMax) return hipErrorInvalidValue;
-  nrc = (int)std::min<size_t>(16, (kJacobiLdsMax - rows_b) / per_round);
-  const size_t sm = rows_b + per_round * nrc;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_jacobi_vectors),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_jacobi_vectors, dim3((S + wv - 1) / wv), dim3(64 * wv), sm, st, V, S, lg,
-                     sweeps_out, nrc);
+  const size_t lds_h =
+      sizeof(c64d) * ((size_t)n + (i_lds < n ? (size_t)(n - i_lds) * (n - i_lds) : 0));
+  if (lds_h > kEighLdsMax) return hipErrorInvalidValue;
+  auto hetrd = [&](auto kern) -> hipError_t {
+    hipError_t e2 = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_h);
+    if (e2 != hipSuccess) return e2;
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kHT), lds_h, st, G, n, i_lds, evals, e, tau, R);
+    return hipGetLastError();
+  };
+  const int nv = (n + 63) / 64;
+  if (nv <= 1) err = hetrd(k_hetrd<1>);
+  else if (nv <= 2) err = hetrd(k_hetrd<2>);
+  else if (nv <= 3) err = hetrd(k_hetrd<3>);
+  else if (nv <= 4) err = hetrd(k_hetrd<4>);
+  else if (nv <= 8) err = hetrd(k_hetrd<8>);
+  else if (nv <= 16) err = hetrd(k_hetrd<16>);
+  else return hipErrorInvalidValue;  // more than 1024 source points
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_tql, dim3(1), dim3(64), sizeof(double) * (2 * (size_t)n + 3), st, evals,
+                     e, n, sw, rots, (int)max_sw, (int)max_rot, counts);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  // two rows per lane (128 per block) where they fit in LDS, else one
+  const bool two = sizeof(double) * 128 * (size_t)(n + 1) <= kEighLdsMax;
+  const int rpb = two ? 128 : 64;
+  const size_t lds_r = sizeof(double) * (size_t)rpb * (n + 1);
+  if (lds_r > kEighLdsMax) return hipErrorInvalidValue;
+  const void* kr = two ? reinterpret_cast<const void*>(k_ql_rows<2>)
+                       : reinterpret_cast<const void*>(k_ql_rows<1>);
+  if ((err = hipFuncSetAttribute(kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r)) !=
+      hipSuccess)
+    return err;
+  if (two)
+    hipLaunchKernelGGL(k_ql_rows<2>, dim3((n + rpb - 1) / rpb), dim3(64), lds_r, st, n, sw, rots,
+                       counts, Z, 0);
+  else
+    hipLaunchKernelGGL(k_ql_rows<1>, dim3((n + rpb - 1) / rpb), dim3(64), lds_r, st, n, sw, rots,
+                       counts, Z, 0);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  const size_t lds_b = sizeof(c64d) * kQBW * (size_t)n;
+  if (lds_b > kEighLdsMax) return hipErrorInvalidValue;
+  if ((err = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ql_back),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b)) !=
+      hipSuccess)
+    return err;
+  hipLaunchKernelGGL(k_ql_back, dim3((n + kQBW - 1) / kQBW), dim3(64 * kQBW), lds_b, st, n, R,
+                     tau, Z, V);
   return hipGetLastError();
 }
 

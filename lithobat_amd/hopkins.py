@@ -1,8 +1,8 @@
 """Sum of coherent systems (expose/hopkins.py of the reference), on the device.
 
 ``SOCSKernels.build`` decomposes the optics in FP64 on the device (Gram
-matrix of the TCC rows and a Hermitian Jacobi eigensolver); ``raw`` / ``image``
-image masks through the kept kernels with the same coherent-system kernel as
+matrix of the TCC rows, Householder tridiagonalisation and implicit QL);
+``raw`` / ``image`` image masks through the kept kernels with the same coherent-system kernel as
 the Abbe sum, in the precision the kernels were built for.
 """
 from __future__ import annotations
