@@ -943,7 +943,7 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
   LB_HIP_C(hipMalloc(&G, sizeof(cx<double>) * Sa * Sa));
   LB_HIP_C(hipMalloc(&V, sizeof(cx<double>) * Sa * Sa));
   LB_HIP_C(hipMalloc(&ev, sizeof(double) * Sa));
-  LB_HIP_C(hipMalloc(&counts, 4 * sizeof(int)));
+  LB_HIP_C(hipMalloc(&counts, 8 * sizeof(int)));
   LB_HIP_C(hipMalloc(&ework, std::max(eigh_work_bytes(Sa), gram_work_bytes(Sa, F))));
   LB_HIP_C(hipMalloc(&sys_dev, sizeof(PupilSys) * Sa));
   LB_HIP_C(hipMalloc(&dc, sizeof(double) * Sa));
@@ -956,15 +956,16 @@ lb_status lb_socs_plan_create(const lb_imaging_desc* d, double tol, int32_t max_
     LB_HIP_C(hermitian_eigh_f64(G, V, S, ework, ev, counts, 0));
     LB_HIP_C(hipDeviceSynchronize());
     LB_HIP_C(hipMemcpy(evals.data(), ev, sizeof(double) * S, hipMemcpyDeviceToHost));
-    int cnt[4] = {0, 0, 0, 0};
-    LB_HIP_C(hipMemcpy(cnt, counts, 3 * sizeof(int), hipMemcpyDeviceToHost));
-    if (cnt[2] != 0) {
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    LB_HIP_C(hipMemcpy(cnt, counts, 8 * sizeof(int), hipMemcpyDeviceToHost));
+    const int bad = cnt[2] != 0 ? cnt[2] : cnt[6];
+    if (bad != 0) {
       cleanup();
       delete h;
-      return fail(LB_ERR_HIP, cnt[2] == 1 ? "SOCS eigensolve: the QL iteration did not converge"
-                                          : "SOCS eigensolve: rotation log full");
+      return fail(LB_ERR_HIP, bad == 1 ? "SOCS eigensolve: the QL iteration did not converge"
+                                       : "SOCS eigensolve: rotation log full");
     }
-    p.eig_sweeps = cnt[0];
+    p.eig_sweeps = cnt[0] + cnt[4];
   }
   // order, truncation: hopkins.py:172-182
   std::vector<int> order(S);

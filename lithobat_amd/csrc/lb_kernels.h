@@ -267,10 +267,11 @@ hipError_t pupil_tables(const PupilSys* sys_dev, int n_sys, const PupilParams& p
 size_t gram_work_bytes(int S, long long F);
 hipError_t gram_f64(const cx<double>* A, int S, long long F, cx<double>* G, void* work,
                     hipStream_t st);
-// Hermitian eigensolve (Householder + implicit QL): G destroyed, evals (S,
+// Hermitian eigensolve (Householder tridiagonalisation; implicit QL, on two
+// halves merged by divide and conquer from 32 rows on): G destroyed, evals (S,
 // unordered), V = eigenvectors (columns); work: eigh_work_bytes(S) of scratch;
-// counts (4 ints) = {QL sweeps, rotations, status (0 ok, 1 no convergence,
-// 2 rotation log full)}
+// counts (8 ints, 4 per QL part) = {QL sweeps, rotations, status (0 ok, 1 no
+// convergence, 2 rotation log full), unused}
 size_t eigh_work_bytes(int S);
 hipError_t hermitian_eigh_f64(cx<double>* G, cx<double>* V, int S, void* work, double* evals,
                               int* counts, hipStream_t st);

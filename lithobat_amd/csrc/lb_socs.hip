@@ -267,15 +267,32 @@ __device__ __forceinline__ double rsqrt_refined(double x) {
 // rotations form one dependent chain, so each is kept short: the next
 // rotation's d and e are loaded ahead and 1 / r comes from a refined
 // reciprocal square root instead of a square root and a division.
-__global__ void k_tql(double* __restrict__ d_io, const double* __restrict__ e_in, int n,
-                      QLSweep* __restrict__ sweeps, double2* __restrict__ rots, int max_sweeps,
-                      int max_rots, int* __restrict__ counts) {
+//
+// With a split point h > 0 (divide and conquer), block b solves part b of
+// the torn matrix, T = diag(T1', T2') + beta u u^T with beta = e[h-1] and
+// u = e_{h-1} + e_h: part 0 is rows [0, h) with d[h-1] - beta, part 1 rows
+// [h, n) with d[h] - beta.  Each part has its own log (local indices) and
+// counts.
+__global__ void k_tql(double* __restrict__ d_io, const double* __restrict__ e_in, int n_all,
+                      int h, QLSweep* __restrict__ sweeps_all, double2* __restrict__ rots_all,
+                      int max_sweeps, int max_rots, int* __restrict__ counts_all) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int part = blockIdx.x;
+  const int off = part == 0 ? 0 : h;
+  const int n = h == 0 ? n_all : (part == 0 ? h : n_all - h);
+  QLSweep* sweeps = sweeps_all + (long long)part * max_sweeps;
+  double2* rots = rots_all + (long long)part * max_rots;
+  int* counts = counts_all + 4 * part;
   double* d = reinterpret_cast<double*>(smem_raw);  // n
   double* e = d + n;                                // n + 3 (e[n-1 ...] = 0)
+  const double beta = h > 0 ? e_in[h - 1] : 0.0;
   for (int i = threadIdx.x; i < n + 3; i += blockDim.x) {
-    if (i < n) d[i] = d_io[i];
-    e[i] = i + 1 < n ? e_in[i] : 0.0;
+    if (i < n) {
+      double di = d_io[off + i];
+      if (h > 0 && ((part == 0 && i == n - 1) || (part == 1 && i == 0))) di -= beta;
+      d[i] = di;
+    }
+    e[i] = i + 1 < n ? e_in[off + i] : 0.0;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -378,7 +395,7 @@ __global__ void k_tql(double* __restrict__ d_io, const double* __restrict__ e_in
       e[m] = 0.0;
     }
   }
-  for (int i = 0; i < n; ++i) d_io[i] = d[i];
+  for (int i = 0; i < n; ++i) d_io[off + i] = d[i];
   counts[0] = n_sw;
   counts[1] = n_rot;
   counts[2] = status;
@@ -394,12 +411,22 @@ __global__ void k_tql(double* __restrict__ d_io, const double* __restrict__ e_in
 // uniform, zero at run time): scalar loads share the LDS wait counter and
 // return out of order, so every wait for a row element would also wait for
 // them.
+// With a split h > 0, blockIdx.y is the part: its rows and columns
+// [off, off + n) of the n_all x n_all Z (zero outside the part's block).
 template <int RPL>
-__global__ __launch_bounds__(64) void k_ql_rows(int n, const QLSweep* __restrict__ sweeps,
-                                                const double2* __restrict__ rots,
-                                                const int* __restrict__ counts,
+__global__ __launch_bounds__(64) void k_ql_rows(int n_all, int h, int max_sweeps, int max_rots,
+                                                const QLSweep* __restrict__ sweeps_all,
+                                                const double2* __restrict__ rots_all,
+                                                const int* __restrict__ counts_all,
                                                 double* __restrict__ Z, int zero) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int part = blockIdx.y;
+  const int off = part == 0 ? 0 : h;
+  const int n = h == 0 ? n_all : (part == 0 ? h : n_all - h);
+  if ((int)blockIdx.x * 64 * RPL >= n) return;  // uniform: this part has fewer rows
+  const QLSweep* sweeps = sweeps_all + (long long)part * max_sweeps;
+  const double2* rots = rots_all + (long long)part * max_rots;
+  const int* counts = counts_all + 4 * part;
   const int ld = n + 1;
   double* z[RPL];
   int row[RPL];
@@ -453,8 +480,263 @@ __global__ __launch_bounds__(64) void k_ql_rows(int n, const QLSweep* __restrict
   }
 #pragma unroll
   for (int u = 0; u < RPL; ++u)
-    if (row[u] < n)
-      for (int c = 0; c < n; ++c) Z[(long long)row[u] * n + c] = z[u][c];
+    if (row[u] < n) {
+      double* zr = Z + (long long)(off + row[u]) * n_all;
+      for (int c = 0; c < n_all; ++c)
+        zr[c] = (c >= off && c < off + n) ? z[u][c - off] : 0.0;
+    }
+}
+
+// ------------------------------------------------ divide and conquer merge
+// The two parts' eigenpairs (T1' = Z1 D1 Z1^T, T2' = Z2 D2 Z2^T) give
+// T = Q (D + beta z z^T) Q^T with Q = diag(Z1, Z2) and z = Q^T u (row h-1 of
+// Z1 beside row 0 of Z2).  The rank-one problem is solved as in Cuppen, "A
+// divide and conquer method for the symmetric tridiagonal eigenproblem"
+// (Numer. Math. 36, 1981), with the eigenvectors from the recomputed z of Gu
+// and Eisenstat, "A divide-and-conquer algorithm for the symmetric tridiagonal
+// eigenproblem" (SIAM J. Matrix Anal. Appl. 16, 1995), which keeps them
+// orthogonal: beta < 0 is handled by negating the problem, z is normalised
+// (rho = |beta| |z|^2), D sorted ascending, entries with |rho z_i| below the
+// tolerance deflate (eigenvalue d_i, unit eigenvector), and of two kept
+// entries closer than the tolerance a rotation zeroes one z and deflates it.
+// The K kept eigenvalues are the roots of 1 + rho sum z_i^2 / (d_i - lambda),
+// one in each gap and one above the last d, found by bisection on
+// tau = lambda - d_o from the nearer pole d_o (so lambda - d_i is formed from
+// exact pole differences).
+struct DCInfo {
+  int K, n_giv;
+  double rho, sgn;
+};
+struct DCGivens {
+  int a, b;
+  double c, s;
+};
+
+constexpr int kDCT = 256;
+
+__global__ __launch_bounds__(kDCT) void k_dc_prep(const double* __restrict__ lam0,
+                                                  const double* __restrict__ Z,
+                                                  const double* __restrict__ e, int n, int h,
+                                                  double* __restrict__ delta,
+                                                  double* __restrict__ zz, int* __restrict__ perm,
+                                                  int* __restrict__ kidx, int* __restrict__ kpos,
+                                                  DCGivens* __restrict__ giv,
+                                                  DCInfo* __restrict__ info) {
+  __shared__ double red[kDCT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double beta = e[h - 1];
+  const double sgn = beta < 0.0 ? -1.0 : 1.0;
+  double part = 0;
+  for (int i = tid; i < n; i += kDCT) {
+    const double zi = Z[(long long)(h - 1) * n + i] + Z[(long long)h * n + i];
+    part += zi * zi;
+  }
+  part = wave_sum(part);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  double nz2 = 0;
+  for (int w = 0; w < kDCT / 64; ++w) nz2 += red[w];
+  const double inz = nz2 > 0.0 ? 1.0 / sqrt(nz2) : 0.0;
+  // rank sort, ascending (ties by index)
+  for (int i = tid; i < n; i += kDCT) {
+    const double di = sgn * lam0[i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = sgn * lam0[j];
+      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    perm[rank] = i;
+    delta[rank] = di;
+    zz[rank] = (Z[(long long)(h - 1) * n + i] + Z[(long long)h * n + i]) * inz;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  const double rho = fabs(beta) * nz2;
+  double dmax = 0;
+  for (int i = 0; i < n; ++i) dmax = fmax(dmax, fabs(delta[i]));
+  const double tol = 8.0 * 2.220446049250313e-16 * fmax(dmax, rho);
+  int K = 0, ng = 0, last = -1;
+  for (int i = 0; i < n; ++i) {
+    kpos[i] = -1;
+    if (!(fabs(rho * zz[i]) > tol)) {
+      zz[i] = 0.0;
+      continue;
+    }
+    if (last >= 0 && delta[i] - delta[last] <= tol) {
+      // rotate (last, i) so that z[last] becomes zero; last deflates
+      const double zl = zz[last], zi = zz[i];
+      const double r = sqrt(zl * zl + zi * zi);
+      const double c = zi / r, sn = zl / r;
+      zz[i] = r;
+      zz[last] = 0.0;
+      const double dl = delta[last], di = delta[i];
+      delta[last] = c * c * dl + sn * sn * di;
+      delta[i] = sn * sn * dl + c * c * di;
+      giv[ng++] = DCGivens{last, i, c, sn};
+      kpos[last] = -1;
+      --K;
+    }
+    kpos[i] = K;
+    last = i;
+    ++K;
+  }
+  // kept indices in order (kpos was assigned before later deflations: renumber)
+  int k = 0;
+  for (int i = 0; i < n; ++i)
+    if (kpos[i] >= 0) {
+      kpos[i] = k;
+      kidx[k++] = i;
+    }
+  info->K = k;
+  info->n_giv = ng;
+  info->rho = rho;
+  info->sgn = sgn;
+}
+
+// One wavefront per kept root j; the secular function's sum split over lanes.
+__global__ __launch_bounds__(64) void k_dc_roots(const double* __restrict__ delta,
+                                                 const double* __restrict__ zz,
+                                                 const int* __restrict__ kidx,
+                                                 const DCInfo* __restrict__ info,
+                                                 int* __restrict__ ro, double* __restrict__ rt) {
+  const int j = blockIdx.x, lane = threadIdx.x;
+  const int K = info->K;
+  if (j >= K) return;
+  const double rho = info->rho;
+  auto dk = [&](int i) { return delta[kidx[i]]; };
+  // f(o, t) = 1 + rho sum_i z_i^2 / ((d_i - d_o) - t)
+  auto f = [&](int o, double t) {
+    const double dko = dk(o);
+    double acc = 0;
+    for (int i = lane; i < K; i += 64) {
+      const double zi = zz[kidx[i]];
+      acc += zi * zi / ((dk(i) - dko) - t);
+    }
+    return 1.0 + rho * wave_sum(acc);
+  };
+  double gap;
+  if (j < K - 1) {
+    gap = dk(j + 1) - dk(j);
+  } else {
+    double acc = 0;
+    for (int i = lane; i < K; i += 64) acc += zz[kidx[i]] * zz[kidx[i]];
+    gap = rho * wave_sum(acc);
+  }
+  int o = j;
+  double a = 0.0, b = gap;
+  if (j < K - 1) {
+    const double mid = 0.5 * gap;
+    if (f(j, mid) > 0.0) {
+      b = mid;  // the root lies in (d_j, midpoint): origin d_j
+    } else {
+      o = j + 1;  // in [midpoint, d_{j+1}): origin d_{j+1}
+      a = -mid;
+      b = 0.0;
+    }
+  }
+  for (int it = 0; it < 400; ++it) {
+    const double t = 0.5 * (a + b);
+    if (t == a || t == b) break;
+    if (f(o, t) > 0.0)
+      b = t;
+    else
+      a = t;
+  }
+  if (lane == 0) {
+    ro[j] = o;
+    rt[j] = 0.5 * (a + b);
+  }
+}
+
+// lambda_j - d_i from the root's origin: (d_o - d_i) + tau
+__device__ __forceinline__ double dc_gap(const double* delta, const int* kidx, const int* ro,
+                                         const double* rt, int i, int j) {
+  return (delta[kidx[ro[j]]] - delta[kidx[i]]) + rt[j];
+}
+
+// Recomputed z (Gu-Eisenstat): zhat_i^2 = (lam_K - d_i) prod_{j<i} (lam_j - d_i) /
+// (d_j - d_i) prod_{i<=j<K-1} (lam_j - d_i) / (d_{j+1} - d_i) / rho, signed as z_i.
+__global__ void k_dc_zhat(const double* __restrict__ delta, const double* __restrict__ zz,
+                          const int* __restrict__ kidx, const DCInfo* __restrict__ info,
+                          const int* __restrict__ ro, const double* __restrict__ rt,
+                          double* __restrict__ zh) {
+  const int K = info->K;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  const double di = delta[kidx[i]];
+  double v = dc_gap(delta, kidx, ro, rt, i, K - 1) / info->rho;
+  for (int j = 0; j < i; ++j) v *= dc_gap(delta, kidx, ro, rt, i, j) / (delta[kidx[j]] - di);
+  for (int j = i; j < K - 1; ++j)
+    v *= dc_gap(delta, kidx, ro, rt, i, j) / (delta[kidx[j + 1]] - di);
+  zh[i] = copysign(sqrt(fmax(v, 0.0)), zz[kidx[i]]);
+}
+
+// Column c of U (rank-one eigenvectors in the sorted basis; unit for a
+// deflated c) and its eigenvalue, one wavefront per column.
+__global__ __launch_bounds__(64) void k_dc_u(int n, const double* __restrict__ delta,
+                                             const int* __restrict__ kidx,
+                                             const int* __restrict__ kpos,
+                                             const DCInfo* __restrict__ info,
+                                             const int* __restrict__ ro,
+                                             const double* __restrict__ rt,
+                                             const double* __restrict__ zh,
+                                             double* __restrict__ U, double* __restrict__ lam) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int j = kpos[c];
+  const double sgn = info->sgn;
+  if (j < 0) {
+    for (int r = lane; r < n; r += 64) U[(long long)r * n + c] = r == c ? 1.0 : 0.0;
+    if (lane == 0) lam[c] = sgn * delta[c];
+    return;
+  }
+  const int K = info->K;
+  double nrm = 0;
+  for (int i = lane; i < K; i += 64) {
+    const double u = zh[i] / dc_gap(delta, kidx, ro, rt, i, j);
+    nrm += u * u;
+  }
+  nrm = wave_sum(nrm);
+  const double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+  for (int r = lane; r < n; r += 64) U[(long long)r * n + c] = 0.0;
+  for (int i = lane; i < K; i += 64)
+    U[(long long)kidx[i] * n + c] = -zh[i] / dc_gap(delta, kidx, ro, rt, i, j) * inv;
+  if (lane == 0) lam[c] = sgn * (delta[kidx[ro[j]]] + rt[j]);
+}
+
+// Row r of Q in the sorted basis with the deflation rotations applied.
+__global__ void k_dc_qp(const double* __restrict__ Z, int n, const int* __restrict__ perm,
+                        const DCGivens* __restrict__ giv, const DCInfo* __restrict__ info,
+                        double* __restrict__ Qp) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double* q = Qp + (long long)r * n;
+  for (int c = 0; c < n; ++c) q[c] = Z[(long long)r * n + perm[c]];
+  const int ng = info->n_giv;
+  for (int g = 0; g < ng; ++g) {
+    const DCGivens G = giv[g];
+    const double qa = q[G.a], qb = q[G.b];
+    q[G.a] = G.c * qa - G.s * qb;
+    q[G.b] = G.s * qa + G.c * qb;
+  }
+}
+
+// C = A B (n x n, FP64), 16 x 16 tiles.
+__global__ __launch_bounds__(256) void k_dgemm_nn(const double* __restrict__ A,
+                                                  const double* __restrict__ B, int n,
+                                                  double* __restrict__ C) {
+  __shared__ double As[16][17], Bs[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int r = blockIdx.y * 16 + ty, c = blockIdx.x * 16 + tx;
+  double acc = 0;
+  for (int k0 = 0; k0 < n; k0 += 16) {
+    As[ty][tx] = (r < n && k0 + tx < n) ? A[(long long)r * n + k0 + tx] : 0.0;
+    Bs[ty][tx] = (k0 + ty < n && c < n) ? B[(long long)(k0 + ty) * n + c] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc += As[ty][k] * Bs[k][tx];
+    __syncthreads();
+  }
+  if (r < n && c < n) C[(long long)r * n + c] = acc;
 }
 
 // V = H(0) H(1) ... H(n-2) Z: one wavefront per column of Z (held in LDS),
@@ -488,17 +770,65 @@ __global__ __launch_bounds__(64 * kQBW) void k_ql_back(int n, const c64d* __rest
   for (int r = lane; r < n; r += 64) V[(long long)r * n + j] = z[r];
 }
 
-// Log capacity: the QL takes about n^2 rotations in n + n/2 sweeps (1.4e4 to
-// 2e4 rotations at n = 142); four times that is a generous bound.
+// Log capacity: the QL takes about m^2 rotations on an m-row part in
+// m + m/2 sweeps; four times that is a generous bound.
 static size_t eigh_max_sweeps(size_t n) { return 8 * n + 64; }
 static size_t eigh_max_rots(size_t n) { return 4 * n * n + 1024; }
 
-size_t eigh_work_bytes(int S) {
+// Below this order the tridiagonal goes to one QL; from it on, to two halves
+// and a merge.
+constexpr int kDCMin = 32;
+
+namespace {
+struct EighWork {
+  c64d* R;
+  c64d* tau;
+  double *e, *Z, *Qp, *U, *Zt, *delta, *zz, *rt, *zh;
+  int *perm, *kidx, *kpos, *ro;
+  DCGivens* giv;
+  DCInfo* info;
+  QLSweep* sw;
+  double2* rots;
+  size_t max_sw, max_rot;  // per part
+  size_t bytes;
+};
+
+EighWork eigh_layout(char* base, int S) {
   const size_t n = (size_t)std::max(S, 1);
-  const size_t max_sw = eigh_max_sweeps(n), max_rot = eigh_max_rots(n);
-  return sizeof(c64d) * (n * n + n) + sizeof(double) * (n * n + 2 * n) +
-         sizeof(QLSweep) * max_sw + sizeof(double2) * max_rot + 6 * 256;  // 6 aligned pieces
+  EighWork w{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* q = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  w.max_sw = eigh_max_sweeps(n);
+  w.max_rot = eigh_max_rots(n);
+  w.R = reinterpret_cast<c64d*>(take(sizeof(c64d) * n * n));
+  w.tau = reinterpret_cast<c64d*>(take(sizeof(c64d) * n));
+  w.e = reinterpret_cast<double*>(take(sizeof(double) * n));
+  w.Z = reinterpret_cast<double*>(take(sizeof(double) * n * n));
+  w.Qp = reinterpret_cast<double*>(take(sizeof(double) * n * n));
+  w.U = reinterpret_cast<double*>(take(sizeof(double) * n * n));
+  w.Zt = reinterpret_cast<double*>(take(sizeof(double) * n * n));
+  w.delta = reinterpret_cast<double*>(take(sizeof(double) * n));
+  w.zz = reinterpret_cast<double*>(take(sizeof(double) * n));
+  w.rt = reinterpret_cast<double*>(take(sizeof(double) * n));
+  w.zh = reinterpret_cast<double*>(take(sizeof(double) * n));
+  w.perm = reinterpret_cast<int*>(take(sizeof(int) * n));
+  w.kidx = reinterpret_cast<int*>(take(sizeof(int) * n));
+  w.kpos = reinterpret_cast<int*>(take(sizeof(int) * n));
+  w.ro = reinterpret_cast<int*>(take(sizeof(int) * n));
+  w.giv = reinterpret_cast<DCGivens*>(take(sizeof(DCGivens) * n));
+  w.info = reinterpret_cast<DCInfo*>(take(sizeof(DCInfo)));
+  w.sw = reinterpret_cast<QLSweep*>(take(sizeof(QLSweep) * w.max_sw * 2));
+  w.rots = reinterpret_cast<double2*>(take(sizeof(double2) * w.max_rot * 2));
+  w.bytes = off;
+  return w;
 }
+}  // namespace
+
+size_t eigh_work_bytes(int S) { return eigh_layout(nullptr, S).bytes; }
 
 constexpr size_t kEighLdsMax = 160 * 1024 - 1024;
 
@@ -506,20 +836,8 @@ hipError_t hermitian_eigh_f64(c64d* G, c64d* V, int S, void* work, double* evals
                               hipStream_t st) {
   if (S <= 0) return hipSuccess;
   const int n = S;
-  const size_t max_sw = eigh_max_sweeps(n), max_rot = eigh_max_rots(n);
-  if (max_rot > (size_t)INT32_MAX) return hipErrorInvalidValue;
-  char* w = static_cast<char*>(work);
-  auto take = [&](size_t bytes) {
-    char* q = w;
-    w += (bytes + 255) & ~(size_t)255;
-    return q;
-  };
-  c64d* R = reinterpret_cast<c64d*>(take(sizeof(c64d) * n * n));
-  c64d* tau = reinterpret_cast<c64d*>(take(sizeof(c64d) * n));
-  double* e = reinterpret_cast<double*>(take(sizeof(double) * n));
-  double* Z = reinterpret_cast<double*>(take(sizeof(double) * n * n));
-  QLSweep* sw = reinterpret_cast<QLSweep*>(take(sizeof(QLSweep) * max_sw));
-  double2* rots = reinterpret_cast<double2*>(take(sizeof(double2) * max_rot));
+  const EighWork W = eigh_layout(static_cast<char*>(work), S);
+  if (W.max_rot > (size_t)INT32_MAX) return hipErrorInvalidValue;
   hipError_t err;
   // the trailing matrix moves into LDS at the first step where it fits there
   int i_lds = n;
@@ -535,7 +853,7 @@ hipError_t hermitian_eigh_f64(c64d* G, c64d* V, int S, void* work, double* evals
     hipError_t e2 = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_h);
     if (e2 != hipSuccess) return e2;
-    hipLaunchKernelGGL(kern, dim3(1), dim3(kHT), lds_h, st, G, n, i_lds, evals, e, tau, R);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kHT), lds_h, st, G, n, i_lds, evals, W.e, W.tau, W.R);
     return hipGetLastError();
   };
   const int nv = (n + 63) / 64;
@@ -547,34 +865,59 @@ hipError_t hermitian_eigh_f64(c64d* G, c64d* V, int S, void* work, double* evals
   else if (nv <= 16) err = hetrd(k_hetrd<16>);
   else return hipErrorInvalidValue;  // more than 1024 source points
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(k_tql, dim3(1), dim3(64), sizeof(double) * (2 * (size_t)n + 3), st, evals,
-                     e, n, sw, rots, (int)max_sw, (int)max_rot, counts);
+  // QL on the whole tridiagonal, or on its two halves (one block each)
+  const int h = n >= kDCMin ? n / 2 : 0;
+  const int parts = h > 0 ? 2 : 1;
+  const int len = h > 0 ? n - h : n;  // the longer part
+  // both parts' counts start at zero (one part leaves the second's unwritten);
+  // plan set-up is never graph-captured
+  if ((err = hipMemsetAsync(counts, 0, 8 * sizeof(int), st)) != hipSuccess) return err;
+  hipLaunchKernelGGL(k_tql, dim3(parts), dim3(64), sizeof(double) * (2 * (size_t)len + 3), st,
+                     evals, W.e, n, h, W.sw, W.rots, (int)W.max_sw, (int)W.max_rot, counts);
   if ((err = hipGetLastError()) != hipSuccess) return err;
   // two rows per lane (128 per block) where they fit in LDS, else one
-  const bool two = sizeof(double) * 128 * (size_t)(n + 1) <= kEighLdsMax;
+  const bool two = sizeof(double) * 128 * (size_t)(len + 1) <= kEighLdsMax;
   const int rpb = two ? 128 : 64;
-  const size_t lds_r = sizeof(double) * (size_t)rpb * (n + 1);
+  const size_t lds_r = sizeof(double) * (size_t)rpb * (len + 1);
   if (lds_r > kEighLdsMax) return hipErrorInvalidValue;
   const void* kr = two ? reinterpret_cast<const void*>(k_ql_rows<2>)
                        : reinterpret_cast<const void*>(k_ql_rows<1>);
   if ((err = hipFuncSetAttribute(kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_r)) !=
       hipSuccess)
     return err;
+  const dim3 gr((len + rpb - 1) / rpb, parts);
   if (two)
-    hipLaunchKernelGGL(k_ql_rows<2>, dim3((n + rpb - 1) / rpb), dim3(64), lds_r, st, n, sw, rots,
-                       counts, Z, 0);
+    hipLaunchKernelGGL(k_ql_rows<2>, gr, dim3(64), lds_r, st, n, h, (int)W.max_sw,
+                       (int)W.max_rot, W.sw, W.rots, counts, W.Z, 0);
   else
-    hipLaunchKernelGGL(k_ql_rows<1>, dim3((n + rpb - 1) / rpb), dim3(64), lds_r, st, n, sw, rots,
-                       counts, Z, 0);
+    hipLaunchKernelGGL(k_ql_rows<1>, gr, dim3(64), lds_r, st, n, h, (int)W.max_sw,
+                       (int)W.max_rot, W.sw, W.rots, counts, W.Z, 0);
   if ((err = hipGetLastError()) != hipSuccess) return err;
+  const double* Zt = W.Z;
+  if (h > 0) {
+    hipLaunchKernelGGL(k_dc_prep, dim3(1), dim3(kDCT), 0, st, evals, W.Z, W.e, n, h, W.delta,
+                       W.zz, W.perm, W.kidx, W.kpos, W.giv, W.info);
+    hipLaunchKernelGGL(k_dc_roots, dim3(n), dim3(64), 0, st, W.delta, W.zz, W.kidx, W.info, W.ro,
+                       W.rt);
+    hipLaunchKernelGGL(k_dc_zhat, dim3((n + 255) / 256), dim3(256), 0, st, W.delta, W.zz, W.kidx,
+                       W.info, W.ro, W.rt, W.zh);
+    hipLaunchKernelGGL(k_dc_u, dim3(n), dim3(64), 0, st, n, W.delta, W.kidx, W.kpos, W.info,
+                       W.ro, W.rt, W.zh, W.U, evals);
+    hipLaunchKernelGGL(k_dc_qp, dim3((n + 255) / 256), dim3(256), 0, st, W.Z, n, W.perm, W.giv,
+                       W.info, W.Qp);
+    hipLaunchKernelGGL(k_dgemm_nn, dim3((n + 15) / 16, (n + 15) / 16), dim3(256), 0, st, W.Qp,
+                       W.U, n, W.Zt);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    Zt = W.Zt;
+  }
   const size_t lds_b = sizeof(c64d) * kQBW * (size_t)n;
   if (lds_b > kEighLdsMax) return hipErrorInvalidValue;
   if ((err = hipFuncSetAttribute(reinterpret_cast<const void*>(k_ql_back),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b)) !=
       hipSuccess)
     return err;
-  hipLaunchKernelGGL(k_ql_back, dim3((n + kQBW - 1) / kQBW), dim3(64 * kQBW), lds_b, st, n, R,
-                     tau, Z, V);
+  hipLaunchKernelGGL(k_ql_back, dim3((n + kQBW - 1) / kQBW), dim3(64 * kQBW), lds_b, st, n, W.R,
+                     W.tau, Zt, V);
   return hipGetLastError();
 }
 
