@@ -653,7 +653,10 @@ def main():
     ilt = ImagingPlan(optics_cfg4(), grid, precision=32)
     if args.tune:
         from lithobat_amd import autotune
-        res = autotune.tune(plan, masks, adjoint=False, save=False)
+        # the coarse grid (80, 96 or 128 points here) and its kernel shapes
+        res = autotune.tune_grid(
+            lambda m: ImagingPlan(optics, grid, planes, precision=32, coarse_m=m), masks,
+            adjoint=False, save=False)
         th = torch.randn((1, N, N), device=dev)
         res.update(autotune.tune(ilt, torch.sigmoid(8 * th), adjoint=True, save=False))
         thb = torch.randn((ILT_BATCH, N, N), device=dev)
@@ -670,6 +673,7 @@ def main():
         table.update(res)
         autotune.save_table(table, args.tune)
         print(json.dumps(res), file=sys.stderr)
+        plan = ImagingPlan(optics, grid, planes, precision=32)  # on the chosen grid
     plan._use_tuning(N_MASKS)
     n_img = N_MASKS * len(planes)
     out = torch.empty((N_MASKS, len(planes), N, N), device=dev, dtype=torch.float32)

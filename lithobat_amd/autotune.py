@@ -8,6 +8,12 @@ a JSON table keyed on (precision, coarse grid M, pupil box W, images per call
 and systems per image, both bucketed to powers of two), so one tuning run
 serves every later plan of the same shape.  The table lives at
 ``$LITHOBAT_TUNING`` or ``lithobat_amd/tuning.json``.
+
+``tune_grid`` also chooses the coarse grid itself (and with it the radix
+factorisation of its transforms: 80 = 10 x 8, 96 = 12 x 8, 128 = 16 x 8, ...):
+it builds the plan on every admissible grid (side >= 2W - 1, coarser than the
+image), tunes each, and records the fastest under (precision, N, W); a plan
+created without an explicit ``coarse_m`` is built on the recorded grid.
 """
 from __future__ import annotations
 
@@ -103,6 +109,64 @@ def apply(plan, n_masks: int) -> bool:
         if not done:
             _set(plan, adjoint, 0, 0, 0)
     return hit
+
+
+# coarse grid sides with compiled coherent-system kernels (radix plans)
+FIELD_SIZES = (16, 32, 48, 64, 80, 96, 128, 256)
+
+
+def grid_key(precision: int, n: int, box: int) -> str:
+    return f"p{precision}-N{n}-W{box}-grid"
+
+
+def grid_candidates(n: int, box: int, limit: int = 3) -> list[int]:
+    """Admissible coarse grids for an n-pixel image with a pupil box of ``box`` bins: the
+    smallest ``limit`` compiled sides >= 2 box - 1 that are coarser than the image."""
+    return [m for m in FIELD_SIZES if m >= 2 * box - 1 and m < n][:limit]
+
+
+def recorded_grid(precision: int, n: int, box: int) -> int:
+    """The coarse grid recorded by ``tune_grid`` for this geometry, or 0."""
+    e = load_table().get(grid_key(precision, n, box))
+    try:
+        return int(e["coarse_m"]) if e else 0
+    except (KeyError, TypeError, ValueError):
+        return 0
+
+
+def tune_grid(make_plan, masks: torch.Tensor, adjoint: bool = True, reps: int = 5,
+              save: bool = True, candidates=None) -> dict:
+    """Choose the coarse grid: for each admissible side M, ``make_plan(M)`` builds the
+    plan on it, its kernel shapes are tuned for this batch (``tune``), and the forward
+    (plus adjoint) time with the winning shapes decides.  Returns the shape winners of
+    every grid and, under ``grid_key``, the chosen side with every candidate's time."""
+    probe = make_plan(0)
+    i = probe.info
+    prec, n, box = probe.precision, probe.n, i["box"]
+    cands = list(candidates) if candidates else grid_candidates(n, box)
+    if i["m"] not in cands and i["m"] < n:
+        cands.append(i["m"])
+    del probe
+    results, times = {}, {}
+    for m in sorted(set(cands)):
+        plan = make_plan(m)
+        if plan.info["m"] != m:
+            continue
+        res = tune(plan, masks, adjoint=adjoint, reps=reps, save=False)
+        results.update(res)
+        times[m] = sum(v["ms"] for v in res.values())
+    if not times:
+        raise ValueError("no admissible coarse grid")
+    best = min(times, key=times.get)
+    results[grid_key(prec, n, box)] = {
+        "coarse_m": best, "ms": round(times[best], 4),
+        "candidates": {str(m): round(t, 4) for m, t in sorted(times.items())}}
+    if save:
+        table = load_table()
+        with _lock:
+            table.update(results)
+        save_table(table)
+    return results
 
 
 def _time(fn, reps: int) -> float:

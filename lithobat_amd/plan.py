@@ -128,6 +128,8 @@ class ImagingPlan:
         handle = C.c_void_p()
         if kind == "abbe":
             _lib.check(lib.lb_abbe_plan_create(C.byref(d), C.byref(handle)))
+            if coarse_m == 0:
+                handle = self._tuned_grid(lib, d, handle, precision)
         elif kind == "socs":
             mk = -1 if max_kernels is None else int(max_kernels)
             _lib.check(lib.lb_socs_plan_create(C.byref(d), float(tol), mk, C.byref(handle)))
@@ -158,6 +160,24 @@ class ImagingPlan:
             self.energy_kept = float(e.value)
             self.n_band = int(nb.value)
             self.n_source = int(ns.value)
+
+    def _tuned_grid(self, lib, d, handle, precision: int):
+        """The plan rebuilt on the coarse grid ``autotune.tune_grid`` recorded for this
+        geometry (precision, image side, pupil box), when that differs from the automatic
+        one; the automatic plan where nothing is recorded or the rebuild fails."""
+        from . import autotune
+        info = _lib.PlanInfo()
+        _lib.check(lib.lb_plan_info_get(handle, C.byref(info)))
+        m = autotune.recorded_grid(precision, self.n, int(info.box))
+        if not m or m == int(info.m):
+            return handle
+        d.coarse_m = m
+        tuned = C.c_void_p()
+        if lib.lb_abbe_plan_create(C.byref(d), C.byref(tuned)) != _lib.LB_OK:
+            d.coarse_m = 0
+            return handle
+        lib.lb_plan_destroy(handle)
+        return tuned
 
     # ------------------------------------------------------------------
     def __del__(self):

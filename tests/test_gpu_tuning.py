@@ -44,3 +44,36 @@ def test_shapes_follow_the_batch_size(device):
     autotune.unpin(plan)
     plan.forward(m[:1])
     assert shape() == want1
+
+
+def test_grid_choice_is_recorded_and_used(device, tmp_path, monkeypatch):
+    """tune_grid times the plan on every admissible coarse grid (side >= 2W - 1, coarser
+    than the image), records the fastest under (precision, N, W), and a plan created
+    without an explicit grid is built on it; images agree across grids."""
+    import torch
+    from lithobat_amd import autotune
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.plan import ImagingPlan
+    monkeypatch.setattr(autotune, "TABLE_PATH", str(tmp_path / "table.json"))
+    monkeypatch.setattr(autotune, "_table", None)
+    optics = OpticsConfig(NA=0.93, source_type="quadrupole", sigma_outer=0.85, sigma_inner=0.55,
+                          source_grid=21)
+    grid = GridConfig(n_pixels=256, pixel_size=4e-9)
+
+    def make(m):
+        return ImagingPlan(optics, grid, precision=32, coarse_m=m)
+
+    auto = make(0)  # empty table: the automatic grid
+    box = auto.info["box"]
+    cands = autotune.grid_candidates(256, box)
+    assert len(cands) >= 2 and auto.info["m"] == cands[0]
+    masks = torch.rand((4, 256, 256), device=device)
+    res = autotune.tune_grid(make, masks, adjoint=True, reps=1, save=True)
+    rec = res[autotune.grid_key(32, 256, box)]
+    assert rec["coarse_m"] in cands and set(rec["candidates"]) == {str(m) for m in cands}
+    monkeypatch.setattr(autotune, "_table", None)  # read back from the file
+    tuned = ImagingPlan(optics, grid, precision=32)
+    assert tuned.info["m"] == rec["coarse_m"]
+    a = auto.forward(masks, with_peak=False)[0]
+    b = tuned.forward(masks, with_peak=False)[0]
+    assert float((a - b).abs().max() / a.abs().max()) <= 1e-5
