@@ -171,7 +171,7 @@ def ilt_leg(args, ilt, dev, sync):
     step)."""
     import torch
 
-    from lithobat_amd.ilt import abbe_intensity, mask_of, relaxed_resist_loss
+    from lithobat_amd.ilt import abbe_intensity, ilt_objective, mask_of, relaxed_resist_loss
     from lithobat_amd.masks import manhattan
     scale = 1.0 / float(ilt.clear[0])
     rng = np.random.default_rng(3000)
@@ -199,6 +199,18 @@ def ilt_leg(args, ilt, dev, sync):
         loss.backward()
 
     batched = timed(batch_step, max(2, args.ilt_steps // 2), 1, sync) / ILT_BATCH
+
+    # the same objective as one autograd function (ILTObjective): the sigmoid inside the
+    # band analysis, the loss gradient applied with the sigmoid derivative on the device
+    def fused_step():
+        theta.grad = None
+        ilt_objective(theta, ilt, target, scale, 0.3, 50.0, beta=8.0).backward()
+
+    def fused_batch_step():
+        thetas.grad = None
+        ilt_objective(thetas, ilt, targets, scale, 0.3, 50.0, beta=8.0).backward()
+
+    fused_batched = timed(fused_batch_step, max(2, args.ilt_steps // 2), 1, sync) / ILT_BATCH
     # the adjoint coherent-system kernel of the batched step, from HIP events on its stream
     ilt.profile(True)
     for _ in range(max(3, args.ilt_steps // 2)):
@@ -248,8 +260,18 @@ def ilt_leg(args, ilt, dev, sync):
     # replayed: the single-mask step is bound by host launches when run eagerly
     g_one = graph_step_seconds(ilt_step, lambda: theta.grad, args, sync)
     g_batch = graph_step_seconds(batch_step, lambda: thetas.grad, args, sync)
+    gf_one = graph_step_seconds(fused_step, lambda: theta.grad, args, sync)
+    gf_batch = graph_step_seconds(fused_batch_step, lambda: thetas.grad, args, sync)
+    fused = {"batched_steps_per_s": ratio(1.0, fused_batched),
+             "graph_steps_per_s": ratio(1.0, gf_one),
+             "graph_batched_steps_per_s": (None if gf_batch is None
+                                           else ILT_BATCH / gf_batch),
+             "definition": "ILTObjective: theta -> relaxed-resist cost of the Abbe image of "
+                           "sigmoid(8 theta) in one autograd function (sigmoid inside the band "
+                           "analysis, loss gradient and sigmoid derivative in one device pass); "
+                           "same value and gradient as the composed path"}
     return (one, batched, dense, blurred, g_one,
-            None if g_batch is None else g_batch / ILT_BATCH, roof)
+            None if g_batch is None else g_batch / ILT_BATCH, roof, fused)
 
 
 def graph_step_seconds(fn, result, args, sync):
@@ -750,10 +772,10 @@ def main():
     ilt_s = ilt_batched_s = ilt_dense_s = ilt_blur_s = vol_s = None
     ilt_graph_s = ilt_graph_batched_s = None
     clip_ilt_s = clip_plain_s = dense_images_per_s = None
-    ilt_roof = None
+    ilt_roof = ilt_fused = None
     if "ilt" in legs:
         (ilt_s, ilt_batched_s, ilt_dense_s, ilt_blur_s, ilt_graph_s,
-         ilt_graph_batched_s, ilt_roof) = ilt_leg(args, ilt, dev, sync)
+         ilt_graph_batched_s, ilt_roof, ilt_fused) = ilt_leg(args, ilt, dev, sync)
     if "volume" in legs:
         vol_s = volume_leg(args, masks, sync)
     solve = run_ilt_leg(args, sync) if "solve" in legs else None
@@ -836,6 +858,7 @@ def main():
         "ilt_blur_steps_per_s": ratio(1.0, ilt_blur_s),
         "ilt_graph_steps_per_s": ratio(1.0, ilt_graph_s),
         "ilt_graph_batched_steps_per_s": ratio(1.0, ilt_graph_batched_s),
+        "ilt_fused": ilt_fused,
         "exposure_volumes_per_s": ratio(1.0, vol_s),
         "run_ilt_iterations_per_s": None if solve is None else 1.0 / solve[0],
         "run_ilt_config": "512^2 @ 8 nm, NA 1.35 annular (142 pts) SOCS kernels FP32, "

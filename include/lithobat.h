@@ -200,6 +200,16 @@ lb_status lb_backward_spectrum(lb_plan plan, const void* spectrum, int32_t n_mas
                                const void* d_images, void* d_masks, void* workspace,
                                size_t workspace_bytes, void* stream);
 
+/* lb_mask_spectrum of the ILT mask m = sigmoid(clip(beta * theta, -60, 60))
+ * (opc/ilt.py:368-378, _mask_of without its Gaussian) from the real parameter
+ * theta [n_masks][N][N]: the sigmoid is applied as the band analysis loads
+ * theta, so no mask array is written.  LB_ERR_UNSUPPORTED for a grid side
+ * without the persistent analysis row pass (then lb_mask_sigmoid followed by
+ * lb_mask_spectrum gives the same spectra).  Workspace as lb_mask_spectrum. */
+lb_status lb_mask_spectrum_sigmoid(lb_plan plan, const void* theta, int32_t n_masks, double beta,
+                                   void* spectrum, void* workspace, size_t workspace_bytes,
+                                   void* stream);
+
 /* lb_backward_spectrum for complex masks: d_masks[mask][N][N] (interleaved complex) =
  * dJ/dRe m + i dJ/dIm m of J = sum d_images * I, the gradient torch's autograd uses for a
  * complex input; its real part is the reference's projection (opc/ilt.py:235-266 takes
@@ -330,6 +340,14 @@ lb_status lb_mask_sigmoid(const void* x, int64_t n, double beta, void* mask, int
                           void* stream);
 lb_status lb_mask_sigmoid_grad(const void* grad_mask, const void* mask, int64_t n, double beta,
                                void* grad_x, int32_t precision, void* stream);
+/* opc/ilt.py:381-389 (_grad_theta without its Gaussian) from theta rather than
+ * the mask: grad_x = g * grad_mask * beta * m * (1 - m) with m recomputed as
+ * lb_mask_sigmoid does, g a scalar read from device memory (the incoming
+ * gradient of a scalar loss, of the same precision; null: 1), so neither the
+ * mask nor a loss-scaled intensity derivative needs a pass of its own. */
+lb_status lb_mask_sigmoid_grad_theta(const void* grad_mask, const void* theta, int64_t n,
+                                     double beta, const void* g, void* grad_x,
+                                     int32_t precision, void* stream);
 
 /* No counterpart in the reference: the coherent-system kernel shape of a
  * batch's forward or adjoint launch (single-launch shapes only; the work

@@ -62,7 +62,8 @@ EXPORTS = [
     "lb_mask_sigmoid_grad", "lb_mask_penalties", "lb_adam_step", "lb_adam_step_dev",
     "lb_ilt_schedule", "lb_mask_sigmoid_dev", "lb_mask_sigmoid_grad_dev", "lb_batch_set_shape",
     "lb_volume_interp", "lb_mask_spectrum_bytes", "lb_mask_spectrum", "lb_forward_spectrum",
-    "lb_backward_spectrum", "lb_batch_backward_reuse",
+    "lb_backward_spectrum", "lb_batch_backward_reuse", "lb_mask_spectrum_sigmoid",
+    "lb_mask_sigmoid_grad_theta",
 ]
 
 _lib = None
@@ -135,6 +136,9 @@ def load() -> C.CDLL:
         "lb_mask_spectrum": ([vp, vp, i32, i32, vp, vp, sz, vp], C.c_int),
         "lb_forward_spectrum": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
         "lb_backward_spectrum": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
+        "lb_mask_spectrum_sigmoid": ([vp, vp, i32, C.c_double, vp, vp, sz, vp], C.c_int),
+        "lb_mask_sigmoid_grad_theta": ([vp, vp, C.c_int64, C.c_double, vp, vp, i32, vp],
+                                       C.c_int),
         "lb_batch_backward_reuse": ([vp, C.POINTER(vp), C.POINTER(vp), vp, sz, vp, vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),
         "lb_batch_forward": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, sz, vp],

@@ -1318,6 +1318,35 @@ lb_status lb_mask_spectrum(lb_plan plan, const void* masks, int32_t cplx, int32_
   return LB_OK;
 }
 
+lb_status lb_mask_spectrum_sigmoid(lb_plan plan, const void* theta, int32_t n_masks, double beta,
+                                   void* spectrum, void* workspace, size_t ws_bytes,
+                                   void* stream) {
+  g_err.clear();
+  if (!plan || !theta || !spectrum) return fail(LB_ERR_ARG, "null plan, theta or spectrum");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  Work w = layout(p, n_masks, false);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  const double invN2 = 1.0 / ((double)p.N * (double)p.N);
+  const hipError_t e =
+      p.precision == 64
+          ? band_analysis_sigmoid<double>(p.N, (const double*)theta, n_masks, p.nu0m, p.Wbm,
+                                          invN2, beta, (cx<double>*)(ws + w.RTm),
+                                          (cx<double>*)spectrum, st)
+          : band_analysis_sigmoid<float>(p.N, (const float*)theta, n_masks, p.nu0m, p.Wbm,
+                                         (float)invN2, (float)beta, (cx<float>*)(ws + w.RTm),
+                                         (cx<float>*)spectrum, st);
+  if (e == hipErrorNotSupported)
+    return fail(LB_ERR_UNSUPPORTED, "no fused sigmoid analysis for a " + std::to_string(p.N) +
+                                        "-point grid (apply lb_mask_sigmoid, then "
+                                        "lb_mask_spectrum)");
+  LB_HIP(e);
+  return LB_OK;
+}
+
 lb_status lb_forward_spectrum(lb_plan plan, const void* spectrum, int32_t n_masks, void* images,
                               void* peak, void* workspace, size_t ws_bytes, void* stream) {
   g_err.clear();
@@ -2030,6 +2059,24 @@ lb_status lb_mask_sigmoid_grad_dev(const void* grad_mask, const void* mask, int6
                                    void* stream) {
   if (!beta) return fail(LB_ERR_ARG, "null beta");
   return mask_sigmoid_grad_any(grad_mask, mask, n, 0.0, beta, grad_x, precision, stream);
+}
+
+lb_status lb_mask_sigmoid_grad_theta(const void* grad_mask, const void* theta, int64_t n,
+                                     double beta, const void* gscale, void* grad_x,
+                                     int32_t precision, void* stream) {
+  g_err.clear();
+  if (!grad_mask || !theta || !grad_x) return fail(LB_ERR_ARG, "null argument");
+  if (n < 0) return fail(LB_ERR_ARG, "n must be >= 0");
+  if (precision != 32 && precision != 64) return fail(LB_ERR_ARG, "precision must be 32 or 64");
+  hipStream_t st = (hipStream_t)stream;
+  if (precision == 64)
+    LB_HIP(mask_sigmoid_grad_theta<double>((const double*)grad_mask, (const double*)theta, n, beta,
+                                           nullptr, (const double*)gscale, (double*)grad_x, st));
+  else
+    LB_HIP(mask_sigmoid_grad_theta<float>((const float*)grad_mask, (const float*)theta, n,
+                                          (float)beta, nullptr, (const float*)gscale,
+                                          (float*)grad_x, st));
+  return LB_OK;
 }
 
 lb_status lb_forward_dense(lb_plan plan, const void* mask, int32_t plane, int32_t batch,

@@ -428,14 +428,18 @@ static size_t ap_smem_bytes() {
   return sizeof(cx<T>) * (size_t)(padded<T, N>() * (1 + ap_waves<T, N>() * tpw<N>()));
 }
 
-template <typename T, int N, bool CPLX>
+// SIG: the real input is the ILT parameter theta and the row pass transforms
+// the mask sigmoid(clip(beta theta, +-60)) (applied as the unit's values are
+// taken from the prefetch registers).
+template <typename T, int N, bool CPLX, bool SIG = false>
 __global__ __launch_bounds__((64 * ap_waves<T, N>()),
                              (ap_blocks_per_cu<T, N>() * ap_waves<T, N>() / 4 < 1
                                   ? 1
                                   : ap_blocks_per_cu<T, N>() * ap_waves<T, N>() /
                                         4)) void k_analysis_rows_p(
     const T* __restrict__ in, long long img_stride, int nu0, int Wb,
-    const cx<T>* __restrict__ twg, cx<T>* __restrict__ RT) {
+    const cx<T>* __restrict__ twg, cx<T>* __restrict__ RT, T beta) {
+  static_assert(!(SIG && CPLX), "the sigmoid applies to a real parameter");
   constexpr int TPW = tpw<N>();
   constexpr int NW = ap_waves<T, N>();
   constexpr int PL = padded<T, N>();
@@ -496,7 +500,9 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
 #pragma unroll
     for (int i = 0; i < TPL; ++i)
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = cx<T>{ra[i][r], rb[i][r]};
+      for (int r = 0; r < R; ++r)
+        v[i][r] = SIG ? cx<T>{sigmoid_clipped(beta * ra[i][r]), sigmoid_clipped(beta * rb[i][r])}
+                      : cx<T>{ra[i][r], rb[i][r]};
     if (u + stride < NU) fetch(u + stride);
     const int ntr = min(TPW, NTR - u * TPW);
 #pragma unroll
@@ -855,7 +861,7 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
       const size_t smp = ap_smem_bytes<T, N>();
       if ((e = set_smem(k, smp)) != hipSuccess) return e;
       hipLaunchKernelGGL(k, dim3(gx, nimg), dim3(64 * nw), smp, st, (const T*)in, img_stride,
-                         nu0, Wb, twg, RT);
+                         nu0, Wb, twg, RT, (T)0);
       done = true;
     }
   }
@@ -878,6 +884,37 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
   hipLaunchKernelGGL(k2, dim3(blocks_for(Wb, cpb<T, N>()), nimg), dim3(64 * col_waves<T, N>()),
                      smc, st, (const cx<T>*)RT, nu0, Wb, s, twg, X);
   return hipGetLastError();
+}
+
+template <typename T, int N>
+static hipError_t analysis_sigmoid_n(const T* theta, int nimg, int nu0, int Wb, T s, T beta,
+                                     cx<T>* RT, cx<T>* X, hipStream_t st) {
+  if constexpr (Plan<N>::R1 > 1) {
+    const cx<T>* twg = nullptr;
+    hipError_t e = table_for<T>(N, &twg);
+    if (e != hipSuccess) return e;
+    const int units = (N / 2 + tpw<N>() - 1) / tpw<N>();
+    const int nw = ap_waves<T, N>();
+    const int upw = (int)std::min<long long>(
+        kUnitsPerWave, std::max<long long>(1, (long long)units * nimg / (nw * kRowWavesTarget)));
+    const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
+    auto k = k_analysis_rows_p<T, N, false, true>;
+    const size_t smp = ap_smem_bytes<T, N>();
+    if ((e = set_smem(k, smp)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(gx, nimg), dim3(64 * nw), smp, st, theta, (long long)N * N, nu0,
+                       Wb, twg, RT, beta);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    auto k2 = k_analysis_cols<T, N>;
+    const size_t smc = col_smem_bytes<T, N>();
+    if ((e = set_smem(k2, smc)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k2, dim3(blocks_for(Wb, cpb<T, N>()), nimg), dim3(64 * col_waves<T, N>()),
+                       smc, st, (const cx<T>*)RT, nu0, Wb, s, twg, X);
+    return hipGetLastError();
+  } else {
+    (void)theta, (void)nimg, (void)nu0, (void)Wb, (void)s, (void)beta, (void)RT, (void)X,
+        (void)st;
+    return hipErrorNotSupported;
+  }
 }
 
 template <typename T, int N>
@@ -979,6 +1016,25 @@ hipError_t band_synthesis(int N, const cx<T>* Y, int nimg, int nu0, int Wb, int 
   }
 }
 
+template <typename T>
+hipError_t band_analysis_sigmoid(int N, const T* theta, int nimg, int nu0, int Wb, T s, T beta,
+                                 cx<T>* RT, cx<T>* X, hipStream_t st) {
+  if (!band_size_supported(N) || Wb > N) return hipErrorNotSupported;
+  switch (N) {
+#define LB_CASE(n) \
+  case n:          \
+    return analysis_sigmoid_n<T, n>(theta, nimg, nu0, Wb, s, beta, RT, X, st);
+    LB_BAND_SIZES(LB_CASE)
+#undef LB_CASE
+    default:
+      return hipErrorNotSupported;
+  }
+}
+
+template hipError_t band_analysis_sigmoid<float>(int, const float*, int, int, int, float, float,
+                                                 cx<float>*, cx<float>*, hipStream_t);
+template hipError_t band_analysis_sigmoid<double>(int, const double*, int, int, int, double,
+                                                  double, cx<double>*, cx<double>*, hipStream_t);
 template hipError_t band_analysis<float>(int, const void*, int, long long, long long, int, int,
                                          int, int, float, cx<float>*, cx<float>*, hipStream_t);
 template hipError_t band_analysis<double>(int, const void*, int, long long, long long, int, int,

@@ -39,6 +39,14 @@ __device__ __forceinline__ cx<T> mulc(cx<T> a, cx<T> b) {
 template <typename T>
 __device__ __forceinline__ T norm2(cx<T> a) { return a.re * a.re + a.im * a.im; }
 
+// The ILT mask and relaxed-resist sigmoid, 1 / (1 + e^{-v}) with v clipped to +-60
+// (opc/ilt.py:311-378 clips the same way).
+template <typename T>
+__device__ __forceinline__ T sigmoid_clipped(T v) {
+  v = v < (T)-60 ? (T)-60 : (v > (T)60 ? (T)60 : v);
+  return (T)1 / ((T)1 + exp(-v));
+}
+
 // Wait until every vector-memory access of this wave has completed
 // (s_waitcnt vmcnt(0), other counters unconstrained).  Placed after the
 // prologue prefetch of a persistent loop: the compiler's wait analysis merges

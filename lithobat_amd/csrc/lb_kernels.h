@@ -51,6 +51,13 @@ hipError_t band_analysis(int N, const void* in, int cplx, long long img_stride,
                          long long slice_stride, int nsum, int nimg, int nu0, int Wb, T s,
                          cx<T>* RT, cx<T>* X, hipStream_t st);
 
+// band_analysis of the ILT mask sigmoid(clip(beta theta, +-60)) of real theta
+// [img][N][N] (contiguous images), the sigmoid applied as the row pass loads theta;
+// hipErrorNotSupported where the side has no persistent row pass.
+template <typename T>
+hipError_t band_analysis_sigmoid(int N, const T* theta, int nimg, int nu0, int Wb, T s, T beta,
+                                 cx<T>* RT, cx<T>* X, hipStream_t st);
+
 // out[img] = s * Re sum X e^{+...} (cout = 0; X Hermitianised when herm) or the
 // complex sum (cout = 1, out holds complex).  TT: scratch img x N x band_ld(Wb).
 // peak (nullable, real output only): running max per image, must start at 0.
@@ -211,6 +218,10 @@ hipError_t mask_sigmoid(const T* x, long long n, T beta, const double* pbeta, T*
 template <typename T>
 hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, const double* pbeta,
                              T* gx, hipStream_t st);
+// gx = gscale[0] * gm * beta m (1 - m), m = sigmoid(clip(beta x, +-60)) from x (gscale null: 1)
+template <typename T>
+hipError_t mask_sigmoid_grad_theta(const T* gm, const T* x, long long n, T beta,
+                                   const double* pbeta, const T* gscale, T* gx, hipStream_t st);
 template <typename T>
 hipError_t mask_penalties(const T* m, int n_img, int ny, int nx, T wtv, T wd, T* cost, T* total,
                           T* grad, double* part_cost, int* part_zero, hipStream_t st);

@@ -21,12 +21,6 @@ namespace lb {
 
 constexpr int kObjThreads = 256;
 
-template <typename T>
-__device__ __forceinline__ T sigmoid_clipped(T v) {
-  v = v < (T)-60 ? (T)-60 : (v > (T)60 ? (T)60 : v);
-  return (T)1 / ((T)1 + exp(-v));
-}
-
 // Block (blockIdx.x, image blockIdx.y) reduces its share of one image into
 // part_cost / part_err [image][block]; d_f and signed_f are optional.
 template <typename T>
@@ -184,6 +178,36 @@ __global__ void k_mask_sigmoid_grad(const T* __restrict__ gm, const T* __restric
   }
 }
 
+// d theta = g * d m * beta m (1 - m) with m = sigmoid(clip(beta theta, +-60)) recomputed
+// from theta, g the incoming gradient of a scalar loss read from device memory (null: 1):
+// the mask array need not be kept, and the loss gradient scales here instead of in a pass
+// over the intensity derivative (the adjoint between them is linear).
+template <typename T, bool VEC>
+__global__ void k_mask_sigmoid_grad_theta(const T* __restrict__ gm, const T* __restrict__ x,
+                                          long long n, T beta_arg, const double* __restrict__ pbeta,
+                                          const T* __restrict__ gscale, T* __restrict__ gx) {
+  const T beta = pbeta ? (T)*pbeta : beta_arg;
+  const T gs = gscale ? *gscale : (T)1;
+  using V = Vec16<T>;
+  const long long nv = VEC ? n / V::K : 0;
+  const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (long long i = t0; i < nv; i += st) {
+    V g = reinterpret_cast<const V*>(gm)[i];
+    const V a = reinterpret_cast<const V*>(x)[i];
+#pragma unroll
+    for (int k = 0; k < V::K; ++k) {
+      const T m = sigmoid_clipped(beta * a.v[k]);
+      g.v[k] *= beta * m * ((T)1 - m) * gs;
+    }
+    reinterpret_cast<V*>(gx)[i] = g;
+  }
+  for (long long i = nv * V::K + t0; i < n; i += st) {
+    const T m = sigmoid_clipped(beta * x[i]);
+    gx[i] = gm[i] * (beta * m * ((T)1 - m) * gs);
+  }
+}
+
 static unsigned elementwise_blocks(long long n) {
   long long b = (n + 4 * 256 - 1) / (4 * 256);  // a few vectors per thread
   if (b > 16384) b = 16384;
@@ -208,6 +232,17 @@ hipError_t mask_sigmoid_grad(const T* gm, const T* m, long long n, T beta, const
   auto k = aligned16(gm) && aligned16(m) && aligned16(gx) ? k_mask_sigmoid_grad<T, true>
                                                           : k_mask_sigmoid_grad<T, false>;
   hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, gm, m, n, beta, pbeta, gx);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t mask_sigmoid_grad_theta(const T* gm, const T* x, long long n, T beta,
+                                   const double* pbeta, const T* gscale, T* gx, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  auto k = aligned16(gm) && aligned16(x) && aligned16(gx) ? k_mask_sigmoid_grad_theta<T, true>
+                                                          : k_mask_sigmoid_grad_theta<T, false>;
+  hipLaunchKernelGGL(k, dim3(elementwise_blocks(n)), dim3(256), 0, st, gm, x, n, beta, pbeta,
+                     gscale, gx);
   return hipGetLastError();
 }
 
@@ -364,6 +399,8 @@ hipError_t volume_lerp(const T* raw, int n_masks, int n_planes, long long npix, 
   template hipError_t mask_sigmoid<T>(const T*, long long, T, const double*, T*, hipStream_t);  \
   template hipError_t mask_sigmoid_grad<T>(const T*, const T*, long long, T, const double*, T*,   \
                                            hipStream_t);                                        \
+  template hipError_t mask_sigmoid_grad_theta<T>(const T*, const T*, long long, T, const double*, \
+                                                 const T*, T*, hipStream_t);                     \
   template hipError_t mask_penalties<T>(const T*, int, int, int, T, T, T*, T*, T*, double*, int*, \
                                         hipStream_t);                                           \
   template hipError_t adam_step<T>(T*, T*, T*, const T*, long long, T, T, T, T, T, T, T, T,      \

@@ -18,11 +18,12 @@ import torch
 
 from .blur import blur
 from .expose import _to_device
-from .objective import MaskPenalties, MaskSigmoid, RelaxedResist
+from .objective import (MaskPenalties, MaskSigmoid, RelaxedResist, mask_sigmoid_grad_theta,
+                        resist_objective)
 from .plan import ImagingPlan
 
 __all__ = ["AbbeIntensity", "abbe_intensity", "socs_fields", "socs_gradient",
-           "relaxed_resist_loss", "mask_of"]
+           "relaxed_resist_loss", "mask_of", "ILTObjective", "ilt_objective"]
 
 
 class AbbeIntensity(torch.autograd.Function):
@@ -78,6 +79,46 @@ def relaxed_resist_loss(raw: torch.Tensor, target: torch.Tensor, scale: float, l
         return RelaxedResist.apply(raw, target, scale, level, steepness, sign, side)
     return RelaxedResist.apply(blur(raw * scale, sigma_px), target, 1.0, level, steepness, sign,
                                side)
+
+
+class ILTObjective(torch.autograd.Function):
+    """theta [B, N, N] -> the relaxed-resist cost (opc/ilt.py:311-330, without penalties
+    and blurs), summed over images, of the image of the mask sigmoid(clip(beta theta,
+    +-60)) (ilt.py:368-378): the value and gradient of
+    ``relaxed_resist_loss(abbe_intensity(mask_of(theta, beta), plan), target, ...)``
+    in one function.  The sigmoid runs inside the masks' band analysis; the objective
+    pass gives the intensity derivative; the backward runs the recomputing adjoint from
+    the kept band spectra, then the sigmoid derivative recomputed from theta and scaled by
+    the incoming gradient, read on the device.  No mask array is written and no pass
+    scales the intensity derivative (the adjoint between them is linear)."""
+
+    @staticmethod
+    def forward(ctx, theta, plan, target, scale, level, steepness, beta, sign):
+        spec = plan.mask_spectrum_sigmoid(theta, beta)
+        raw, _ = plan.forward_spectrum(spec, with_peak=False)
+        t = target if target.dim() < 3 else target.reshape(target.shape[0], 1,
+                                                           *target.shape[-2:])
+        r = resist_objective(raw, t, scale, level, steepness, sign, grad=True)
+        ctx.plan, ctx.beta, ctx.dtype = plan, float(beta), theta.dtype
+        ctx.squeeze = theta.dim() == 2
+        ctx.save_for_backward(spec, r["d_intensity"], theta)
+        return r["total"]
+
+    @staticmethod
+    def backward(ctx, g):
+        spec, d, theta = ctx.saved_tensors
+        dm = ctx.plan.backward_spectrum(spec, d)
+        th = theta[None] if ctx.squeeze else theta
+        dth = mask_sigmoid_grad_theta(dm, th.to(dm.dtype), ctx.beta, g)
+        return ((dth[0] if ctx.squeeze else dth).to(ctx.dtype), None, None, None, None, None,
+                None, None)
+
+
+def ilt_objective(theta: torch.Tensor, plan: ImagingPlan, target: torch.Tensor, scale: float,
+                  level: float, steepness: float, beta: float = 8.0,
+                  sign: float = 1.0) -> torch.Tensor:
+    """The ILT objective of theta in one autograd function (``ILTObjective``)."""
+    return ILTObjective.apply(theta, plan, target, scale, level, steepness, beta, sign)
 
 
 def mask_penalties(mask: torch.Tensor, weight_tv: float = 0.0,

@@ -132,6 +132,28 @@ def mask_sigmoid_grad(grad_mask: torch.Tensor, mask: torch.Tensor, beta,
     return res
 
 
+def mask_sigmoid_grad_theta(grad_mask: torch.Tensor, theta: torch.Tensor, beta: float,
+                            g: torch.Tensor | None = None,
+                            out: torch.Tensor | None = None) -> torch.Tensor:
+    """d theta = g * grad_mask * beta m (1 - m), m = sigmoid(clip(beta theta, +-60))
+    recomputed from theta; ``g`` a 0-d device tensor of theta's dtype read when the pass
+    runs (None: 1)."""
+    _on_device(theta, "theta")
+    th = theta.contiguous()
+    gm = grad_mask.to(th.dtype).contiguous()
+    res = torch.empty_like(th) if out is None else out
+    gp = None
+    if g is not None:
+        gt = g.to(th.dtype).reshape(()).contiguous()
+        _on_device(gt, "g")
+        gp = gt.data_ptr()
+    st = torch.cuda.current_stream(th.device).cuda_stream
+    _lib.check(_lib.load().lb_mask_sigmoid_grad_theta(gm.data_ptr(), th.data_ptr(), th.numel(),
+                                                      float(beta), gp, res.data_ptr(),
+                                                      _prec(th), st))
+    return res
+
+
 def ilt_schedule(schedule: torch.Tensor, growth: float, beta1: float = 0.9,
                  beta2: float = 0.999) -> None:
     """Advance ``schedule`` = [beta, it, 1 - beta1^it, 1 - beta2^it, beta_next] (float64,

@@ -279,6 +279,30 @@ class ImagingPlan:
                                                   st))
         return spec
 
+    def mask_spectrum_sigmoid(self, theta: torch.Tensor, beta: float) -> torch.Tensor:
+        """``mask_spectrum`` of the ILT mask sigmoid(clip(beta theta, +-60)) of real theta
+        [B, N, N] (opc/ilt.py:368-378): the sigmoid is applied inside the band analysis, no
+        mask array is written.  Grid sides without the fused analysis apply the sigmoid
+        first (``lb_mask_sigmoid``) and analyse the mask."""
+        from .objective import mask_sigmoid
+        th, cplx = self._check_masks(theta)
+        if cplx:
+            raise TypeError("theta must be real")
+        B = th.shape[0]
+        wb = self._band_side()
+        spec = torch.empty((B, wb, wb), dtype=self.complex_dtype, device=th.device)
+        with self._lock:
+            ws = self._workspace(self.workspace_bytes(B, False), th.device)
+            st = torch.cuda.current_stream(th.device).cuda_stream
+            rc = self._lib.lb_mask_spectrum_sigmoid(self._h, th.data_ptr(), B, float(beta),
+                                                    spec.data_ptr(), ws.data_ptr(), ws.numel(),
+                                                    st)
+            if rc == _lib.LB_OK:
+                return spec
+            if rc != _lib.LB_ERR_UNSUPPORTED:
+                _lib.check(rc)
+        return self.mask_spectrum(mask_sigmoid(th, beta))
+
     def _band_side(self) -> int:
         if self._band is None:
             self._band = int(self.info["band"])
