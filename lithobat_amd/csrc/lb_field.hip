@@ -502,6 +502,26 @@ __device__ __forceinline__ int next_item(int* queue, int item, int* s_next) {
   return *s_next;
 }
 
+// Column rounds taken in pairs: where a round's stage-2 tasks (CPR columns x
+// T2) do not fill whole passes of the block but two rounds' do (M = 96 at 256
+// threads: 384 tasks, 1.5 passes; two rounds 3), both rounds' stage 1 write
+// the two halves of the round buffer and their stage 2 runs as one group of
+// 2 CPR columns, so no pass runs on half the waves while the others wait at
+// the next barrier.  Group g covers rounds 2g and 2g + 1 (the last alone when
+// the round count is odd).
+template <class F, int NT>
+struct RoundPairs {
+  static constexpr int G = (F::ROUNDS + 1) / 2;
+  static constexpr int cols(int g) { return (2 * g + 1 < F::ROUNDS ? 2 : 1) * F::CPR; }
+  static constexpr int passes(int g) { return (cols(g) * F::T2 + NT - 1) / NT; }
+  static constexpr int off(int g) {
+    int o = 0;
+    for (int i = 0; i < g; ++i) o += passes(i) * F::R2;
+    return o;
+  }
+  static constexpr int NACC = off(G);
+};
+
 template <typename T, int M, int NT, int CSPLIT, typename Src>
 __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restrict__ twg) {
   using F = FieldShape<T, M, NT, CSPLIT>;
@@ -530,10 +550,13 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
     const int rl1 = stage1_row<M>(tid / F::T1), j1 = tid - (tid / F::T1) * F::T1;
     if (Pre::ON && s0 < s1)
       pre.fetch(Xm, Wb, wrapN, H + (long long)s0 * W * W, W, box[s0], rl1, j1);
+    using RP = RoundPairs<F, NT>;
+    constexpr bool PAIR = Pre::DB && F::NT2 % NT != 0 && (2 * F::NT2) % NT == 0;
+    constexpr int NACC = PAIR ? RP::NACC : F::NACC;
     for (int cs = 0; cs < CSPLIT; ++cs) {
-      T acc[F::NACC];
+      T acc[NACC];
 #pragma unroll
-      for (int i = 0; i < F::NACC; ++i) acc[i] = 0;
+      for (int i = 0; i < NACC; ++i) acc[i] = 0;
       for (int j = s0; j < s1; ++j) {
         const T w = wsys[j];
         if constexpr (Pre::ON) {
@@ -545,6 +568,49 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
           row_pass<T, M, NT, CSPLIT>(Xm, Wb, wrapN, H + (long long)j * W * W, W, box[j], tw, Cb,
                                      Tb, cs, tid, twr);
         }
+        if constexpr (PAIR) {
+#pragma unroll
+          for (int g = 0; g < RP::G; ++g) {
+            const int nr = 2 * g + 1 < F::ROUNDS ? 2 : 1;
+            // stage 1 of the group's rounds into the halves of the round buffer
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              if (h < nr) {
+                cx<T> v[F::R1];
+                const int xl = (2 * g + h) * F::CPR + c1;
+#pragma unroll
+                for (int r = 0; r < F::R1; ++r) {
+                  const int a = jc1 + r * F::T1;
+                  v[r] = (r < F::NRW && a < W) ? Tb[a * F::MC + xl] : cx<T>{0, 0};
+                }
+                dft<F::R1, +1>(v);
+#pragma unroll
+                for (int r = 0; r < F::R1; ++r)
+                  Cb[(h * F::CPR + c1) * F::LDC + jc1 * F::R1 + r] = v[r];
+              }
+            }
+            __syncthreads();
+            const int ncol = nr * F::CPR;
+#pragma unroll
+            for (int q = 0; q < RP::passes(g); ++q) {
+              const int t = tid + q * NT;
+              if (t < ncol * F::T2) {
+                const int c2 = t % ncol, j2 = t / ncol;
+                cx<T> v[F::R2];
+#pragma unroll
+                for (int r = 0; r < F::R2; ++r) {
+                  v[r] = Cb[c2 * F::LDC + j2 + r * F::R1];
+                  if (r > 0) v[r] = v[r] * tw[r * j2];
+                }
+                dft<F::R2, +1>(v);
+#pragma unroll
+                for (int r = 0; r < F::R2; ++r)
+                  acc[RP::off(g) + q * F::R2 + r] += w * norm2(v[r]);
+              }
+            }
+            __syncthreads();  // the next group's stage 1 (or row pass) rewrites Cb
+          }
+        } else {
         // Column rounds.  With Pre::DB round rd uses half (rd & 1) of Cb: a
         // round's stage 1 may overwrite the other half while slower threads
         // still read this one, and every thread has left round rd - 2 when it
@@ -585,21 +651,40 @@ __global__ __launch_bounds__(NT) void k_field_fwd(Src src, const cx<T>* __restri
           if (!Pre::DB) __syncthreads();
         }
         if (Pre::DB) __syncthreads();  // the next row pass writes over both halves
+        }
       }
       T* out = iv.part;
+      if constexpr (PAIR) {
 #pragma unroll
-      for (int rd = 0; rd < F::ROUNDS; ++rd)
+        for (int g = 0; g < RP::G; ++g) {
+          const int ncol = (2 * g + 1 < F::ROUNDS ? 2 : 1) * F::CPR;
 #pragma unroll
-        for (int q = 0; q < F::Q2; ++q) {
-          const int t = tid + q * NT;
-          if (F::NT2 % NT == 0 || t < F::NT2) {
-            const int c2 = t % F::CPR, j2 = t / F::CPR;
-            const int x = cs * F::MC + rd * F::CPR + c2;
+          for (int q = 0; q < RP::passes(g); ++q) {
+            const int t = tid + q * NT;
+            if (t < ncol * F::T2) {
+              const int c2 = t % ncol, j2 = t / ncol;
+              const int x = cs * F::MC + 2 * g * F::CPR + c2;
 #pragma unroll
-            for (int r = 0; r < F::R2; ++r)
-              out[(j2 + r * F::R1) * M + x] = acc[(rd * F::Q2 + q) * F::R2 + r];
+              for (int r = 0; r < F::R2; ++r)
+                out[(j2 + r * F::R1) * M + x] = acc[RP::off(g) + q * F::R2 + r];
+            }
           }
         }
+      } else {
+#pragma unroll
+        for (int rd = 0; rd < F::ROUNDS; ++rd)
+#pragma unroll
+          for (int q = 0; q < F::Q2; ++q) {
+            const int t = tid + q * NT;
+            if (F::NT2 % NT == 0 || t < F::NT2) {
+              const int c2 = t % F::CPR, j2 = t / F::CPR;
+              const int x = cs * F::MC + rd * F::CPR + c2;
+#pragma unroll
+              for (int r = 0; r < F::R2; ++r)
+                out[(j2 + r * F::R1) * M + x] = acc[(rd * F::Q2 + q) * F::R2 + r];
+            }
+          }
+      }
     }
   }
 }
