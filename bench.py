@@ -394,7 +394,7 @@ def socs_leg(args, dev, sync):
             "n_source": k24.n_source, "n_band": k24.n_band,
             "energy_kept": float(k24.energy_kept), "rel_err_vs_abbe": err,
             "config": "256 masks 512^2 @ 8 nm, NA 1.35 annular 0.6-0.9 (142 points), "
-                      "K = 24 of the FP64 Gram/Jacobi decomposition, complex64 imaging"}
+                      "K = 24 of the FP64 Gram / Householder + divide-and-conquer QL decomposition, complex64 imaging"}
 
 
 def volume_leg(args, masks, sync):
