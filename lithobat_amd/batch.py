@@ -214,15 +214,41 @@ class ILTStep:
                                    dtype=torch.uint8, device=dev) for u in self.units]
         batch.workspace(False, dev)
         batch.workspace(True, dev)
+        # the units' elementwise passes run concurrently, one stream each (default
+        # priority), forked from and joined to the caller's stream
+        self.side = [torch.cuda.Stream(device=dev) for _ in self.units] if len(self.units) > 1 \
+            else []
+
+    def _each(self, fn) -> None:
+        if not self.side:
+            for u in range(len(self.units)):
+                fn(u)
+            return
+        cur = torch.cuda.current_stream()
+        for u, s in enumerate(self.side):
+            s.wait_stream(cur)
+            with torch.cuda.stream(s):
+                fn(u)
+        for s in self.side:
+            cur.wait_stream(s)
+
+    def _sigmoid(self, u: int) -> None:
+        th, m, *_ = self.units[u]
+        objective.mask_sigmoid(th, self.beta, out=m)
+
+    def _objective(self, u: int) -> None:
+        _, _, raw, t, d, _, _, s = self.units[u]
+        objective.resist_objective_into(raw, t, s, self.level, self.steep, 1.0, None,
+                                        self.costs[u], d, None, None, self.obj_ws[u])
+
+    def _sigmoid_grad(self, u: int) -> None:
+        _, m, _, _, _, dm, g, _ = self.units[u]
+        objective.mask_sigmoid_grad(dm, m, self.beta, out=g)
 
     def run(self) -> None:
-        for th, m, *_ in self.units:
-            objective.mask_sigmoid(th, self.beta, out=m)
+        self._each(self._sigmoid)
         self.batch.forward_into(self.mask, self.raw)
-        for u, (_, _, raw, t, d, _, _, s) in enumerate(self.units):
-            objective.resist_objective_into(raw, t, s, self.level, self.steep, 1.0, None,
-                                            self.costs[u], d, None, None, self.obj_ws[u])
+        self._each(self._objective)
         torch.sum(self.costs, 0, out=self.loss)
         self.batch.backward_into(self.mask, self.d_raw, self.d_mask, after_forward=True)
-        for _, m, _, _, _, dm, g, _ in self.units:
-            objective.mask_sigmoid_grad(dm, m, self.beta, out=g)
+        self._each(self._sigmoid_grad)

@@ -681,11 +681,49 @@ struct lb_batch_s {
   ItemDesc* items_f = nullptr;
   ItemDesc* items_b = nullptr;
   int n_items_f = 0, n_items_b = 0;
+  // the groups' band passes run concurrently between the coherent-system
+  // launches: one side stream each (default priority, non-blocking), forked
+  // from and joined to the caller's stream with events (graph capture records
+  // them as parallel branches)
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork_ev = nullptr;
   ~lb_batch_s() {
     if (items_f) (void)hipFree(items_f);
     if (items_b) (void)hipFree(items_b);
+    for (hipStream_t s : side) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
   }
 };
+
+constexpr size_t kBatchSideStreams = 4;  // groups beyond this share streams round-robin
+
+// body(group, stream) for every group of the batch: on the caller's stream
+// when there is one group (or no side streams), else each group on a side
+// stream between a fork from and a join to the caller's stream.  The join
+// also runs when a group fails, so no forked work is left outside it.
+template <class F>
+static lb_status for_groups(lb_batch_s& b, hipStream_t st, F&& body) {
+  const size_t G = b.groups.size();
+  if (G < 2 || b.side.empty()) {
+    for (BatchGroup& g : b.groups) {
+      lb_status rc = body(g, st);
+      if (rc != LB_OK) return rc;
+    }
+    return LB_OK;
+  }
+  const size_t S = std::min(G, b.side.size());
+  LB_HIP(hipEventRecord(b.fork_ev, st));
+  for (size_t k = 0; k < S; ++k) LB_HIP(hipStreamWaitEvent(b.side[k], b.fork_ev, 0));
+  lb_status rc = LB_OK;
+  for (size_t g = 0; g < G && rc == LB_OK; ++g) rc = body(b.groups[g], b.side[g % S]);
+  for (size_t k = 0; k < S; ++k) {
+    LB_HIP(hipEventRecord(b.joins[k], b.side[k]));
+    LB_HIP(hipStreamWaitEvent(st, b.joins[k], 0));
+  }
+  return rc;
+}
 
 extern "C" {
 
@@ -1543,6 +1581,22 @@ lb_status lb_batch_create(const lb_plan* plans, int32_t n_plans, lb_batch* out) 
     return fail(e == hipErrorOutOfMemory ? LB_ERR_OOM : LB_ERR_HIP,
                 std::string("batch tables: ") + hipGetErrorString(e));
   }
+  if (b->groups.size() > 1) {
+    const size_t S = std::min(b->groups.size(), kBatchSideStreams);
+    e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
+    for (size_t k = 0; k < S && e == hipSuccess; ++k) {
+      hipStream_t sk = nullptr;
+      hipEvent_t ek = nullptr;
+      e = hipStreamCreateWithFlags(&sk, hipStreamNonBlocking);
+      if (e == hipSuccess) b->side.push_back(sk);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&ek, hipEventDisableTiming);
+      if (e == hipSuccess) b->joins.push_back(ek);
+    }
+    if (e != hipSuccess) {
+      delete b;
+      return fail(LB_ERR_HIP, std::string("batch streams: ") + hipGetErrorString(e));
+    }
+  }
   *out = b;
   return LB_OK;
 }
@@ -1612,8 +1666,8 @@ static hipError_t scatter(void* const* dst, const char* src, const std::vector<i
 
 template <typename T>
 static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* const* images,
-                                 void* const* peaks, char* ws, hipStream_t st) {
-  for (const BatchGroup& g : b.groups) {
+                                 void* const* peaks, char* ws, hipStream_t st0) {
+  lb_status rc0 = for_groups(b, st0, [&](BatchGroup& g, hipStream_t st) -> lb_status {
     const Plan& p = b.plans[g.members[0]]->p;
     const size_t mb = sizeof(T) * (size_t)p.N * p.N;
     const void* m0 = masks[g.members[0]];
@@ -1621,16 +1675,17 @@ static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* 
       LB_HIP(gather(ws + g.stage_f, masks, g.members, mb, st));
       m0 = ws + g.stage_f;
     }
-    lb_status rc = fwd_pre<T>(b.plans[g.members[0]]->p, m0, 0, (int)g.members.size(),
-                              ws + g.base_f, g.wf, st);
-    if (rc != LB_OK) return rc;
-  }
+    return fwd_pre<T>(b.plans[g.members[0]]->p, m0, 0, (int)g.members.size(), ws + g.base_f,
+                      g.wf, st);
+  });
+  if (rc0 != LB_OK) return rc0;
+  hipStream_t st = st0;
   FieldConfig fc = b.fwd;
   fc.grid = batch_grid(b, fc, b.n_items_f, false);
   int* q = (int*)(ws + b.groups[0].base_f + b.groups[0].wf.queue);
   LB_HIP(fill<float>((float*)q, 0.0f, 1, st));
   LB_HIP(field_multi(b.precision, fc, b.items_f, b.n_items_f, b.Wmax, ws, q, false, st));
-  for (const BatchGroup& g : b.groups) {
+  return for_groups(b, st0, [&](BatchGroup& g, hipStream_t sg) -> lb_status {
     Plan& p = b.plans[g.members[0]]->p;
     const int cnt = (int)g.members.size();
     const size_t nn = (size_t)p.N * p.N;
@@ -1641,12 +1696,12 @@ static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* 
     const bool pk_in_place = !peaks || consecutive(peaks, g.members, pb);
     void* out = img_in_place ? images[g.members[0]] : (void*)stage_img;
     void* pk = peaks ? (pk_in_place ? peaks[g.members[0]] : (void*)stage_pk) : nullptr;
-    lb_status rc = fwd_post<T>(p, out, pk, ws + g.base_f, g.wf, st);
+    lb_status rc = fwd_post<T>(p, out, pk, ws + g.base_f, g.wf, sg);
     if (rc != LB_OK) return rc;
-    if (!img_in_place) LB_HIP(scatter(images, stage_img, g.members, ib, st));
-    if (peaks && !pk_in_place) LB_HIP(scatter(peaks, stage_pk, g.members, pb, st));
-  }
-  return LB_OK;
+    if (!img_in_place) LB_HIP(scatter(images, stage_img, g.members, ib, sg));
+    if (peaks && !pk_in_place) LB_HIP(scatter(peaks, stage_pk, g.members, pb, sg));
+    return LB_OK;
+  });
 }
 
 // fws (nullable): the workspace of the lb_batch_forward call that imaged these
@@ -1654,8 +1709,8 @@ static lb_status batch_forward_t(lb_batch_s& b, const void* const* masks, void* 
 template <typename T>
 static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
                                   const void* const* dimg, void* const* dmask, char* ws,
-                                  hipStream_t st, const char* fws = nullptr) {
-  for (const BatchGroup& g : b.groups) {
+                                  hipStream_t st0, const char* fws = nullptr) {
+  lb_status rc0 = for_groups(b, st0, [&](BatchGroup& g, hipStream_t st) -> lb_status {
     const Plan& p = b.plans[g.members[0]]->p;
     const int cnt = (int)g.members.size();
     const size_t nn = (size_t)p.N * p.N;
@@ -1676,16 +1731,17 @@ static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
       g0 = stage_g;
     }
     const T* gM = nullptr;
-    lb_status rc = bwd_pre<T>(b.plans[g.members[0]]->p, m0, cnt, g0, ws + g.base_b, g.wb, &gM,
-                              st, fws != nullptr);
-    if (rc != LB_OK) return rc;
-  }
+    return bwd_pre<T>(b.plans[g.members[0]]->p, m0, cnt, g0, ws + g.base_b, g.wb, &gM, st,
+                      fws != nullptr);
+  });
+  if (rc0 != LB_OK) return rc0;
+  hipStream_t st = st0;
   FieldConfig fc = b.adj;
   fc.grid = batch_grid(b, fc, b.n_items_b, true);
   int* q = (int*)(ws + b.groups[0].base_b + b.groups[0].wb.queue);
   LB_HIP(fill<float>((float*)q, 0.0f, 1, st));
   LB_HIP(field_multi(b.precision, fc, b.items_b, b.n_items_b, b.Wmax, ws, q, true, st));
-  for (const BatchGroup& g : b.groups) {
+  return for_groups(b, st0, [&](BatchGroup& g, hipStream_t sg) -> lb_status {
     Plan& p = b.plans[g.members[0]]->p;
     const int cnt = (int)g.members.size();
     const size_t nn = (size_t)p.N * p.N;
@@ -1694,11 +1750,11 @@ static lb_status batch_backward_t(lb_batch_s& b, const void* const* masks,
                     (sizeof(T) * cnt * p.n_planes * nn + 255) / 256 * 256;
     const bool in_place = consecutive(dmask, g.members, mb);
     void* out = in_place ? dmask[g.members[0]] : (void*)stage_d;
-    lb_status rc = bwd_post<T>(p, cnt, out, ws + g.base_b, g.wb, st);
+    lb_status rc = bwd_post<T>(p, cnt, out, ws + g.base_b, g.wb, sg);
     if (rc != LB_OK) return rc;
-    if (!in_place) LB_HIP(scatter(dmask, stage_d, g.members, mb, st));
-  }
-  return LB_OK;
+    if (!in_place) LB_HIP(scatter(dmask, stage_d, g.members, mb, sg));
+    return LB_OK;
+  });
 }
 
 extern "C" {
