@@ -520,14 +520,26 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
     const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
     cx<T>* Y = (cx<T>*)(ws + w.Y);
     T* IM = (T*)(ws + w.IM);
-    if (w.spread)
-      LB_HIP(spread_sum<T>(partial, w.n_img, 1, p.spp, total, w.runs, w.n_chunks,
-                           (long long)p.M * p.M, IM, (T*)nullptr, st));
-    else
-      LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.M * p.M, IM, (T*)nullptr,
-                            (T*)(ws + w.seg), st));
-    LB_HIP(band_analysis<T>(p.M, IM, 0, (long long)p.M * p.M, 0, 1, w.n_img, -p.QI, p.WbI, invM2,
-                            (cx<T>*)(ws + w.RTc), Y, st));
+    // the partial sums and the coarse band spectrum in one pass (one block per
+    // image) when the images are enough to occupy the device; fewer images
+    // spread their partial sums over more blocks in the separate passes
+    const hipError_t fused =
+        2 * w.n_img >= p.cus
+            ? coarse_band<T>(p.M, partial, w.spread, w.n_chunks, p.spp, total, w.runs,
+                             w.n_chunks, w.n_img, p.QI, invM2, Y, st)
+            : hipErrorNotSupported;
+    if (fused != hipErrorNotSupported) {
+      LB_HIP(fused);
+    } else {
+      if (w.spread)
+        LB_HIP(spread_sum<T>(partial, w.n_img, 1, p.spp, total, w.runs, w.n_chunks,
+                             (long long)p.M * p.M, IM, (T*)nullptr, st));
+      else
+        LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.M * p.M, IM,
+                              (T*)nullptr, (T*)(ws + w.seg), st));
+      LB_HIP(band_analysis<T>(p.M, IM, 0, (long long)p.M * p.M, 0, 1, w.n_img, -p.QI, p.WbI,
+                              invM2, (cx<T>*)(ws + w.RTc), Y, st));
+    }
     LB_HIP(band_synthesis<T>(p.N, Y, w.n_img, -p.QI, p.WbI, 1, (T)1, 0, (cx<T>*)(ws + w.TT),
                              (T*)images, pk, st));
   }

@@ -590,7 +590,12 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(cons
 // gathers only the in-band inputs from Y (Hermitianised on load when herm).
 // The CPB columns of a block are written out cooperatively as row runs,
 // TT[img][y][v].
-template <typename T, int N, bool SP>
+//
+// HALF (real output of a Hermitian band symmetric about 0, Wb = 2Q + 1): each
+// row of the output is real, so its spectrum at -nu is the conjugate of that
+// at nu and only the columns nu = 0..Q are transformed and stored, TT[img][y]
+// holding Q + 1 values (the row pass takes the others as conjugates).
+template <typename T, int N, bool SP, bool HALF = false>
 __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const cx<T>* __restrict__ Y, int nu0, int Wb, int herm,
                              const cx<T>* __restrict__ twg, cx<T>* __restrict__ TT) {
   constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = col_ld<T, N>();
@@ -601,7 +606,9 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const c
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int img = blockIdx.y;
   const int vb = blockIdx.x * CPB;
-  const int ncol = max(0, min(CPB, Wb - vb));
+  const int Qh = (Wb - 1) / 2;
+  const int nstored = HALF ? Qh + 1 : Wb;  // stored column v is band column v + Qh when HALF
+  const int ncol = max(0, min(CPB, nstored - vb));
   load_roots_padded(tw, twg, N, tid, NT);
   __syncthreads();
   const int c0 = wave * TPW;
@@ -611,7 +618,7 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const c
   auto load = [&](int tr, int b) -> cx<T> {
     const int u = band_pos(b, N, nu0, Wb);
     if (u < 0) return cx<T>{0, 0};
-    const int v = vb + c0 + tr;
+    const int v = vb + c0 + tr + (HALF ? Qh : 0);
     cx<T> val = src[(long long)u * Wb + v];
     if (herm) {
       const cx<T> m = src[(long long)mirror_of(u, nu0, Wb) * Wb + mirror_of(v, nu0, Wb)];
@@ -622,7 +629,7 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_synth_cols(const c
   auto store = [&](int tr, int y, cx<T> val) { wb[tr * LD + pidx<T>(y)] = val; };
   wave_fft<T, +1, N, TPW, LD, SP>(wb, tw, lane, ntr, load, store);
   __syncthreads();
-  const int ld = band_ld(Wb);
+  const int ld = band_ld(nstored);
   cx<T>* out = TT + (long long)img * N * ld;
   for (int t = tid; t < N * CPB; t += NT) {
     const int y = t / CPB, c = t - y * CPB;
@@ -692,17 +699,23 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
 // transform units (TPW transforms each, `stride` apart): the next unit's band
 // values are loaded while the current one is transformed, the roots table is
 // loaded once per block, and the peak is one atomic per wavefront.
-template <typename T, int N, bool COUT>
+//
+// HALF: TT holds the columns nu = 0..Q of a band symmetric about 0 (see
+// k_synth_cols); the input at nu = j - S < 0 is the conjugate of the stored
+// column S - j.
+template <typename T, int N, bool COUT, bool HALF = false>
 __global__ __launch_bounds__((64 * row_waves<T, N>()),
                              (row_waves_per_simd<T, N>())) void k_synth_rows_sp(
     const cx<T>* __restrict__ TT, int nu0, int Wb, T s, const cx<T>* __restrict__ twg,
     T* __restrict__ out, T* __restrict__ peak) {
+  static_assert(!(HALF && COUT), "the half band is for real output");
   LB_BAND_PROLOGUE(T, N)
   constexpr int R = Plan<N>::R0, S = N / R;
   constexpr int NTR = COUT ? N : N / 2;
   constexpr int NU = (NTR + TPW - 1) / TPW;
   constexpr int TASKS = TPW * S, TPL = (TASKS + 63) / 64;
-  const int ld = band_ld(Wb);
+  const int Qh = (Wb - 1) / 2;
+  const int ld = band_ld(HALF ? Qh + 1 : Wb);
   const cx<T>* src = TT + (long long)img * N * ld;
   // per task slot: transform within the unit, and the band positions of its
   // two inputs (nu = j and nu = j - S).  The loads below always use an
@@ -717,7 +730,11 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
     const bool ok = t < TASKS;
     trs[i] = ok ? tr : TPW;
     jj[i] = j;
-    const int lo = j - nu0, hi = j - S - nu0;
+    int lo = j - nu0, hi = j - S - nu0;
+    if constexpr (HALF) {
+      lo = j <= Qh ? j : -1;          // nu = j >= 0: stored column j
+      hi = S - j <= Qh ? S - j : -1;  // nu = j - S < 0: conjugate of stored column S - j
+    }
     vlo[i] = ok && lo >= 0 && lo < Wb;
     vhi[i] = ok && hi >= 0 && hi < Wb;
     plo[i] = vlo[i] ? lo : 0;
@@ -745,7 +762,9 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
     for (int i = 0; i < TPL; ++i) {
       // real output: row 2p + i row 2p+1 (the band is Hermitian per row pair)
       const cx<T> l = COUT ? a0[i] : cx<T>{a0[i].re - a1[i].im, a0[i].im + a1[i].re};
-      const cx<T> h = COUT ? b0[i] : cx<T>{b0[i].re - b1[i].im, b0[i].im + b1[i].re};
+      const cx<T> h = COUT   ? b0[i]
+                      : HALF ? cx<T>{b0[i].re + b1[i].im, b1[i].re - b0[i].im}  // conj b0 + i conj b1
+                             : cx<T>{b0[i].re - b1[i].im, b0[i].im + b1[i].re};
       lo[i] = vlo[i] ? scale(l, s) : cx<T>{0, 0};
       hi[i] = vhi[i] ? scale(h, s) : cx<T>{0, 0};
     }
@@ -802,6 +821,142 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
   if (!COUT && peak) {
     for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
     if (lane == 0) atomic_max_nonneg(peak + img, lmax);
+  }
+}
+
+// ------------------------------------------------------- coarse image band
+// One block per image: the sum of the image's coarse partials (the spread or
+// segment layout of the coherent-system pass) and its band spectrum
+//   Y[u][v] = s * sum_{y,x} I[y][x] e^{-2 pi i ((u - Q) y + (v - Q) x) / M},
+// u, v < 2Q + 1, in one pass, the summed image never leaving the registers of
+// the row transforms.  Rows are packed in pairs (even + i odd); since every
+// row is real, its spectrum at -nu is the conjugate of that at nu, so only
+// nu = 0..Q is extracted (RT, M x (Q + 1) in LDS) and column-transformed, and
+// the columns v < Q are written as conjugate mirrors of v > Q.  The partials
+// are added in the order the separate sum pass adds them, so the image is the
+// same to the bit.
+constexpr int kCoarseWaves = 6;
+constexpr int kCoarseMaxParts = 64;
+
+__host__ __device__ constexpr int coarse_ldr(int Q) { return (Q + 1) % 2 == 0 ? Q + 2 : Q + 1; }
+
+template <typename T, int M>
+__global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_band(
+    const T* __restrict__ parts, int spread, int cnt, int spp, long long total, int blocks,
+    int segs, int Q, T s, const cx<T>* __restrict__ twg, cx<T>* __restrict__ Y) {
+  constexpr int TPW = tpw<M>(), PL = padded<T, M>(), NT = 64 * kCoarseWaves;
+  constexpr long long LEN = (long long)M * M;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  cx<T>* wb = tw + PL + wave * TPW * PL;
+  const int LDR = coarse_ldr(Q);
+  cx<T>* rt = tw + PL + kCoarseWaves * TPW * PL;  // M x LDR
+  int* plist = reinterpret_cast<int*>(rt + M * LDR);
+  const int g = blockIdx.x;
+  if (tid == 0) {
+    int n = 0;
+    if (spread) {
+      // the runs that overlap image g, each holding one segment of it
+      const long long q0 = (long long)g * spp, q1 = min(total, q0 + spp);
+      const int bf = spread_run_of(q0, total, blocks), bl = spread_run_of(q1 - 1, total, blocks);
+      for (int b = bf; b <= bl; ++b) {
+        const long long lo = (long long)b * total / blocks, hi = (long long)(b + 1) * total / blocks;
+        if (lo >= hi) continue;
+        const long long k = g - lo / spp;
+        if (k >= 0 && k < segs && n < kCoarseMaxParts) plist[n++] = (int)(k * blocks + b);
+      }
+    } else {
+      for (int c = 0; c < cnt && n < kCoarseMaxParts; ++c) plist[n++] = g * cnt + c;
+    }
+    plist[kCoarseMaxParts] = n;
+  }
+  load_roots_padded(tw, twg, M, tid, NT);
+  __syncthreads();
+  const int np = plist[kCoarseMaxParts];
+  // row pairs: forward transforms of (even + i odd), spectra at nu = 0..Q.
+  // A unit's 2 TPW rows are contiguous in every partial: the wave sums them
+  // with 16-byte loads (one round trip per partial), packs the sums into its
+  // LDS buffer and transforms them there.
+  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+  constexpr int VE = 16 / (int)sizeof(T);  // values per 16-byte load
+  constexpr int NPAIR = M / 2, NUR = (NPAIR + TPW - 1) / TPW;
+  constexpr int VPL = (2 * TPW * M / VE + 63) / 64;  // loads per lane per partial
+  static_assert(M % VE == 0, "rows are whole 16-byte runs");
+  for (int u = wave; u < NUR; u += kCoarseWaves) {
+    const int p0 = u * TPW, ntr = min(TPW, NPAIR - p0);
+    const int nv = 2 * ntr * M / VE;
+    V acc[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) acc[k] = V{};
+    for (int i = 0; i < np; ++i) {
+      const V* src = reinterpret_cast<const V*>(parts + (long long)plist[i] * LEN +
+                                                (long long)(2 * p0) * M);
+#pragma unroll
+      for (int k = 0; k < VPL; ++k) {
+        const int e = lane + 64 * k;
+        if (e < nv) {
+          const V a = src[e];
+          if constexpr (VE == 4) {
+            acc[k].x += a.x, acc[k].y += a.y, acc[k].z += a.z, acc[k].w += a.w;
+          } else {
+            acc[k].x += a.x, acc[k].y += a.y;
+          }
+        }
+      }
+    }
+    T* wr = reinterpret_cast<T*>(wb);
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) {
+      const int e = lane + 64 * k;
+      if (e < nv) {
+        T vals[VE];
+        if constexpr (VE == 4) {
+          vals[0] = acc[k].x, vals[1] = acc[k].y, vals[2] = acc[k].z, vals[3] = acc[k].w;
+        } else {
+          vals[0] = acc[k].x, vals[1] = acc[k].y;
+        }
+#pragma unroll
+        for (int c = 0; c < VE; ++c) {
+          const int f = e * VE + c, rr = f / M, x = f - rr * M;
+          wr[2 * ((rr >> 1) * PL + pidx<T>(x)) + (rr & 1)] = vals[c];
+        }
+      }
+    }
+    auto load = [&](int tr, int x) -> cx<T> { return wb[tr * PL + pidx<T>(x)]; };
+    auto store = [&](int tr, int x, cx<T> v) { wb[tr * PL + pidx<T>(x)] = v; };
+    wave_fft<T, -1, M, TPW>(wb, tw, lane, ntr, load, store);
+    for (int t = lane; t < 2 * ntr * (Q + 1); t += 64) {
+      const int yy = t / (Q + 1), nu = t - yy * (Q + 1), tr = yy >> 1;
+      const cx<T> a = wb[tr * PL + pidx<T>(nu)];
+      const cx<T> b = conj(wb[tr * PL + pidx<T>(bin_of(-nu, M))]);
+      cx<T> r;
+      if ((yy & 1) == 0) {
+        r = scale(a + b, (T)0.5);  // transform of the even row
+      } else {
+        const cx<T> d = a - b;     // (a - b) / (2i)
+        r = cx<T>{d.im * (T)0.5, -d.re * (T)0.5};
+      }
+      rt[(2 * p0 + yy) * LDR + nu] = r;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // columns nu_v = 0..Q: forward transforms along y, band rows kept
+  const int Wb = 2 * Q + 1, ncol = Q + 1, NUC = (ncol + TPW - 1) / TPW;
+  cx<T>* out = Y + (long long)g * Wb * Wb;
+  for (int u = wave; u < NUC; u += kCoarseWaves) {
+    const int c0 = u * TPW, ntr = min(TPW, ncol - c0);
+    auto load = [&](int tr, int y) -> cx<T> { return rt[y * LDR + c0 + tr]; };
+    auto store = [&](int tr, int b, cx<T> val) {
+      const int nu = nu_of_bin(b, M);
+      if (nu < -Q || nu > Q) return;
+      const int nv = c0 + tr;
+      const cx<T> y = scale(val, s);
+      out[(long long)(nu + Q) * Wb + (nv + Q)] = y;
+      if (nv > 0) out[(long long)(Q - nu) * Wb + (Q - nv)] = conj(y);
+    };
+    wave_fft<T, -1, M, TPW>(wb, tw, lane, ntr, load, store);
   }
 }
 
@@ -929,11 +1084,14 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
   // the band lies within |nu| < N / R0: sparse first radix stage
   constexpr int S0 = N / Plan<N>::R0;
   const bool sp = nu0 > -S0 && nu0 + Wb - 1 < S0;
-  auto k1 = sp ? k_synth_cols<T, N, true> : k_synth_cols<T, N, false>;
+  // real output of a Hermitian band symmetric about 0: half the columns
+  const bool half = Plan<N>::R1 > 1 && sp && herm && !cout && Wb % 2 == 1 && 2 * nu0 + Wb - 1 == 0;
+  auto k1 = half ? k_synth_cols<T, N, true, true>
+                 : (sp ? k_synth_cols<T, N, true> : k_synth_cols<T, N, false>);
   const size_t smc = col_smem_bytes<T, N>();
   if ((e = set_smem(k1, smc)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k1, dim3(blocks_for(Wb, cpb<T, N>()), nimg), dim3(64 * col_waves<T, N>()),
-                     smc, st, Y, nu0, Wb, herm, twg, TT);
+  hipLaunchKernelGGL(k1, dim3(blocks_for(half ? (Wb + 1) / 2 : Wb, cpb<T, N>()), nimg),
+                     dim3(64 * col_waves<T, N>()), smc, st, Y, nu0, Wb, herm, twg, TT);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if constexpr (Plan<N>::R1 > 1) {
     if (sp) {
@@ -945,7 +1103,8 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
           kSynthUnitsPerWave,
           std::max<long long>(1, (long long)units * nimg / (nw * kSynthWavesTarget)));
       const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
-      auto k2 = cout ? k_synth_rows_sp<T, N, true> : k_synth_rows_sp<T, N, false>;
+      auto k2 = cout ? k_synth_rows_sp<T, N, true>
+                     : (half ? k_synth_rows_sp<T, N, false, true> : k_synth_rows_sp<T, N, false>);
       if ((e = set_smem(k2, sm)) != hipSuccess) return e;
       hipLaunchKernelGGL(k2, dim3(gx, nimg), dim3(NT), sm, st, (const cx<T>*)TT, nu0, Wb, s, twg,
                          out, cout ? (T*)nullptr : peak);
@@ -963,6 +1122,24 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
     hipLaunchKernelGGL(k2, dim3(blocks_for(N / 2, TPB), nimg), dim3(NT), sm, st,
                        (const cx<T>*)TT, nu0, Wb, s, twg, out, peak);
   }
+  return hipGetLastError();
+}
+
+template <typename T, int M>
+static hipError_t coarse_band_n(const T* parts, int spread, int cnt, int spp, long long total,
+                                int blocks, int segs, int nimg, int Q, T s, cx<T>* Y,
+                                hipStream_t st) {
+  const size_t sm = sizeof(cx<T>) * ((size_t)padded<T, M>() * (1 + kCoarseWaves * tpw<M>()) +
+                                     (size_t)M * coarse_ldr(Q)) +
+                    sizeof(int) * (kCoarseMaxParts + 1);
+  if (sm > 160 * 1024) return hipErrorNotSupported;
+  const cx<T>* twg = nullptr;
+  hipError_t e = table_for<T>(M, &twg);
+  if (e != hipSuccess) return e;
+  auto k = k_coarse_band<T, M>;
+  if ((e = set_smem(k, sm)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)nimg), dim3(64 * kCoarseWaves), sm, st, parts, spread, cnt,
+                     spp, total, blocks, segs, Q, s, twg, Y);
   return hipGetLastError();
 }
 
@@ -1031,6 +1208,31 @@ hipError_t band_analysis_sigmoid(int N, const T* theta, int nimg, int nu0, int W
   }
 }
 
+template <typename T>
+hipError_t coarse_band(int M, const T* parts, int spread, int cnt, int spp, long long total,
+                       int blocks, int segs, int nimg, int Q, T s, cx<T>* Y, hipStream_t st) {
+  if (nimg <= 0) return hipSuccess;
+  if (Q < 0 || 2 * Q + 1 > M) return hipErrorNotSupported;
+  // every image's partials must fit the block's list: a spread image meets at
+  // most (spp - 1) * blocks / total + 2 runs
+  const long long most = spread ? (long long)(spp - 1) * blocks / total + 2 : cnt;
+  if (most > kCoarseMaxParts) return hipErrorNotSupported;
+  switch (M) {
+#define LB_CASE(m) \
+  case m:          \
+    return coarse_band_n<T, m>(parts, spread, cnt, spp, total, blocks, segs, nimg, Q, s, Y, st);
+    LB_CASE(16) LB_CASE(32) LB_CASE(48) LB_CASE(64) LB_CASE(80) LB_CASE(96) LB_CASE(128)
+        LB_CASE(256)
+#undef LB_CASE
+    default:
+      return hipErrorNotSupported;
+  }
+}
+
+template hipError_t coarse_band<float>(int, const float*, int, int, int, long long, int, int, int,
+                                       int, float, cx<float>*, hipStream_t);
+template hipError_t coarse_band<double>(int, const double*, int, int, int, long long, int, int,
+                                        int, int, double, cx<double>*, hipStream_t);
 template hipError_t band_analysis_sigmoid<float>(int, const float*, int, int, int, float, float,
                                                  cx<float>*, cx<float>*, hipStream_t);
 template hipError_t band_analysis_sigmoid<double>(int, const double*, int, int, int, double,

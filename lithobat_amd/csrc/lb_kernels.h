@@ -84,6 +84,11 @@ struct ItemMap {
   long long total;
 };
 
+// Block whose spread run [b total / blocks, (b+1) total / blocks) holds system q.
+__host__ __device__ inline int spread_run_of(long long q, long long total, int blocks) {
+  return (int)(((q + 1) * blocks + total - 1) / total) - 1;
+}
+
 struct FieldArgs {
   const void* X;      // band spectra [n_masks][Wb][Wb] complex
   int Wb;
@@ -162,6 +167,15 @@ hipError_t spread_sum(const T* in, int groups, int ipg, int spp, long long total
 template <typename T>
 hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, T* peak,
                        T* scratch, hipStream_t st);
+// The coarse partials of each image (spread layout when `spread`, else `cnt`
+// consecutive partials per image) summed and taken to their band spectrum on
+// the M x M grid in one pass: Y[img][u][v] = s * sum_{y,x} I[y][x]
+// e^{-2 pi i ((u - Q) y + (v - Q) x) / M}, u, v < 2Q + 1 (the result of
+// spread_sum / segment_sum followed by band_analysis with nu0 = -Q).
+// hipErrorNotSupported, with nothing launched, when M, Q or the LDS rule it out.
+template <typename T>
+hipError_t coarse_band(int M, const T* parts, int spread, int cnt, int spp, long long total,
+                       int blocks, int segs, int nimg, int Q, T s, cx<T>* Y, hipStream_t st);
 
 template <typename T>
 hipError_t scatter_band(const cx<T>* in, int K, int Wb, int nu0, int N, cx<T>* out,

@@ -77,18 +77,13 @@ __global__ void k_segment_sum(const T* __restrict__ in, int cnt, long long len, 
   }
 }
 
-// Block whose run [b T / B, (b+1) T / B) holds system q.
-__device__ __forceinline__ int run_of(long long q, long long total, int blocks) {
-  return (int)(((q + 1) * blocks + total - 1) / total) - 1;
-}
-
 template <typename T>
 __global__ void k_spread_sum(const T* __restrict__ in, int ipg, int spp, long long total,
                              int blocks, int segs, long long len, T* __restrict__ out,
                              T* __restrict__ peak) {
   const int g = blockIdx.y;
   const long long q0 = (long long)g * ipg * spp, q1 = min(total, q0 + (long long)ipg * spp);
-  const int bf = run_of(q0, total, blocks), bl = run_of(q1 - 1, total, blocks);
+  const int bf = spread_run_of(q0, total, blocks), bl = spread_run_of(q1 - 1, total, blocks);
   T lmax = 0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < len;
        e += (long long)gridDim.x * blockDim.x) {
