@@ -431,7 +431,10 @@ static size_t ap_smem_bytes() {
 // SIG: the real input is the ILT parameter theta and the row pass transforms
 // the mask sigmoid(clip(beta theta, +-60)) (applied as the unit's values are
 // taken from the prefetch registers).
-template <typename T, int N, bool CPLX, bool SIG = false>
+// HALF (real input, band symmetric about 0, Wb = 2Q + 1): a real row's
+// spectrum at -nu is the conjugate of that at nu, so only nu = 0..Q is
+// stored, RT[img][y] holding Q + 1 values (k_analysis_cols<..., HALF>).
+template <typename T, int N, bool CPLX, bool SIG = false, bool HALF = false>
 __global__ __launch_bounds__((64 * ap_waves<T, N>()),
                              (ap_blocks_per_cu<T, N>() * ap_waves<T, N>() / 4 < 1
                                   ? 1
@@ -440,6 +443,7 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
     const T* __restrict__ in, long long img_stride, int nu0, int Wb,
     const cx<T>* __restrict__ twg, cx<T>* __restrict__ RT, T beta) {
   static_assert(!(SIG && CPLX), "the sigmoid applies to a real parameter");
+  static_assert(!(HALF && CPLX), "the half band is for real input");
   constexpr int TPW = tpw<N>();
   constexpr int NW = ap_waves<T, N>();
   constexpr int PL = padded<T, N>();
@@ -493,7 +497,8 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
   wait_vmem_all();  // no loads in flight at the loop head (see wait_vmem_all)
   TailTw<T, -1, N, TPW, 32> rt;  // 3 waves per SIMD: room for 64 more VGPRs
   rt.load(tw, lane);
-  const int ld = band_ld(Wb);
+  const int ncols = HALF ? (Wb - 1) / 2 + 1 : Wb;  // stored column vv is nu = vv when HALF
+  const int ld = band_ld(ncols);
   cx<T>* out = RT + (long long)img * ld * N;
   for (; u < NU; u += stride) {
     cx<T> v[TPL][R];
@@ -527,9 +532,9 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
     // band columns of this unit's rows, RT[img][y][v]
     const int y0 = u * ROWS_U;
     const int nrows = min(ROWS_U, N - y0);
-    for (int t = lane; t < Wb * nrows; t += 64) {
-      const int yy = t / Wb, vv = t - yy * Wb;
-      const int nu = nu0 + vv;
+    for (int t = lane; t < ncols * nrows; t += 64) {
+      const int yy = t / ncols, vv = t - yy * ncols;
+      const int nu = HALF ? vv : nu0 + vv;
       cx<T> r;
       if (CPLX) {
         r = wb[yy * PL + pidx<T>(bin_of(nu, N))];
@@ -553,7 +558,9 @@ __global__ __launch_bounds__((64 * ap_waves<T, N>()),
 // Pass 2: one forward transform per band column.  The block loads CPB
 // columns of RT[img][y][v] (contiguous runs of a row) into padded LDS; only
 // the band rows of each result are stored, X[img][u][v] = s * Z[bin(nu0 + u)].
-template <typename T, int N>
+// HALF: RT holds the columns nu = 0..Q of a real plane; column nu > 0 also
+// gives the mirrored column, X[2Q - u][Q - nu] = conj(X[u][Q + nu]).
+template <typename T, int N, bool HALF = false>
 __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(const cx<T>* __restrict__ RT, int nu0, int Wb, T s,
                                 const cx<T>* __restrict__ twg, cx<T>* __restrict__ X) {
   constexpr int CPB = cpb<T, N>(), TPW = col_tpw<T, N>(), LD = col_ld<T, N>();
@@ -564,9 +571,11 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(cons
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int img = blockIdx.y;
   const int vb = blockIdx.x * CPB;
-  const int ncol = max(0, min(CPB, Wb - vb));
+  const int Qh = (Wb - 1) / 2;
+  const int nstored = HALF ? Qh + 1 : Wb;
+  const int ncol = max(0, min(CPB, nstored - vb));
   load_roots_padded(tw, twg, N, tid, NT);
-  const int ld = band_ld(Wb);
+  const int ld = band_ld(nstored);
   const cx<T>* src = RT + (long long)img * N * ld;
   for (int t = tid; t < N * CPB; t += NT) {
     const int y = t / CPB, c = t - y * CPB;
@@ -580,7 +589,15 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(cons
   auto load = [&](int tr, int y) -> cx<T> { return wb[tr * LD + pidx<T>(y)]; };
   auto store = [&](int tr, int b, cx<T> val) {
     const int u = band_pos(b, N, nu0, Wb);
-    if (u >= 0) out[(long long)u * Wb + vb + c0 + tr] = scale(val, s);
+    if (u < 0) return;
+    const int v = vb + c0 + tr;
+    if (HALF) {
+      const cx<T> y = scale(val, s);
+      out[(long long)u * Wb + Qh + v] = y;
+      if (v > 0) out[(long long)(2 * Qh - u) * Wb + Qh - v] = conj(y);
+    } else {
+      out[(long long)u * Wb + v] = scale(val, s);
+    }
   };
   wave_fft<T, -1, N, TPW, LD>(wb, tw, lane, ntr, load, store);
 }
@@ -649,6 +666,24 @@ __device__ __forceinline__ void atomic_max_nonneg<double>(double* addr, double v
             (unsigned long long)__double_as_longlong(v > 0.0 ? v : 0.0));
 }
 
+// The block's running maximum into *peak with one atomic per block: the
+// image's blocks all update one address (and the whole batch's peaks share a
+// few cache lines), so one atomic per wavefront serialises many of them on the
+// same L2 channel.  Each wave leaves its maximum in the first slot of its own
+// LDS buffer (no longer in use), at bufs + wave * stride.
+template <typename T, int NW, int STRIDE>
+__device__ __forceinline__ void block_peak(T lmax, cx<T>* bufs, int lane, int wave, int tid,
+                                           T* peak) {
+  for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
+  if (lane == 0) bufs[wave * STRIDE].re = lmax;
+  __syncthreads();
+  if (tid == 0) {
+    T m = bufs[0].re;
+    for (int w = 1; w < NW; ++w) m = fmax(m, bufs[w * STRIDE].re);
+    atomic_max_nonneg(peak, m);
+  }
+}
+
 // Pass 2: one inverse transform per output row (complex output) or row pair
 // (real output: row y + i row y+1, valid because the band is Hermitian);
 // sparse first stage as above, outputs scaled and stored straight to HBM,
@@ -688,10 +723,8 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
     }
   };
   wave_fft<T, +1, N, TPW, PL, SP>(wb, tw, lane, ntr, load, store);
-  if (!COUT && peak) {
-    for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
-    if (lane == 0) atomic_max_nonneg(peak + img, lmax);
-  }
+  if (!COUT && peak)
+    block_peak<T, row_waves<T, N>(), TPW * PL>(lmax, tw + PL, lane, wave, tid, peak + img);
 }
 
 // Row pass for a band inside |nu| < S = N / R0 (the usual case): only two
@@ -818,10 +851,8 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
     wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store, rt);
     take();
   }
-  if (!COUT && peak) {
-    for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
-    if (lane == 0) atomic_max_nonneg(peak + img, lmax);
-  }
+  if (!COUT && peak)
+    block_peak<T, row_waves<T, N>(), TPW * PL>(lmax, tw + PL, lane, wave, tid, peak + img);
 }
 
 // ------------------------------------------------------- coarse image band
@@ -981,13 +1012,14 @@ static hipError_t set_smem(K kern, size_t bytes) {
 
 // Persistent row kernels: units per wavefront, and the wavefront count
 // (about 8 resident per SIMD over 256 CUs) below which fewer units per
-// wavefront are used.  The synthesis rows take more units per wavefront:
-// each block loads the roots table and the register twiddles once, and a
-// longer loop keeps the next unit's band values in flight (a 320-image batch
-// at 1024^2: 349 us with 2 units per wavefront, 276 us with 8).
+// wavefront are used.  The synthesis rows take 2 units per wavefront: with
+// the half band (about 40 stored columns) their reads are light, and short
+// blocks balance the store-bound tail better than long loops (a 320-image
+// batch at 1024^2: the whole forward 0.962 ms at 2, 0.973 at 4, 0.988 at 8,
+// 1.012 at 1).
 constexpr int kUnitsPerWave = 4;
 constexpr int kRowWavesTarget = 8192;
-constexpr int kSynthUnitsPerWave = 8;
+constexpr int kSynthUnitsPerWave = 2;
 constexpr int kSynthWavesTarget = 1024;
 
 static unsigned blocks_for(int lines, int per_block) {
@@ -1004,7 +1036,7 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
   const cx<T>* twg = nullptr;
   hipError_t e = table_for<T>(N, &twg);
   if (e != hipSuccess) return e;
-  bool done = false;
+  bool done = false, half = false;
   if constexpr (Plan<N>::R1 > 1) {
     if (nsum == 1) {
       const int units = ((cplx ? N : N / 2) + tpw<N>() - 1) / tpw<N>();
@@ -1012,7 +1044,11 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
       const int upw = (int)std::min<long long>(
           kUnitsPerWave, std::max<long long>(1, (long long)units * nimg / (nw * kRowWavesTarget)));
       const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
-      auto k = cplx ? k_analysis_rows_p<T, N, true> : k_analysis_rows_p<T, N, false>;
+      // a real plane over a band symmetric about 0: half the columns
+      half = !cplx && Wb % 2 == 1 && 2 * nu0 + Wb - 1 == 0;
+      auto k = cplx   ? k_analysis_rows_p<T, N, true>
+               : half ? k_analysis_rows_p<T, N, false, false, true>
+                      : k_analysis_rows_p<T, N, false>;
       const size_t smp = ap_smem_bytes<T, N>();
       if ((e = set_smem(k, smp)) != hipSuccess) return e;
       hipLaunchKernelGGL(k, dim3(gx, nimg), dim3(64 * nw), smp, st, (const T*)in, img_stride,
@@ -1033,11 +1069,11 @@ static hipError_t analysis_n(const void* in, int cplx, long long img_stride,
                        (const T*)in, img_stride, slice_stride, nsum, nu0, Wb, twg, RT);
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  auto k2 = k_analysis_cols<T, N>;
+  auto k2 = half ? k_analysis_cols<T, N, true> : k_analysis_cols<T, N>;
   const size_t smc = col_smem_bytes<T, N>();
   if ((e = set_smem(k2, smc)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k2, dim3(blocks_for(Wb, cpb<T, N>()), nimg), dim3(64 * col_waves<T, N>()),
-                     smc, st, (const cx<T>*)RT, nu0, Wb, s, twg, X);
+  hipLaunchKernelGGL(k2, dim3(blocks_for(half ? (Wb + 1) / 2 : Wb, cpb<T, N>()), nimg),
+                     dim3(64 * col_waves<T, N>()), smc, st, (const cx<T>*)RT, nu0, Wb, s, twg, X);
   return hipGetLastError();
 }
 
@@ -1053,17 +1089,18 @@ static hipError_t analysis_sigmoid_n(const T* theta, int nimg, int nu0, int Wb, 
     const int upw = (int)std::min<long long>(
         kUnitsPerWave, std::max<long long>(1, (long long)units * nimg / (nw * kRowWavesTarget)));
     const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
-    auto k = k_analysis_rows_p<T, N, false, true>;
+    const bool half = Wb % 2 == 1 && 2 * nu0 + Wb - 1 == 0;
+    auto k = half ? k_analysis_rows_p<T, N, false, true, true> : k_analysis_rows_p<T, N, false, true>;
     const size_t smp = ap_smem_bytes<T, N>();
     if ((e = set_smem(k, smp)) != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(gx, nimg), dim3(64 * nw), smp, st, theta, (long long)N * N, nu0,
                        Wb, twg, RT, beta);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    auto k2 = k_analysis_cols<T, N>;
+    auto k2 = half ? k_analysis_cols<T, N, true> : k_analysis_cols<T, N>;
     const size_t smc = col_smem_bytes<T, N>();
     if ((e = set_smem(k2, smc)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k2, dim3(blocks_for(Wb, cpb<T, N>()), nimg), dim3(64 * col_waves<T, N>()),
-                       smc, st, (const cx<T>*)RT, nu0, Wb, s, twg, X);
+    hipLaunchKernelGGL(k2, dim3(blocks_for(half ? (Wb + 1) / 2 : Wb, cpb<T, N>()), nimg),
+                       dim3(64 * col_waves<T, N>()), smc, st, (const cx<T>*)RT, nu0, Wb, s, twg, X);
     return hipGetLastError();
   } else {
     (void)theta, (void)nimg, (void)nu0, (void)Wb, (void)s, (void)beta, (void)RT, (void)X,
