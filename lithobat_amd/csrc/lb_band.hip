@@ -736,12 +736,16 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
 // HALF: TT holds the columns nu = 0..Q of a band symmetric about 0 (see
 // k_synth_cols); the input at nu = j - S < 0 is the conjugate of the stored
 // column S - j.
-template <typename T, int N, bool COUT, bool HALF = false>
+// STAGED (real output, `out` 16-byte aligned): the last stage's values go to
+// the wave's LDS buffer as the two image rows and are written out in 16-byte
+// runs, four times fewer store instructions than the stage's own outputs,
+// which lie N / R apart.
+template <typename T, int N, bool COUT, bool HALF = false, bool STAGED = false>
 __global__ __launch_bounds__((64 * row_waves<T, N>()),
                              (row_waves_per_simd<T, N>())) void k_synth_rows_sp(
     const cx<T>* __restrict__ TT, int nu0, int Wb, T s, const cx<T>* __restrict__ twg,
     T* __restrict__ out, T* __restrict__ peak) {
-  static_assert(!(HALF && COUT), "the half band is for real output");
+  static_assert(!((HALF || STAGED) && COUT), "the half band and staged rows are for real output");
   LB_BAND_PROLOGUE(T, N)
   constexpr int R = Plan<N>::R0, S = N / R;
   constexpr int NTR = COUT ? N : N / 2;
@@ -836,19 +840,39 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
     }
     __builtin_amdgcn_wave_barrier();
     const int p0 = u * TPW;
-    auto store = [&](int tr, int x, cx<T> val) {
-      const int p = p0 + tr;
-      if (COUT) {
-        reinterpret_cast<cx<T>*>(out)[((long long)img * N + p) * N + x] = val;
-      } else {
-        // the image is written once and not read back here: streaming stores
-        T* o = out + (long long)img * N * N + (long long)(2 * p) * N + x;
-        __builtin_nontemporal_store(val.re, o);
-        __builtin_nontemporal_store(val.im, o + N);
+    if constexpr (STAGED) {
+      // the two image rows (real parts, then imaginary parts) in LDS
+      T* rows = reinterpret_cast<T*>(wb);
+      auto store = [&](int tr, int x, cx<T> val) {
+        rows[tr * 2 * PL + x] = val.re;
+        rows[tr * 2 * PL + N + x] = val.im;
         lmax = fmax(lmax, fmax(val.re, val.im));
+      };
+      wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store, rt);
+      using V = T __attribute__((ext_vector_type(16 / sizeof(T))));
+      constexpr int K = 16 / (int)sizeof(T), NV = 2 * N / K;
+      for (int tr = 0; tr < ntr; ++tr) {
+        const V* src = reinterpret_cast<const V*>(rows + tr * 2 * PL);
+        V* dst = reinterpret_cast<V*>(out + (long long)img * N * N + (long long)(2 * (p0 + tr)) * N);
+#pragma unroll
+        for (int k = lane; k < NV; k += 64) __builtin_nontemporal_store(src[k], dst + k);
       }
-    };
-    wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store, rt);
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      auto store = [&](int tr, int x, cx<T> val) {
+        const int p = p0 + tr;
+        if (COUT) {
+          reinterpret_cast<cx<T>*>(out)[((long long)img * N + p) * N + x] = val;
+        } else {
+          // the image is written once and not read back here: streaming stores
+          T* o = out + (long long)img * N * N + (long long)(2 * p) * N + x;
+          __builtin_nontemporal_store(val.re, o);
+          __builtin_nontemporal_store(val.im, o + N);
+          lmax = fmax(lmax, fmax(val.re, val.im));
+        }
+      };
+      wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, store, rt);
+    }
     take();
   }
   if (!COUT && peak)
@@ -1140,8 +1164,11 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
           kSynthUnitsPerWave,
           std::max<long long>(1, (long long)units * nimg / (nw * kSynthWavesTarget)));
       const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
-      auto k2 = cout ? k_synth_rows_sp<T, N, true>
-                     : (half ? k_synth_rows_sp<T, N, false, true> : k_synth_rows_sp<T, N, false>);
+      const bool staged = !cout && ((uintptr_t)out & 15) == 0;
+      auto k2 = cout     ? k_synth_rows_sp<T, N, true>
+                : staged ? (half ? k_synth_rows_sp<T, N, false, true, true>
+                                 : k_synth_rows_sp<T, N, false, false, true>)
+                         : (half ? k_synth_rows_sp<T, N, false, true> : k_synth_rows_sp<T, N, false>);
       if ((e = set_smem(k2, sm)) != hipSuccess) return e;
       hipLaunchKernelGGL(k2, dim3(gx, nimg), dim3(NT), sm, st, (const cx<T>*)TT, nu0, Wb, s, twg,
                          out, cout ? (T*)nullptr : peak);

@@ -21,9 +21,19 @@ namespace lb {
 
 constexpr int kObjThreads = 256;
 
-// Block (blockIdx.x, image blockIdx.y) reduces its share of one image into
-// part_cost / part_err [image][block]; d_f and signed_f are optional.
+// Elementwise passes: each thread handles 16-byte vectors (4 floats or 2
+// doubles) when every pointer is 16-byte aligned (VEC), then a scalar tail.
 template <typename T>
+struct Vec16 {
+  static constexpr int K = 16 / sizeof(T);
+  T v[K];
+};
+
+// Block (blockIdx.x, image blockIdx.y) reduces its share of one image into
+// part_cost / part_err [image][block]; d_f and signed_f are optional.  VEC:
+// the pixels go through in 16-byte vectors (every pointer 16-byte aligned
+// and npix a whole number of vectors).
+template <typename T, bool VEC>
 __global__ __launch_bounds__(kObjThreads) void k_resist_partial(
     const T* __restrict__ f, const T* __restrict__ target, long long npix, T scale, T level, T a,
     T sign, T* __restrict__ d_f, T* __restrict__ signed_f, double* __restrict__ part_cost,
@@ -32,16 +42,37 @@ __global__ __launch_bounds__(kObjThreads) void k_resist_partial(
   const T dscale = sign * (T)2 / (T)npix * a * scale;
   double c = 0;
   int err = 0;
-  for (long long e = blockIdx.x * (long long)kObjThreads + threadIdx.x; e < npix;
-       e += (long long)gridDim.x * kObjThreads) {
-    const T s = sign * (f[base + e] * scale - level);
+  auto pixel = [&](T fv, T t, T& d, T& sv) {
+    const T s = sign * (fv * scale - level);
     const T z = sigmoid_clipped(a * s);
-    const T t = target[base + e];
     const T r = z - t;
     c += (double)(r * r);
     err += (s > (T)0) != (t > (T)0.5);
-    if (d_f) d_f[base + e] = dscale * r * z * ((T)1 - z);
-    if (signed_f) signed_f[base + e] = s;
+    d = dscale * r * z * ((T)1 - z);
+    sv = s;
+  };
+  if constexpr (VEC) {
+    using V = Vec16<T>;
+    const long long nv = npix / V::K;
+    const V* fv = reinterpret_cast<const V*>(f + base);
+    const V* tv = reinterpret_cast<const V*>(target + base);
+    for (long long i = blockIdx.x * (long long)kObjThreads + threadIdx.x; i < nv;
+         i += (long long)gridDim.x * kObjThreads) {
+      const V x = fv[i], t = tv[i];
+      V d, sv;
+#pragma unroll
+      for (int k = 0; k < V::K; ++k) pixel(x.v[k], t.v[k], d.v[k], sv.v[k]);
+      if (d_f) reinterpret_cast<V*>(d_f + base)[i] = d;
+      if (signed_f) reinterpret_cast<V*>(signed_f + base)[i] = sv;
+    }
+  } else {
+    for (long long e = blockIdx.x * (long long)kObjThreads + threadIdx.x; e < npix;
+         e += (long long)gridDim.x * kObjThreads) {
+      T d, sv;
+      pixel(f[base + e], target[base + e], d, sv);
+      if (d_f) d_f[base + e] = d;
+      if (signed_f) signed_f[base + e] = sv;
+    }
   }
   __shared__ double sc[kObjThreads / 64];
   __shared__ int se[kObjThreads / 64];
@@ -112,6 +143,8 @@ int resist_blocks(long long npix) {
   return (int)b;
 }
 
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 // part_cost holds n_img * (nblk + 1) doubles: the block partials, then the
 // per-image costs.
 template <typename T>
@@ -119,8 +152,11 @@ hipError_t resist_objective(const T* f, const T* target, int n_img, long long np
                             T level, T a, T sign, T* cost, T* total, T* d_f, T* signed_f,
                             int* mismatch, double* part_cost, int* part_err, hipStream_t st) {
   const int nblk = resist_blocks(npix);
-  hipLaunchKernelGGL(k_resist_partial<T>, dim3(nblk, n_img), dim3(kObjThreads), 0, st, f, target,
-                     npix, scale, level, a, sign, d_f, signed_f, part_cost, part_err);
+  const bool vec = npix % (16 / (long long)sizeof(T)) == 0 && aligned16(f) && aligned16(target) &&
+                   (!d_f || aligned16(d_f)) && (!signed_f || aligned16(signed_f));
+  auto k = vec ? k_resist_partial<T, true> : k_resist_partial<T, false>;
+  hipLaunchKernelGGL(k, dim3(nblk, n_img), dim3(kObjThreads), 0, st, f, target, npix, scale,
+                     level, a, sign, d_f, signed_f, part_cost, part_err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_resist_final<T>, dim3(1), dim3(64 * kFinalWaves), 0, st, part_cost,
@@ -128,14 +164,6 @@ hipError_t resist_objective(const T* f, const T* target, int n_img, long long np
                      mismatch);
   return hipGetLastError();
 }
-
-// Elementwise passes: each thread handles 16-byte vectors (4 floats or 2
-// doubles) when every pointer is 16-byte aligned (VEC), then a scalar tail.
-template <typename T>
-struct Vec16 {
-  static constexpr int K = 16 / sizeof(T);
-  T v[K];
-};
 
 // pbeta (device, optional): beta read from device memory instead of the
 // argument, so a captured graph can follow a schedule that changes per call.
@@ -214,7 +242,6 @@ static unsigned elementwise_blocks(long long n) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
-static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 template <typename T>
 hipError_t mask_sigmoid(const T* x, long long n, T beta, const double* pbeta, T* m,
