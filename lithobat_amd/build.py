@@ -36,9 +36,17 @@ def device_flags():
             "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include")]
 
 
+# Per-source extra flags.  The coherent-system kernels feed compile-time zeros
+# (box rows and columns past the pupil support) into their first radix stage;
+# without signed zeros x + 0 folds to x, so those additions drop out (the
+# sign of a zero is never observed there: only |E|^2 and products leave).
+EXTRA_FLAGS = {"lb_field.hip": ["-fno-signed-zeros"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-    cmd = [hipcc(), *device_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *device_flags(), *EXTRA_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+           "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -507,9 +507,9 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
   // the per-image peak is a running maximum (atomics) in the final pass:
   // computed only when the caller asks for it
   T* pk = (T*)peak;
-  if (pk) LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
   const long long total = (long long)w.n_img * p.spp;
   if (p.direct) {
+    if (pk) LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
     if (w.spread)
       LB_HIP(spread_sum<T>(partial, w.n_img, 1, p.spp, total, w.runs, w.n_chunks,
                            (long long)p.N * p.N, (T*)images, pk, st));
@@ -526,11 +526,12 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
     const hipError_t fused =
         2 * w.n_img >= p.cus
             ? coarse_band<T>(p.M, partial, w.spread, w.n_chunks, p.spp, total, w.runs,
-                             w.n_chunks, w.n_img, p.QI, invM2, Y, st)
+                             w.n_chunks, w.n_img, p.QI, invM2, Y, pk, st)
             : hipErrorNotSupported;
     if (fused != hipErrorNotSupported) {
-      LB_HIP(fused);
+      LB_HIP(fused);  // the peaks are zeroed there
     } else {
+      if (pk) LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
       if (w.spread)
         LB_HIP(spread_sum<T>(partial, w.n_img, 1, p.spp, total, w.runs, w.n_chunks,
                              (long long)p.M * p.M, IM, (T*)nullptr, st));

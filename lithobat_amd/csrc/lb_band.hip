@@ -898,7 +898,8 @@ __host__ __device__ constexpr int coarse_ldr(int Q) { return (Q + 1) % 2 == 0 ? 
 template <typename T, int M>
 __global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_band(
     const T* __restrict__ parts, int spread, int cnt, int spp, long long total, int blocks,
-    int segs, int Q, T s, const cx<T>* __restrict__ twg, cx<T>* __restrict__ Y) {
+    int segs, int Q, T s, const cx<T>* __restrict__ twg, cx<T>* __restrict__ Y,
+    T* __restrict__ peak) {
   constexpr int TPW = tpw<M>(), PL = padded<T, M>(), NT = 64 * kCoarseWaves;
   constexpr long long LEN = (long long)M * M;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -925,6 +926,7 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void k_coarse_band(
       for (int c = 0; c < cnt && n < kCoarseMaxParts; ++c) plist[n++] = g * cnt + c;
     }
     plist[kCoarseMaxParts] = n;
+    if (peak) peak[g] = 0;  // the synthesis rows' running maximum starts here
   }
   load_roots_padded(tw, twg, M, tid, NT);
   __syncthreads();
@@ -1191,7 +1193,7 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
 
 template <typename T, int M>
 static hipError_t coarse_band_n(const T* parts, int spread, int cnt, int spp, long long total,
-                                int blocks, int segs, int nimg, int Q, T s, cx<T>* Y,
+                                int blocks, int segs, int nimg, int Q, T s, cx<T>* Y, T* peak,
                                 hipStream_t st) {
   const size_t sm = sizeof(cx<T>) * ((size_t)padded<T, M>() * (1 + kCoarseWaves * tpw<M>()) +
                                      (size_t)M * coarse_ldr(Q)) +
@@ -1203,7 +1205,7 @@ static hipError_t coarse_band_n(const T* parts, int spread, int cnt, int spp, lo
   auto k = k_coarse_band<T, M>;
   if ((e = set_smem(k, sm)) != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3((unsigned)nimg), dim3(64 * kCoarseWaves), sm, st, parts, spread, cnt,
-                     spp, total, blocks, segs, Q, s, twg, Y);
+                     spp, total, blocks, segs, Q, s, twg, Y, peak);
   return hipGetLastError();
 }
 
@@ -1274,7 +1276,8 @@ hipError_t band_analysis_sigmoid(int N, const T* theta, int nimg, int nu0, int W
 
 template <typename T>
 hipError_t coarse_band(int M, const T* parts, int spread, int cnt, int spp, long long total,
-                       int blocks, int segs, int nimg, int Q, T s, cx<T>* Y, hipStream_t st) {
+                       int blocks, int segs, int nimg, int Q, T s, cx<T>* Y, T* peak,
+                       hipStream_t st) {
   if (nimg <= 0) return hipSuccess;
   if (Q < 0 || 2 * Q + 1 > M) return hipErrorNotSupported;
   // every image's partials must fit the block's list: a spread image meets at
@@ -1284,7 +1287,8 @@ hipError_t coarse_band(int M, const T* parts, int spread, int cnt, int spp, long
   switch (M) {
 #define LB_CASE(m) \
   case m:          \
-    return coarse_band_n<T, m>(parts, spread, cnt, spp, total, blocks, segs, nimg, Q, s, Y, st);
+    return coarse_band_n<T, m>(parts, spread, cnt, spp, total, blocks, segs, nimg, Q, s, Y, peak, \
+                               st);
     LB_CASE(16) LB_CASE(32) LB_CASE(48) LB_CASE(64) LB_CASE(80) LB_CASE(96) LB_CASE(128)
         LB_CASE(256)
 #undef LB_CASE
@@ -1294,9 +1298,9 @@ hipError_t coarse_band(int M, const T* parts, int spread, int cnt, int spp, long
 }
 
 template hipError_t coarse_band<float>(int, const float*, int, int, int, long long, int, int, int,
-                                       int, float, cx<float>*, hipStream_t);
+                                       int, float, cx<float>*, float*, hipStream_t);
 template hipError_t coarse_band<double>(int, const double*, int, int, int, long long, int, int,
-                                        int, int, double, cx<double>*, hipStream_t);
+                                        int, int, double, cx<double>*, double*, hipStream_t);
 template hipError_t band_analysis_sigmoid<float>(int, const float*, int, int, int, float, float,
                                                  cx<float>*, cx<float>*, hipStream_t);
 template hipError_t band_analysis_sigmoid<double>(int, const double*, int, int, int, double,

@@ -172,10 +172,12 @@ hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, 
 // the M x M grid in one pass: Y[img][u][v] = s * sum_{y,x} I[y][x]
 // e^{-2 pi i ((u - Q) y + (v - Q) x) / M}, u, v < 2Q + 1 (the result of
 // spread_sum / segment_sum followed by band_analysis with nu0 = -Q).
+// peak (nullable): set to 0 per image, ready for the synthesis's running maximum.
 // hipErrorNotSupported, with nothing launched, when M, Q or the LDS rule it out.
 template <typename T>
 hipError_t coarse_band(int M, const T* parts, int spread, int cnt, int spp, long long total,
-                       int blocks, int segs, int nimg, int Q, T s, cx<T>* Y, hipStream_t st);
+                       int blocks, int segs, int nimg, int Q, T s, cx<T>* Y, T* peak,
+                       hipStream_t st);
 
 template <typename T>
 hipError_t scatter_band(const cx<T>* in, int K, int Wb, int nu0, int N, cx<T>* out,
