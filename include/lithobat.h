@@ -349,6 +349,32 @@ lb_status lb_mask_sigmoid_grad_theta(const void* grad_mask, const void* theta, i
                                      double beta, const void* g, void* grad_x,
                                      int32_t precision, void* stream);
 
+/* The relaxed-resist objective of the images a spectrum gives (opc/ilt.py:
+ * 311-330, _Objective's image term; lb_forward_spectrum then
+ * lb_resist_objective), evaluated where the images are synthesised: per image
+ * the cost, their total and the mismatch counts exactly as lb_resist_objective
+ * returns them (each output nullable), and sens_band [n_masks * n_planes][Wb]
+ * [Wb] complex (lb_sensitivity_band_bytes; Wb = 2 box - 1) = the band
+ * spectrum of the intensity derivative d_intensity, from which
+ * lb_backward_sensitivity_band runs the adjoint (opc/ilt.py:235-266) as
+ * lb_backward_spectrum does from d_intensity.  target [n_img][N][N] real;
+ * images and d_images (nullable) also receive the intensity and its
+ * derivative.  Neither array needs a pass of its own.  LB_ERR_UNSUPPORTED for
+ * a plan imaged directly on its N x N grid, an image band reaching |nu| >=
+ * N / 16, or a pointer that is not 16-byte aligned (then lb_forward_spectrum,
+ * lb_resist_objective and lb_backward_spectrum give the same results).
+ * lb_forward_objective_workspace_bytes returns 0 when unsupported. */
+size_t lb_forward_objective_workspace_bytes(lb_plan plan, int32_t n_masks);
+size_t lb_sensitivity_band_bytes(lb_plan plan, int32_t n_masks);
+lb_status lb_forward_objective(lb_plan plan, const void* spectrum, int32_t n_masks,
+                               const void* target, double scale, double level, double steepness,
+                               double sign, void* cost, void* total, int32_t* mismatch,
+                               void* sens_band, void* images, void* d_images, void* workspace,
+                               size_t workspace_bytes, void* stream);
+lb_status lb_backward_sensitivity_band(lb_plan plan, const void* spectrum, int32_t n_masks,
+                                       const void* sens_band, void* d_masks, void* workspace,
+                                       size_t workspace_bytes, void* stream);
+
 /* No counterpart in the reference: the coherent-system kernel shape of a
  * batch's forward or adjoint launch (single-launch shapes only; the work
  * items and workspace do not depend on it). */

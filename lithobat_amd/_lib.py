@@ -63,7 +63,8 @@ EXPORTS = [
     "lb_ilt_schedule", "lb_mask_sigmoid_dev", "lb_mask_sigmoid_grad_dev", "lb_batch_set_shape",
     "lb_volume_interp", "lb_mask_spectrum_bytes", "lb_mask_spectrum", "lb_forward_spectrum",
     "lb_backward_spectrum", "lb_batch_backward_reuse", "lb_mask_spectrum_sigmoid",
-    "lb_mask_sigmoid_grad_theta",
+    "lb_mask_sigmoid_grad_theta", "lb_forward_objective_workspace_bytes",
+    "lb_sensitivity_band_bytes", "lb_forward_objective", "lb_backward_sensitivity_band",
 ]
 
 _lib = None
@@ -139,6 +140,11 @@ def load() -> C.CDLL:
         "lb_mask_spectrum_sigmoid": ([vp, vp, i32, C.c_double, vp, vp, sz, vp], C.c_int),
         "lb_mask_sigmoid_grad_theta": ([vp, vp, C.c_int64, C.c_double, vp, vp, i32, vp],
                                        C.c_int),
+        "lb_forward_objective_workspace_bytes": ([vp, i32], sz),
+        "lb_sensitivity_band_bytes": ([vp, i32], sz),
+        "lb_forward_objective": ([vp, vp, i32, vp, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, vp, vp, vp, vp, vp, vp, vp, sz, vp], C.c_int),
+        "lb_backward_sensitivity_band": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
         "lb_batch_backward_reuse": ([vp, C.POINTER(vp), C.POINTER(vp), vp, sz, vp, vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),
         "lb_batch_forward": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, sz, vp],

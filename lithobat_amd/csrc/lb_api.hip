@@ -499,10 +499,27 @@ lb_status fwd_pre(Plan& p, const void* masks, int cplx, int n_masks, char* ws, c
   return LB_OK;
 }
 
+// The relaxed-resist objective evaluated where the images are synthesised
+// (lb_forward_objective): targets [n_img][N][N], outputs as
+// lb_resist_objective's, the band spectrum of the intensity derivative
+// (sens_band) and, optionally, the derivative itself; rt / pc / pe are
+// workspace offsets of the scratch that pass uses.
+struct ObjArgs {
+  const void* target;
+  double scale, level, a, sign;
+  void* cost;
+  void* total;
+  int32_t* mismatch;
+  void* sens_band;
+  void* d_images;
+  size_t rt, pc, pe;
+};
+
 // Forward, phase 3: reduce the coarse partials, lift them to the N x N
 // images (or sum them directly when the coarse grid is the image grid).
 template <typename T>
-lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, hipStream_t st) {
+lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, hipStream_t st,
+                   const ObjArgs* obj = nullptr) {
   T* partial = (T*)(ws + w.partial);
   // the per-image peak is a running maximum (atomics) in the final pass:
   // computed only when the caller asks for it
@@ -541,8 +558,19 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
       LB_HIP(band_analysis<T>(p.M, IM, 0, (long long)p.M * p.M, 0, 1, w.n_img, -p.QI, p.WbI,
                               invM2, (cx<T>*)(ws + w.RTc), Y, st));
     }
-    LB_HIP(band_synthesis<T>(p.N, Y, w.n_img, -p.QI, p.WbI, 1, (T)1, 0, (cx<T>*)(ws + w.TT),
-                             (T*)images, pk, st));
+    if (obj) {
+      const double npix = (double)p.N * (double)p.N;
+      const ResistPix<T> rp{(T)obj->scale, (T)obj->level, (T)obj->a, (T)obj->sign,
+                            (T)(obj->sign * 2.0 / npix * obj->a * obj->scale)};
+      LB_HIP(synth_objective<T>(p.N, Y, w.n_img, -p.QI, p.WbI, (cx<T>*)(ws + w.TT),
+                                (const T*)obj->target, rp, (T*)images, (T*)obj->d_images,
+                                (cx<T>*)(ws + obj->rt), (cx<T>*)obj->sens_band,
+                                (double*)(ws + obj->pc), (int*)(ws + obj->pe), (T*)obj->cost,
+                                (T*)obj->total, obj->mismatch, st));
+    } else {
+      LB_HIP(band_synthesis<T>(p.N, Y, w.n_img, -p.QI, p.WbI, 1, (T)1, 0, (cx<T>*)(ws + w.TT),
+                               (T*)images, pk, st));
+    }
   }
   return LB_OK;
 }
@@ -552,7 +580,7 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
 template <typename T>
 lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* images, void* peak,
                     char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr,
-                    bool may_profile = true) {
+                    bool may_profile = true, const ObjArgs* obj = nullptr) {
   if (!spectrum) {
     lb_status rc = fwd_pre<T>(p, masks, cplx, n_masks, ws, w, st);
     if (rc != LB_OK) return rc;
@@ -577,14 +605,17 @@ lb_status forward_t(Plan& p, const void* masks, int cplx, int n_masks, void* ima
     LB_HIP(field_forward(p.precision, fc, a, ws + w.partial, st));
   }
   if (prof) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
-  return fwd_post<T>(p, images, peak, ws, w, st);
+  return fwd_post<T>(p, images, peak, ws, w, st, obj);
 }
 
 // Backward, phase 1: band spectra of the masks and the coarse sensitivity
 // gM (the exact adjoint of the lift from M x M to N x N).
+// gy: the band spectrum of dimg (lb_forward_objective's sens_band), given
+// instead of dimg (coarse grids only).
 template <typename T>
 lb_status bwd_pre(Plan& p, const void* masks, int n_masks, const void* dimg, char* ws,
-                  const Work& w, const T** gM, hipStream_t st, bool have_spectrum = false) {
+                  const Work& w, const T** gM, hipStream_t st, bool have_spectrum = false,
+                  const void* gy = nullptr) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
   if (!have_spectrum)
     LB_HIP(band_analysis<T>(p.N, masks, 0, (long long)p.N * p.N, 0, 1, n_masks, p.nu0m, p.Wbm,
@@ -592,9 +623,12 @@ lb_status bwd_pre(Plan& p, const void* masks, int n_masks, const void* dimg, cha
   *gM = (const T*)dimg;
   if (!p.direct) {
     const T invM2 = (T)(1.0 / ((double)p.M * (double)p.M));
-    cx<T>* GY = (cx<T>*)(ws + w.GY);
-    LB_HIP(band_analysis<T>(p.N, dimg, 0, (long long)p.N * p.N, 0, 1, w.n_img, -p.QI, p.WbI,
-                            (T)1, (cx<T>*)(ws + w.RTg), GY, st));
+    const cx<T>* GY = (const cx<T>*)gy;
+    if (!GY) {
+      LB_HIP(band_analysis<T>(p.N, dimg, 0, (long long)p.N * p.N, 0, 1, w.n_img, -p.QI, p.WbI,
+                              (T)1, (cx<T>*)(ws + w.RTg), (cx<T>*)(ws + w.GY), st));
+      GY = (const cx<T>*)(ws + w.GY);
+    }
     LB_HIP(band_synthesis<T>(p.M, GY, w.n_img, -p.QI, p.WbI, 1, invM2, 0, (cx<T>*)(ws + w.TTc),
                              (T*)(ws + w.gM), (T*)nullptr, st));
     *gM = (const T*)(ws + w.gM);
@@ -625,9 +659,9 @@ lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, h
 template <typename T>
 lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, void* dmask,
                      char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr,
-                     bool cout = false) {
+                     bool cout = false, const void* gy = nullptr) {
   const T* gM = nullptr;
-  lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st, spectrum != nullptr);
+  lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st, spectrum != nullptr, gy);
   if (rc != LB_OK) return rc;
   FieldConfig fc = p.adj;
   fc.grid = w.spread ? w.runs : grid_for(p, fc, w.n_items, true);
@@ -1433,6 +1467,97 @@ lb_status lb_backward_spectrum(lb_plan plan, const void* spectrum, int32_t n_mas
                                   spectrum)
              : backward_t<float>(p, nullptr, n_masks, dimg, dmask, (char*)workspace, w, st,
                                  spectrum);
+}
+
+namespace {
+
+// Workspace of lb_forward_objective: the forward's, then the derivative's
+// row spectra and the cost partials.
+struct ObjWork {
+  Work w;
+  size_t rt, pc, pe, total;
+};
+
+ObjWork objective_layout(const Plan& p, int n_masks) {
+  ObjWork o;
+  o.w = layout(p, n_masks, false);
+  Layout L;
+  L.off = o.w.total;
+  const int parts = synth_objective_part_count(p.N, o.w.n_img);
+  o.rt = L.take(p.csz() * (size_t)o.w.n_img * band_ld((p.WbI + 1) / 2) * p.N);
+  o.pc = L.take(sizeof(double) * (size_t)parts);
+  o.pe = L.take(sizeof(int) * (size_t)parts);
+  o.total = L.off;
+  return o;
+}
+
+bool objective_supported(const Plan& p) {
+  return !p.direct && p.spp > 0 && synth_objective_supported(p.N, -p.QI, p.WbI);
+}
+
+}  // namespace
+
+size_t lb_forward_objective_workspace_bytes(lb_plan plan, int32_t n_masks) {
+  if (!plan || n_masks < 1 || !objective_supported(plan->p)) return 0;
+  return objective_layout(plan->p, n_masks).total;
+}
+
+size_t lb_sensitivity_band_bytes(lb_plan plan, int32_t n_masks) {
+  if (!plan || n_masks < 1 || plan->p.direct) return 0;
+  const Plan& p = plan->p;
+  return p.csz() * (size_t)n_masks * p.n_planes * p.WbI * p.WbI;
+}
+
+lb_status lb_forward_objective(lb_plan plan, const void* spectrum, int32_t n_masks,
+                               const void* target, double scale, double level, double steepness,
+                               double sign, void* cost, void* total, int32_t* mismatch,
+                               void* sens_band, void* images, void* d_images, void* workspace,
+                               size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !spectrum || !target || !sens_band)
+    return fail(LB_ERR_ARG, "null plan, spectrum, target or sens_band");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  auto al = [](const void* q) { return q == nullptr || ((uintptr_t)q & 15) == 0; };
+  if (!objective_supported(p))
+    return fail(LB_ERR_UNSUPPORTED,
+                "the fused objective needs a coarse grid and an image band inside |nu| < N/16");
+  if (!al(target) || !al(images) || !al(d_images))
+    return fail(LB_ERR_UNSUPPORTED, "target, images and d_images must be 16-byte aligned");
+  const ObjWork o = objective_layout(p, n_masks);
+  if (!workspace || ws_bytes < o.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(o.total) + " bytes");
+  const double a = steepness / std::max(std::fabs(level), 1e-12);  // as lb_resist_objective
+  const ObjArgs obj{target, scale, level, a, sign, cost, total, mismatch, sens_band, d_images,
+                    o.rt, o.pc, o.pe};
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? forward_t<double>(p, nullptr, 0, n_masks, images, nullptr, (char*)workspace, o.w,
+                                 st, spectrum, true, &obj)
+             : forward_t<float>(p, nullptr, 0, n_masks, images, nullptr, (char*)workspace, o.w,
+                                st, spectrum, true, &obj);
+}
+
+lb_status lb_backward_sensitivity_band(lb_plan plan, const void* spectrum, int32_t n_masks,
+                                       const void* sens_band, void* dmask, void* workspace,
+                                       size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !spectrum || !sens_band || !dmask) return fail(LB_ERR_ARG, "null argument");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  if (p.direct)
+    return fail(LB_ERR_UNSUPPORTED, "the sensitivity band form needs a coarse grid");
+  Work w = layout(p, n_masks, true);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  return p.precision == 64
+             ? backward_t<double>(p, nullptr, n_masks, nullptr, dmask, (char*)workspace, w, st,
+                                  spectrum, false, sens_band)
+             : backward_t<float>(p, nullptr, n_masks, nullptr, dmask, (char*)workspace, w, st,
+                                 spectrum, false, sens_band);
 }
 
 lb_status lb_backward_spectrum_complex(lb_plan plan, const void* spectrum, int32_t n_masks,

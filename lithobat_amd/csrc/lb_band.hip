@@ -879,6 +879,199 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
     block_peak<T, row_waves<T, N>(), TPW * PL>(lmax, tw + PL, lane, wave, tid, peak + img);
 }
 
+// ------------------------------------------- synthesis + resist objective
+// The synthesis rows of a real image over a Hermitian half band (as
+// k_synth_rows_sp<T, N, false, true>) fused with the relaxed-resist objective
+// and the row pass of the band analysis of the intensity derivative.  Per row
+// pair the image rows land in the wave's LDS buffer; the objective of
+// opc/ilt.py:311-330 turns them in place into the derivative rows
+//   v = sign (scale I - level), z = sigmoid(a v), cost += (z - t)^2,
+//   mismatch += (v > 0) != (t > 0.5), dI = dscale (z - t) z (1 - z),
+// (cost and mismatch summed per block in a fixed order), and one forward
+// transform of (even + i odd) gives the derivative's band columns
+// nu = 0..Q, RT[img][y][nu] (the input of k_analysis_cols<..., HALF>).
+// Neither the image nor its derivative goes to HBM unless image / dI are
+// given.  Four-wave blocks, three per CU.
+constexpr int kObjRowWaves = 4;
+
+template <typename T, int N>
+__global__ __launch_bounds__(64 * kObjRowWaves, 3) void k_synth_objective_rows(
+    const cx<T>* __restrict__ TT, int Wb, T s, const cx<T>* __restrict__ twg,
+    const T* __restrict__ target, ResistPix<T> rp, T* __restrict__ image, T* __restrict__ dI,
+    cx<T>* __restrict__ RT, double* __restrict__ part_cost, int* __restrict__ part_err) {
+  constexpr int NW = kObjRowWaves, TPW = tpw<N>(), PL = padded<T, N>();
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  cx<T>* wb = tw + PL + wave * TPW * PL;
+  const int img = blockIdx.y;
+  load_roots_padded(tw, twg, N, tid, 64 * NW);
+  __syncthreads();
+  constexpr int R = Plan<N>::R0, S = N / R;
+  constexpr int NTR = N / 2, NU = (NTR + TPW - 1) / TPW;
+  constexpr int TASKS = TPW * S, TPL = (TASKS + 63) / 64;
+  using V = T __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int VK = 16 / (int)sizeof(T), NV = (2 * TPW * N / VK + 63) / 64;
+  const int Qh = (Wb - 1) / 2, ld = band_ld(Qh + 1);
+  const cx<T>* src = TT + (long long)img * N * ld;
+  const long long ibase = (long long)img * N * N;
+  int trs[TPL], plo[TPL], phi[TPL], jj[TPL];
+  bool vlo[TPL], vhi[TPL];
+#pragma unroll
+  for (int i = 0; i < TPL; ++i) {
+    const int t = lane + 64 * i;
+    const int tr = t / S, j = t - (t / S) * S;
+    const bool ok = t < TASKS;
+    trs[i] = ok ? tr : TPW;
+    jj[i] = j;
+    vlo[i] = ok && j <= Qh;      // nu = j >= 0: stored column j
+    vhi[i] = ok && S - j <= Qh;  // nu = j - S < 0: conjugate of stored column S - j
+    plo[i] = vlo[i] ? j : 0;
+    phi[i] = vhi[i] ? S - j : 0;
+  }
+  cx<T> a0[TPL], a1[TPL], b0[TPL], b1[TPL];
+  auto fetch = [&](int u) {  // any u < NU
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      const int p = min(u * TPW + min(trs[i], TPW - 1), NTR - 1);
+      const long long r0 = (long long)(2 * p) * ld;
+      a0[i] = src[r0 + plo[i]];
+      b0[i] = src[r0 + phi[i]];
+      a1[i] = src[r0 + ld + plo[i]];
+      b1[i] = src[r0 + ld + phi[i]];
+    }
+  };
+  cx<T> lo[TPL], hi[TPL];
+  auto take = [&]() {
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      const cx<T> l = cx<T>{a0[i].re - a1[i].im, a0[i].im + a1[i].re};
+      const cx<T> h = cx<T>{b0[i].re + b1[i].im, b1[i].re - b0[i].im};  // conj b0 + i conj b1
+      lo[i] = vlo[i] ? scale(l, s) : cx<T>{0, 0};
+      hi[i] = vhi[i] ? scale(h, s) : cx<T>{0, 0};
+    }
+  };
+  const int stride = gridDim.x * NW;
+  int u = blockIdx.x * NW + wave;
+  fetch(min(u, NU - 1));
+  take();
+  TailTw<T, +1, N, TPW, 0> rt;  // twiddles from the roots table: registers go to the rows
+  double cost = 0;
+  int err = 0;
+  T* rows = reinterpret_cast<T*>(wb);  // transform tr: rows 2 tr, 2 tr + 1 at tr * 2 PL
+  for (; u < NU; u += stride) {
+    fetch(min(u + stride, NU - 1));
+    const int ntr = NTR % TPW == 0 ? TPW : min(TPW, NTR - u * TPW);
+    const int nvu = 2 * ntr * N / VK;  // the unit's rows are contiguous in the image
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+      if (trs[i] < ntr) {
+        cx<T>* w = wb + trs[i] * PL;
+        cx<T>* w0 = w + pidx<T>(jj[i] * R);
+        constexpr bool LIN = (1 << kPadShift<T>) % R == 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const cx<T> val = lo[i] + mul_root<R, +1, T>(hi[i], (R - 1) * k);
+          if constexpr (LIN)
+            w0[k] = val;
+          else
+            w[pidx<T>(jj[i] * R + k)] = val;
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    auto put = [&](int tr, int x, cx<T> val) {
+      rows[tr * 2 * PL + x] = val.re;
+      rows[tr * 2 * PL + N + x] = val.im;
+    };
+    wave_fft_tail<T, +1, N, TPW, PL>(wb, tw, lane, ntr, put, rt);
+    // the unit's target rows
+    V tv[NV];
+    const V* tsrc = reinterpret_cast<const V*>(target + ibase + (long long)(2 * u * TPW) * N);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int e = lane + 64 * k;
+      tv[k] = e < nvu ? tsrc[e] : V{};
+    }
+    // the objective, in place: LDS row x of transform tr sits at tr * 2 PL + x,
+    // the image row at (2 tr + {0, 1}) N + x: vector e covers both the same way
+    // when each transform's two rows are whole vectors (2 N % VK == 0)
+    V* rv = reinterpret_cast<V*>(rows);
+    V* img_v = image ? reinterpret_cast<V*>(image + ibase + (long long)(2 * u * TPW) * N) : nullptr;
+    V* d_v = dI ? reinterpret_cast<V*>(dI + ibase + (long long)(2 * u * TPW) * N) : nullptr;
+    constexpr int VPT = 2 * N / VK;  // vectors per transform's row pair
+#pragma unroll 2  // fully unrolled, the loop needs about 240 registers (spills at three waves)
+    for (int k = 0; k < NV; ++k) {
+      const int e = lane + 64 * k;
+      if (e < nvu) {
+        const int tr = e / VPT, q = e - tr * VPT;
+        V x = rv[tr * (2 * PL / VK) + q];
+        if (img_v) __builtin_nontemporal_store(x, img_v + e);
+        V d;
+#pragma unroll
+        for (int c = 0; c < VK; ++c) {
+          const T t = tv[k][c];
+          const T v = rp.sign * (x[c] * rp.scale - rp.level);
+          const T z = sigmoid_clipped(rp.a * v);
+          const T r = z - t;
+          cost += (double)(r * r);
+          err += (v > (T)0) != (t > (T)0.5);
+          d[c] = rp.dscale * r * z * ((T)1 - z);
+        }
+        rv[tr * (2 * PL / VK) + q] = d;
+        if (d_v) __builtin_nontemporal_store(d, d_v + e);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // forward transform of (dI even + i dI odd), band columns nu = 0..Q
+    auto load = [&](int tr, int x) -> cx<T> {
+      return cx<T>{rows[tr * 2 * PL + x], rows[tr * 2 * PL + N + x]};
+    };
+    auto store = [&](int tr, int x, cx<T> val) { wb[tr * PL + pidx<T>(x)] = val; };
+    wave_fft<T, -1, N, TPW>(wb, tw, lane, ntr, load, store);
+    const int y0 = 2 * u * TPW;
+    cx<T>* out = RT + (long long)img * ld * N;
+    for (int t = lane; t < (Qh + 1) * 2 * ntr; t += 64) {
+      const int yy = t / (Qh + 1), nu = t - yy * (Qh + 1), tr = yy >> 1;
+      const cx<T> a = wb[tr * PL + pidx<T>(nu)];
+      const cx<T> b = conj(wb[tr * PL + pidx<T>(bin_of(-nu, N))]);
+      cx<T> r;
+      if ((yy & 1) == 0) {
+        r = scale(a + b, (T)0.5);
+      } else {
+        const cx<T> d = a - b;
+        r = cx<T>{d.im * (T)0.5, -d.re * (T)0.5};
+      }
+      out[(long long)(y0 + yy) * ld + nu] = r;
+    }
+    __builtin_amdgcn_wave_barrier();
+    take();
+  }
+  // block partials in a fixed order: lanes (shuffle tree), then waves
+  for (int o = 32; o > 0; o >>= 1) {
+    cost += __shfl_xor(cost, o, 64);
+    err += __shfl_xor(err, o, 64);
+  }
+  __syncthreads();  // every wave is done with its buffer
+  double* pc = reinterpret_cast<double*>(tw + PL);
+  int* pe = reinterpret_cast<int*>(pc + NW);
+  if (lane == 0) {
+    pc[wave] = cost;
+    pe[wave] = err;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double c = 0;
+    int e = 0;
+    for (int w = 0; w < NW; ++w) {
+      c += pc[w];
+      e += pe[w];
+    }
+    part_cost[(long long)img * gridDim.x + blockIdx.x] = c;
+    part_err[(long long)img * gridDim.x + blockIdx.x] = e;
+  }
+}
+
 // ------------------------------------------------------- coarse image band
 // One block per image: the sum of the image's coarse partials (the spread or
 // segment layout of the coherent-system pass) and its band spectrum
@@ -1166,7 +1359,10 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
           kSynthUnitsPerWave,
           std::max<long long>(1, (long long)units * nimg / (nw * kSynthWavesTarget)));
       const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
-      const bool staged = !cout && ((uintptr_t)out & 15) == 0;
+      // staging pays once the batch gives every wavefront its full share of
+      // units (a store-bound pass); small batches are latency-bound and skip
+      // the extra LDS round trip (16 images at 1024^2: 15 us direct, 20 staged)
+      const bool staged = !cout && ((uintptr_t)out & 15) == 0 && upw == kSynthUnitsPerWave;
       auto k2 = cout     ? k_synth_rows_sp<T, N, true>
                 : staged ? (half ? k_synth_rows_sp<T, N, false, true, true>
                                  : k_synth_rows_sp<T, N, false, false, true>)
@@ -1207,6 +1403,64 @@ static hipError_t coarse_band_n(const T* parts, int spread, int cnt, int spp, lo
   hipLaunchKernelGGL(k, dim3((unsigned)nimg), dim3(64 * kCoarseWaves), sm, st, parts, spread, cnt,
                      spp, total, blocks, segs, Q, s, twg, Y, peak);
   return hipGetLastError();
+}
+
+// Worst-case blocks per image of the fused synthesis-objective rows (one unit
+// per wave), for sizing the cost partials.
+template <int N>
+static int synth_objective_blocks() {
+  const int units = (N / 2 + tpw<N>() - 1) / tpw<N>();
+  return (units + kObjRowWaves - 1) / kObjRowWaves;
+}
+
+template <typename T, int N>
+static hipError_t synth_objective_n(const cx<T>* Y, int nimg, int nu0, int Wb, cx<T>* TT,
+                                    const T* target, ResistPix<T> rp, T* image, T* dI,
+                                    cx<T>* RT, cx<T>* GY, double* part_cost, int* part_err,
+                                    T* cost, T* total, int* mismatch, hipStream_t st) {
+  if constexpr (Plan<N>::R1 > 1) {
+    constexpr int S0 = N / Plan<N>::R0, VK = 16 / (int)sizeof(T);
+    const bool sp = nu0 > -S0 && nu0 + Wb - 1 < S0;
+    const bool half = Wb % 2 == 1 && 2 * nu0 + Wb - 1 == 0;
+    auto al = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+    if (!sp || !half || N % VK != 0 || !al(target) || (image && !al(image)) || (dI && !al(dI)))
+      return hipErrorNotSupported;
+    const cx<T>* twg = nullptr;
+    hipError_t e = table_for<T>(N, &twg);
+    if (e != hipSuccess) return e;
+    auto k1 = k_synth_cols<T, N, true, true>;
+    const size_t smc = col_smem_bytes<T, N>();
+    if ((e = set_smem(k1, smc)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k1, dim3(blocks_for((Wb + 1) / 2, cpb<T, N>()), nimg),
+                       dim3(64 * col_waves<T, N>()), smc, st, Y, nu0, Wb, 1, twg, TT);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int units = (N / 2 + tpw<N>() - 1) / tpw<N>();
+    const int upw = (int)std::min<long long>(
+        kSynthUnitsPerWave,
+        std::max<long long>(1, (long long)units * nimg / (kObjRowWaves * kSynthWavesTarget)));
+    const unsigned gx = (unsigned)((units + kObjRowWaves * upw - 1) / (kObjRowWaves * upw));
+    auto k2 = k_synth_objective_rows<T, N>;
+    const size_t sm = sizeof(cx<T>) * (size_t)padded<T, N>() * (1 + kObjRowWaves * tpw<N>());
+    if ((e = set_smem(k2, sm)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k2, dim3(gx, nimg), dim3(64 * kObjRowWaves), sm, st, (const cx<T>*)TT, Wb,
+                       (T)1, twg, target, rp, image, dI, RT, part_cost, part_err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = resist_final<T>(part_cost, part_err, nimg, (int)gx, (long long)N * N,
+                             part_cost + (long long)nimg * synth_objective_blocks<N>(), cost,
+                             total, mismatch, st)) != hipSuccess)
+      return e;
+    auto k3 = k_analysis_cols<T, N, true>;
+    if ((e = set_smem(k3, smc)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k3, dim3(blocks_for((Wb + 1) / 2, cpb<T, N>()), nimg),
+                       dim3(64 * col_waves<T, N>()), smc, st, (const cx<T>*)RT, nu0, Wb, (T)1, twg,
+                       GY);
+    return hipGetLastError();
+  } else {
+    (void)Y, (void)nimg, (void)nu0, (void)Wb, (void)TT, (void)target, (void)rp, (void)image,
+        (void)dI, (void)RT, (void)GY, (void)part_cost, (void)part_err, (void)cost, (void)total,
+        (void)mismatch, (void)st;
+    return hipErrorNotSupported;
+  }
 }
 
 #define LB_BAND_SIZES(X) \
@@ -1301,6 +1555,57 @@ template hipError_t coarse_band<float>(int, const float*, int, int, int, long lo
                                        int, float, cx<float>*, float*, hipStream_t);
 template hipError_t coarse_band<double>(int, const double*, int, int, int, long long, int, int,
                                         int, int, double, cx<double>*, double*, hipStream_t);
+bool synth_objective_supported(int N, int nu0, int Wb) {
+  if (!band_size_supported(N) || Wb > N || Wb % 2 == 0 || 2 * nu0 + Wb - 1 != 0) return false;
+  switch (N) {
+#define LB_CASE(n) \
+  case n:          \
+    return Plan<n>::R1 > 1 && nu0 > -(n / Plan<n>::R0) && nu0 + Wb - 1 < n / Plan<n>::R0;
+    LB_BAND_SIZES(LB_CASE)
+#undef LB_CASE
+    default:
+      return false;
+  }
+}
+
+int synth_objective_part_count(int N, int nimg) {
+  switch (N) {
+#define LB_CASE(n) \
+  case n:          \
+    return nimg * (synth_objective_blocks<n>() + 1);
+    LB_BAND_SIZES(LB_CASE)
+#undef LB_CASE
+    default:
+      return 0;
+  }
+}
+
+template <typename T>
+hipError_t synth_objective(int N, const cx<T>* Y, int nimg, int nu0, int Wb, cx<T>* TT,
+                           const T* target, ResistPix<T> rp, T* image, T* dI, cx<T>* RT,
+                           cx<T>* GY, double* part_cost, int* part_err, T* cost, T* total,
+                           int* mismatch, hipStream_t st) {
+  if (!band_size_supported(N) || Wb > N) return hipErrorNotSupported;
+  switch (N) {
+#define LB_CASE(n)                                                                            \
+  case n:                                                                                     \
+    return synth_objective_n<T, n>(Y, nimg, nu0, Wb, TT, target, rp, image, dI, RT, GY,       \
+                                   part_cost, part_err, cost, total, mismatch, st);
+    LB_BAND_SIZES(LB_CASE)
+#undef LB_CASE
+    default:
+      return hipErrorNotSupported;
+  }
+}
+
+template hipError_t synth_objective<float>(int, const cx<float>*, int, int, int, cx<float>*,
+                                           const float*, ResistPix<float>, float*, float*,
+                                           cx<float>*, cx<float>*, double*, int*, float*, float*,
+                                           int*, hipStream_t);
+template hipError_t synth_objective<double>(int, const cx<double>*, int, int, int, cx<double>*,
+                                            const double*, ResistPix<double>, double*, double*,
+                                            cx<double>*, cx<double>*, double*, int*, double*,
+                                            double*, int*, hipStream_t);
 template hipError_t band_analysis_sigmoid<float>(int, const float*, int, int, int, float, float,
                                                  cx<float>*, cx<float>*, hipStream_t);
 template hipError_t band_analysis_sigmoid<double>(int, const double*, int, int, int, double,

@@ -167,6 +167,36 @@ hipError_t spread_sum(const T* in, int groups, int ipg, int spp, long long total
 template <typename T>
 hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, T* peak,
                        T* scratch, hipStream_t st);
+// Per-pixel relaxed-resist objective (opc/ilt.py:311-330): v = sign (scale I -
+// level), z = sigmoid(a v), cost (z - t)^2, dI = dscale (z - t) z (1 - z) with
+// dscale = sign 2 a scale / npix.
+template <typename T>
+struct ResistPix {
+  T scale, level, a, sign, dscale;
+};
+// The deterministic reduction of resist_objective's block partials (part_cost,
+// part_err [n_img][nblk]) into per-image costs (img_cost: n_img doubles of
+// scratch) and the optional outputs.
+template <typename T>
+hipError_t resist_final(const double* part_cost, const int* part_err, int n_img, int nblk,
+                        long long npix, double* img_cost, T* cost, T* total, int* mismatch,
+                        hipStream_t st);
+// Real-output band synthesis (band_synthesis with herm = 1, s = 1) fused with
+// the relaxed-resist objective and the band analysis (nu0, Wb, s = 1) of the
+// intensity derivative: GY[img][u][v] as band_analysis of dI would give it,
+// per-image costs (cost), their sum (total) and mismatch counts; image and dI
+// (nullable) receive the intensity and its derivative.  TT and RT are
+// scratch of band_ld((Wb + 1) / 2) * N complex per image; part_cost holds
+// synth_objective_part_count(N, nimg) doubles and part_err as many ints.
+// hipErrorNotSupported, nothing launched, unless the band is symmetric about
+// 0 inside |nu| < N / 16 and target, image and dI are 16-byte aligned.
+int synth_objective_part_count(int N, int nimg);
+bool synth_objective_supported(int N, int nu0, int Wb);  // the band conditions above
+template <typename T>
+hipError_t synth_objective(int N, const cx<T>* Y, int nimg, int nu0, int Wb, cx<T>* TT,
+                           const T* target, ResistPix<T> rp, T* image, T* dI, cx<T>* RT,
+                           cx<T>* GY, double* part_cost, int* part_err, T* cost, T* total,
+                           int* mismatch, hipStream_t st);
 // The coarse partials of each image (spread layout when `spread`, else `cnt`
 // consecutive partials per image) summed and taken to their band spectrum on
 // the M x M grid in one pass: Y[img][u][v] = s * sum_{y,x} I[y][x]

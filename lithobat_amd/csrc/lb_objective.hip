@@ -159,11 +159,22 @@ hipError_t resist_objective(const T* f, const T* target, int n_img, long long np
                      level, a, sign, d_f, signed_f, part_cost, part_err);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  return resist_final<T>(part_cost, part_err, n_img, nblk, npix,
+                         part_cost + (long long)n_img * nblk, cost, total, mismatch, st);
+}
+
+template <typename T>
+hipError_t resist_final(const double* part_cost, const int* part_err, int n_img, int nblk,
+                        long long npix, double* img_cost, T* cost, T* total, int* mismatch,
+                        hipStream_t st) {
   hipLaunchKernelGGL(k_resist_final<T>, dim3(1), dim3(64 * kFinalWaves), 0, st, part_cost,
-                     part_err, n_img, nblk, npix, part_cost + (long long)n_img * nblk, cost, total,
-                     mismatch);
+                     part_err, n_img, nblk, npix, img_cost, cost, total, mismatch);
   return hipGetLastError();
 }
+template hipError_t resist_final<float>(const double*, const int*, int, int, long long, double*,
+                                        float*, float*, int*, hipStream_t);
+template hipError_t resist_final<double>(const double*, const int*, int, int, long long, double*,
+                                         double*, double*, int*, hipStream_t);
 
 // pbeta (device, optional): beta read from device memory instead of the
 // argument, so a captured graph can follow a schedule that changes per call.

@@ -86,28 +86,39 @@ class ILTObjective(torch.autograd.Function):
     and blurs), summed over images, of the image of the mask sigmoid(clip(beta theta,
     +-60)) (ilt.py:368-378): the value and gradient of
     ``relaxed_resist_loss(abbe_intensity(mask_of(theta, beta), plan), target, ...)``
-    in one function.  The sigmoid runs inside the masks' band analysis; the objective
-    pass gives the intensity derivative; the backward runs the recomputing adjoint from
-    the kept band spectra, then the sigmoid derivative recomputed from theta and scaled by
-    the incoming gradient, read on the device.  No mask array is written and no pass
-    scales the intensity derivative (the adjoint between them is linear)."""
+    in one function.  The sigmoid runs inside the masks' band analysis; the objective is
+    evaluated where the images are synthesised, which hands the backward the band spectrum
+    of the intensity derivative (plans without that pass write the images and run the
+    objective pass on them); the backward runs the recomputing adjoint from the kept band
+    spectra, then the sigmoid derivative recomputed from theta and scaled by the incoming
+    gradient, read on the device.  No mask, image or intensity-derivative array is
+    written and no pass scales the derivative (the adjoint between them is linear)."""
 
     @staticmethod
     def forward(ctx, theta, plan, target, scale, level, steepness, beta, sign):
         spec = plan.mask_spectrum_sigmoid(theta, beta)
-        raw, _ = plan.forward_spectrum(spec, with_peak=False)
-        t = target if target.dim() < 3 else target.reshape(target.shape[0], 1,
-                                                           *target.shape[-2:])
-        r = resist_objective(raw, t, scale, level, steepness, sign, grad=True)
         ctx.plan, ctx.beta, ctx.dtype = plan, float(beta), theta.dtype
         ctx.squeeze = theta.dim() == 2
+        t = target if target.dim() < 3 else target.reshape(target.shape[0], 1,
+                                                           *target.shape[-2:])
+        # the objective where the images are synthesised: the backward then starts from the
+        # band spectrum of the intensity derivative (neither array is written)
+        fused = plan.forward_objective(spec, t, scale, level, steepness, sign)
+        ctx.band = fused is not None
+        if ctx.band:
+            total, _, sens = fused
+            ctx.save_for_backward(spec, sens, theta)
+            return total
+        raw, _ = plan.forward_spectrum(spec, with_peak=False)
+        r = resist_objective(raw, t, scale, level, steepness, sign, grad=True)
         ctx.save_for_backward(spec, r["d_intensity"], theta)
         return r["total"]
 
     @staticmethod
     def backward(ctx, g):
         spec, d, theta = ctx.saved_tensors
-        dm = ctx.plan.backward_spectrum(spec, d)
+        dm = (ctx.plan.backward_sensitivity_band(spec, d) if ctx.band
+              else ctx.plan.backward_spectrum(spec, d))
         th = theta[None] if ctx.squeeze else theta
         dth = mask_sigmoid_grad_theta(dm, th.to(dm.dtype), ctx.beta, g)
         return ((dth[0] if ctx.squeeze else dth).to(ctx.dtype), None, None, None, None, None,

@@ -353,6 +353,56 @@ class ImagingPlan:
                           ws.data_ptr(), ws.numel(), st))
         return out
 
+    def forward_objective(self, spec: torch.Tensor, target: torch.Tensor, scale: float,
+                          level: float, steepness: float, sign: float = 1.0):
+        """The relaxed-resist objective (``objective.resist_objective``) of the images of band
+        spectra ``spec`` against ``target`` ([N, N], [B, N, N] or [B, P, N, N]), evaluated where the
+        images are synthesised (lb_forward_objective): (total (0-d), cost [B, P],
+        sens_band), sens_band being the band spectrum of the intensity derivative that
+        ``backward_sensitivity_band`` takes.  None when the plan has no such pass (an image
+        grid used directly, or an image band wider than N / 8)."""
+        B = self._check_spectrum(spec)
+        P, n = len(self.planes), self.n
+        t = target.to(self.real_dtype)
+        t = t.reshape(-1 if t.dim() > 2 else 1, 1 if t.dim() < 4 else t.shape[1], n, n)
+        t = t.expand(B, P, n, n).contiguous()
+        cost = torch.empty((B, P), dtype=self.real_dtype, device=spec.device)
+        total = torch.empty((), dtype=self.real_dtype, device=spec.device)
+        with self._lock:
+            self._use_tuning(B)
+            nb = int(self._lib.lb_forward_objective_workspace_bytes(self._h, B))
+            sb = int(self._lib.lb_sensitivity_band_bytes(self._h, B))
+            if nb == 0 or sb == 0 or t.data_ptr() % 16:
+                return None
+            wb = int(round((sb // (B * P * spec.element_size())) ** 0.5))
+            sens = torch.empty((B, P, wb, wb), dtype=self.complex_dtype, device=spec.device)
+            ws = self._workspace(nb, spec.device)
+            st = torch.cuda.current_stream(spec.device).cuda_stream
+            rc = self._lib.lb_forward_objective(
+                self._h, spec.data_ptr(), B, t.data_ptr(), float(scale), float(level),
+                float(steepness), float(sign), cost.data_ptr(), total.data_ptr(), None,
+                sens.data_ptr(), None, None, ws.data_ptr(), ws.numel(), st)
+            if rc == _lib.LB_ERR_UNSUPPORTED:
+                return None
+            _lib.check(rc)
+        return total, cost, sens
+
+    def backward_sensitivity_band(self, spec: torch.Tensor, sens: torch.Tensor) -> torch.Tensor:
+        """``backward_spectrum`` from the band spectrum of the intensity derivative
+        (``forward_objective``'s sens_band) instead of the derivative itself: real [B, N, N]."""
+        B = self._check_spectrum(spec)
+        if sens.dtype != self.complex_dtype or not sens.is_contiguous() or sens.shape[0] != B:
+            raise ValueError("sens must be forward_objective's sensitivity band spectrum")
+        out = torch.empty((B, self.n, self.n), dtype=self.real_dtype, device=spec.device)
+        with self._lock:
+            self._use_tuning(B)
+            ws = self._workspace(self.workspace_bytes(B, True), spec.device)
+            st = torch.cuda.current_stream(spec.device).cuda_stream
+            _lib.check(self._lib.lb_backward_sensitivity_band(
+                self._h, spec.data_ptr(), B, sens.data_ptr(), out.data_ptr(), ws.data_ptr(),
+                ws.numel(), st))
+        return out
+
     def fields_backward(self, fields: torch.Tensor, sensitivity: torch.Tensor,
                         complex_out: bool = False) -> torch.Tensor:
         """SOCS plans: socs_gradient on the fields given, [K, N, N] or [B, K, N, N] complex,

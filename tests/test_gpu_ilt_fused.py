@@ -6,7 +6,10 @@ test_gpu_cfg4.py / test_gpu_ilt.py pin to the oracle and to central differences
 (opc/ilt.py:311-389).
 
 The loss is scaled by 2.5 before the backward, so the incoming gradient is not 1.  n = 100
-has no fused analysis row pass and takes the sigmoid-then-analysis route.  Tolerances: FP64
+has no fused analysis row pass and takes the sigmoid-then-analysis route; n = 512 at 4 nm
+(image band inside |nu| < N/16) evaluates the objective where the images are synthesised
+and runs the adjoint from the band spectrum of the intensity derivative, n = 128 at 8 nm
+(band too wide for it) writes the images and runs the objective pass.  Tolerances: FP64
 <= 1e-12 (value) / 1e-10 (gradient, relative to its largest entry), FP32 <= 1e-6 / 2e-5.
 """
 import numpy as np
@@ -20,24 +23,29 @@ def _rel(a, b):
     return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300))
 
 
-@pytest.mark.parametrize("precision,n,tol_v,tol_g", [(64, 128, 1e-12, 1e-10),
-                                                     (32, 128, 1e-6, 2e-5),
-                                                     (64, 100, 1e-12, 1e-10)])
-def test_fused_objective_matches_the_composed_path(device, precision, n, tol_v, tol_g):
+@pytest.mark.parametrize("precision,n,px,tol_v,tol_g", [(64, 128, 8e-9, 1e-12, 1e-10),
+                                                        (32, 128, 8e-9, 1e-6, 2e-5),
+                                                        (64, 100, 8e-9, 1e-12, 1e-10),
+                                                        (64, 512, 4e-9, 1e-12, 1e-10),
+                                                        (32, 512, 4e-9, 1e-6, 2e-5)])
+def test_fused_objective_matches_the_composed_path(device, precision, n, px, tol_v, tol_g):
     import torch
     from lithobat_amd.config import GridConfig, OpticsConfig
     from lithobat_amd.ilt import abbe_intensity, ilt_objective, mask_of, relaxed_resist_loss
     from lithobat_amd.plan import ImagingPlan
-    print(f"fused ILT objective FP{precision} n={n}", flush=True)
+    print(f"fused ILT objective FP{precision} n={n} px={px}", flush=True)
     optics = OpticsConfig(wavelength=193e-9, NA=1.35, n_immersion=1.44, source_type="quadrupole",
                           sigma_outer=0.85, sigma_inner=0.55, source_grid=15)
-    plan = ImagingPlan(optics, GridConfig(n_pixels=n, pixel_size=8e-9), precision=precision)
+    plan = ImagingPlan(optics, GridConfig(n_pixels=n, pixel_size=px), precision=precision)
     dt = plan.real_dtype
     rng = np.random.default_rng(40 + n)
     theta = torch.tensor(rng.standard_normal((3, n, n)), device=device, dtype=dt)
     target = torch.tensor((rng.random((3, n, n)) > 0.6).astype(np.float64), device=device,
                           dtype=dt)
     scale = 1.0 / float(plan.clear[0])
+    # which objective pass runs: the one inside the synthesis exactly when the band allows
+    spec = plan.mask_spectrum_sigmoid(theta, 8.0)
+    assert (plan.forward_objective(spec, target, scale, 0.3, 50.0) is not None) == (n == 512)
     th1 = theta.clone().requires_grad_(True)
     loss1 = relaxed_resist_loss(abbe_intensity(mask_of(th1, 8.0), plan), target[:, None], scale,
                                 0.3, 50.0)
