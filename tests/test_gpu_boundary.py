@@ -287,3 +287,54 @@ def test_dense_pupil_tilts_the_mask_the_other_way(device):
         return np.abs(np.fft.ifft2(np.fft.fft2(mask * t) * on_axis)) ** 2
     assert np.abs(eng - imaged(-1)).max() < 1e-12 * eng.max()
     assert np.abs(eng - imaged(+1)).max() > 1e-3 * eng.max()
+
+
+def test_objective_entry_points_refuse_what_they_cannot_run(device):
+    """lb_forward_objective: LB_ERR_UNSUPPORTED where the image band reaches |nu| >= N/16
+    (64^2 at 8 nm), for a target that is not 16-byte aligned, LB_ERR_WORKSPACE below its
+    workspace size (zero when unsupported); lb_backward_sensitivity_band refuses a plan
+    imaged directly on its own grid.  Each refusal leaves a message and launches nothing."""
+    import torch
+    from lithobat_amd import _lib
+    from lithobat_amd.config import GridConfig, OpticsConfig
+    from lithobat_amd.plan import ImagingPlan
+    lib = _lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    optics, grid = _small()
+    small = ImagingPlan(optics, grid, precision=32)
+    spec = small.mask_spectrum(torch.ones((1, 64, 64), device=device))
+    tgt = torch.zeros((1, 64, 64), device=device)
+    sb = torch.empty((1, 64, 64), dtype=torch.complex64, device=device)
+    ws = torch.empty(1 << 20, dtype=torch.uint8, device=device)
+    assert lib.lb_forward_objective_workspace_bytes(small._h, 1) == 0
+    assert lib.lb_forward_objective(small._h, spec.data_ptr(), 1, tgt.data_ptr(), 1.0, 0.3, 50.0,
+                                    1.0, None, None, None, sb.data_ptr(), None, None,
+                                    ws.data_ptr(), ws.numel(), st) == _lib.LB_ERR_UNSUPPORTED
+    assert "N/16" in lib.lb_last_error().decode()
+    assert small.forward_objective(spec, tgt, 1.0, 0.3, 50.0) is None
+    if small.info["m"] == small.info["n"]:
+        assert lib.lb_backward_sensitivity_band(small._h, spec.data_ptr(), 1, sb.data_ptr(),
+                                                tgt.data_ptr(), ws.data_ptr(), ws.numel(),
+                                                st) == _lib.LB_ERR_UNSUPPORTED
+    o = OpticsConfig(NA=1.35, n_immersion=1.44, source_type="quadrupole", sigma_outer=0.85,
+                     sigma_inner=0.55, source_grid=9)
+    plan = ImagingPlan(o, GridConfig(n_pixels=512, pixel_size=4e-9), precision=32)
+    spec = plan.mask_spectrum(torch.ones((1, 512, 512), device=device))
+    need = int(lib.lb_forward_objective_workspace_bytes(plan._h, 1))
+    assert need > 0
+    wb = 2 * plan.info["box"] - 1
+    assert lib.lb_sensitivity_band_bytes(plan._h, 1) == 8 * wb * wb
+    sb = torch.empty((1, wb, wb), dtype=torch.complex64, device=device)
+    tgt = torch.zeros(512 * 512 + 4, device=device)
+    big = torch.empty(need, dtype=torch.uint8, device=device)
+    args = (1.0, 0.3, 50.0, 1.0, None, None, None, sb.data_ptr(), None, None)
+    assert lib.lb_forward_objective(plan._h, spec.data_ptr(), 1, tgt.data_ptr() + 4, *args,
+                                    big.data_ptr(), need, st) == _lib.LB_ERR_UNSUPPORTED
+    assert "aligned" in lib.lb_last_error().decode()
+    assert lib.lb_forward_objective(plan._h, spec.data_ptr(), 1, tgt.data_ptr(), *args,
+                                    big.data_ptr(), need - 256, st) == _lib.LB_ERR_WORKSPACE
+    assert str(need) in lib.lb_last_error().decode()
+    assert lib.lb_forward_objective(plan._h, spec.data_ptr(), 1, tgt.data_ptr(), *args,
+                                    big.data_ptr(), need, st) == _lib.LB_OK
+    torch.cuda.synchronize()
+    assert torch.isfinite(torch.view_as_real(sb)).all()
