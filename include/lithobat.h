@@ -374,6 +374,18 @@ lb_status lb_forward_objective(lb_plan plan, const void* spectrum, int32_t n_mas
 lb_status lb_backward_sensitivity_band(lb_plan plan, const void* spectrum, int32_t n_masks,
                                        const void* sens_band, void* d_masks, void* workspace,
                                        size_t workspace_bytes, void* stream);
+/* lb_backward_sensitivity_band followed by lb_mask_sigmoid_grad_theta in one
+ * pass: the gradient of the ILT parameter theta [n_masks][N][N] (the mask is
+ * sigmoid(clip(beta theta, +-60)), opc/ilt.py:368-389 without the Gaussian),
+ * grad_theta = d/dm (beta m (1 - m) g), g a scalar in device memory (null: 1),
+ * written by the mask-gradient synthesis itself.  LB_ERR_UNSUPPORTED for a
+ * plan imaged directly, a mask band reaching |nu| >= N / 16, or theta /
+ * grad_theta not 16-byte aligned (then the two calls give the same result).
+ * Workspace as lb_backward_spectrum. */
+lb_status lb_backward_objective_theta(lb_plan plan, const void* spectrum, int32_t n_masks,
+                                      const void* sens_band, const void* theta, double beta,
+                                      const void* g, void* grad_theta, void* workspace,
+                                      size_t workspace_bytes, void* stream);
 
 /* No counterpart in the reference: the coherent-system kernel shape of a
  * batch's forward or adjoint launch (single-launch shapes only; the work

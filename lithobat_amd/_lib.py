@@ -65,6 +65,7 @@ EXPORTS = [
     "lb_backward_spectrum", "lb_batch_backward_reuse", "lb_mask_spectrum_sigmoid",
     "lb_mask_sigmoid_grad_theta", "lb_forward_objective_workspace_bytes",
     "lb_sensitivity_band_bytes", "lb_forward_objective", "lb_backward_sensitivity_band",
+    "lb_backward_objective_theta",
 ]
 
 _lib = None
@@ -145,6 +146,8 @@ def load() -> C.CDLL:
         "lb_forward_objective": ([vp, vp, i32, vp, C.c_double, C.c_double, C.c_double,
                                   C.c_double, vp, vp, vp, vp, vp, vp, vp, sz, vp], C.c_int),
         "lb_backward_sensitivity_band": ([vp, vp, i32, vp, vp, vp, sz, vp], C.c_int),
+        "lb_backward_objective_theta": ([vp, vp, i32, vp, vp, C.c_double, vp, vp, vp, sz, vp],
+                                        C.c_int),
         "lb_batch_backward_reuse": ([vp, C.POINTER(vp), C.POINTER(vp), vp, sz, vp, vp], C.c_int),
         "lb_batch_workspace_bytes": ([vp, i32], sz),
         "lb_batch_forward": ([vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, sz, vp],

@@ -639,9 +639,11 @@ lb_status bwd_pre(Plan& p, const void* masks, int n_masks, const void* dimg, cha
 // Backward, phase 3: sum the band-gradient partials per mask and synthesise
 // the real mask gradient.
 // cout: the complex gradient d/dRe + i d/dIm (complex masks) instead of its real part.
+// te: write d/dtheta of the ILT parameter instead of the mask gradient
+// (band_synthesis_theta; real masks over a symmetric band inside N/16).
 template <typename T>
 lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, hipStream_t st,
-                   bool cout = false) {
+                   bool cout = false, const ThetaEpi<T>* te = nullptr) {
   const T invN2 = (T)(1.0 / ((double)p.N * (double)p.N));
   cx<T>* GX = (cx<T>*)(ws + w.GX);
   if (w.spread)
@@ -651,15 +653,20 @@ lb_status bwd_post(Plan& p, int n_masks, void* dmask, char* ws, const Work& w, h
   else
     LB_HIP(segment_sum<T>((const T*)(ws + w.gX), n_masks, p.n_planes * w.n_chunks * w.nrc,
                           2LL * p.Wbm * p.Wbm, (T*)GX, (T*)nullptr, (T*)(ws + w.seg), st));
-  LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, cout ? 0 : 1, invN2, cout ? 1 : 0,
-                           (cx<T>*)(ws + w.TTm), (T*)dmask, (T*)nullptr, st));
+  if (te)
+    LB_HIP(band_synthesis_theta<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, invN2, (cx<T>*)(ws + w.TTm),
+                                   *te, (T*)dmask, st));
+  else
+    LB_HIP(band_synthesis<T>(p.N, GX, n_masks, p.nu0m, p.Wbm, cout ? 0 : 1, invN2, cout ? 1 : 0,
+                             (cx<T>*)(ws + w.TTm), (T*)dmask, (T*)nullptr, st));
   return LB_OK;
 }
 
 template <typename T>
 lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, void* dmask,
                      char* ws, const Work& w, hipStream_t st, const void* spectrum = nullptr,
-                     bool cout = false, const void* gy = nullptr) {
+                     bool cout = false, const void* gy = nullptr,
+                     const ThetaEpi<T>* te = nullptr) {
   const T* gM = nullptr;
   lb_status rc = bwd_pre<T>(p, masks, n_masks, dimg, ws, w, &gM, st, spectrum != nullptr, gy);
   if (rc != LB_OK) return rc;
@@ -681,7 +688,7 @@ lb_status backward_t(Plan& p, const void* masks, int n_masks, const void* dimg, 
     LB_HIP(field_adjoint(p.precision, fc, a, gM, ws + w.gX, st));
   }
   if (p.profile) LB_HIP(hipEventRecord(p.ev_pool[e1], st));
-  return bwd_post<T>(p, n_masks, dmask, ws, w, st, cout);
+  return bwd_post<T>(p, n_masks, dmask, ws, w, st, cout, te);
 }
 
 // mask band covering every box: symmetric [-Q, Q], or the whole grid
@@ -1558,6 +1565,35 @@ lb_status lb_backward_sensitivity_band(lb_plan plan, const void* spectrum, int32
                                   spectrum, false, sens_band)
              : backward_t<float>(p, nullptr, n_masks, nullptr, dmask, (char*)workspace, w, st,
                                  spectrum, false, sens_band);
+}
+
+lb_status lb_backward_objective_theta(lb_plan plan, const void* spectrum, int32_t n_masks,
+                                      const void* sens_band, const void* theta, double beta,
+                                      const void* g, void* grad_theta, void* workspace,
+                                      size_t ws_bytes, void* stream) {
+  g_err.clear();
+  if (!plan || !spectrum || !sens_band || !theta || !grad_theta)
+    return fail(LB_ERR_ARG, "null argument");
+  if (n_masks < 1) return fail(LB_ERR_ARG, "n_masks must be >= 1");
+  Plan& p = plan->p;
+  if (p.spp < 1) return fail(LB_ERR_ARG, "plan has no coherent systems (empty source)");
+  if (p.direct || !synth_objective_supported(p.N, p.nu0m, p.Wbm))
+    return fail(LB_ERR_UNSUPPORTED,
+                "the theta gradient pass needs a coarse grid and a mask band inside |nu| < N/16");
+  if (((uintptr_t)theta & 15) || ((uintptr_t)grad_theta & 15))
+    return fail(LB_ERR_UNSUPPORTED, "theta and grad_theta must be 16-byte aligned");
+  Work w = layout(p, n_masks, true);
+  if (!workspace || ws_bytes < w.total)
+    return fail(LB_ERR_WORKSPACE, "workspace needs " + std::to_string(w.total) + " bytes");
+  hipStream_t st = (hipStream_t)stream;
+  if (p.precision == 64) {
+    const ThetaEpi<double> te{(const double*)theta, beta, (const double*)g};
+    return backward_t<double>(p, nullptr, n_masks, nullptr, grad_theta, (char*)workspace, w, st,
+                              spectrum, false, sens_band, &te);
+  }
+  const ThetaEpi<float> te{(const float*)theta, (float)beta, (const float*)g};
+  return backward_t<float>(p, nullptr, n_masks, nullptr, grad_theta, (char*)workspace, w, st,
+                           spectrum, false, sens_band, &te);
 }
 
 lb_status lb_backward_spectrum_complex(lb_plan plan, const void* spectrum, int32_t n_masks,

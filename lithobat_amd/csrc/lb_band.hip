@@ -740,13 +740,20 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
 // the wave's LDS buffer as the two image rows and are written out in 16-byte
 // runs, four times fewer store instructions than the stage's own outputs,
 // which lie N / R apart.
-template <typename T, int N, bool COUT, bool HALF = false, bool STAGED = false>
+// EPI (staged): the rows are a mask gradient d/dm and leave as the gradient of
+// the ILT parameter, d/dtheta = d/dm (beta m (1 - m) g) with m =
+// sigmoid(clip(beta theta, +-60)) recomputed from te.theta (16-byte aligned,
+// like out) and g = *te.g (1 when null): lb_mask_sigmoid_grad_theta's pass
+// fused into the synthesis.
+template <typename T, int N, bool COUT, bool HALF = false, bool STAGED = false, bool EPI = false>
 __global__ __launch_bounds__((64 * row_waves<T, N>()),
                              (row_waves_per_simd<T, N>())) void k_synth_rows_sp(
     const cx<T>* __restrict__ TT, int nu0, int Wb, T s, const cx<T>* __restrict__ twg,
-    T* __restrict__ out, T* __restrict__ peak) {
+    T* __restrict__ out, T* __restrict__ peak, ThetaEpi<T> te) {
   static_assert(!((HALF || STAGED) && COUT), "the half band and staged rows are for real output");
+  static_assert(!EPI || STAGED, "the theta epilogue works on staged rows");
   LB_BAND_PROLOGUE(T, N)
+  const T gsc = EPI && te.g ? te.g[0] : (T)1;
   constexpr int R = Plan<N>::R0, S = N / R;
   constexpr int NTR = COUT ? N : N / 2;
   constexpr int NU = (NTR + TPW - 1) / TPW;
@@ -853,9 +860,25 @@ __global__ __launch_bounds__((64 * row_waves<T, N>()),
       constexpr int K = 16 / (int)sizeof(T), NV = 2 * N / K;
       for (int tr = 0; tr < ntr; ++tr) {
         const V* src = reinterpret_cast<const V*>(rows + tr * 2 * PL);
-        V* dst = reinterpret_cast<V*>(out + (long long)img * N * N + (long long)(2 * (p0 + tr)) * N);
+        const long long r0 = (long long)img * N * N + (long long)(2 * (p0 + tr)) * N;
+        V* dst = reinterpret_cast<V*>(out + r0);
+        if constexpr (EPI) {
+          const V* th = reinterpret_cast<const V*>(te.theta + r0);
 #pragma unroll
-        for (int k = lane; k < NV; k += 64) __builtin_nontemporal_store(src[k], dst + k);
+          for (int k = lane; k < NV; k += 64) {
+            V d = src[k];
+            const V x = th[k];
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+              const T m = sigmoid_clipped(te.beta * x[c]);
+              d[c] *= te.beta * m * ((T)1 - m) * gsc;
+            }
+            __builtin_nontemporal_store(d, dst + k);
+          }
+        } else {
+#pragma unroll
+          for (int k = lane; k < NV; k += 64) __builtin_nontemporal_store(src[k], dst + k);
+        }
       }
       __builtin_amdgcn_wave_barrier();
     } else {
@@ -1369,7 +1392,7 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
                          : (half ? k_synth_rows_sp<T, N, false, true> : k_synth_rows_sp<T, N, false>);
       if ((e = set_smem(k2, sm)) != hipSuccess) return e;
       hipLaunchKernelGGL(k2, dim3(gx, nimg), dim3(NT), sm, st, (const cx<T>*)TT, nu0, Wb, s, twg,
-                         out, cout ? (T*)nullptr : peak);
+                         out, cout ? (T*)nullptr : peak, ThetaEpi<T>{});
       return hipGetLastError();
     }
   }
@@ -1403,6 +1426,46 @@ static hipError_t coarse_band_n(const T* parts, int spread, int cnt, int spp, lo
   hipLaunchKernelGGL(k, dim3((unsigned)nimg), dim3(64 * kCoarseWaves), sm, st, parts, spread, cnt,
                      spp, total, blocks, segs, Q, s, twg, Y, peak);
   return hipGetLastError();
+}
+
+// Real-output synthesis of a mask gradient over a symmetric half band with the
+// theta epilogue (k_synth_rows_sp<..., EPI>).
+template <typename T, int N>
+static hipError_t synthesis_theta_n(const cx<T>* Y, int nimg, int nu0, int Wb, T s, cx<T>* TT,
+                                    ThetaEpi<T> te, T* grad_theta, hipStream_t st) {
+  if constexpr (Plan<N>::R1 > 1) {
+    constexpr int S0 = N / Plan<N>::R0, VK = 16 / (int)sizeof(T);
+    const bool sp = nu0 > -S0 && nu0 + Wb - 1 < S0;
+    const bool half = Wb % 2 == 1 && 2 * nu0 + Wb - 1 == 0;
+    auto al = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+    if (!sp || !half || N % VK != 0 || !al(te.theta) || !al(grad_theta))
+      return hipErrorNotSupported;
+    const cx<T>* twg = nullptr;
+    hipError_t e = table_for<T>(N, &twg);
+    if (e != hipSuccess) return e;
+    auto k1 = k_synth_cols<T, N, true, true>;
+    const size_t smc = col_smem_bytes<T, N>();
+    if ((e = set_smem(k1, smc)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k1, dim3(blocks_for((Wb + 1) / 2, cpb<T, N>()), nimg),
+                       dim3(64 * col_waves<T, N>()), smc, st, Y, nu0, Wb, 1, twg, TT);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int units = (N / 2 + tpw<N>() - 1) / tpw<N>();
+    const int nw = row_waves<T, N>();
+    const int upw = (int)std::min<long long>(
+        kSynthUnitsPerWave,
+        std::max<long long>(1, (long long)units * nimg / (nw * kSynthWavesTarget)));
+    const unsigned gx = (unsigned)((units + nw * upw - 1) / (nw * upw));
+    auto k2 = k_synth_rows_sp<T, N, false, true, true, true>;
+    const size_t sm = smem_bytes<T, N>();
+    if ((e = set_smem(k2, sm)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k2, dim3(gx, nimg), dim3(64 * nw), sm, st, (const cx<T>*)TT, nu0, Wb, s,
+                       twg, grad_theta, (T*)nullptr, te);
+    return hipGetLastError();
+  } else {
+    (void)Y, (void)nimg, (void)nu0, (void)Wb, (void)s, (void)TT, (void)te, (void)grad_theta,
+        (void)st;
+    return hipErrorNotSupported;
+  }
 }
 
 // Worst-case blocks per image of the fused synthesis-objective rows (one unit
@@ -1555,6 +1618,27 @@ template hipError_t coarse_band<float>(int, const float*, int, int, int, long lo
                                        int, float, cx<float>*, float*, hipStream_t);
 template hipError_t coarse_band<double>(int, const double*, int, int, int, long long, int, int,
                                         int, int, double, cx<double>*, double*, hipStream_t);
+template <typename T>
+hipError_t band_synthesis_theta(int N, const cx<T>* Y, int nimg, int nu0, int Wb, T s, cx<T>* TT,
+                                ThetaEpi<T> te, T* grad_theta, hipStream_t st) {
+  if (!band_size_supported(N) || Wb > N) return hipErrorNotSupported;
+  switch (N) {
+#define LB_CASE(n) \
+  case n:          \
+    return synthesis_theta_n<T, n>(Y, nimg, nu0, Wb, s, TT, te, grad_theta, st);
+    LB_BAND_SIZES(LB_CASE)
+#undef LB_CASE
+    default:
+      return hipErrorNotSupported;
+  }
+}
+
+template hipError_t band_synthesis_theta<float>(int, const cx<float>*, int, int, int, float,
+                                                cx<float>*, ThetaEpi<float>, float*, hipStream_t);
+template hipError_t band_synthesis_theta<double>(int, const cx<double>*, int, int, int, double,
+                                                 cx<double>*, ThetaEpi<double>, double*,
+                                                 hipStream_t);
+
 bool synth_objective_supported(int N, int nu0, int Wb) {
   if (!band_size_supported(N) || Wb > N || Wb % 2 == 0 || 2 * nu0 + Wb - 1 != 0) return false;
   switch (N) {

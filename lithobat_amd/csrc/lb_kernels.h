@@ -167,6 +167,22 @@ hipError_t spread_sum(const T* in, int groups, int ipg, int spp, long long total
 template <typename T>
 hipError_t segment_sum(const T* in, int groups, int cnt, long long len, T* out, T* peak,
                        T* scratch, hipStream_t st);
+// The ILT parameter behind a mask gradient (opc/ilt.py:381-389 without the
+// Gaussian): d/dtheta = d/dm (beta m (1 - m) g), m = sigmoid(clip(beta theta,
+// +-60)), g read from device memory (null: 1).
+template <typename T>
+struct ThetaEpi {
+  const T* theta;
+  T beta;
+  const T* g;
+};
+// band_synthesis (herm = 1, real output) of a mask gradient written as
+// d/dtheta (ThetaEpi) in the same pass; hipErrorNotSupported, nothing
+// launched, unless the band is symmetric about 0 inside |nu| < N / 16 and
+// theta and grad_theta are 16-byte aligned.  TT as band_synthesis.
+template <typename T>
+hipError_t band_synthesis_theta(int N, const cx<T>* Y, int nimg, int nu0, int Wb, T s, cx<T>* TT,
+                                ThetaEpi<T> te, T* grad_theta, hipStream_t st);
 // Per-pixel relaxed-resist objective (opc/ilt.py:311-330): v = sign (scale I -
 // level), z = sigmoid(a v), cost (z - t)^2, dI = dscale (z - t) z (1 - z) with
 // dscale = sign 2 a scale / npix.

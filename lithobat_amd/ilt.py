@@ -117,10 +117,14 @@ class ILTObjective(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         spec, d, theta = ctx.saved_tensors
-        dm = (ctx.plan.backward_sensitivity_band(spec, d) if ctx.band
-              else ctx.plan.backward_spectrum(spec, d))
         th = theta[None] if ctx.squeeze else theta
-        dth = mask_sigmoid_grad_theta(dm, th.to(dm.dtype), ctx.beta, g)
+        # the sigmoid derivative inside the mask-gradient synthesis where the band allows
+        dth = (ctx.plan.backward_objective_theta(spec, d, th, ctx.beta, g) if ctx.band
+               else None)
+        if dth is None:
+            dm = (ctx.plan.backward_sensitivity_band(spec, d) if ctx.band
+                  else ctx.plan.backward_spectrum(spec, d))
+            dth = mask_sigmoid_grad_theta(dm, th.to(dm.dtype), ctx.beta, g)
         return ((dth[0] if ctx.squeeze else dth).to(ctx.dtype), None, None, None, None, None,
                 None, None)
 

@@ -403,6 +403,29 @@ class ImagingPlan:
                 ws.numel(), st))
         return out
 
+    def backward_objective_theta(self, spec: torch.Tensor, sens: torch.Tensor,
+                                 theta: torch.Tensor, beta: float, g: torch.Tensor | None = None):
+        """``backward_sensitivity_band`` followed by ``objective.mask_sigmoid_grad_theta`` in
+        one pass (lb_backward_objective_theta): d/dtheta [B, N, N] of the ILT parameter
+        whose mask sigmoid(clip(beta theta, +-60)) gave ``spec``, scaled by the device
+        scalar ``g``.  None when the plan has no such pass."""
+        B = self._check_spectrum(spec)
+        th = theta.to(self.real_dtype).reshape(B, self.n, self.n).contiguous()
+        gs = None if g is None else g.to(self.real_dtype).reshape(()).contiguous()
+        out = torch.empty((B, self.n, self.n), dtype=self.real_dtype, device=spec.device)
+        with self._lock:
+            self._use_tuning(B)
+            ws = self._workspace(self.workspace_bytes(B, True), spec.device)
+            st = torch.cuda.current_stream(spec.device).cuda_stream
+            rc = self._lib.lb_backward_objective_theta(
+                self._h, spec.data_ptr(), B, sens.data_ptr(), th.data_ptr(), float(beta),
+                gs.data_ptr() if gs is not None else None, out.data_ptr(), ws.data_ptr(),
+                ws.numel(), st)
+            if rc == _lib.LB_ERR_UNSUPPORTED:
+                return None
+            _lib.check(rc)
+        return out
+
     def fields_backward(self, fields: torch.Tensor, sensitivity: torch.Tensor,
                         complex_out: bool = False) -> torch.Tensor:
         """SOCS plans: socs_gradient on the fields given, [K, N, N] or [B, K, N, N] complex,

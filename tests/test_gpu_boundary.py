@@ -338,3 +338,18 @@ def test_objective_entry_points_refuse_what_they_cannot_run(device):
                                     big.data_ptr(), need, st) == _lib.LB_OK
     torch.cuda.synchronize()
     assert torch.isfinite(torch.view_as_real(sb)).all()
+    # the theta-gradient pass: unaligned theta refused, aligned theta runs
+    bw = torch.empty(plan.workspace_bytes(1, True), dtype=torch.uint8, device=device)
+    gt = torch.empty(512 * 512 + 4, device=device)
+    assert lib.lb_backward_objective_theta(plan._h, spec.data_ptr(), 1, sb.data_ptr(),
+                                           tgt.data_ptr() + 4, 8.0, None, gt.data_ptr(),
+                                           bw.data_ptr(), bw.numel(), st) == _lib.LB_ERR_UNSUPPORTED
+    assert "aligned" in lib.lb_last_error().decode()
+    assert lib.lb_backward_objective_theta(plan._h, spec.data_ptr(), 1, sb.data_ptr(),
+                                           tgt.data_ptr(), 8.0, None, gt.data_ptr(),
+                                           bw.data_ptr(), bw.numel(), st) == _lib.LB_OK
+    torch.cuda.synchronize()
+    assert torch.isfinite(gt[:512 * 512]).all()
+    assert lib.lb_backward_objective_theta(small._h, spec.data_ptr(), 1, sb.data_ptr(),
+                                           tgt.data_ptr(), 8.0, None, gt.data_ptr(),
+                                           bw.data_ptr(), bw.numel(), st) == _lib.LB_ERR_UNSUPPORTED
