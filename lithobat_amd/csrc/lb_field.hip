@@ -905,9 +905,12 @@ __global__ __launch_bounds__(NT) void k_rows_staged(MapSource<T, M> src, int img
   const int nrc = (W + F::CPR - 1) / F::CPR;
   const long long n_tasks = (long long)n_img * spp * nrc;
   const int rl1 = tid / F::T1, j1 = tid - rl1 * F::T1;
-  for (long long task = blockIdx.x; task < n_tasks; task += gridDim.x) {
+  // this thread's stage-1 inputs of a task (mask band and transfer values),
+  // loaded one task ahead so their latency overlaps the current task
+  cx<T> xv[F::NRW], hv[F::NRW];
+  auto fetch = [&](long long task) {
     const int rc = (int)(task % nrc);
-    const long long slot = task / nrc;  // (image in pass) * spp + system in plane
+    const long long slot = task / nrc;
     const int jl = (int)(slot % spp);
     const int img = img0 + (int)(slot / spp);
     const int plane = img % n_planes, mask = img / n_planes;
@@ -915,21 +918,29 @@ __global__ __launch_bounds__(NT) void k_rows_staged(MapSource<T, M> src, int img
     const cx<T>* Xm = src.X + (long long)mask * src.Wb * src.Wb;
     const cx<T>* Hj = src.H + (long long)j * W * W;
     const int2 bx = src.box[j];
+    const int a = rc * F::CPR + rl1;
+    const int xr = wrap_idx(bx.x + (a < W ? a : 0), src.wrapN);
+#pragma unroll
+    for (int r = 0; r < F::NRW; ++r) {
+      const int b = j1 + r * F::T1;
+      const bool in = a < W && b < W;
+      xv[r] = in ? Xm[(long long)xr * src.Wb + wrap_idx(bx.y + b, src.wrapN)] : cx<T>{0, 0};
+      hv[r] = in ? Hj[a * W + b] : cx<T>{0, 0};
+    }
+  };
+  if (blockIdx.x < n_tasks) fetch(blockIdx.x);
+  for (long long task = blockIdx.x; task < n_tasks; task += gridDim.x) {
+    const int rc = (int)(task % nrc);
+    const long long slot = task / nrc;  // (image in pass) * spp + system in plane
     cx<T>* Gt = G + slot * W * M;
     const int a0 = rc * F::CPR;
     {
       const int a = a0 + rl1;
-      if (a < W) {
-        cx<T> v[F::R1];
-        const int xr = wrap_idx(bx.x + a, src.wrapN);
+      cx<T> v[F::R1];
 #pragma unroll
-        for (int r = 0; r < F::R1; ++r) {
-          const int b = j1 + r * F::T1;
-          if (r < F::NRW && b < W)
-            v[r] = Xm[(long long)xr * src.Wb + wrap_idx(bx.y + b, src.wrapN)] * Hj[a * W + b];
-          else
-            v[r] = cx<T>{0, 0};
-        }
+      for (int r = 0; r < F::R1; ++r) v[r] = r < F::NRW ? xv[r] * hv[r] : cx<T>{0, 0};
+      if (task + gridDim.x < n_tasks) fetch(task + gridDim.x);
+      if (a < W) {
         dft<F::R1, +1>(v);
 #pragma unroll
         for (int r = 0; r < F::R1; ++r) S[rl1 * L::LDR + j1 * L::SP + r] = v[r];
