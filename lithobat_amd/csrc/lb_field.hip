@@ -1183,15 +1183,40 @@ __global__ __launch_bounds__(NT) void k_rows_adj_staged(MapSource<T, M> src, int
     for (int e = tid; e < Wb * Wb; e += NT) gout[e] = cx<T>{0, 0};
     __syncthreads();
     const int a0 = rc * F::CPR;
-    for (int j = iv.s0; j < iv.s1; ++j) {
+    // this thread's share of a system's CPR table rows, loaded one system ahead
+    constexpr int NPT = (F::CPR * M + NT - 1) / NT;
+    cx<T> nt[NPT];
+    auto fetch = [&](int j) {
       const cx<T>* Gt = Gi + (long long)j * W * M;
+#pragma unroll
+      for (int i = 0; i < NPT; ++i) {
+        const int e = tid + i * NT, rl = e / M, x = e - rl * M;
+        nt[i] = e < F::CPR * M && a0 + rl < W ? Gt[(long long)(a0 + rl) * M + x] : cx<T>{0, 0};
+      }
+    };
+    if (iv.s0 < iv.s1) fetch(iv.s0);
+    for (int j = iv.s0; j < iv.s1; ++j) {
       const int2 bx = src.box[j];
       const T w2 = (T)2 * src.wsys[j];
       const cx<T>* Hj = src.H + (long long)j * W * W;
-      for (int e = tid; e < F::CPR * M; e += NT) {
-        const int rl = e / M, x = e - rl * M;
-        const int a = a0 + rl;
-        S[rl * RL::LDR + rpos<M, F::R1>(x)] = a < W ? Gt[(long long)a * M + x] : cx<T>{0, 0};
+#pragma unroll
+      for (int i = 0; i < NPT; ++i) {
+        const int e = tid + i * NT, rl = e / M, x = e - rl * M;
+        if (e < F::CPR * M) S[rl * RL::LDR + rpos<M, F::R1>(x)] = nt[i];
+      }
+      if (j + 1 < iv.s1) fetch(j + 1);
+      // this system's transfer values and accumulator entries for the stage 1
+      // below, loaded before the stage-2 work so their latency overlaps it (the
+      // previous system's stores to the entries are visible after its barrier)
+      const int a1 = a0 + rl1;
+      const int xr1 = wrap_idx(bx.x + (a1 < W ? a1 : 0), wrapN);
+      cx<T> hv[F::NRW], old[F::NRW];
+#pragma unroll
+      for (int r = 0; r < F::NRW; ++r) {
+        const int b = j1r + r * F::T1;
+        const bool in = a1 < W && b < W;
+        hv[r] = in ? Hj[a1 * W + b] : cx<T>{0, 0};
+        old[r] = in ? gout[(long long)xr1 * Wb + wrap_idx(bx.y + b, wrapN)] : cx<T>{0, 0};
       }
       __syncthreads();
 #pragma unroll
@@ -1211,21 +1236,17 @@ __global__ __launch_bounds__(NT) void k_rows_adj_staged(MapSource<T, M> src, int
       }
       __syncthreads();
       {
-        const int a = a0 + rl1;
         cx<T> v[F::R1];
 #pragma unroll
         for (int r = 0; r < F::R1; ++r) v[r] = S[rl1 * RL::LDR + j1r * RL::SP + r];
         dft<F::R1, -1>(v);
-        if (a < W) {
-          const int xr = wrap_idx(bx.x + a, wrapN);
+        if (a1 < W) {
 #pragma unroll
           for (int r = 0; r < F::NRW; ++r) {
             const int b = j1r + r * F::T1;
-            if (b < W) {
-              const cx<T> gc = scale(mulc(v[r], Hj[a * W + b]), w2);
-              cx<T>* dst = gout + (long long)xr * Wb + wrap_idx(bx.y + b, wrapN);
-              *dst = *dst + gc;
-            }
+            if (b < W)
+              gout[(long long)xr1 * Wb + wrap_idx(bx.y + b, wrapN)] =
+                  old[r] + scale(mulc(v[r], hv[r]), w2);
           }
         }
       }
