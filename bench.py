@@ -268,8 +268,10 @@ def ilt_leg(args, ilt, dev, sync):
                                            else ILT_BATCH / gf_batch),
              "definition": "ILTObjective: theta -> relaxed-resist cost of the Abbe image of "
                            "sigmoid(8 theta) in one autograd function (sigmoid inside the band "
-                           "analysis, loss gradient and sigmoid derivative in one device pass); "
-                           "same value and gradient as the composed path"}
+                           "analysis, objective inside the image synthesis, adjoint from the "
+                           "intensity derivative's band spectrum, sigmoid derivative inside the "
+                           "mask-gradient synthesis); same value and gradient as the composed "
+                           "path"}
     return (one, batched, dense, blurred, g_one,
             None if g_batch is None else g_batch / ILT_BATCH, roof, fused)
 
