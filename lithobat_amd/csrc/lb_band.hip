@@ -688,25 +688,43 @@ __device__ __forceinline__ void block_peak(T lmax, cx<T>* bufs, int lane, int wa
 // (real output: row y + i row y+1, valid because the band is Hermitian);
 // sparse first stage as above, outputs scaled and stored straight to HBM,
 // per-image running maximum for the peak.
-template <typename T, int N, bool COUT, bool SP>
+// HALF (real output): TT holds the columns nu = 0..Q of a band symmetric
+// about 0 (see k_synth_cols); the input at nu < 0 is the conjugate of the
+// stored column -nu, for each row of the pair.
+template <typename T, int N, bool COUT, bool SP, bool HALF = false>
 __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const cx<T>* __restrict__ TT,
                                                              int nu0, int Wb, T s,
                                                              const cx<T>* __restrict__ twg,
                                                              T* __restrict__ out,
                                                              T* __restrict__ peak) {
+  static_assert(!(HALF && COUT), "the half band is for real output");
   LB_BAND_PROLOGUE(T, N)
   constexpr int NTR = COUT ? N : N / 2;
   const int tr0 = blockIdx.x * TPB + wave * TPW;
   const int ntr = max(0, min(TPW, NTR - tr0));
-  const int ld = band_ld(Wb);
+  const int Qh = (Wb - 1) / 2;
+  const int ld = band_ld(HALF ? Qh + 1 : Wb);
   const cx<T>* src = TT + (long long)img * N * ld;
   auto load = [&](int tr, int b) -> cx<T> {
-    const int v = band_pos(b, N, nu0, Wb);
-    if (v < 0) return cx<T>{0, 0};
+    int v;
+    bool neg = false;
+    if constexpr (HALF) {
+      const int nu = nu_of_bin(b, N);
+      neg = nu < 0;
+      v = neg ? -nu : nu;
+      if (v > Qh) return cx<T>{0, 0};
+    } else {
+      v = band_pos(b, N, nu0, Wb);
+      if (v < 0) return cx<T>{0, 0};
+    }
     const int p = tr0 + tr;
     if (COUT) return src[(long long)p * ld + v];
-    const cx<T> a = src[(long long)(2 * p) * ld + v];
-    const cx<T> c = src[(long long)(2 * p + 1) * ld + v];
+    cx<T> a = src[(long long)(2 * p) * ld + v];
+    cx<T> c = src[(long long)(2 * p + 1) * ld + v];
+    if (HALF && neg) {
+      a = conj(a);
+      c = conj(c);
+    }
     return cx<T>{a.re - c.im, a.im + c.re};  // a + i c
   };
   T lmax = 0;
@@ -1364,8 +1382,8 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
   constexpr int S0 = N / Plan<N>::R0;
   const bool sp = nu0 > -S0 && nu0 + Wb - 1 < S0;
   // real output of a Hermitian band symmetric about 0: half the columns
-  const bool half = Plan<N>::R1 > 1 && sp && herm && !cout && Wb % 2 == 1 && 2 * nu0 + Wb - 1 == 0;
-  auto k1 = half ? k_synth_cols<T, N, true, true>
+  const bool half = herm && !cout && Wb % 2 == 1 && 2 * nu0 + Wb - 1 == 0;
+  auto k1 = half ? (sp ? k_synth_cols<T, N, true, true> : k_synth_cols<T, N, false, true>)
                  : (sp ? k_synth_cols<T, N, true> : k_synth_cols<T, N, false>);
   const size_t smc = col_smem_bytes<T, N>();
   if ((e = set_smem(k1, smc)) != hipSuccess) return e;
@@ -1402,7 +1420,9 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
     hipLaunchKernelGGL(k2, dim3(blocks_for(N, TPB), nimg), dim3(NT), sm, st,
                        (const cx<T>*)TT, nu0, Wb, s, twg, out, (T*)nullptr);
   } else {
-    auto k2 = sp ? k_synth_rows<T, N, false, true> : k_synth_rows<T, N, false, false>;
+    auto k2 = half ? (sp ? k_synth_rows<T, N, false, true, true>
+                         : k_synth_rows<T, N, false, false, true>)
+                   : (sp ? k_synth_rows<T, N, false, true> : k_synth_rows<T, N, false, false>);
     if ((e = set_smem(k2, sm)) != hipSuccess) return e;
     hipLaunchKernelGGL(k2, dim3(blocks_for(N / 2, TPB), nimg), dim3(NT), sm, st,
                        (const cx<T>*)TT, nu0, Wb, s, twg, out, peak);

@@ -66,6 +66,45 @@ def test_staged_matches_single_launch_and_oracle_m256(device, precision):
         assert _rel(img[3, 1].cpu(), o) <= 1e-12
 
 
+def _passes(plan, n_masks):
+    """Passes of row tables a staged forward of n_masks masks runs (160 MiB per pass)."""
+    info = plan.info
+    per_img = info["box"] * info["m"] * (16 if info["precision"] == 64 else 8) * info["n_systems"]
+    n_img = n_masks * info["n_planes"]
+    return -(-n_img // max(1, min(n_img, (160 << 20) // per_img)))
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_staged_several_passes_m256(device, precision):
+    """A batch of several passes of row tables (the last partly filled) against calls of
+    2 masks (one pass each) and the single-launch kernel on 2 of the masks, with whole
+    items (each column block runs its image's every system, inputs loaded ahead) and
+    items of 3 systems."""
+    import torch
+    from lithobat_amd.config import GridConfig
+    from lithobat_amd.masks import manhattan
+    from lithobat_amd.plan import ImagingPlan
+    grid = GridConfig(n_pixels=512, pixel_size=4e-9)
+    plan = ImagingPlan(_euv(), grid, [(-40e-9, 1.0), (25e-9, 1.0)], precision=precision)
+    assert plan.info["m"] == 256
+    n = 19 if precision == 64 else 37
+    print(f"staged passes: FP{precision}, {n} masks, {_passes(plan, n)} passes", flush=True)
+    assert _passes(plan, n) >= 2 and _passes(plan, 2) == 1
+    dt = torch.float64 if precision == 64 else torch.float32
+    masks_np = np.stack([manhattan(512, 4e-9, seed=400 + i) for i in range(n)])
+    masks = torch.tensor(masks_np, device=device, dtype=dt)
+    tol = 1e-12 if precision == 64 else 1e-5
+    for shape in [(256, 0, 0), (512, 0, 3)]:
+        big, big_pk = _forward(plan, masks, shape)
+        parts = [_forward(plan, masks[i:i + 2], shape) for i in range(0, n, 2)]
+        small = torch.cat([p[0] for p in parts]).cpu()
+        small_pk = torch.cat([p[1] for p in parts]).cpu()
+        assert _rel(big.cpu(), small) <= tol, shape
+        assert _rel(big_pk.cpu(), small_pk) <= tol, shape
+    ref, _ = _forward(plan, masks[n - 2:], _single_launch(plan))
+    assert _rel(big[n - 2:].cpu(), ref.cpu()) <= tol
+
+
 def test_staged_m128_and_single_image(device):
     """The staged shape on a 128-point coarse grid (radix 16 x 8, two stage-2 tasks per
     thread) and a single mask (one pass, chunked items reduced by the split sum)."""
