@@ -549,13 +549,15 @@ lb_status fwd_post(Plan& p, void* images, void* peak, char* ws, const Work& w, h
       LB_HIP(fused);  // the peaks are zeroed there
     } else {
       if (pk) LB_HIP(fill<T>(pk, (T)0, w.n_img, st));
+      // one work item per image: its partial is the image's coarse intensity
+      const T* im = w.spread || w.n_chunks > 1 ? IM : partial;
       if (w.spread)
         LB_HIP(spread_sum<T>(partial, w.n_img, 1, p.spp, total, w.runs, w.n_chunks,
                              (long long)p.M * p.M, IM, (T*)nullptr, st));
-      else
+      else if (w.n_chunks > 1)
         LB_HIP(segment_sum<T>(partial, w.n_img, w.n_chunks, (long long)p.M * p.M, IM,
                               (T*)nullptr, (T*)(ws + w.seg), st));
-      LB_HIP(band_analysis<T>(p.M, IM, 0, (long long)p.M * p.M, 0, 1, w.n_img, -p.QI, p.WbI,
+      LB_HIP(band_analysis<T>(p.M, im, 0, (long long)p.M * p.M, 0, 1, w.n_img, -p.QI, p.WbI,
                               invM2, (cx<T>*)(ws + w.RTc), Y, st));
     }
     if (obj) {

@@ -585,21 +585,26 @@ __global__ __launch_bounds__((64 * col_waves<T, N>())) void k_analysis_cols(cons
   const int c0 = wave * TPW;
   const int ntr = max(0, min(TPW, ncol - c0));
   cx<T>* wb = buf + c0 * LD;
-  cx<T>* out = X + (long long)img * Wb * Wb;
   auto load = [&](int tr, int y) -> cx<T> { return wb[tr * LD + pidx<T>(y)]; };
-  auto store = [&](int tr, int b, cx<T> val) {
-    const int u = band_pos(b, N, nu0, Wb);
-    if (u < 0) return;
-    const int v = vb + c0 + tr;
+  auto store = [&](int tr, int b, cx<T> val) { wb[tr * LD + pidx<T>(b)] = val; };
+  wave_fft<T, -1, N, TPW, LD>(wb, tw, lane, ntr, load, store);
+  __syncthreads();
+  // the band rows of the block's columns, written out cooperatively as runs
+  // of ncol values (a lane per band row would scatter single values Wb apart)
+  cx<T>* out = X + (long long)img * Wb * Wb;
+  for (int t = tid; t < Wb * CPB; t += NT) {
+    const int u = t / CPB, c = t - u * CPB;
+    const int b = bin_of(nu0 + u, N);
+    if (c >= ncol || band_pos(b, N, nu0, Wb) != u) continue;
+    const cx<T> y = scale(buf[c * LD + pidx<T>(b)], s);
+    const int v = vb + c;
     if (HALF) {
-      const cx<T> y = scale(val, s);
       out[(long long)u * Wb + Qh + v] = y;
       if (v > 0) out[(long long)(2 * Qh - u) * Wb + Qh - v] = conj(y);
     } else {
-      out[(long long)u * Wb + v] = scale(val, s);
+      out[(long long)u * Wb + v] = y;
     }
-  };
-  wave_fft<T, -1, N, TPW, LD>(wb, tw, lane, ntr, load, store);
+  }
 }
 
 // --------------------------------------------------------------- synthesis
