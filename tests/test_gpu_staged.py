@@ -79,7 +79,8 @@ def test_staged_several_passes_m256(device, precision):
     """A batch of several passes of row tables (the last partly filled) against calls of
     2 masks (one pass each) and the single-launch kernel on 2 of the masks, with whole
     items (each column block runs its image's every system, inputs loaded ahead) and
-    items of 3 systems."""
+    items of 3 systems, and one item per image (whose partials are the coarse intensities,
+    taken by the band analysis as they are); the last against the single-launch kernel."""
     import torch
     from lithobat_amd.config import GridConfig
     from lithobat_amd.masks import manhattan
@@ -94,8 +95,11 @@ def test_staged_several_passes_m256(device, precision):
     masks_np = np.stack([manhattan(512, 4e-9, seed=400 + i) for i in range(n)])
     masks = torch.tensor(masks_np, device=device, dtype=dt)
     tol = 1e-12 if precision == 64 else 1e-5
-    for shape in [(256, 0, 0), (512, 0, 3)]:
+    spp = plan.info["n_systems"]
+    for shape in [(256, 0, 0), (512, 0, 3), (256, 0, spp)]:
         big, big_pk = _forward(plan, masks, shape)
+        if shape[2] == spp:
+            assert plan.work_items(n) == (spp, 2 * n)  # one work item per image
         parts = [_forward(plan, masks[i:i + 2], shape) for i in range(0, n, 2)]
         small = torch.cat([p[0] for p in parts]).cpu()
         small_pk = torch.cat([p[1] for p in parts]).cpu()
