@@ -738,10 +738,11 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
     if (COUT) {
       reinterpret_cast<cx<T>*>(out)[((long long)img * N + p) * N + x] = scale(val, s);
     } else {
+      // the image is written once and not read back here: streaming stores
       const T a = val.re * s, b = val.im * s;
-      T* o = out + (long long)img * N * N;
-      o[(long long)(2 * p) * N + x] = a;
-      o[(long long)(2 * p + 1) * N + x] = b;
+      T* o = out + (long long)img * N * N + (long long)(2 * p) * N + x;
+      __builtin_nontemporal_store(a, o);
+      __builtin_nontemporal_store(b, o + N);
       lmax = fmax(lmax, fmax(a, b));
     }
   };
