@@ -696,7 +696,9 @@ __device__ __forceinline__ void block_peak(T lmax, cx<T>* bufs, int lane, int wa
 // HALF (real output): TT holds the columns nu = 0..Q of a band symmetric
 // about 0 (see k_synth_cols); the input at nu < 0 is the conjugate of the
 // stored column -nu, for each row of the pair.
-template <typename T, int N, bool COUT, bool SP, bool HALF = false>
+// STREAM (real output): the image leaves in streaming stores, for batches
+// too large to be read back from the caches.
+template <typename T, int N, bool COUT, bool SP, bool HALF = false, bool STREAM = false>
 __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const cx<T>* __restrict__ TT,
                                                              int nu0, int Wb, T s,
                                                              const cx<T>* __restrict__ twg,
@@ -738,11 +740,15 @@ __global__ __launch_bounds__((64 * row_waves<T, N>())) void k_synth_rows(const c
     if (COUT) {
       reinterpret_cast<cx<T>*>(out)[((long long)img * N + p) * N + x] = scale(val, s);
     } else {
-      // the image is written once and not read back here: streaming stores
       const T a = val.re * s, b = val.im * s;
       T* o = out + (long long)img * N * N + (long long)(2 * p) * N + x;
-      __builtin_nontemporal_store(a, o);
-      __builtin_nontemporal_store(b, o + N);
+      if constexpr (STREAM) {
+        __builtin_nontemporal_store(a, o);
+        __builtin_nontemporal_store(b, o + N);
+      } else {
+        o[0] = a;
+        o[N] = b;
+      }
       lmax = fmax(lmax, fmax(a, b));
     }
   };
@@ -1426,9 +1432,18 @@ static hipError_t synthesis_n(const cx<T>* Y, int nimg, int nu0, int Wb, int her
     hipLaunchKernelGGL(k2, dim3(blocks_for(N, TPB), nimg), dim3(NT), sm, st,
                        (const cx<T>*)TT, nu0, Wb, s, twg, out, (T*)nullptr);
   } else {
-    auto k2 = half ? (sp ? k_synth_rows<T, N, false, true, true>
-                         : k_synth_rows<T, N, false, false, true>)
-                   : (sp ? k_synth_rows<T, N, false, true> : k_synth_rows<T, N, false, false>);
+    // a batch of 64 MB or more of images is not read back from the caches:
+    // streaming stores (SOCS, 256 images at 512^2: 1.11 -> 1.08 ms per forward;
+    // a single small image read back at once by an ILT step is faster cached)
+    const bool stream = (long long)nimg * N * N * (long long)sizeof(T) >= (64LL << 20);
+    auto k2 = half ? (sp ? (stream ? k_synth_rows<T, N, false, true, true, true>
+                                   : k_synth_rows<T, N, false, true, true>)
+                         : (stream ? k_synth_rows<T, N, false, false, true, true>
+                                   : k_synth_rows<T, N, false, false, true>))
+                   : (sp ? (stream ? k_synth_rows<T, N, false, true, false, true>
+                                   : k_synth_rows<T, N, false, true>)
+                         : (stream ? k_synth_rows<T, N, false, false, false, true>
+                                   : k_synth_rows<T, N, false, false>));
     if ((e = set_smem(k2, sm)) != hipSuccess) return e;
     hipLaunchKernelGGL(k2, dim3(blocks_for(N / 2, TPB), nimg), dim3(NT), sm, st,
                        (const cx<T>*)TT, nu0, Wb, s, twg, out, peak);
